@@ -1,0 +1,200 @@
+"""ctypes binding of libbimsim_hip.so, the gfx950 BIMSim near-field and
+imaging library (C ABI in include/bimsim_hip.h).
+
+Device buffers are torch tensors on the GPU (torch supplies device memory and
+the stream); every compute call goes to the HIP library and raises
+BimsimError when the library reports a failure.  There is no CPU fallback:
+loading fails loudly when libbimsim_hip.so has not been built.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbimsim_hip.so")
+
+c_int, c_double, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+
+# bimsim_status / enums (include/bimsim_hip.h)
+OK, E_INVALID, E_HIP, E_NOMEM, E_UNSUPPORTED, E_NODEVICE = 0, -1, -2, -3, -4, -5
+FP64_PARITY, FP64, FP32_FAST = 0, 1, 2
+QUIRKS_OFF, QUIRKS_REF = 0, 1
+UF_SERIES, UF_PLANE, UF_LUT = 0, 1, 2
+US_SERIES, US_LUT = 0, 1
+RNG_GLIBC, RNG_SPLITMIX64 = 0, 1
+PI_F = 3.1415901184082031   # float(3.14159f), reference dataTypes.h:17
+
+EXPORTS = (
+    "bimsim_abi_version", "bimsim_ctx_create", "bimsim_ctx_destroy", "bimsim_sync",
+    "bimsim_last_error", "bimsim_set_timing", "bimsim_get_timings", "bimsim_sphere_order",
+    "bimsim_mc_sample_na", "bimsim_mie_coeffs", "bimsim_nf_uf", "bimsim_nf_us",
+    "bimsim_nf_sum_uf", "bimsim_detector",
+)
+
+
+class BimsimError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"bimsim error {code}: {msg}")
+        self.code = code
+
+
+class Quad(ctypes.Structure):
+    """bimsim_quad (rts::quad): corner A, edges X and Y."""
+    _fields_ = [("A", c_double * 3), ("X", c_double * 3), ("Y", c_double * 3)]
+
+    @classmethod
+    def xy(cls, size, z=0.0):
+        """The --xy size z slice (arguments.h:340-360 via rts::quad(pMin, pMax, n))."""
+        q = cls()
+        q.A[:] = [-size / 2.0, -size / 2.0, z]
+        q.X[:] = [size, 0.0, 0.0]
+        q.Y[:] = [0.0, size, 0.0]
+        return q
+
+
+class Sphere(ctypes.Structure):
+    """bimsim_sphere."""
+    _fields_ = [("p", c_double * 3), ("a", c_double), ("n", c_double * 2),
+                ("Nl", c_int), ("coeff_off", c_int)]
+
+
+def sphere_array(spheres):
+    arr = (Sphere * max(len(spheres), 1))()
+    for i, s in enumerate(spheres):
+        arr[i].p[:] = list(s["p"])
+        arr[i].a = s["a"]
+        n = s["n"]
+        arr[i].n[:] = [n.real, n.imag] if isinstance(n, complex) else list(n)
+        arr[i].Nl = s["Nl"]
+        arr[i].coeff_off = s.get("coeff_off", 0)
+    return arr
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run `make -C bimsim-cuda_amd` "
+                              "(or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        L.bimsim_last_error.restype = ctypes.c_char_p
+        L.bimsim_last_error.argtypes = [c_void_p]
+        L.bimsim_ctx_create.argtypes = [c_int, c_void_p, ctypes.POINTER(c_void_p)]
+        for name in ("bimsim_ctx_destroy", "bimsim_sync"):
+            getattr(L, name).argtypes = [c_void_p]
+        L.bimsim_set_timing.argtypes = [c_void_p, c_int]
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.bimsim_get_timings.argtypes = [c_void_p, fp, fp, fp]
+        L.bimsim_sphere_order.argtypes = [c_double, c_double]
+        L.bimsim_mc_sample_na.argtypes = [c_void_p, c_int, c_double * 3, c_double, c_double,
+                                          c_int, ctypes.c_ulonglong]
+        L.bimsim_mie_coeffs.argtypes = [c_void_p, c_void_p, c_int, c_double, c_int, c_void_p,
+                                        c_void_p]
+        L.bimsim_nf_uf.argtypes = [c_void_p, ctypes.POINTER(Quad), c_int, c_int, c_double * 3,
+                                   c_double * 3, c_double, c_double, c_double, c_double, c_int,
+                                   c_int, c_int, c_int, c_void_p]
+        L.bimsim_nf_us.argtypes = [c_void_p, ctypes.POINTER(Quad), c_int, c_int, c_double * 3,
+                                   c_double, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int,
+                                   c_double, c_int, c_int, c_void_p, c_int]
+        L.bimsim_nf_sum_uf.argtypes = [c_void_p, ctypes.POINTER(Quad), c_int, c_int, c_void_p,
+                                       c_int, c_int, c_void_p, c_void_p]
+        L.bimsim_detector.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_int,
+                                      c_double, c_double, c_double, c_double, c_double, c_int,
+                                      c_int, c_int, c_void_p, c_void_p, c_void_p, c_int]
+        _lib = L
+    return _lib
+
+
+def sphere_order(a, lam):
+    """sphere::calcNl (sphere.h:71-74)."""
+    return lib().bimsim_sphere_order(a, lam)
+
+
+def mc_sample_na(N, k, NAin, NAout, rng=RNG_GLIBC, seed=0):
+    """mcSampleNA (montecarlo.cpp:9-71); returns a list of N (x, y, z)."""
+    out = (c_double * (3 * N))()
+    rc = lib().bimsim_mc_sample_na(out, N, (c_double * 3)(*k), NAin, NAout, rng, seed)
+    if rc != OK:
+        raise BimsimError(rc, "invalid Monte-Carlo sampling arguments")
+    return [tuple(out[3 * i:3 * i + 3]) for i in range(N)]
+
+
+def _ptr(t):
+    return c_void_p(t.data_ptr()) if t is not None else None
+
+
+class Context:
+    """A bimsim_ctx bound to a device and a stream (torch's current stream by default)."""
+
+    def __init__(self, device=0, stream=None):
+        L = lib()
+        self.L = L
+        h = c_void_p()
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream(device).cuda_stream
+        rc = L.bimsim_ctx_create(device, c_void_p(stream), ctypes.byref(h))
+        if rc != OK:
+            raise BimsimError(rc, "bimsim_ctx_create failed (no gfx950 device?)")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.bimsim_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != OK:
+            raise BimsimError(rc, self.L.bimsim_last_error(self.h).decode())
+
+    def sync(self):
+        self._check(self.L.bimsim_sync(self.h))
+
+    def set_timing(self, on):
+        self._check(self.L.bimsim_set_timing(self.h, 1 if on else 0))
+
+    def timings(self):
+        a, b, c = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+        self._check(self.L.bimsim_get_timings(self.h, ctypes.byref(a), ctypes.byref(b),
+                                              ctypes.byref(c)))
+        return {"t_uf_ms": a.value, "t_us_ms": b.value, "t_det_ms": c.value}
+
+    # ---- entry points (device tensors)
+    def mie_coeffs(self, spheres, kmag, A, B, quirks=QUIRKS_REF):
+        S = sphere_array(spheres)
+        self._check(self.L.bimsim_mie_coeffs(self.h, S, len(spheres), kmag, quirks, _ptr(A),
+                                             _ptr(B)))
+
+    def nf_uf(self, q, uR, vR, focus, kdir, kmag, amp, ca, cb, m, Uf, mode=UF_SERIES,
+              prec=FP64_PARITY, quirks=QUIRKS_REF):
+        self._check(self.L.bimsim_nf_uf(self.h, ctypes.byref(q), uR, vR, (c_double * 3)(*focus),
+                                        (c_double * 3)(*kdir), kmag, amp, ca, cb, m, mode, prec,
+                                        quirks, _ptr(Uf)))
+
+    def nf_us(self, q, uR, vR, focus, kmag, spheres, A, B, waves, subA, U, prec=FP64_PARITY,
+              quirks=QUIRKS_REF, accumulate=False):
+        S = sphere_array(spheres)
+        nk = waves.numel() // 3
+        self._check(self.L.bimsim_nf_us(self.h, ctypes.byref(q), uR, vR, (c_double * 3)(*focus),
+                                        kmag, S, len(spheres), _ptr(A), _ptr(B), _ptr(waves), nk,
+                                        subA, prec, quirks, _ptr(U), 1 if accumulate else 0))
+
+    def nf_sum_uf(self, q, uR, vR, spheres, Uf, U, prec=FP64_PARITY):
+        S = sphere_array(spheres)
+        self._check(self.L.bimsim_nf_sum_uf(self.h, ctypes.byref(q), uR, vR, S, len(spheres),
+                                            prec, _ptr(Uf), _ptr(U)))
+
+    def detector(self, U, Uf, uR, vR, du, dv, NAin, NAout, lam, padding, Nd, ss, D, Di,
+                 Abs=None, prec=FP64_PARITY, quirks=QUIRKS_REF, write_filtered=True):
+        self._check(self.L.bimsim_detector(self.h, prec, quirks, _ptr(U), _ptr(Uf), uR, vR, du,
+                                           dv, NAin, NAout, lam, padding, Nd, ss, _ptr(D),
+                                           _ptr(Di), _ptr(Abs), 1 if write_filtered else 0))

@@ -1,0 +1,180 @@
+// Context lifecycle, errors, scratch, timers, and the host-side helpers of the
+// C ABI (sphere order, Monte-Carlo directions).
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+
+#include "internal.h"
+
+namespace bimsim {
+
+int set_error(bimsim_ctx* ctx, int code, const char* fmt, ...) {
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return code;
+}
+
+int scratch_get(bimsim_ctx* ctx, int slot, size_t bytes, void** out) {
+    auto& s = ctx->scratch[slot];
+    if (s.second < bytes) {
+        if (s.first) {
+            // the previous user of the slot may still be in flight
+            BIMSIM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+            BIMSIM_HIP_CHECK(ctx, hipFree(s.first));
+            s.first = nullptr;
+            s.second = 0;
+        }
+        size_t want = bytes + bytes / 8 + 256;
+        if (hipMalloc(&s.first, want) != hipSuccess) {
+            s.first = nullptr;
+            (void)hipGetLastError();
+            return set_error(ctx, BIMSIM_E_NOMEM, "device scratch of %zu bytes unavailable", want);
+        }
+        s.second = want;
+    }
+    *out = s.first;
+    return BIMSIM_OK;
+}
+
+int twiddles_get(bimsim_ctx* ctx, int n, const double** d, const float** f) {
+    auto it = ctx->twiddles.find(n);
+    if (it == ctx->twiddles.end()) {
+        std::vector<double> hd(2 * (size_t)n);
+        std::vector<float> hf(2 * (size_t)n);
+        for (int t = 0; t < n; ++t) {
+            // angle reduced to the first octant pattern through the exact
+            // integer fraction t/n; cos/sin evaluated in double
+            const double ang = -2.0 * M_PI * (double)t / (double)n;
+            hd[2 * t] = std::cos(ang);
+            hd[2 * t + 1] = std::sin(ang);
+            hf[2 * t] = (float)hd[2 * t];
+            hf[2 * t + 1] = (float)hd[2 * t + 1];
+        }
+        double* dd = nullptr;
+        float* df = nullptr;
+        if (hipMalloc(&dd, sizeof(double) * 2 * n) != hipSuccess ||
+            hipMalloc(&df, sizeof(float) * 2 * n) != hipSuccess) {
+            (void)hipGetLastError();
+            return set_error(ctx, BIMSIM_E_NOMEM, "twiddle table allocation failed");
+        }
+        BIMSIM_HIP_CHECK(ctx, hipMemcpyAsync(dd, hd.data(), sizeof(double) * 2 * n,
+                                             hipMemcpyHostToDevice, ctx->stream));
+        BIMSIM_HIP_CHECK(ctx, hipMemcpyAsync(df, hf.data(), sizeof(float) * 2 * n,
+                                             hipMemcpyHostToDevice, ctx->stream));
+        BIMSIM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        it = ctx->twiddles.emplace(n, std::make_pair(dd, df)).first;
+    }
+    *d = it->second.first;
+    *f = it->second.second;
+    return BIMSIM_OK;
+}
+
+void timer_begin(bimsim_ctx* ctx) {
+    if (ctx->timing) (void)hipEventRecord(ctx->ev0, ctx->stream);
+}
+
+float timer_end(bimsim_ctx* ctx) {
+    if (!ctx->timing) return 0.f;
+    float ms = 0.f;
+    (void)hipEventRecord(ctx->ev1, ctx->stream);
+    (void)hipEventSynchronize(ctx->ev1);
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    return ms;
+}
+
+}  // namespace bimsim
+
+using namespace bimsim;
+
+extern "C" {
+
+int bimsim_abi_version(void) { return BIMSIM_ABI_VERSION; }
+
+int bimsim_ctx_create(int device, void* hip_stream, bimsim_ctx** out) {
+    if (!out) return BIMSIM_E_INVALID;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        return BIMSIM_E_NODEVICE;
+    }
+    if (device < 0 || device >= ndev) return BIMSIM_E_INVALID;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return BIMSIM_E_HIP;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return BIMSIM_E_NODEVICE;
+    if (hipSetDevice(device) != hipSuccess) return BIMSIM_E_HIP;
+    bimsim_ctx* ctx = new bimsim_ctx();
+    ctx->device = device;
+    ctx->cu_count = prop.multiProcessorCount;
+    ctx->lds_max = prop.sharedMemPerBlock;
+    if (hip_stream) {
+        ctx->stream = (hipStream_t)hip_stream;
+    } else {
+        if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete ctx;
+            return BIMSIM_E_HIP;
+        }
+        ctx->own_stream = true;
+    }
+    if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+        delete ctx;
+        return BIMSIM_E_HIP;
+    }
+    *out = ctx;
+    return BIMSIM_OK;
+}
+
+int bimsim_ctx_destroy(bimsim_ctx* ctx) {
+    if (!ctx) return BIMSIM_E_INVALID;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->scratch)
+        if (kv.second.first) (void)hipFree(kv.second.first);
+    for (auto& kv : ctx->twiddles) {
+        (void)hipFree(kv.second.first);
+        (void)hipFree(kv.second.second);
+    }
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return BIMSIM_OK;
+}
+
+int bimsim_sync(bimsim_ctx* ctx) {
+    if (!ctx) return BIMSIM_E_INVALID;
+    BIMSIM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return BIMSIM_OK;
+}
+
+const char* bimsim_last_error(const bimsim_ctx* ctx) {
+    return ctx ? ctx->err.c_str() : "null context";
+}
+
+int bimsim_set_timing(bimsim_ctx* ctx, int enable) {
+    if (!ctx) return BIMSIM_E_INVALID;
+    ctx->timing = enable != 0;
+    return BIMSIM_OK;
+}
+
+int bimsim_get_timings(const bimsim_ctx* ctx, float* t_uf, float* t_us, float* t_det) {
+    if (!ctx) return BIMSIM_E_INVALID;
+    if (t_uf) *t_uf = ctx->t_uf;
+    if (t_us) *t_us = ctx->t_us;
+    if (t_det) *t_det = ctx->t_det;
+    return BIMSIM_OK;
+}
+
+// sphere::calcNl, sphere.h:71-74 (float arithmetic, PI = 3.14159f)
+int bimsim_sphere_order(double a, double lambda) {
+    const float x = (2 * kPIf * (float)a) / (float)lambda;
+    return (int)std::ceil(x + 4 * std::pow(x, (float)(1.0 / 3.0)) + 2);
+}
+
+}  // extern "C"

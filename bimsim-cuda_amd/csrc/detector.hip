@@ -1,0 +1,328 @@
+// Detector: microscopeStruct::SimulateImaging (reference microscope.cu:
+// 608-662, 722-791; fieldslice.cpp:25-94; fieldslice.cu:71-153,319-362).
+//
+// 2D FFT as two passes of batched 1D Stockham FFTs staged in LDS (one
+// workgroup owns B lines of length N; radix 4/2/3/5 stages ping-pong between
+// two LDS buffers).  The objective band-pass is applied in the store phase of
+// the forward column pass, the 1/(uR vR) scale in the store phase of the
+// inverse row pass.  A final kernel crops, forms |U|^2 averaged over ss x ss
+// into D and Di (accumulating, like resample_intensity) and -log10(D/Di).
+// FP64 instantiations run in FP64 end to end (-ffp-contract=off for the
+// whole translation unit); FP32 ones in float with FP64-accurate twiddles.
+#include <type_traits>
+
+#include "internal.h"
+
+namespace bimsim {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMaxElems = 8192;   // B*N per workgroup
+constexpr int kMaxStages = 16;
+
+struct FftPlan {
+    int n;
+    int nstages;
+    int radix[kMaxStages];
+};
+
+int make_plan(int n, FftPlan* p) {
+    p->n = n;
+    p->nstages = 0;
+    int m = n;
+    while (m > 1) {
+        int r;
+        if (m % 4 == 0) r = 4;
+        else if (m % 2 == 0) r = 2;
+        else if (m % 3 == 0) r = 3;
+        else if (m % 5 == 0) r = 5;
+        else return -1;
+        if (p->nstages >= kMaxStages) return -1;
+        p->radix[p->nstages++] = r;
+        m /= r;
+    }
+    return 0;
+}
+
+template <class R> struct C2;
+template <> struct C2<float> { using T = float2; };
+template <> struct C2<double> { using T = double2; };
+
+template <class C>
+__device__ inline C cmk(decltype(C::x) a, decltype(C::x) b) {
+    C c;
+    c.x = a;
+    c.y = b;
+    return c;
+}
+
+// Post-operation applied when a pass stores its results.
+struct PostOp {
+    int kind;        // 0 none, 1 objective band-pass (freq. index = (elem, line)), 2 scale
+    int uR, vR;
+    int quirks;
+    double du, dv, fmin, fmax;   // fmin = NAin/lambda, fmax = NAout/lambda
+    double scale;
+};
+
+// bandpass kernel of microscope.cu:495-526 (frequencies in double)
+__device__ inline bool passband(const PostOp& op, int iu, int iv) {
+    double u, v;
+    if (iu <= op.uR / 2) u = (double)iu * op.du;
+    else u = -(double)(op.quirks ? op.uR - 1 - iu : op.uR - iu) * op.du;
+    if (iv <= op.vR / 2) v = (double)iv * op.dv;
+    else v = -(double)(op.quirks ? op.vR - 1 - iv : op.vR - iv) * op.dv;
+    const double fmag = sqrt(u * u + v * v);
+    return !(fmag < op.fmin || fmag > op.fmax);
+}
+
+template <class R>
+__device__ inline void butterfly(typename C2<R>::T* x, int r, R sgn) {
+    using C = typename C2<R>::T;
+    if (r == 2) {
+        const C a = x[0], b = x[1];
+        x[0] = cmk<C>(a.x + b.x, a.y + b.y);
+        x[1] = cmk<C>(a.x - b.x, a.y - b.y);
+    } else if (r == 4) {
+        const C a = cmk<C>(x[0].x + x[2].x, x[0].y + x[2].y);
+        const C b = cmk<C>(x[0].x - x[2].x, x[0].y - x[2].y);
+        const C c = cmk<C>(x[1].x + x[3].x, x[1].y + x[3].y);
+        const C t = cmk<C>(x[1].x - x[3].x, x[1].y - x[3].y);
+        const C d = cmk<C>(-sgn * t.y, sgn * t.x);   // (x1 - x3) * (sgn i)
+        x[0] = cmk<C>(a.x + c.x, a.y + c.y);
+        x[2] = cmk<C>(a.x - c.x, a.y - c.y);
+        x[1] = cmk<C>(b.x + d.x, b.y + d.y);
+        x[3] = cmk<C>(b.x - d.x, b.y - d.y);
+    } else if (r == 3) {
+        const R h = (R)0.86602540378443864676;   // sqrt(3)/2
+        const C t1 = cmk<C>(x[1].x + x[2].x, x[1].y + x[2].y);
+        const C t = cmk<C>(x[1].x - x[2].x, x[1].y - x[2].y);
+        const C t2 = cmk<C>(-sgn * h * t.y, sgn * h * t.x);
+        const C m = cmk<C>(x[0].x - (R)0.5 * t1.x, x[0].y - (R)0.5 * t1.y);
+        x[0] = cmk<C>(x[0].x + t1.x, x[0].y + t1.y);
+        x[1] = cmk<C>(m.x + t2.x, m.y + t2.y);
+        x[2] = cmk<C>(m.x - t2.x, m.y - t2.y);
+    } else {   // r == 5, direct DFT with exact constants
+        const R c1 = (R)0.30901699437494742410, c2 = (R)-0.80901699437494742410;
+        const R s1 = (R)0.95105651629515357212, s2 = (R)0.58778525229247312917;
+        C y[5];
+        y[0] = cmk<C>(x[0].x + x[1].x + x[2].x + x[3].x + x[4].x,
+                      x[0].y + x[1].y + x[2].y + x[3].y + x[4].y);
+        const C a1 = cmk<C>(x[1].x + x[4].x, x[1].y + x[4].y);
+        const C b1 = cmk<C>(x[1].x - x[4].x, x[1].y - x[4].y);
+        const C a2 = cmk<C>(x[2].x + x[3].x, x[2].y + x[3].y);
+        const C b2 = cmk<C>(x[2].x - x[3].x, x[2].y - x[3].y);
+        const C m1 = cmk<C>(x[0].x + c1 * a1.x + c2 * a2.x, x[0].y + c1 * a1.y + c2 * a2.y);
+        const C m2 = cmk<C>(x[0].x + c2 * a1.x + c1 * a2.x, x[0].y + c2 * a1.y + c1 * a2.y);
+        // sgn i (s1 b1 + s2 b2) and sgn i (s2 b1 - s1 b2)
+        const C n1 = cmk<C>(-sgn * (s1 * b1.y + s2 * b2.y), sgn * (s1 * b1.x + s2 * b2.x));
+        const C n2 = cmk<C>(-sgn * (s2 * b1.y - s1 * b2.y), sgn * (s2 * b1.x - s1 * b2.x));
+        y[1] = cmk<C>(m1.x + n1.x, m1.y + n1.y);
+        y[4] = cmk<C>(m1.x - n1.x, m1.y - n1.y);
+        y[2] = cmk<C>(m2.x + n2.x, m2.y + n2.y);
+        y[3] = cmk<C>(m2.x - n2.x, m2.y - n2.y);
+        for (int i = 0; i < 5; ++i) x[i] = y[i];
+    }
+}
+
+// Batched 1D FFTs: line `g` of the array has its element e at
+// base[g*line_stride + e*elem_stride].  Each workgroup owns B lines.
+template <class R>
+__global__ __launch_bounds__(kThreads) void fft_lines_kernel(
+    typename C2<R>::T* __restrict__ data, int nlines, int B, int P, long long line_stride,
+    long long elem_stride, FftPlan plan, const typename C2<R>::T* __restrict__ tw, int inverse,
+    PostOp op) {
+    using C = typename C2<R>::T;
+    extern __shared__ unsigned char smem_raw[];
+    const int N = plan.n;   // P >= N: line stride in LDS (N + 1 breaks bank conflicts)
+    C* buf0 = reinterpret_cast<C*>(smem_raw);
+    C* buf1 = buf0 + (size_t)B * P;
+    const int g0 = blockIdx.x * B;
+    const int nb = min(B, nlines - g0);
+    const int tot = nb * N;
+    // load: consecutive threads walk the unit-stride dimension
+    const bool lines_fast = line_stride == 1;
+    for (int t = threadIdx.x; t < tot; t += kThreads) {
+        int g, e;
+        if (lines_fast) { g = t % nb; e = t / nb; }
+        else { e = t % N; g = t / N; }
+        buf0[g * P + e] = data[(long long)(g0 + g) * line_stride + (long long)e * elem_stride];
+    }
+    __syncthreads();
+    const R sgn = inverse ? (R)1 : (R)-1;
+    C* src = buf0;
+    C* dst = buf1;
+    int Ns = 1;
+    for (int st = 0; st < plan.nstages; ++st) {
+        const int r = plan.radix[st];
+        const int nq = N / r;
+        const int tstep = N / (Ns * r);
+        for (int t = threadIdx.x; t < nb * nq; t += kThreads) {
+            const int g = t / nq, j = t - g * nq;
+            const int k = j % Ns;
+            C x[5];
+            for (int q = 0; q < r; ++q) {
+                C v = src[g * P + j + q * nq];
+                if (q > 0 && k > 0) {
+                    C w = tw[k * q * tstep];   // k*q*tstep < N
+                    if (inverse) w.y = -w.y;
+                    v = cmk<C>(v.x * w.x - v.y * w.y, v.x * w.y + v.y * w.x);
+                }
+                x[q] = v;
+            }
+            butterfly<R>(x, r, sgn);
+            const int o = (j / Ns) * Ns * r + k;
+            for (int q = 0; q < r; ++q) dst[g * P + o + q * Ns] = x[q];
+        }
+        __syncthreads();
+        C* tmp = src;
+        src = dst;
+        dst = tmp;
+        Ns *= r;
+    }
+    for (int t = threadIdx.x; t < tot; t += kThreads) {
+        int g, e;
+        if (lines_fast) { g = t % nb; e = t / nb; }
+        else { e = t % N; g = t / N; }
+        C v = src[g * P + e];
+        if (op.kind == 1) {
+            // frequency indices: elements run along v (column pass), lines along u
+            const int iu = g0 + g, iv = e;
+            if (!passband(op, iu, iv)) v = cmk<C>((R)0, (R)0);
+        } else if (op.kind == 2) {
+            v = cmk<C>(v.x * (R)op.scale, v.y * (R)op.scale);
+        }
+        data[(long long)(g0 + g) * line_stride + (long long)e * elem_stride] = v;
+    }
+}
+
+// crop + resample_intensity (+=) + calc_absorbance, per detector pixel
+template <class R>
+__global__ void detect_kernel(const typename C2<R>::T* __restrict__ U,
+                              const typename C2<R>::T* __restrict__ Uf, int uR, int off, int Nd,
+                              int ss, R* __restrict__ D, R* __restrict__ Di, R* __restrict__ Abs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Nd * Nd) return;
+    const int iv = i / Nd, iu = i - iv * Nd;
+    R I = 0, Ii = 0;
+    for (int ix = 0; ix < ss; ++ix)
+        for (int iy = 0; iy < ss; ++iy) {
+            const long long fi = (long long)(off + iv * ss + iy) * uR + (off + iu * ss + ix);
+            const auto a = U[fi], b = Uf[fi];
+            const R am = sqrt(a.x * a.x + a.y * a.y), bm = sqrt(b.x * b.x + b.y * b.y);
+            I += am * am;
+            Ii += bm * bm;
+        }
+    const R d = D[i] + I / (R)(ss * ss);
+    const R di = Di[i] + Ii / (R)(ss * ss);
+    D[i] = d;
+    Di[i] = di;
+    if (Abs) Abs[i] = -log10(d / di);
+}
+
+template <class R>
+int run_fft_pass(bimsim_ctx* ctx, typename C2<R>::T* data, int nlines, int n, long long line_stride,
+                 long long elem_stride, const typename C2<R>::T* tw, int inverse, const PostOp& op) {
+    using C = typename C2<R>::T;
+    FftPlan plan;
+    if (make_plan(n, &plan) != 0)
+        return set_error(ctx, BIMSIM_E_UNSUPPORTED, "FFT length %d is not a product of 2, 3 and 5", n);
+    // lines per workgroup: as many as fit the LDS ping-pong buffers
+    int B = std::min(16, std::min(nlines, std::max(1, kMaxElems / n)));
+    size_t lds = 0;
+    int P = n;
+    for (; B >= 1; --B) {
+        P = B == 1 ? n : n + 1;
+        lds = sizeof(C) * 2 * (size_t)B * P;
+        if (lds <= ctx->lds_max) break;
+    }
+    if (B < 1)
+        return set_error(ctx, BIMSIM_E_UNSUPPORTED, "FFT length %d does not fit in LDS", n);
+    const dim3 grid((nlines + B - 1) / B);
+    hipLaunchKernelGGL(fft_lines_kernel<R>, grid, dim3(kThreads), lds, ctx->stream, data, nlines,
+                       B, P, line_stride, elem_stride, plan, tw, inverse, op);
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return BIMSIM_OK;
+}
+
+template <class R>
+int detector_impl(bimsim_ctx* ctx, void* Uin, void* Ufin, int uR, int vR, double du, double dv,
+                  double NAin, double NAout, double lambda, int padding, int Nd, int ss, void* D,
+                  void* Di, void* Abs, int write_filtered, int quirks) {
+    using C = typename C2<R>::T;
+    const size_t npix = (size_t)uR * vR;
+    C* U = (C*)Uin;
+    C* Uf = (C*)Ufin;
+    if (!write_filtered) {
+        void *a = nullptr, *b = nullptr;
+        BIMSIM_TRY(scratch_get(ctx, kSlotFftA, sizeof(C) * npix, &a));
+        BIMSIM_TRY(scratch_get(ctx, kSlotFftB, sizeof(C) * npix, &b));
+        BIMSIM_HIP_CHECK(ctx, hipMemcpyAsync(a, U, sizeof(C) * npix, hipMemcpyDeviceToDevice, ctx->stream));
+        BIMSIM_HIP_CHECK(ctx, hipMemcpyAsync(b, Uf, sizeof(C) * npix, hipMemcpyDeviceToDevice, ctx->stream));
+        U = (C*)a;
+        Uf = (C*)b;
+    }
+    const double *twd_u, *twd_v;
+    const float *twf_u, *twf_v;
+    BIMSIM_TRY(twiddles_get(ctx, uR, &twd_u, &twf_u));
+    BIMSIM_TRY(twiddles_get(ctx, vR, &twd_v, &twf_v));
+    const C* tw_u = std::is_same<R, double>::value ? (const C*)twd_u : (const C*)twf_u;
+    const C* tw_v = std::is_same<R, double>::value ? (const C*)twd_v : (const C*)twf_v;
+    PostOp none{};
+    PostOp bp{};
+    bp.kind = 1;
+    bp.uR = uR;
+    bp.vR = vR;
+    bp.quirks = quirks;
+    bp.du = du;
+    bp.dv = dv;
+    bp.fmin = NAin / lambda;
+    bp.fmax = NAout / lambda;
+    PostOp sc{};
+    sc.kind = 2;
+    sc.scale = 1.0 / ((unsigned)uR * (unsigned)vR);   // ScaleField(1.0/(R0*R1))
+    C* fields[2] = {U, Uf};
+    for (C* F : fields) {
+        // forward: rows (length uR), then columns (length vR) + band-pass
+        BIMSIM_TRY(run_fft_pass<R>(ctx, F, vR, uR, uR, 1, tw_u, 0, none));
+        BIMSIM_TRY(run_fft_pass<R>(ctx, F, uR, vR, 1, uR, tw_v, 0, bp));
+        // inverse: columns, then rows + 1/(uR vR)
+        BIMSIM_TRY(run_fft_pass<R>(ctx, F, uR, vR, 1, uR, tw_v, 1, none));
+        BIMSIM_TRY(run_fft_pass<R>(ctx, F, vR, uR, uR, 1, tw_u, 1, sc));
+    }
+    const int off = padding * Nd * ss;
+    hipLaunchKernelGGL(detect_kernel<R>, dim3((Nd * Nd + 255) / 256), dim3(256), 0, ctx->stream,
+                       (const C*)U, (const C*)Uf, uR, off, Nd, ss, (R*)D, (R*)Di, (R*)Abs);
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return BIMSIM_OK;
+}
+
+}  // namespace
+}  // namespace bimsim
+
+using namespace bimsim;
+
+extern "C" int bimsim_detector(bimsim_ctx* ctx, int prec, int quirks, void* U, void* Uf, int uR, int vR,
+                               double du, double dv, double NAin, double NAout, double lambda,
+                               int padding, int Nd, int ss, void* D, void* Di, void* Abs,
+                               int write_filtered) {
+    if (!ctx) return BIMSIM_E_INVALID;
+    BIMSIM_CHECK_ARG(ctx, U && Uf && D && Di, "null pointer argument");
+    BIMSIM_CHECK_ARG(ctx, uR > 0 && vR > 0 && Nd > 0 && ss > 0 && padding >= 0, "bad sizes");
+    BIMSIM_CHECK_ARG(ctx, (long long)(2 * padding + 1) * Nd * ss <= std::min(uR, vR),
+                     "detector crop exceeds the near-field grid");
+    BIMSIM_CHECK_ARG(ctx, lambda > 0.0 && du > 0.0 && dv > 0.0, "lambda, du and dv must be positive");
+    BIMSIM_CHECK_ARG(ctx, prec >= BIMSIM_FP64_PARITY && prec <= BIMSIM_FP32_FAST, "unknown precision");
+    BIMSIM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    timer_begin(ctx);
+    int rc;
+    if (prec == BIMSIM_FP32_FAST)
+        rc = detector_impl<float>(ctx, U, Uf, uR, vR, du, dv, NAin, NAout, lambda, padding, Nd, ss,
+                                  D, Di, Abs, write_filtered, quirks);
+    else
+        rc = detector_impl<double>(ctx, U, Uf, uR, vR, du, dv, NAin, NAout, lambda, padding, Nd,
+                                   ss, D, Di, Abs, write_filtered, quirks);
+    if (rc != BIMSIM_OK) return rc;
+    ctx->t_det = timer_end(ctx);
+    return BIMSIM_OK;
+}

@@ -1,0 +1,79 @@
+// Context and error plumbing shared by the C-ABI entry points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+struct bimsim_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+    // stage timers (nearfieldStruct::t_Uf / t_Us)
+    bool timing = false;
+    float t_uf = 0.f, t_us = 0.f, t_det = 0.f;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // grow-only device scratch, split into named slots
+    std::map<int, std::pair<void*, size_t>> scratch;
+    // FFT twiddle tables per length: e^{-2 pi i t/N}, t = 0..N-1
+    std::map<int, std::pair<double*, float*>> twiddles;
+    int cu_count = 256;
+    size_t lds_max = 65536;
+};
+
+namespace bimsim {
+
+enum ScratchSlot {
+    kSlotCoef = 0,    // per-pixel series coefficients
+    kSlotRhat = 1,    // per-pixel unit vectors
+    kSlotWaves = 2,   // per-sphere wave tables
+    kSlotSph = 3,     // device sphere records
+    kSlotPartial = 4, // split partial sums
+    kSlotFftA = 5,    // detector work copies
+    kSlotFftB = 6,
+    kSlotDet = 7,
+    kSlotMisc = 8,
+};
+
+int set_error(bimsim_ctx* ctx, int code, const char* fmt, ...);
+
+// Device scratch of at least `bytes` in `slot` (grown with hipMalloc).
+int scratch_get(bimsim_ctx* ctx, int slot, size_t bytes, void** out);
+
+// Twiddle table for length n on the device (double and float copies).
+int twiddles_get(bimsim_ctx* ctx, int n, const double** d, const float** f);
+
+// Timer helpers: begin records ev0 on the stream, end records ev1, waits and
+// returns the elapsed milliseconds.
+void timer_begin(bimsim_ctx* ctx);
+float timer_end(bimsim_ctx* ctx);
+
+}  // namespace bimsim
+
+#define BIMSIM_HIP_CHECK(ctx, expr)                                                        \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return ::bimsim::set_error((ctx), BIMSIM_E_HIP, "%s failed: %s (%s:%d)", #expr, \
+                                       hipGetErrorString(e_), __FILE__, __LINE__);         \
+    } while (0)
+
+#define BIMSIM_CHECK_ARG(ctx, cond, msg)                                           \
+    do {                                                                           \
+        if (!(cond)) return ::bimsim::set_error((ctx), BIMSIM_E_INVALID, "%s", msg); \
+    } while (0)
+
+#define BIMSIM_TRY(expr)          \
+    do {                          \
+        int rc_ = (expr);         \
+        if (rc_ != BIMSIM_OK) return rc_; \
+    } while (0)

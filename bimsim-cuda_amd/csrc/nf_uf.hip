@@ -1,0 +1,220 @@
+// Incident field Uf and the Uf + Us sum (reference nfScalarUf.cu:9-217,
+// nfSumUf.cu:6-117).  Built with -ffp-contract=off: the series runs in FP64
+// in reference operation order for every precision; FP32_FAST stores the
+// result as float pairs.  The Uf series is O(pixels x m), a small fraction of
+// the scattered-field work of any Monte-Carlo configuration.
+#include "internal.h"
+#include "refmath.h"
+
+namespace bimsim {
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ inline void pixel_of(int pix, int uR, int& iu, int& iv) {
+    iv = pix / uR;
+    iu = pix - iv * uR;
+}
+
+template <class Out>
+__device__ inline void store(Out* U, int i, dc v);
+template <>
+__device__ inline void store<double2>(double2* U, int i, dc v) { U[i] = make_double2(v.r, v.i); }
+template <>
+__device__ inline void store<float2>(float2* U, int i, dc v) {
+    U[i] = make_float2((float)v.r, (float)v.i);
+}
+
+// gpuScalarUf, nfScalarUf.cu:46-175
+template <class Out>
+__global__ __launch_bounds__(kThreads) void uf_series_kernel(QuadArg q, int uR, int vR, d3 f,
+                                                             d3 k, double kmag, double amp,
+                                                             double ca, double cb, int m,
+                                                             Out* __restrict__ Uf) {
+    const int pix = blockIdx.x * kThreads + threadIdx.x;
+    if (pix >= uR * vR) return;
+    int iu, iv;
+    pixel_of(pix, uR, iu, iv);
+    const d3 p = quad_at(q, (double)iu / (double)uR, (double)iv / (double)vR);
+    d3 r = sub(p, f);
+    const double d = len(r);
+    if (d < (double)kEpsFloat) {
+        store(Uf, pix, dc{amp * 2.0 * (double)kPIf * (ca - cb), 0.0});
+        return;
+    }
+    r = norm(r);
+    const dc im = {0.0, 1.0};
+    dc il = {1.0, 0.0};
+    double j[2];
+    const double kd = kmag * d;
+    ref::init_j(kd, j);
+    const double ct = dot(k, r);
+    double P0 = 1.0, P1 = ct;
+    double Pa[3] = {1.0, ca, 1.0}, Pb[3] = {1.0, cb, 1.0};
+    dc sum = {0.0, 0.0};
+    for (int l = 0; l <= m; ++l) {
+        double jl, Pl;
+        if (l == 0) {
+            jl = j[0];
+            Pl = P0;
+        } else if (l == 1) {
+            jl = j[1];
+            Pl = P1;
+            Pa[2] = Pa[0];
+            ref::shift_legendre(l + 1, ca, Pa[0], Pa[1]);
+            Pb[2] = Pb[0];
+            ref::shift_legendre(l + 1, cb, Pb[0], Pb[1]);
+        } else {
+            ref::shift_j(l, kd, j);
+            ref::shift_legendre(l, ct, P0, P1);
+            jl = j[1];
+            Pl = P1;
+            Pa[2] = Pa[0];
+            ref::shift_legendre(l + 1, ca, Pa[0], Pa[1]);
+            Pb[2] = Pb[0];
+            ref::shift_legendre(l + 1, cb, Pb[0], Pb[1]);
+        }
+        const double D = ((Pa[1] - Pa[2]) - Pb[1]) + Pb[2];
+        sum = cadd(sum, cscl(cscl(cscl(il, jl), Pl), D));
+        il = cmul(il, im);
+    }
+    store(Uf, pix, cscl(cscl(cscl(sum, 2.0), (double)kPIf), amp));
+}
+
+// gpuScalarUfp, nfScalarUf.cu:9-43
+template <class Out>
+__global__ __launch_bounds__(kThreads) void uf_plane_kernel(QuadArg q, int uR, int vR, d3 f, d3 k,
+                                                            double kmag, double amp,
+                                                            Out* __restrict__ Uf) {
+    const int pix = blockIdx.x * kThreads + threadIdx.x;
+    if (pix >= uR * vR) return;
+    int iu, iv;
+    pixel_of(pix, uR, iu, iv);
+    const d3 p = quad_at(q, (double)iu / (double)uR, (double)iv / (double)vR);
+    const d3 r = sub(p, f);
+    const double kr = kmag * dot(k, r);
+    store(Uf, pix, cscl(cexp_i(kr), amp));
+}
+
+struct MaskBatch {
+    double p[32][3];
+    double a[32];
+};
+
+// Marks pixels inside any sphere of the batch (d < a, nfSumUf.cu:31-39).
+__global__ __launch_bounds__(kThreads) void inside_mask_kernel(QuadArg q, int uR, int vR,
+                                                               MaskBatch b, int cnt, int first,
+                                                               unsigned char* __restrict__ mask) {
+    const int pix = blockIdx.x * kThreads + threadIdx.x;
+    if (pix >= uR * vR) return;
+    int iu, iv;
+    pixel_of(pix, uR, iu, iv);
+    const d3 p = quad_at(q, (double)iu / (double)uR, (double)iv / (double)vR);
+    unsigned char in = first ? 0 : mask[pix];
+    for (int s = 0; s < cnt && !in; ++s)
+        if (len(sub(p, d3{b.p[s][0], b.p[s][1], b.p[s][2]})) < b.a[s]) in = 1;
+    mask[pix] = in;
+}
+
+template <class C>
+__global__ void add_outside_kernel(int npix, const unsigned char* __restrict__ mask,
+                                   const C* __restrict__ Uf, C* __restrict__ U) {
+    const int pix = blockIdx.x * kThreads + threadIdx.x;
+    if (pix >= npix || (mask && mask[pix])) return;
+    C a = U[pix];
+    const C b = Uf[pix];
+    a.x = a.x + b.x;
+    a.y = a.y + b.y;
+    U[pix] = a;
+}
+
+}  // namespace
+}  // namespace bimsim
+
+using namespace bimsim;
+
+extern "C" int bimsim_nf_uf(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int vR,
+                            const double focus[3], const double kdir[3], double kmag,
+                            double amp, double cos_alpha, double cos_beta, int m, int mode,
+                            int prec, int quirks, void* Uf) {
+    if (!ctx) return BIMSIM_E_INVALID;
+    BIMSIM_CHECK_ARG(ctx, pos && focus && kdir && Uf, "null pointer argument");
+    BIMSIM_CHECK_ARG(ctx, uR > 0 && vR > 0 && (long long)uR * vR <= (1LL << 28),
+                     "slice resolution must be positive and at most 2^28 pixels");
+    BIMSIM_CHECK_ARG(ctx, prec >= BIMSIM_FP64_PARITY && prec <= BIMSIM_FP32_FAST, "unknown precision");
+    BIMSIM_CHECK_ARG(ctx, m >= 1 && m <= 100000, "incident order m must be in [1, 100000]");
+    BIMSIM_CHECK_ARG(ctx, kmag > 0.0 && std::isfinite(kmag), "kmag must be positive");
+    if (mode == BIMSIM_UF_LUT)
+        return set_error(ctx, BIMSIM_E_UNSUPPORTED, "Uf look-up-table mode is not built");
+    BIMSIM_CHECK_ARG(ctx, mode == BIMSIM_UF_SERIES || mode == BIMSIM_UF_PLANE, "unknown Uf mode");
+    if (!quirks && mode == BIMSIM_UF_SERIES)
+        return set_error(ctx, BIMSIM_E_UNSUPPORTED,
+                         "incident-field series without the reference recurrences is not built");
+    BIMSIM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    const int npix = uR * vR;
+    const dim3 grid((npix + kThreads - 1) / kThreads);
+    const QuadArg q = to_quad(*pos);
+    const d3 f = mk(focus), k = mk(kdir);
+    timer_begin(ctx);
+    const bool f32 = prec == BIMSIM_FP32_FAST;
+    if (mode == BIMSIM_UF_PLANE) {
+        if (f32)
+            hipLaunchKernelGGL(uf_plane_kernel<float2>, grid, dim3(kThreads), 0, ctx->stream, q, uR,
+                               vR, f, k, kmag, amp, (float2*)Uf);
+        else
+            hipLaunchKernelGGL(uf_plane_kernel<double2>, grid, dim3(kThreads), 0, ctx->stream, q,
+                               uR, vR, f, k, kmag, amp, (double2*)Uf);
+    } else {
+        if (f32)
+            hipLaunchKernelGGL(uf_series_kernel<float2>, grid, dim3(kThreads), 0, ctx->stream, q,
+                               uR, vR, f, k, kmag, amp, cos_alpha, cos_beta, m, (float2*)Uf);
+        else
+            hipLaunchKernelGGL(uf_series_kernel<double2>, grid, dim3(kThreads), 0, ctx->stream, q,
+                               uR, vR, f, k, kmag, amp, cos_alpha, cos_beta, m, (double2*)Uf);
+    }
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    ctx->t_uf = timer_end(ctx);
+    return BIMSIM_OK;
+}
+
+extern "C" int bimsim_nf_sum_uf(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int vR,
+                                const bimsim_sphere* spheres, int ns, int prec, const void* Uf,
+                                void* U) {
+    if (!ctx) return BIMSIM_E_INVALID;
+    BIMSIM_CHECK_ARG(ctx, pos && Uf && U, "null pointer argument");
+    BIMSIM_CHECK_ARG(ctx, uR > 0 && vR > 0 && (long long)uR * vR <= (1LL << 28),
+                     "slice resolution must be positive and at most 2^28 pixels");
+    BIMSIM_CHECK_ARG(ctx, ns >= 0 && (ns == 0 || spheres), "bad sphere list");
+    BIMSIM_CHECK_ARG(ctx, prec >= BIMSIM_FP64_PARITY && prec <= BIMSIM_FP32_FAST, "unknown precision");
+    BIMSIM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    const int npix = uR * vR;
+    const bool f32 = prec == BIMSIM_FP32_FAST;
+    const size_t esz = f32 ? sizeof(float2) : sizeof(double2);
+    if (ns == 0) {
+        // U = Uf (nfSumUf.cu:59-68)
+        BIMSIM_HIP_CHECK(ctx, hipMemcpyAsync(U, Uf, esz * npix, hipMemcpyDeviceToDevice, ctx->stream));
+        return BIMSIM_OK;
+    }
+    void* mask = nullptr;
+    BIMSIM_TRY(scratch_get(ctx, kSlotMisc, (size_t)npix, &mask));
+    const QuadArg q = to_quad(*pos);
+    const dim3 grid((npix + kThreads - 1) / kThreads);
+    for (int s0 = 0; s0 < ns; s0 += 32) {
+        const int cnt = std::min(32, ns - s0);
+        MaskBatch b{};
+        for (int i = 0; i < cnt; ++i) {
+            for (int c = 0; c < 3; ++c) b.p[i][c] = spheres[s0 + i].p[c];
+            b.a[i] = spheres[s0 + i].a;
+        }
+        hipLaunchKernelGGL(inside_mask_kernel, grid, dim3(kThreads), 0, ctx->stream, q, uR, vR, b,
+                           cnt, s0 == 0 ? 1 : 0, (unsigned char*)mask);
+    }
+    if (f32)
+        hipLaunchKernelGGL(add_outside_kernel<float2>, grid, dim3(kThreads), 0, ctx->stream, npix,
+                           (const unsigned char*)mask, (const float2*)Uf, (float2*)U);
+    else
+        hipLaunchKernelGGL(add_outside_kernel<double2>, grid, dim3(kThreads), 0, ctx->stream, npix,
+                           (const unsigned char*)mask, (const double2*)Uf, (double2*)U);
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return BIMSIM_OK;
+}

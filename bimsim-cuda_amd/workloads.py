@@ -1,0 +1,128 @@
+"""Seeded BIMSim instances (sphere lists, optics, slice, Monte-Carlo waves).
+
+Every parameter is drawn from SplitMix64 with seed 0xB1A50000 + config id and
+rounded to float32, because the reference reads every input as float
+(rts/tools/arguments.h:100-116, ptype = float).  Quantities the reference
+derives on the host in ptype (k = 2*PI/lambda, cos(asin(NA)), the condenser
+amplitude subA, du = 1/|X|) are derived here in double, i.e. the reference
+with ptype = double.
+
+Named workloads (slice of N x N samples over L um at z = 0, focus at the
+origin, k = +z, detector without padding and ss = 1):
+  cfg1   one sphere a=5, n=1.4+0.05i, lambda=2.5, N=128, L=40, condenser 0-0.5,
+         objective 0-0.5, Legendre Uf, direct Us series with 400 waves
+  cfg2   3 spheres a~U(2,8), n~U(1.3,1.6)+iU(0,0.1), centres U(-15,15)^2,
+         1000 waves over NA 0.2-0.62, lambda=3, N=512, L=60, objective 0-0.62
+"""
+import math
+import struct
+
+import numpy as np
+
+PI_F = 3.1415901184082031   # float(3.14159f)
+SEED_BASE = 0xB1A50000
+MASK64 = (1 << 64) - 1
+
+
+def f32(x):
+    """Round a double to the nearest float32 value."""
+    return struct.unpack("f", struct.pack("f", x))[0]
+
+
+class SplitMix64:
+    def __init__(self, seed):
+        self.s = seed & MASK64
+
+    def next(self):
+        self.s = (self.s + 0x9E3779B97F4A7C15) & MASK64
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        return z ^ (z >> 31)
+
+    def uniform(self, lo=0.0, hi=1.0):
+        return lo + (hi - lo) * ((self.next() >> 11) * (1.0 / 9007199254740992.0))
+
+
+def sphere_order(a, lam):
+    """sphere::calcNl (sphere.h:71-74), evaluated by the library's host helper."""
+    from . import sphere_order as lib_order
+    return lib_order(a, lam)
+
+
+def incident_order(kmag, dmax):
+    """m = ceil(kd + 4 kd^(1/3) + 2) at the farthest slice point."""
+    kd = kmag * dmax
+    return int(math.ceil(kd + 4.0 * kd ** (1.0 / 3.0) + 2.0))
+
+
+def make_instance(spheres, lam, N, L, cond, obj, nk, wave_seed, amp=1.0, m=None,
+                  plane_wave=False):
+    lam = f32(lam)
+    kmag = (2.0 * PI_F) / lam
+    spheres = [dict(s) for s in spheres]
+    off = 0
+    for s in spheres:
+        s["p"] = tuple(f32(c) for c in s["p"])
+        s["a"] = f32(s["a"])
+        s["n"] = complex(f32(s["n"].real), f32(s["n"].imag))
+        s["Nl"] = sphere_order(s["a"], lam)
+        s["coeff_off"] = off
+        off += s["Nl"] + 1
+    ca = math.cos(math.asin(cond[0]))
+    cb = math.cos(math.asin(cond[1]))
+    half = L / 2.0
+    if m is None:
+        m = incident_order(kmag, math.hypot(half, half))
+    inst = dict(
+        lam=lam, kmag=kmag, N=N, L=float(L), A=(-half, -half, 0.0), X=(float(L), 0.0, 0.0),
+        Y=(0.0, float(L), 0.0), focus=(0.0, 0.0, 0.0), kdir=(0.0, 0.0, 1.0), amp=amp,
+        cond=(f32(cond[0]), f32(cond[1])), obj=(f32(obj[0]), f32(obj[1])), cos_alpha=ca,
+        cos_beta=cb, m=m, spheres=spheres, ncoef=off, nk=nk, wave_seed=wave_seed,
+        plane_wave=plane_wave, padding=0, Nd=N, ss=1, du=1.0 / L, dv=1.0 / L,
+    )
+    inst["subA"] = (2.0 * PI_F) * amp * ((1.0 - cb) - (1.0 - ca))
+    return inst
+
+
+def mc_waves(inst, sampler):
+    """Monte-Carlo directions for an instance, rounded to float32.
+    `sampler(N, k, NAin, NAout, rng, seed)` is mcSampleNA with SplitMix64."""
+    w = np.asarray(sampler(inst["nk"], inst["kdir"], inst["cond"][0], inst["cond"][1], 1,
+                           inst["wave_seed"]), dtype=np.float64).reshape(-1, 3)
+    return w.astype(np.float32).astype(np.float64)
+
+
+def config(cid, N=None, nk=None):
+    """Named workload `cid` (1 or 2), optionally reduced in N / nk for tests."""
+    g = SplitMix64(SEED_BASE + cid)
+    if cid == 1:
+        sph = [dict(p=(0.0, 0.0, 0.0), a=5.0, n=complex(1.4, 0.05))]
+        return make_instance(sph, 2.5, N or 128, 40.0, (0.0, 0.5), (0.0, 0.5), nk or 400,
+                             SEED_BASE + 0x100 + cid)
+    if cid == 2:
+        sph = []
+        for _ in range(3):
+            a = g.uniform(2.0, 8.0)
+            nr = g.uniform(1.3, 1.6)
+            ni = g.uniform(0.0, 0.1)
+            cx = g.uniform(-15.0, 15.0)
+            cy = g.uniform(-15.0, 15.0)
+            sph.append(dict(p=(cx, cy, 0.0), a=a, n=complex(nr, ni)))
+        return make_instance(sph, 3.0, N or 512, 60.0, (0.2, 0.62), (0.0, 0.62), nk or 1000,
+                             SEED_BASE + 0x100 + cid)
+    raise ValueError(f"unknown configuration {cid}")
+
+
+def terms(inst):
+    """Mie series terms of one image: N^2 nk sum_s (Nl+1) + N^2 (m+1)."""
+    n2 = inst["N"] * inst["N"]
+    return n2 * inst["nk"] * sum(s["Nl"] + 1 for s in inst["spheres"]) + n2 * (inst["m"] + 1)
+
+
+def us_flops(inst):
+    """Algorithmic FLOP of the scattered-field series: per (pixel, sphere)
+    (Nl+1)(8 nk + 12) + 13 nk."""
+    n2 = inst["N"] * inst["N"]
+    nk = inst["nk"]
+    return n2 * sum((s["Nl"] + 1) * (8 * nk + 12) + 13 * nk for s in inst["spheres"])

@@ -1,0 +1,191 @@
+/*
+ * bimsim_hip.h -- C ABI of the MI355X (gfx950) BIMSim near-field and imaging path.
+ *
+ * Plain C: POD structs, pointers and sizes only.  Every entry point returns a
+ * bimsim_status (0 = success, negative = error) and never exits the process;
+ * bimsim_last_error() gives the message of the last failure on a context.
+ *
+ * Buffer conventions (match the reference field containers):
+ *   - fields are complex, interleaved (re, im), row-major with u fastest:
+ *     element i = iv*uR + iu   (fieldslice::x_hat, reference fieldslice.h:16,
+ *     indexing nfScalarUs.cu:346).  FP64 precisions use double pairs,
+ *     BIMSIM_FP32_FAST uses float pairs.
+ *   - detector images are real, Nd*Nd, same indexing (scalarslice::S,
+ *     reference scalarslice.h:10); double for FP64 precisions, float for FP32.
+ *   - every device pointer argument is caller-owned device memory; the library
+ *     never frees caller memory.  Scratch memory belongs to the context.
+ *   - calls are asynchronous on the context stream; bimsim_sync() waits.
+ *     One context per host thread; calls on one context are serialised.
+ */
+#ifndef BIMSIM_HIP_H
+#define BIMSIM_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BIMSIM_ABI_VERSION 1
+
+typedef struct bimsim_ctx bimsim_ctx;
+
+typedef enum {
+    BIMSIM_OK = 0,
+    BIMSIM_E_INVALID = -1,      /* bad argument (shape, pointer, range)        */
+    BIMSIM_E_HIP = -2,          /* HIP runtime error                            */
+    BIMSIM_E_NOMEM = -3,        /* device or host allocation failed             */
+    BIMSIM_E_UNSUPPORTED = -4,  /* mode/size not supported by this build        */
+    BIMSIM_E_NODEVICE = -5      /* no usable gfx950 device                      */
+} bimsim_status;
+
+/* Arithmetic form of a call.
+ *   FP64_PARITY: the reference algorithm in double, reference operation order
+ *                and loop structure (per-wave recurrences), FMA contraction off.
+ *   FP64:        same arithmetic, wave-independent Bessel/Hankel terms hoisted
+ *                out of the wave loop (identical operation sequence per term).
+ *   FP32_FAST:   Bessel terms and Mie products in double, per-wave Legendre
+ *                series, plane-wave sum and FFT in float with compensated
+ *                accumulation. */
+typedef enum {
+    BIMSIM_FP64_PARITY = 0,
+    BIMSIM_FP64 = 1,
+    BIMSIM_FP32_FAST = 2
+} bimsim_precision;
+
+/* Reference behaviours that are numerical defects, reproduced when set
+ * (default).  With 0 the library uses the corrected forms:
+ *   - y_{v0} asymptotic typo for |z|>12 (reference cbessjy.cpp:541)
+ *   - upward j_l/y_l recurrences with the convergence thresholds
+ *     (rts/math/spherical_bessel.h:8-10,102-141); off = Miller downward j_l
+ *   - sqrt(3.14159/2z) scale of the spherical wrappers (bessjy.cpp:18,719-723)
+ */
+typedef enum {
+    BIMSIM_QUIRKS_OFF = 0,
+    BIMSIM_QUIRKS_REF = 1
+} bimsim_quirks;
+
+/* Incident-field evaluation (nearfieldStruct::scalarUf / scalarUfLut /
+ * gpuScalarUfp, reference nfScalarUf.cu:9-217, nfScalarUfLut.cu:11-188). */
+typedef enum {
+    BIMSIM_UF_SERIES = 0,
+    BIMSIM_UF_PLANE = 1,
+    BIMSIM_UF_LUT = 2
+} bimsim_uf_mode;
+
+/* Scattered-field evaluation (scalarUs / scalarUpLut, reference
+ * nfScalarUs.cu:494-658, nfScalarUpLut.cu:342-495). */
+typedef enum {
+    BIMSIM_US_SERIES = 0,
+    BIMSIM_US_LUT = 1
+} bimsim_us_mode;
+
+/* Monte-Carlo direction generators for bimsim_mc_sample_na. */
+typedef enum {
+    BIMSIM_RNG_GLIBC = 0,       /* glibc rand() stream after srand(seed)        */
+    BIMSIM_RNG_SPLITMIX64 = 1   /* SplitMix64(seed) uniform doubles             */
+} bimsim_rng;
+
+/* rts::quad<ptype,3> (reference rts/math/quad.h:30-33): corner A, edges X, Y.
+ * Pixel (iu, iv) of a uR x vR slice sits at A + X*(iu/uR) + Y*(iv/vR). */
+typedef struct {
+    double A[3];
+    double X[3];
+    double Y[3];
+} bimsim_quad;
+
+/* One sphere at one wavelength (reference sphere.h:14-92). */
+typedef struct {
+    double p[3];       /* centre                                         */
+    double a;          /* radius                                         */
+    double n[2];       /* complex refractive index (re, im)              */
+    int    Nl;         /* series order, sphere::calcNl (sphere.h:71-74)  */
+    int    coeff_off;  /* element offset of A_l/B_l in packed arrays     */
+} bimsim_sphere;
+
+/* ---------------------------------------------------------------- context */
+
+/* hip_stream: a hipStream_t to run on, or NULL for a stream of the context's
+ * own.  Replaces the implicit device/stream of main.cpp:46 (cudaSetDevice). */
+int bimsim_ctx_create(int device, void* hip_stream, bimsim_ctx** out);
+int bimsim_ctx_destroy(bimsim_ctx* ctx);
+int bimsim_sync(bimsim_ctx* ctx);
+const char* bimsim_last_error(const bimsim_ctx* ctx);
+int bimsim_abi_version(void);
+
+/* Stage timers of the last call (nearfieldStruct::t_Uf / t_Us,
+ * reference nearfield.h:58-59, rts/cuda/timer.h:4-21); t_us also covers the
+ * scattered-field kernels alone, t_det the detector.  Enabled timing makes
+ * each timed call wait for its own events. */
+int bimsim_set_timing(bimsim_ctx* ctx, int enable);
+int bimsim_get_timings(const bimsim_ctx* ctx, float* t_uf_ms, float* t_us_ms,
+                       float* t_det_ms);
+
+/* ------------------------------------------------------------ host helpers */
+
+/* sphere::calcNl (reference sphere.h:71-74): ceil(x + 4 x^(1/3) + 2),
+ * x = 2*PI*a/lambda evaluated in float with PI = 3.14159f. */
+int bimsim_sphere_order(double a, double lambda);
+
+/* mcSampleNA (reference montecarlo.cpp:9-71): N unit directions uniform in
+ * cos(phi) over the condenser annulus [NAin, NAout], rotated from +z to k.
+ * With BIMSIM_RNG_GLIBC the draws replay glibc rand() after srand(seed) in
+ * float arithmetic; nearfieldStruct::calcWaves uses srand(NULL) (seed 0,
+ * nearfield.cpp:102).  out: N*3 doubles (x, y, z) on the host. */
+int bimsim_mc_sample_na(double* out, int N, const double k[3], double NAin,
+                        double NAout, int rng, unsigned long long seed);
+
+/* ------------------------------------------------------ device entry points */
+
+/* Mie coefficients on the GPU in FP64 (sphere::calcCoeff, reference
+ * sphere.cpp:21-97 with cbessjyva_sph, cbessjy.cpp:456-737).
+ * spheres: host array; A_out/B_out: device complex128 arrays, sphere s writes
+ * elements [coeff_off, coeff_off + Nl] of each.  kmag = 2*PI/lambda. */
+int bimsim_mie_coeffs(bimsim_ctx* ctx, const bimsim_sphere* spheres, int ns,
+                      double kmag, int quirks, void* A_out, void* B_out);
+
+/* Incident field Uf over a uR x vR slice (nearfieldStruct::calcUf,
+ * reference nearfield.cpp:168-192; kernels nfScalarUf.cu:9-175 and
+ * nfScalarUfLut.cu:11-123).  mode = bimsim_uf_mode.  Uf is overwritten. */
+int bimsim_nf_uf(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int vR,
+                 const double focus[3], const double kdir[3], double kmag,
+                 double amp, double cos_alpha, double cos_beta, int m,
+                 int mode, int prec, int quirks, void* Uf);
+
+/* Scattered/internal field of ns spheres summed over nk plane waves
+ * (nearfieldStruct::scalarUs, reference nfScalarUs.cu:494-658: precompute +
+ * optPc_gpuScalarUsp, or gpuScalarUsp for a single plane wave).
+ * waves: device, nk*3 doubles.  subA: amplitude per the reference
+ * (nfScalarUs.cu:572 for Monte-Carlo, A for a plane wave).
+ * A, B: device complex128 packed coefficients (bimsim_mie_coeffs).
+ * accumulate = 0 clears U first (U.clear_gpu(), nfScalarUs.cu:507). */
+int bimsim_nf_us(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int vR,
+                 const double focus[3], double kmag,
+                 const bimsim_sphere* spheres, int ns, const void* A,
+                 const void* B, const double* waves, int nk, double subA,
+                 int prec, int quirks, void* U, int accumulate);
+
+/* U += Uf wherever the pixel lies outside every sphere; U = Uf when ns == 0
+ * (nearfieldStruct::sumUf, reference nfSumUf.cu:6-117). */
+int bimsim_nf_sum_uf(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int vR,
+                     const bimsim_sphere* spheres, int ns, int prec,
+                     const void* Uf, void* U);
+
+/* Detector (microscopeStruct::SimulateImaging, reference microscope.cu:
+ * 608-662 and 722-791; fieldslice.cpp:25-94; fieldslice.cu:71-153,319-362):
+ * FFT of U and Uf, objective band-pass NAin/lambda <= |f| <= NAout/lambda
+ * with du = 1/|X|, dv = 1/|Y|, inverse FFT / (uR*vR), crop of Nd*ss samples
+ * at offset padding*Nd*ss, |.|^2 averaged over ss x ss into D and Di.
+ * D and Di accumulate (+=) like resample_intensity; Abs (may be NULL) gets
+ * -log10(D/Di).  write_filtered != 0 leaves the band-passed fields in U and
+ * Uf like the reference (in-place FFT); otherwise U/Uf are left unchanged.
+ * quirks = BIMSIM_QUIRKS_REF keeps the reference's negative-frequency index
+ * -(uR-1-iu) of the band-pass (microscope.cu:510-518); 0 uses -(uR-iu). */
+int bimsim_detector(bimsim_ctx* ctx, int prec, int quirks, void* U, void* Uf, int uR,
+                    int vR, double du, double dv, double NAin, double NAout,
+                    double lambda, int padding, int Nd, int ss, void* D,
+                    void* Di, void* Abs, int write_filtered);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BIMSIM_HIP_H */
