@@ -1,0 +1,214 @@
+"""Headline benchmark: Mie series terms/s and absorbance images/s on one
+MI355X, batched FP32 form vs the plain FP64 form.
+
+A step is one absorbance image of the workload (incident field, FP64 Mie
+coefficients, scattered field of every sphere under every Monte-Carlo wave,
+Uf + Us, detector FFT / band-pass / intensity / -log10) with inputs resident
+in device memory.  The workload is the 3-sphere, 1000-wave, 512 x 512
+Monte-Carlo focusing configuration (cfg2 in bimsim-cuda_amd/workloads.py).
+
+usage: python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu]
+Prints one JSON line.
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+METRIC = "Mie series terms/s + absorbance images/s, batched FP32 vs plain FP64, 1 GPU"
+PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector, AMD published
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "us_series_f32_cfg2_traffic.json")
+
+
+def load_pkg():
+    d = os.path.join(ROOT, "bimsim-cuda_amd")
+    if "bimsim_amd" in sys.modules:
+        return sys.modules["bimsim_amd"]
+    spec = importlib.util.spec_from_file_location(
+        "bimsim_amd", os.path.join(d, "__init__.py"), submodule_search_locations=[d])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["bimsim_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(pkg, W, inst, waves, A, B, budget_s=12.0):
+    """The oracle (reference algorithm in FP64, OpenMP over rows) on a band of
+    rows of the same workload, scaled to terms/s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+
+    orc = Oracle()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    threads = max(1, min(threads, os.cpu_count() or 1, 16))
+    orc.nthreads = threads
+    N = inst["N"]
+    per_pix = inst["nk"] * sum(s["Nl"] + 1 for s in inst["spheres"])
+
+    def run(rows, r0):
+        q = pkg.Quad()
+        q.A[:] = [inst["A"][0], inst["A"][1] + inst["L"] * r0 / N, inst["A"][2]]
+        q.X[:] = list(inst["X"])
+        q.Y[:] = [0.0, inst["L"] * rows / N, 0.0]
+        t = time.perf_counter()
+        orc.nf_us(q, N, rows, inst["focus"], inst["kmag"], inst["spheres"], A, B, waves,
+                  inst["subA"])
+        return time.perf_counter() - t
+
+    rows = max(threads, 8)
+    dt = run(rows, N // 2)
+    rows = int(min(N, max(rows, rows * budget_s / max(dt, 1e-3))))
+    rows -= rows % threads if rows > threads else 0
+    r0 = max(0, N // 2 - rows // 2)
+    dt = run(rows, r0)
+    tps = rows * N * per_pix / dt
+    return {
+        "value": tps,
+        "unit": "terms/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"scattered-field series (reference FP64 algorithm) of {rows} of {N} rows "
+                  f"(rows {r0}..{r0 + rows - 1}), all {len(inst['spheres'])} spheres and "
+                  f"{inst['nk']} waves, {dt:.1f} s",
+        "images_per_s_est": tps / W.terms(inst),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    pkg = load_pkg()
+    import bimsim_amd.pipeline as P
+    import bimsim_amd.workloads as W
+
+    inst = W.config(2)
+    waves_h = W.mc_waves(inst, pkg.mc_sample_na)
+    waves = torch.as_tensor(waves_h).cuda()
+    ctx = pkg.Context(0)
+    b32 = P.Buffers(inst, pkg.FP32_FAST)
+    b64 = P.Buffers(inst, pkg.FP64_PARITY)
+    terms = W.terms(inst)
+    n2 = inst["N"] ** 2
+    series_flops = n2 * sum(8 * (s["Nl"] + 1) * inst["nk"] + 13 * inst["nk"]
+                            for s in inst["spheres"])
+
+    log(f"bench: workload cfg2 N={inst['N']} nk={inst['nk']} "
+        f"Nl={[s['Nl'] for s in inst['spheres']]} m={inst['m']} terms/image={terms:.3e}")
+    for _ in range(args.warmup):
+        P.simulate(ctx, inst, waves, pkg.FP32_FAST, b32)
+    torch.cuda.synchronize()
+    log("bench: warmup done")
+
+    ctx.set_timing(True)
+    ctx.timings()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        P.simulate(ctx, inst, waves, pkg.FP32_FAST, b32)
+    torch.cuda.synchronize()
+    t32 = (time.perf_counter() - t0) / args.steps
+    tm = ctx.timings()
+    ctx.set_timing(False)
+    ser_ms, ser_n = tm["us_series"]
+    avg_series_ms = ser_ms / max(ser_n, 1)
+    log(f"bench: FP32 {t32 * 1e3:.3f} ms/image, series kernel {avg_series_ms:.3f} ms/launch")
+
+    # plain FP64 form (reference structure), fewer steps
+    P.simulate(ctx, inst, waves, pkg.FP64_PARITY, b64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.fp64_steps):
+        P.simulate(ctx, inst, waves, pkg.FP64_PARITY, b64)
+    torch.cuda.synchronize()
+    t64 = (time.perf_counter() - t0) / args.fp64_steps
+    log(f"bench: FP64 plain {t64 * 1e3:.3f} ms/image")
+
+    # accuracy of the FP32 image against the FP64 form on the full workload
+    # (the detector works on copies, so U still holds the near field Us + Uf)
+    Abs32 = b32.Abs.cpu().numpy().astype(np.float64)
+    Abs64 = b64.Abs.cpu().numpy()
+    Di64 = b64.Di.cpu().numpy()
+    bright = Di64 > 1e-3 * Di64.max()
+    abs_err = float(np.max(np.abs(Abs32[bright] - Abs64[bright])))
+    U32 = b32.U.cpu().numpy().astype(np.complex128)
+    U64 = b64.U.cpu().numpy()
+    field_err = float(np.max(np.abs(U32 - U64)) / np.max(np.abs(U64)))
+
+    traffic = None
+    if os.path.exists(PROFILE_TRAFFIC):
+        with open(PROFILE_TRAFFIC) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+
+    achieved = series_flops / (avg_series_ms * 1e-3) / 1e12 if avg_series_ms > 0 else None
+    out = {
+        "metric": METRIC,
+        "value": terms / t32,
+        "unit": "terms/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t32 * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic: SplitMix64-seeded spheres and Monte-Carlo waves, float32-rounded",
+        "config": {
+            "workload": "cfg2: 3 spheres a~U(2,8) um, n~U(1.3,1.6)+iU(0,0.1), 1000 Monte-Carlo "
+                        "waves over condenser NA 0.2-0.62, lambda 3 um, 512x512 slice over "
+                        "60 um, objective NA 0.62, Legendre incident field",
+            "N": inst["N"], "nk": inst["nk"], "spheres": len(inst["spheres"]),
+            "Nl": [s["Nl"] for s in inst["spheres"]], "m": inst["m"],
+            "terms_per_image": terms,
+        },
+        "images_per_s": 1.0 / t32,
+        "fp64_plain": {"terms_per_s": terms / t64, "images_per_s": 1.0 / t64,
+                       "ms_per_step": t64 * 1e3, "steps": args.fp64_steps},
+        "speedup_vs_fp64_plain": t64 / t32,
+        "fp32_error_vs_fp64": {"field_max_rel": field_err, "absorbance_max_abs": abs_err,
+                               "absorbance_pixels": "incident image >= 1e-3 of its peak"},
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS if achieved else None,
+            "traffic": traffic,
+            "kernel": "us_series_f32",
+            "flops_per_launch": series_flops,
+            "avg_launch_ms": avg_series_ms,
+            "launches": ser_n,
+        },
+    }
+    if not args.no_cpu:
+        log("bench: CPU baseline")
+        A = b64.A.cpu().numpy()
+        B = b64.B.cpu().numpy()
+        out["cpu_baseline"] = cpu_baseline(pkg, W, inst, waves_h, A, B)
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -86,8 +86,8 @@ def lib():
         for name in ("bimsim_ctx_destroy", "bimsim_sync"):
             getattr(L, name).argtypes = [c_void_p]
         L.bimsim_set_timing.argtypes = [c_void_p, c_int]
-        fp = ctypes.POINTER(ctypes.c_float)
-        L.bimsim_get_timings.argtypes = [c_void_p, fp, fp, fp]
+        L.bimsim_get_timings.argtypes = [c_void_p, ctypes.POINTER(ctypes.c_float),
+                                         ctypes.POINTER(c_int)]
         L.bimsim_sphere_order.argtypes = [c_double, c_double]
         L.bimsim_mc_sample_na.argtypes = [c_void_p, c_int, c_double * 3, c_double, c_double,
                                           c_int, ctypes.c_ulonglong]
@@ -162,11 +162,14 @@ class Context:
     def set_timing(self, on):
         self._check(self.L.bimsim_set_timing(self.h, 1 if on else 0))
 
+    STAGES = ("uf", "us", "det", "us_series")
+
     def timings(self):
-        a, b, c = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
-        self._check(self.L.bimsim_get_timings(self.h, ctypes.byref(a), ctypes.byref(b),
-                                              ctypes.byref(c)))
-        return {"t_uf_ms": a.value, "t_us_ms": b.value, "t_det_ms": c.value}
+        """{stage: (summed ms, spans)} since the previous call (waits for the events)."""
+        ms = (ctypes.c_float * 4)()
+        n = (c_int * 4)()
+        self._check(self.L.bimsim_get_timings(self.h, ms, n))
+        return {s: (ms[i], n[i]) for i, s in enumerate(self.STAGES)}
 
     # ---- entry points (device tensors)
     def mie_coeffs(self, spheres, kmag, A, B, quirks=QUIRKS_REF):

@@ -75,17 +75,29 @@ int twiddles_get(bimsim_ctx* ctx, int n, const double** d, const float** f) {
     return BIMSIM_OK;
 }
 
-void timer_begin(bimsim_ctx* ctx) {
-    if (ctx->timing) (void)hipEventRecord(ctx->ev0, ctx->stream);
+static hipEvent_t pooled_event(bimsim_ctx* ctx) {
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
 }
 
-float timer_end(bimsim_ctx* ctx) {
-    if (!ctx->timing) return 0.f;
-    float ms = 0.f;
-    (void)hipEventRecord(ctx->ev1, ctx->stream);
-    (void)hipEventSynchronize(ctx->ev1);
-    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-    return ms;
+int timer_begin(bimsim_ctx* ctx, int stage) {
+    if (!ctx->timing) return -1;
+    bimsim_ctx::Span sp{stage, pooled_event(ctx), pooled_event(ctx)};
+    if (!sp.a || !sp.b) return -1;
+    (void)hipEventRecord(sp.a, ctx->stream);
+    ctx->spans.push_back(sp);
+    return (int)ctx->spans.size() - 1;
+}
+
+void timer_end(bimsim_ctx* ctx, int handle) {
+    if (handle < 0 || handle >= (int)ctx->spans.size()) return;
+    (void)hipEventRecord(ctx->spans[handle].b, ctx->stream);
 }
 
 }  // namespace bimsim
@@ -122,10 +134,6 @@ int bimsim_ctx_create(int device, void* hip_stream, bimsim_ctx** out) {
         }
         ctx->own_stream = true;
     }
-    if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
-        delete ctx;
-        return BIMSIM_E_HIP;
-    }
     *out = ctx;
     return BIMSIM_OK;
 }
@@ -140,8 +148,11 @@ int bimsim_ctx_destroy(bimsim_ctx* ctx) {
         (void)hipFree(kv.second.first);
         (void)hipFree(kv.second.second);
     }
-    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
-    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    for (auto& sp : ctx->spans) {
+        (void)hipEventDestroy(sp.a);
+        (void)hipEventDestroy(sp.b);
+    }
+    for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return BIMSIM_OK;
@@ -163,12 +174,28 @@ int bimsim_set_timing(bimsim_ctx* ctx, int enable) {
     return BIMSIM_OK;
 }
 
-int bimsim_get_timings(const bimsim_ctx* ctx, float* t_uf, float* t_us, float* t_det) {
-    if (!ctx) return BIMSIM_E_INVALID;
-    if (t_uf) *t_uf = ctx->t_uf;
-    if (t_us) *t_us = ctx->t_us;
-    if (t_det) *t_det = ctx->t_det;
-    return BIMSIM_OK;
+int bimsim_get_timings(bimsim_ctx* ctx, float* ms, int* counts) {
+    if (!ctx || !ms) return BIMSIM_E_INVALID;
+    for (int k = 0; k < kNumStages; ++k) {
+        ms[k] = 0.f;
+        if (counts) counts[k] = 0;
+    }
+    int rc = BIMSIM_OK;
+    for (auto& sp : ctx->spans) {
+        float t = 0.f;
+        if (hipEventSynchronize(sp.b) != hipSuccess || hipEventElapsedTime(&t, sp.a, sp.b) != hipSuccess) {
+            rc = set_error(ctx, BIMSIM_E_HIP, "stage timer events failed");
+            t = 0.f;
+        }
+        if (sp.stage >= 0 && sp.stage < kNumStages) {
+            ms[sp.stage] += t;
+            if (counts) counts[sp.stage] += 1;
+        }
+        ctx->event_pool.push_back(sp.a);
+        ctx->event_pool.push_back(sp.b);
+    }
+    ctx->spans.clear();
+    return rc;
 }
 
 // sphere::calcNl, sphere.h:71-74 (float arithmetic, PI = 3.14159f)

@@ -314,7 +314,7 @@ extern "C" int bimsim_detector(bimsim_ctx* ctx, int prec, int quirks, void* U, v
     BIMSIM_CHECK_ARG(ctx, lambda > 0.0 && du > 0.0 && dv > 0.0, "lambda, du and dv must be positive");
     BIMSIM_CHECK_ARG(ctx, prec >= BIMSIM_FP64_PARITY && prec <= BIMSIM_FP32_FAST, "unknown precision");
     BIMSIM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
-    timer_begin(ctx);
+    const int tmr = timer_begin(ctx, kStageDet);
     int rc;
     if (prec == BIMSIM_FP32_FAST)
         rc = detector_impl<float>(ctx, U, Uf, uR, vR, du, dv, NAin, NAout, lambda, padding, Nd, ss,
@@ -323,6 +323,6 @@ extern "C" int bimsim_detector(bimsim_ctx* ctx, int prec, int quirks, void* U, v
         rc = detector_impl<double>(ctx, U, Uf, uR, vR, du, dv, NAin, NAout, lambda, padding, Nd,
                                    ss, D, Di, Abs, write_filtered, quirks);
     if (rc != BIMSIM_OK) return rc;
-    ctx->t_det = timer_end(ctx);
+    timer_end(ctx, tmr);
     return BIMSIM_OK;
 }

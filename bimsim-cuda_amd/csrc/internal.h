@@ -18,10 +18,15 @@ struct bimsim_ctx {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     std::string err;
-    // stage timers (nearfieldStruct::t_Uf / t_Us)
+    // stage timers (nearfieldStruct::t_Uf / t_Us): event pairs recorded on
+    // the stream without waiting, resolved by bimsim_get_timings
     bool timing = false;
-    float t_uf = 0.f, t_us = 0.f, t_det = 0.f;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    struct Span {
+        int stage;
+        hipEvent_t a, b;
+    };
+    std::vector<Span> spans;
+    std::vector<hipEvent_t> event_pool;
     // grow-only device scratch, split into named slots
     std::map<int, std::pair<void*, size_t>> scratch;
     // FFT twiddle tables per length: e^{-2 pi i t/N}, t = 0..N-1
@@ -52,10 +57,13 @@ int scratch_get(bimsim_ctx* ctx, int slot, size_t bytes, void** out);
 // Twiddle table for length n on the device (double and float copies).
 int twiddles_get(bimsim_ctx* ctx, int n, const double** d, const float** f);
 
-// Timer helpers: begin records ev0 on the stream, end records ev1, waits and
-// returns the elapsed milliseconds.
-void timer_begin(bimsim_ctx* ctx);
-float timer_end(bimsim_ctx* ctx);
+// Stages of the timers.
+enum TimerStage { kStageUf = 0, kStageUs = 1, kStageDet = 2, kStageSeries = 3, kNumStages = 4 };
+
+// Timer helpers: timer_begin records a start event for `stage` on the stream
+// (when timing is on) and returns a handle for timer_end; nothing waits.
+int timer_begin(bimsim_ctx* ctx, int stage);
+void timer_end(bimsim_ctx* ctx, int handle);
 
 }  // namespace bimsim
 

@@ -155,7 +155,7 @@ extern "C" int bimsim_nf_uf(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int
     const dim3 grid((npix + kThreads - 1) / kThreads);
     const QuadArg q = to_quad(*pos);
     const d3 f = mk(focus), k = mk(kdir);
-    timer_begin(ctx);
+    const int tmr = timer_begin(ctx, kStageUf);
     const bool f32 = prec == BIMSIM_FP32_FAST;
     if (mode == BIMSIM_UF_PLANE) {
         if (f32)
@@ -173,7 +173,7 @@ extern "C" int bimsim_nf_uf(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int
                                uR, vR, f, k, kmag, amp, cos_alpha, cos_beta, m, (double2*)Uf);
     }
     BIMSIM_HIP_CHECK(ctx, hipGetLastError());
-    ctx->t_uf = timer_end(ctx);
+    timer_end(ctx, tmr);
     return BIMSIM_OK;
 }
 

@@ -168,6 +168,7 @@ int launch_us_series_f32(bimsim_ctx* ctx, int L, const float2* cp, const float4*
         out = (float2*)part;
     }
     dim3 grid(pix_blocks, nsplit);
+    const int tmr = timer_begin(ctx, kStageSeries);
     switch (L) {
 #define BIMSIM_CASE(LL) \
     case LL: launch_L<LL>(grid, ctx->stream, cp, rh, wt, meta, ns, nk_pad, wchunk, npix, out, accumulate, nsplit); break;
@@ -184,6 +185,7 @@ int launch_us_series_f32(bimsim_ctx* ctx, int L, const float2* cp, const float4*
         default:
             return set_error(ctx, BIMSIM_E_UNSUPPORTED, "no FP32 series kernel for %d orders", L);
     }
+    timer_end(ctx, tmr);
     BIMSIM_HIP_CHECK(ctx, hipGetLastError());
     if (nsplit > 1) {
         hipLaunchKernelGGL(us_reduce_splits, dim3(pix_blocks), dim3(kThreads), 0, ctx->stream,

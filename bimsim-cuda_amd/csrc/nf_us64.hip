@@ -336,7 +336,7 @@ extern "C" int bimsim_nf_us(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int
     const d3 f = mk(focus);
     // ptype scale = (1.0f/nk) (nfScalarUs.cu:461): float arithmetic
     const double sA = (double)(1.0f / (float)nk) * subA;
-    timer_begin(ctx);
+    const int tmr = timer_begin(ctx, kStageUs);
     if (prec == BIMSIM_FP64_PARITY) {
         for (int s0 = 0; s0 < ns; s0 += kUsBatch) {
             const int cnt = std::min(kUsBatch, ns - s0);
@@ -401,6 +401,6 @@ extern "C" int bimsim_nf_us(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int
                                             (s0 > 0 || accumulate) ? 1 : 0, s0 == 0));
         }
     }
-    ctx->t_us = timer_end(ctx);
+    timer_end(ctx, tmr);
     return BIMSIM_OK;
 }

@@ -111,13 +111,15 @@ int bimsim_sync(bimsim_ctx* ctx);
 const char* bimsim_last_error(const bimsim_ctx* ctx);
 int bimsim_abi_version(void);
 
-/* Stage timers of the last call (nearfieldStruct::t_Uf / t_Us,
- * reference nearfield.h:58-59, rts/cuda/timer.h:4-21); t_us also covers the
- * scattered-field kernels alone, t_det the detector.  Enabled timing makes
- * each timed call wait for its own events. */
+/* Stage timers (nearfieldStruct::t_Uf / t_Us, reference nearfield.h:58-59,
+ * rts/cuda/timer.h:4-21).  With timing enabled every call records hipEvents
+ * around its stages on the context stream without waiting.
+ * bimsim_get_timings waits for them and returns, per stage, the summed
+ * milliseconds and span counts since the previous call, then resets:
+ * ms[0] incident field, ms[1] scattered field (whole bimsim_nf_us),
+ * ms[2] detector, ms[3] scattered-field series kernel launches only. */
 int bimsim_set_timing(bimsim_ctx* ctx, int enable);
-int bimsim_get_timings(const bimsim_ctx* ctx, float* t_uf_ms, float* t_us_ms,
-                       float* t_det_ms);
+int bimsim_get_timings(bimsim_ctx* ctx, float* ms, int* counts);
 
 /* ------------------------------------------------------------ host helpers */
 
