@@ -42,6 +42,25 @@ int scratch_get(bimsim_ctx* ctx, int slot, size_t bytes, void** out) {
     return BIMSIM_OK;
 }
 
+int int_table_get(bimsim_ctx* ctx, const std::vector<int>& v, const int** out) {
+    auto it = ctx->int_tables.find(v);
+    if (it == ctx->int_tables.end()) {
+        int* d = nullptr;
+        if (hipMalloc(&d, sizeof(int) * std::max<size_t>(v.size(), 1)) != hipSuccess) {
+            (void)hipGetLastError();
+            return set_error(ctx, BIMSIM_E_NOMEM, "table allocation failed");
+        }
+        if (!v.empty()) {
+            BIMSIM_HIP_CHECK(ctx, hipMemcpyAsync(d, v.data(), sizeof(int) * v.size(),
+                                                 hipMemcpyHostToDevice, ctx->stream));
+            BIMSIM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        }
+        it = ctx->int_tables.emplace(v, d).first;
+    }
+    *out = it->second;
+    return BIMSIM_OK;
+}
+
 int twiddles_get(bimsim_ctx* ctx, int n, const double** d, const float** f) {
     auto it = ctx->twiddles.find(n);
     if (it == ctx->twiddles.end()) {
@@ -144,6 +163,7 @@ int bimsim_ctx_destroy(bimsim_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     for (auto& kv : ctx->scratch)
         if (kv.second.first) (void)hipFree(kv.second.first);
+    for (auto& kv : ctx->int_tables) (void)hipFree(kv.second);
     for (auto& kv : ctx->twiddles) {
         (void)hipFree(kv.second.first);
         (void)hipFree(kv.second.second);

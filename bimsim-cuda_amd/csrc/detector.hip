@@ -220,6 +220,336 @@ __global__ void detect_kernel(const typename C2<R>::T* __restrict__ U,
     if (Abs) Abs[i] = -log10(d / di);
 }
 
+// ------------------------------------------------ fixed-length line FFTs
+// Stage plan of a length-N Stockham FFT known at compile time: radix 4 while
+// possible, then 2, 3, 5 (the same order as make_plan).
+constexpr int first_radix(int m) {
+    return m % 4 == 0 ? 4 : (m % 2 == 0 ? 2 : (m % 3 == 0 ? 3 : (m % 5 == 0 ? 5 : 0)));
+}
+template <int N>
+struct Shape {
+    static constexpr int nst() {
+        int m = N, s = 0;
+        while (m > 1) {
+            m /= first_radix(m);
+            ++s;
+        }
+        return s;
+    }
+    static constexpr int rad(int st) {
+        int m = N;
+        for (int i = 0; i < st; ++i) m /= first_radix(m);
+        return first_radix(m);
+    }
+    static constexpr int span(int st) {   // product of the radices before stage st
+        int p = 1, m = N;
+        for (int i = 0; i < st; ++i) {
+            const int r = first_radix(m);
+            p *= r;
+            m /= r;
+        }
+        return p;
+    }
+};
+
+template <class R, int r>
+__device__ __forceinline__ void bfly_fixed(typename C2<R>::T* x, R sgn) {
+    butterfly<R>(x, r, sgn);
+}
+
+template <class R, int N, int ST>
+__device__ __forceinline__ void fixed_stage(const typename C2<R>::T* __restrict__ src,
+                                            typename C2<R>::T* __restrict__ dst, int nb,
+                                            const typename C2<R>::T* __restrict__ tw, R sgn,
+                                            bool inverse) {
+    using C = typename C2<R>::T;
+    constexpr int r = Shape<N>::rad(ST);
+    constexpr int Ns = Shape<N>::span(ST);
+    constexpr int nq = N / r;
+    constexpr int tstep = N / (Ns * r);
+    for (int t = threadIdx.x; t < nb * nq; t += kThreads) {
+        const int g = t / nq, j = t - g * nq;
+        const int k = j % Ns;
+        C x[r];
+#pragma unroll
+        for (int q = 0; q < r; ++q) {
+            C v = src[g * N + j + q * nq];
+            if (Ns > 1 && q > 0) {
+                C w = tw[k * q * tstep];
+                if (inverse) w.y = -w.y;
+                v = cmk<C>(v.x * w.x - v.y * w.y, v.x * w.y + v.y * w.x);
+            }
+            x[q] = v;
+        }
+        bfly_fixed<R, r>(x, sgn);
+        const int o = (j / Ns) * Ns * r + k;
+#pragma unroll
+        for (int q = 0; q < r; ++q) dst[g * N + o + q * Ns] = x[q];
+    }
+}
+
+// All stages; returns the buffer holding the transform (natural order).
+template <class R, int N, int ST = 0>
+__device__ __forceinline__ typename C2<R>::T* fixed_fft(typename C2<R>::T* a,
+                                                       typename C2<R>::T* b, int nb,
+                                                       const typename C2<R>::T* tw, bool inverse) {
+    if constexpr (ST == Shape<N>::nst()) {
+        return a;
+    } else {
+        fixed_stage<R, N, ST>(a, b, nb, tw, inverse ? (R)1 : (R)-1, inverse);
+        __syncthreads();
+        return fixed_fft<R, N, ST + 1>(b, a, nb, tw, inverse);
+    }
+}
+
+// Forward FFT of B rows (length N) of U (blockIdx.y = 0) or Uf (= 1); the
+// pass-band columns cols[0..Nc) go to the compact spectrum
+// G[(field*Nc + c)*nrows + row].
+template <class R, int N>
+__global__ __launch_bounds__(kThreads) void rows_fwd_kernel(
+    const typename C2<R>::T* __restrict__ U, const typename C2<R>::T* __restrict__ Uf, int nrows,
+    int B, const int* __restrict__ cols, int Nc, typename C2<R>::T* __restrict__ G,
+    const typename C2<R>::T* __restrict__ tw) {
+    using C = typename C2<R>::T;
+    extern __shared__ unsigned char smem_raw[];
+    C* a = reinterpret_cast<C*>(smem_raw);
+    C* b = a + (size_t)B * N;
+    const int f = blockIdx.y;
+    const C* F = f ? Uf : U;
+    const int row0 = blockIdx.x * B;
+    const int nb = min(B, nrows - row0);
+    for (int t = threadIdx.x; t < nb * N; t += kThreads) a[t] = F[(size_t)row0 * N + t];
+    __syncthreads();
+    const C* res = fixed_fft<R, N>(a, b, nb, tw, false);
+    for (int t = threadIdx.x; t < nb * Nc; t += kThreads) {
+        const int g = t / Nc, c = t - g * Nc;
+        G[((size_t)f * Nc + c) * nrows + row0 + g] = res[g * N + cols[c]];
+    }
+}
+
+// Pass-band columns of the compact spectrum, each: forward FFT along v,
+// objective band-pass, inverse FFT, scale 1/(uR vR).  In place in G.
+template <class R, int M>
+__global__ __launch_bounds__(kThreads) void cols_band_kernel(typename C2<R>::T* __restrict__ G,
+                                                             int Nc, int B,
+                                                             const int* __restrict__ cols,
+                                                             PostOp bp,
+                                                             const typename C2<R>::T* __restrict__ tw) {
+    using C = typename C2<R>::T;
+    extern __shared__ unsigned char smem_raw[];
+    C* a = reinterpret_cast<C*>(smem_raw);
+    C* b = a + (size_t)B * M;
+    const int f = blockIdx.y;
+    const int c0 = blockIdx.x * B;
+    const int nb = min(B, Nc - c0);
+    C* line = G + ((size_t)f * Nc + c0) * M;
+    for (int t = threadIdx.x; t < nb * M; t += kThreads) a[t] = line[t];
+    __syncthreads();
+    C* res = fixed_fft<R, M>(a, b, nb, tw, false);
+    C* other = res == a ? b : a;
+    for (int t = threadIdx.x; t < nb * M; t += kThreads) {
+        const int g = t / M, iv = t - g * M;
+        if (!passband(bp, cols[c0 + g], iv)) res[t] = cmk<C>((R)0, (R)0);
+    }
+    __syncthreads();
+    res = fixed_fft<R, M>(res, other, nb, tw, true);
+    const R sc = (R)bp.scale;
+    for (int t = threadIdx.x; t < nb * M; t += kThreads) {
+        const C v = res[t];
+        line[t] = cmk<C>(v.x * sc, v.y * sc);
+    }
+}
+
+// Inverse FFT of rows [row_begin, row_begin + nrows) of U/Uf from the compact
+// spectrum (zeros outside the pass band).  Writes |.|^2 (as |.| squared, like
+// resample_intensity) into I[(field*nrows + row - row_begin)*N + iu] and, when
+// Uw/Ufw are given, the band-passed field itself.
+template <class R, int N>
+__global__ __launch_bounds__(kThreads) void rows_inv_kernel(
+    const typename C2<R>::T* __restrict__ G, int Nc, int vR, const int* __restrict__ cols,
+    int row_begin, int nrows, int B, R* __restrict__ I, typename C2<R>::T* __restrict__ Uw,
+    typename C2<R>::T* __restrict__ Ufw, const typename C2<R>::T* __restrict__ tw) {
+    using C = typename C2<R>::T;
+    extern __shared__ unsigned char smem_raw[];
+    C* a = reinterpret_cast<C*>(smem_raw);
+    C* b = a + (size_t)B * N;
+    const int f = blockIdx.y;
+    const int r0 = row_begin + blockIdx.x * B;
+    const int nb = min(B, row_begin + nrows - r0);
+    for (int t = threadIdx.x; t < nb * N; t += kThreads) a[t] = cmk<C>((R)0, (R)0);
+    __syncthreads();
+    for (int t = threadIdx.x; t < nb * Nc; t += kThreads) {
+        const int g = t / Nc, c = t - g * Nc;
+        a[g * N + cols[c]] = G[((size_t)f * Nc + c) * vR + r0 + g];
+    }
+    __syncthreads();
+    const C* res = fixed_fft<R, N>(a, b, nb, tw, true);
+    C* W = f ? Ufw : Uw;
+    for (int t = threadIdx.x; t < nb * N; t += kThreads) {
+        const C v = res[t];
+        if (W) W[(size_t)r0 * N + t] = v;
+        const R m = sqrt(v.x * v.x + v.y * v.y);
+        I[((size_t)f * nrows + (r0 - row_begin)) * N + t] = m * m;
+    }
+}
+
+template <class R>
+__global__ void fill_zero(R* p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (R)0;
+}
+
+// crop + resample_intensity (+=) + calc_absorbance from intensity rows
+template <class R>
+__global__ void detect_from_intensity(const R* __restrict__ I, int nrows, int uR, int off, int Nd,
+                                      int ss, R* __restrict__ D, R* __restrict__ Di,
+                                      R* __restrict__ Abs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Nd * Nd) return;
+    const int iv = i / Nd, iu = i - iv * Nd;
+    const R* I0 = I;
+    const R* I1 = I + (size_t)nrows * uR;
+    R s = 0, si = 0;
+    for (int ix = 0; ix < ss; ++ix)
+        for (int iy = 0; iy < ss; ++iy) {
+            const size_t fi = (size_t)(iv * ss + iy) * uR + (off + iu * ss + ix);
+            s += I0[fi];
+            si += I1[fi];
+        }
+    const R d = D[i] + s / (R)(ss * ss);
+    const R di = Di[i] + si / (R)(ss * ss);
+    D[i] = d;
+    Di[i] = di;
+    if (Abs) Abs[i] = -log10(d / di);
+}
+
+#define BIMSIM_FFT_SIZES(X) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) \
+    X(1536) X(2048) X(3072) X(4096)
+
+bool fixed_size(int n) {
+    switch (n) {
+#define BIMSIM_X(NN) case NN:
+        BIMSIM_FFT_SIZES(BIMSIM_X)
+#undef BIMSIM_X
+        return true;
+        default:
+            return false;
+    }
+}
+
+template <class R>
+int launch_rows_fwd(bimsim_ctx* ctx, int N, dim3 grid, size_t lds, const typename C2<R>::T* U,
+                    const typename C2<R>::T* Uf, int nrows, int B, const int* cols, int Nc,
+                    typename C2<R>::T* G, const typename C2<R>::T* tw) {
+    switch (N) {
+#define BIMSIM_X(NN)                                                                            \
+    case NN:                                                                                    \
+        hipLaunchKernelGGL((rows_fwd_kernel<R, NN>), grid, dim3(kThreads), lds, ctx->stream, U, \
+                           Uf, nrows, B, cols, Nc, G, tw);                                      \
+        break;
+        BIMSIM_FFT_SIZES(BIMSIM_X)
+#undef BIMSIM_X
+    }
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return BIMSIM_OK;
+}
+
+template <class R>
+int launch_cols_band(bimsim_ctx* ctx, int M, dim3 grid, size_t lds, typename C2<R>::T* G, int Nc,
+                     int B, const int* cols, const PostOp& bp, const typename C2<R>::T* tw) {
+    switch (M) {
+#define BIMSIM_X(NN)                                                                            \
+    case NN:                                                                                    \
+        hipLaunchKernelGGL((cols_band_kernel<R, NN>), grid, dim3(kThreads), lds, ctx->stream, G, \
+                           Nc, B, cols, bp, tw);                                                \
+        break;
+        BIMSIM_FFT_SIZES(BIMSIM_X)
+#undef BIMSIM_X
+    }
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return BIMSIM_OK;
+}
+
+template <class R>
+int launch_rows_inv(bimsim_ctx* ctx, int N, dim3 grid, size_t lds, const typename C2<R>::T* G,
+                    int Nc, int vR, const int* cols, int row_begin, int nrows, int B, R* I,
+                    typename C2<R>::T* Uw, typename C2<R>::T* Ufw, const typename C2<R>::T* tw) {
+    switch (N) {
+#define BIMSIM_X(NN)                                                                            \
+    case NN:                                                                                    \
+        hipLaunchKernelGGL((rows_inv_kernel<R, NN>), grid, dim3(kThreads), lds, ctx->stream, G, \
+                           Nc, vR, cols, row_begin, nrows, B, I, Uw, Ufw, tw);                  \
+        break;
+        BIMSIM_FFT_SIZES(BIMSIM_X)
+#undef BIMSIM_X
+    }
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return BIMSIM_OK;
+}
+
+// Band-limited detector for FFT lengths with a fixed plan.
+template <class R>
+int detector_band(bimsim_ctx* ctx, const PostOp& bp, void* Uin, void* Ufin, int uR, int vR,
+                  int padding, int Nd, int ss, void* D, void* Di, void* Abs, int write_filtered,
+                  const typename C2<R>::T* tw_u, const typename C2<R>::T* tw_v) {
+    using C = typename C2<R>::T;
+    // pass-band columns: |f_u| <= NAout/lambda with the band-pass's own index map
+    std::vector<int> cols;
+    for (int iu = 0; iu < uR; ++iu) {
+        double u;
+        if (iu <= uR / 2) u = (double)iu * bp.du;
+        else u = -(double)(bp.quirks ? uR - 1 - iu : uR - iu) * bp.du;
+        if (!(std::fabs(u) > bp.fmax)) cols.push_back(iu);
+    }
+    const int Nc = (int)cols.size();
+    const int S = Nd * ss, off = padding * S;
+    const int row_begin = write_filtered ? 0 : off;
+    const int nrows = write_filtered ? vR : S;
+    const int* colsd = nullptr;
+    void *G = nullptr, *I = nullptr;
+    BIMSIM_TRY(int_table_get(ctx, cols, &colsd));
+    BIMSIM_TRY(scratch_get(ctx, kSlotFftA, sizeof(C) * 2 * (size_t)std::max(Nc, 1) * vR, &G));
+    BIMSIM_TRY(scratch_get(ctx, kSlotFftB, sizeof(R) * 2 * (size_t)nrows * uR, &I));
+    if (Nc > 0) {
+        int B = std::max(1, std::min(8, 1024 / uR));
+        size_t lds = sizeof(C) * 2 * (size_t)B * uR;
+        BIMSIM_TRY(launch_rows_fwd<R>(ctx, uR, dim3((vR + B - 1) / B, 2), lds, (const C*)Uin,
+                                      (const C*)Ufin, vR, B, colsd, Nc, (C*)G, tw_u));
+        int Bc = std::max(1, std::min(8, 2048 / vR));
+        Bc = std::min(Bc, Nc);
+        lds = sizeof(C) * 2 * (size_t)Bc * vR;
+        BIMSIM_TRY(launch_cols_band<R>(ctx, vR, dim3((Nc + Bc - 1) / Bc, 2), lds, (C*)G, Nc, Bc,
+                                       colsd, bp, tw_v));
+    }
+    int B = std::max(1, std::min(8, 1024 / uR));
+    size_t lds = sizeof(C) * 2 * (size_t)B * uR;
+    if (Nc > 0) {
+        BIMSIM_TRY(launch_rows_inv<R>(ctx, uR, dim3((nrows + B - 1) / B, 2), lds, (const C*)G, Nc,
+                                      vR, colsd, row_begin, nrows, B, (R*)I,
+                                      write_filtered ? (C*)Uin : nullptr,
+                                      write_filtered ? (C*)Ufin : nullptr, tw_u));
+    } else {
+        // nothing passes the objective: zero intensities (and fields)
+        const size_t ni = 2 * (size_t)nrows * uR;
+        hipLaunchKernelGGL(fill_zero<R>, dim3((unsigned)((ni + 255) / 256)), dim3(256), 0,
+                           ctx->stream, (R*)I, ni);
+        if (write_filtered) {
+            const size_t nf = 2 * (size_t)uR * vR;   // complex values as pairs of R
+            hipLaunchKernelGGL(fill_zero<R>, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0,
+                               ctx->stream, (R*)Uin, nf);
+            hipLaunchKernelGGL(fill_zero<R>, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0,
+                               ctx->stream, (R*)Ufin, nf);
+        }
+        BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    }
+    const R* Ic = (const R*)I + (size_t)(off - row_begin) * uR;
+    // detect_from_intensity indexes rows relative to the crop start
+    hipLaunchKernelGGL(detect_from_intensity<R>, dim3((Nd * Nd + 255) / 256), dim3(256), 0,
+                       ctx->stream, Ic, nrows, uR, off, Nd, ss, (R*)D, (R*)Di, (R*)Abs);
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return BIMSIM_OK;
+}
+
 template <class R>
 int run_fft_pass(bimsim_ctx* ctx, typename C2<R>::T* data, int nlines, int n, long long line_stride,
                  long long elem_stride, const typename C2<R>::T* tw, int inverse, const PostOp& op) {
@@ -253,6 +583,27 @@ int detector_impl(bimsim_ctx* ctx, void* Uin, void* Ufin, int uR, int vR, double
     const size_t npix = (size_t)uR * vR;
     C* U = (C*)Uin;
     C* Uf = (C*)Ufin;
+    if (fixed_size(uR) && fixed_size(vR) &&
+        sizeof(C) * 2 * (size_t)std::max(uR, vR) <= ctx->lds_max) {
+        const double *twd_u, *twd_v;
+        const float *twf_u, *twf_v;
+        BIMSIM_TRY(twiddles_get(ctx, uR, &twd_u, &twf_u));
+        BIMSIM_TRY(twiddles_get(ctx, vR, &twd_v, &twf_v));
+        PostOp bp{};
+        bp.kind = 1;
+        bp.uR = uR;
+        bp.vR = vR;
+        bp.quirks = quirks;
+        bp.du = du;
+        bp.dv = dv;
+        bp.fmin = NAin / lambda;
+        bp.fmax = NAout / lambda;
+        bp.scale = 1.0 / ((unsigned)uR * (unsigned)vR);   // ScaleField(1.0/(R0*R1))
+        const bool dbl = std::is_same<R, double>::value;
+        return detector_band<R>(ctx, bp, Uin, Ufin, uR, vR, padding, Nd, ss, D, Di, Abs,
+                                write_filtered, dbl ? (const C*)twd_u : (const C*)twf_u,
+                                dbl ? (const C*)twd_v : (const C*)twf_v);
+    }
     if (!write_filtered) {
         void *a = nullptr, *b = nullptr;
         BIMSIM_TRY(scratch_get(ctx, kSlotFftA, sizeof(C) * npix, &a));

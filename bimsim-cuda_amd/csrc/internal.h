@@ -31,6 +31,9 @@ struct bimsim_ctx {
     std::map<int, std::pair<void*, size_t>> scratch;
     // FFT twiddle tables per length: e^{-2 pi i t/N}, t = 0..N-1
     std::map<int, std::pair<double*, float*>> twiddles;
+    // device copies of small host tables, keyed by content (e.g. the
+    // detector's pass-band column list)
+    std::map<std::vector<int>, int*> int_tables;
     int cu_count = 256;
     size_t lds_max = 65536;
 };
@@ -53,6 +56,9 @@ int set_error(bimsim_ctx* ctx, int code, const char* fmt, ...);
 
 // Device scratch of at least `bytes` in `slot` (grown with hipMalloc).
 int scratch_get(bimsim_ctx* ctx, int slot, size_t bytes, void** out);
+
+// Device copy of an int table (cached by content; uploaded once).
+int int_table_get(bimsim_ctx* ctx, const std::vector<int>& v, const int** out);
 
 // Twiddle table for length n on the device (double and float copies).
 int twiddles_get(bimsim_ctx* ctx, int n, const double** d, const float** f);

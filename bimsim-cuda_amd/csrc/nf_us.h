@@ -24,6 +24,7 @@ struct WaveF {
 // Per-sphere offsets into the FP32 series buffers.
 struct SeriesMeta {
     int nb[kUsBatch];         // blocks of 8 orders (ceil((Nl+1)/8))
+    int nl1[kUsBatch];        // orders Nl + 1
     long long coff[kUsBatch]; // element offset of c'[l][pix] for the sphere
     long long roff[kUsBatch]; // element offset of r-hat[pix]
     long long woff[kUsBatch]; // element offset of the sphere's wave table

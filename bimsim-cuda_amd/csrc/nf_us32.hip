@@ -20,6 +20,59 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kW = 2;   // waves in flight per thread
 
+// One step of the series for order l on kW waves (l >= 2).
+template <int l>
+__device__ __forceinline__ void series_step(const float* cr, const float* ci, const float* x2,
+                                            float* q0, float* q1, float* sr, float* si) {
+    const float g = -(float)legq_g(l);
+#pragma unroll
+    for (int u = 0; u < kW; ++u) {
+        const float qn = fmaf(g, q0[u], x2[u] * q1[u]);
+        q0[u] = q1[u];
+        q1[u] = qn;
+        sr[u] = fmaf(cr[l], qn, sr[u]);
+        si[u] = fmaf(ci[l], qn, si[u]);
+    }
+}
+
+// Step j (0..7) of the block starting at order l0, when that order is >= 2
+// (orders 0 and 1 are folded into the initial sum).
+template <int l0, int j>
+__device__ __forceinline__ void block_step(const float* cr, const float* ci, const float* x2,
+                                           float* q0, float* q1, float* sr, float* si) {
+    if constexpr (l0 + j >= 2) series_step<l0 + j>(cr, ci, x2, q0, q1, sr, si);
+}
+
+// Orders in blocks of 8: blocks below nfull run whole, block nfull runs its
+// first rem orders, each behind a wavefront-uniform test.
+template <int L, int B = 0>
+__device__ __forceinline__ void series_blocks(const float* cr, const float* ci, const float* x2,
+                                              float* q0, float* q1, float* sr, float* si,
+                                              int nfull, int rem) {
+    if constexpr (B < L / 8) {
+        constexpr int l0 = 8 * B;
+        if (B < nfull) {
+            block_step<l0, 0>(cr, ci, x2, q0, q1, sr, si);
+            block_step<l0, 1>(cr, ci, x2, q0, q1, sr, si);
+            block_step<l0, 2>(cr, ci, x2, q0, q1, sr, si);
+            block_step<l0, 3>(cr, ci, x2, q0, q1, sr, si);
+            block_step<l0, 4>(cr, ci, x2, q0, q1, sr, si);
+            block_step<l0, 5>(cr, ci, x2, q0, q1, sr, si);
+            block_step<l0, 6>(cr, ci, x2, q0, q1, sr, si);
+            block_step<l0, 7>(cr, ci, x2, q0, q1, sr, si);
+            series_blocks<L, B + 1>(cr, ci, x2, q0, q1, sr, si, nfull, rem);
+        } else if (B == nfull) {
+            if (0 < rem) block_step<l0, 0>(cr, ci, x2, q0, q1, sr, si);
+            if (1 < rem) block_step<l0, 1>(cr, ci, x2, q0, q1, sr, si);
+            if (2 < rem) block_step<l0, 2>(cr, ci, x2, q0, q1, sr, si);
+            if (3 < rem) block_step<l0, 3>(cr, ci, x2, q0, q1, sr, si);
+            if (4 < rem) block_step<l0, 4>(cr, ci, x2, q0, q1, sr, si);
+            if (5 < rem) block_step<l0, 5>(cr, ci, x2, q0, q1, sr, si);
+            if (6 < rem) block_step<l0, 6>(cr, ci, x2, q0, q1, sr, si);
+        }
+    }
+}
+
 template <int L>
 __global__ __launch_bounds__(kThreads) void us_series_f32(
     const float2* __restrict__ cp, const float4* __restrict__ rh, const WaveF* __restrict__ wt,
@@ -32,6 +85,8 @@ __global__ __launch_bounds__(kThreads) void us_series_f32(
     float acc_r = 0.f, acc_i = 0.f, cmp_r = 0.f, cmp_i = 0.f;
     for (int s = 0; s < ns; ++s) {
         const int nb = meta.nb[s];
+        const int nl1 = meta.nl1[s];        // Nl + 1 orders
+        const int nfull = nl1 >> 3, rem = nl1 & 7;
         float cr[L], ci[L];
         const float2* c = cp + meta.coff[s] + pix;
 #pragma unroll
@@ -52,11 +107,15 @@ __global__ __launch_bounds__(kThreads) void us_series_f32(
         const WaveF* W = wt + meta.woff[s];
         float ch_r = 0.f, ch_i = 0.f;
         int in_chunk = 0;
+        if (w_begin >= w_end) continue;
+        WaveF nxt[kW];
+#pragma unroll
+        for (int u = 0; u < kW; ++u) nxt[u] = W[w_begin + u];
         for (int w = w_begin; w < w_end; w += kW) {
             float x2[kW], q0[kW], q1[kW], sr[kW], si[kW], pr[kW], pi[kW];
 #pragma unroll
             for (int u = 0; u < kW; ++u) {
-                const WaveF e = W[w + u];
+                const WaveF e = nxt[u];
                 x2[u] = e.kx2 * r.x + e.ky2 * r.y + e.kz2 * r.z;
                 pr[u] = e.pr;
                 pi[u] = e.pi;
@@ -65,26 +124,11 @@ __global__ __launch_bounds__(kThreads) void us_series_f32(
                 sr[u] = fmaf(cr[1], x2[u], cr[0]);
                 si[u] = fmaf(ci[1], x2[u], ci[0]);
             }
-            // orders in blocks of 8; blocks beyond the sphere's order are
-            // skipped by a wavefront-uniform branch
+            // prefetch the next wave records (the last iteration re-reads its own)
+            const int wn = w + kW < w_end ? w + kW : w;
 #pragma unroll
-            for (int b = 0; b < L / 8; ++b) {
-                if (b < nb) {
-#pragma unroll
-                    for (int j = (b == 0 ? 2 : 0); j < 8; ++j) {
-                        const int l = 8 * b + j;
-                        const float g = -(float)legq_g(l);
-#pragma unroll
-                        for (int u = 0; u < kW; ++u) {
-                            const float qn = fmaf(g, q0[u], x2[u] * q1[u]);
-                            q0[u] = q1[u];
-                            q1[u] = qn;
-                            sr[u] = fmaf(cr[l], qn, sr[u]);
-                            si[u] = fmaf(ci[l], qn, si[u]);
-                        }
-                    }
-                }
-            }
+            for (int u = 0; u < kW; ++u) nxt[u] = W[wn + u];
+            series_blocks<L>(cr, ci, x2, q0, q1, sr, si, nfull, rem);
 #pragma unroll
             for (int u = 0; u < kW; ++u) {
                 ch_r = fmaf(pr[u], sr[u], ch_r);

@@ -381,6 +381,7 @@ extern "C" int bimsim_nf_us(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int
             for (int i = 0; i < cnt; ++i) {
                 b.s[i] = to_dev(spheres[s0 + i]);
                 meta.nb[i] = (spheres[s0 + i].Nl + 1 + 7) / 8;
+                meta.nl1[i] = spheres[s0 + i].Nl + 1;
                 meta.coff[i] = coff;
                 coff += (long long)meta.nb[i] * 8 * npix;
                 meta.roff[i] = (long long)i * npix;

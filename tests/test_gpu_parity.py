@@ -117,6 +117,36 @@ def test_scattered_field_matches_oracle(pkg, ctx, orc, W, cid, N, nk):
     assert normrel(outs[pkg.FP32_FAST], ref) <= 2e-5
 
 
+def test_scattered_field_mixed_orders(pkg, ctx, orc, W):
+    """Spheres whose Nl+1 is below 8, a multiple of 8 and ragged, one pixel at
+    a sphere centre (calc_Ui's A_0 branch) and pixels inside every sphere."""
+    import bimsim_amd.pipeline as P
+    sph = [dict(p=(0.0, 0.0, 0.0), a=0.2, n=complex(1.5, 0.05)),
+           dict(p=(-6.0, 4.0, 0.0), a=3.0, n=complex(1.33, 0.0)),
+           dict(p=(-9.0, -9.0, 0.0), a=2.6, n=complex(1.45, 0.03)),
+           dict(p=(7.0, -5.0, 0.5), a=6.0, n=complex(1.6, 0.1)),
+           dict(p=(2.0, 9.0, -1.0), a=1.9, n=complex(1.2, 0.02))]
+    inst = W.make_instance(sph, 3.0, 32, 30.0, (0.1, 0.5), (0.0, 0.5), 18, 77)
+    orders = [s["Nl"] + 1 for s in inst["spheres"]]
+    assert min(orders) < 8 and any(o % 8 == 0 for o in orders)
+    waves = instance_waves(pkg, W, inst)
+    q = P.quad_of(inst)
+    N = inst["N"]
+    A = torch.empty(inst["ncoef"], dtype=torch.complex128, device="cuda")
+    B = torch.empty(inst["ncoef"], dtype=torch.complex128, device="cuda")
+    ctx.mie_coeffs(inst["spheres"], inst["kmag"], A, B)
+    Ah, Bh = host(A), host(B)
+    ref = orc.nf_us(q, N, N, inst["focus"], inst["kmag"], inst["spheres"], Ah, Bh, waves,
+                    inst["subA"]).ravel()
+    wd = dev(waves, torch.float64)
+    for prec, dt, tol in [(pkg.FP64_PARITY, torch.complex128, 1e-10),
+                          (pkg.FP32_FAST, torch.complex64, 2e-5)]:
+        U = torch.empty(N * N, dtype=dt, device="cuda")
+        ctx.nf_us(q, N, N, inst["focus"], inst["kmag"], inst["spheres"], A, B, wd, inst["subA"],
+                  U, prec=prec)
+        assert normrel(host(U), ref) <= tol, (prec, orders)
+
+
 def test_scattered_field_accumulates_and_single_plane_wave(pkg, ctx, orc, W):
     import bimsim_amd.pipeline as P
     inst, _ = _us_case(pkg, W, 1, 32, 1)
