@@ -27,7 +27,8 @@ EXPORTS = (
     "bimsim_abi_version", "bimsim_ctx_create", "bimsim_ctx_destroy", "bimsim_sync",
     "bimsim_last_error", "bimsim_set_timing", "bimsim_get_timings", "bimsim_sphere_order",
     "bimsim_mc_sample_na", "bimsim_mie_coeffs", "bimsim_nf_uf", "bimsim_nf_us",
-    "bimsim_nf_sum_uf", "bimsim_detector",
+    "bimsim_nf_sum_uf", "bimsim_detector", "bimsim_uf_bessel_lut", "bimsim_nf_uf_lut",
+    "bimsim_sphere_lut_dims", "bimsim_sphere_lut", "bimsim_nf_us_lut", "bimsim_nf_interpolate",
 )
 
 
@@ -104,6 +105,24 @@ def lib():
         L.bimsim_detector.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_int,
                                       c_double, c_double, c_double, c_double, c_double, c_int,
                                       c_int, c_int, c_void_p, c_void_p, c_void_p, c_int]
+        L.bimsim_uf_bessel_lut.argtypes = [c_void_p, c_int, c_double, c_double, c_double, c_int,
+                                           c_int, c_void_p]
+        L.bimsim_nf_uf_lut.argtypes = [c_void_p, ctypes.POINTER(Quad), c_int, c_int,
+                                       c_double * 3, c_double * 3, c_double, c_double, c_double,
+                                       c_int, c_void_p, c_int, c_double, c_double, c_int,
+                                       c_void_p]
+        ip, dp = ctypes.POINTER(c_int), ctypes.POINTER(c_double)
+        L.bimsim_sphere_lut_dims.argtypes = [ctypes.POINTER(Quad), c_void_p, c_int, c_int, ip, ip,
+                                             dp, dp]
+        L.bimsim_sphere_lut.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_double,
+                                        c_double, c_double, c_int, c_int, c_int, c_int, c_void_p,
+                                        c_void_p]
+        L.bimsim_nf_us_lut.argtypes = [c_void_p, ctypes.POINTER(Quad), c_int, c_int,
+                                       c_double * 3, c_double, c_void_p, c_void_p, c_int,
+                                       c_void_p, c_int, c_int, c_double, c_double, c_void_p,
+                                       c_int, c_double, c_int, c_void_p]
+        L.bimsim_nf_interpolate.argtypes = [c_void_p, c_int, c_double, c_int, c_int, c_void_p,
+                                            c_void_p, c_void_p, c_void_p]
         _lib = L
     return _lib
 
@@ -120,6 +139,18 @@ def mc_sample_na(N, k, NAin, NAout, rng=RNG_GLIBC, seed=0):
     if rc != OK:
         raise BimsimError(rc, "invalid Monte-Carlo sampling arguments")
     return [tuple(out[3 * i:3 * i + 3]) for i in range(N)]
+
+
+def sphere_lut_dims(q, sphere, R, dR_override=0):
+    """sphere::calcUp table sizes -> (aR, dR, d_min, d_max)."""
+    S = sphere_array([sphere])
+    aR, dR = c_int(), c_int()
+    dmin, dmax = c_double(), c_double()
+    rc = lib().bimsim_sphere_lut_dims(ctypes.byref(q), S, R, dR_override, ctypes.byref(aR),
+                                      ctypes.byref(dR), ctypes.byref(dmin), ctypes.byref(dmax))
+    if rc != OK:
+        raise BimsimError(rc, "bad sphere LUT arguments")
+    return aR.value, dR.value, dmin.value, dmax.value
 
 
 def _ptr(t):
@@ -195,6 +226,34 @@ class Context:
         S = sphere_array(spheres)
         self._check(self.L.bimsim_nf_sum_uf(self.h, ctypes.byref(q), uR, vR, S, len(spheres),
                                             prec, _ptr(Uf), _ptr(U)))
+
+    def uf_bessel_lut(self, m, kmag, dmin, dmax, dR, lut, prec=FP64_PARITY):
+        self._check(self.L.bimsim_uf_bessel_lut(self.h, m, kmag, dmin, dmax, dR, prec, _ptr(lut)))
+
+    def nf_uf_lut(self, q, uR, vR, focus, kdir, amp, ca, cb, m, lut, dR, dmin, dmax, Uf,
+                  prec=FP64_PARITY):
+        self._check(self.L.bimsim_nf_uf_lut(self.h, ctypes.byref(q), uR, vR,
+                                            (c_double * 3)(*focus), (c_double * 3)(*kdir), amp,
+                                            ca, cb, m, _ptr(lut), dR, dmin, dmax, prec, _ptr(Uf)))
+
+    def sphere_lut(self, sphere, A, B, kmag, dmin, dmax, dR, aR, thetaR, Usp, Uip,
+                   prec=FP64_PARITY):
+        S = sphere_array([sphere])
+        self._check(self.L.bimsim_sphere_lut(self.h, S, _ptr(A), _ptr(B), kmag, dmin, dmax, dR,
+                                             aR, thetaR, prec, _ptr(Usp), _ptr(Uip)))
+
+    def nf_us_lut(self, q, uR, vR, focus, kmag, sphere, Usp, dR, Uip, aR, thetaR, dmin, dmax,
+                  waves, subA, U, prec=FP64_PARITY):
+        S = sphere_array([sphere])
+        nk = waves.numel() // 3
+        self._check(self.L.bimsim_nf_us_lut(self.h, ctypes.byref(q), uR, vR,
+                                            (c_double * 3)(*focus), kmag, S, _ptr(Usp), dR,
+                                            _ptr(Uip), aR, thetaR, dmin, dmax, _ptr(waves), nk,
+                                            subA, prec, _ptr(U)))
+
+    def nf_interpolate(self, deg, uR, vR, U_src, Uf_src, U, Uf, prec=FP64_PARITY):
+        self._check(self.L.bimsim_nf_interpolate(self.h, prec, deg, uR, vR, _ptr(U_src),
+                                                 _ptr(Uf_src), _ptr(U), _ptr(Uf)))
 
     def detector(self, U, Uf, uR, vR, du, dv, NAin, NAout, lam, padding, Nd, ss, D, Di,
                  Abs=None, prec=FP64_PARITY, quirks=QUIRKS_REF, write_filtered=True):

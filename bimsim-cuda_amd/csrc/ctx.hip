@@ -144,15 +144,9 @@ int bimsim_ctx_create(int device, void* hip_stream, bimsim_ctx** out) {
     ctx->device = device;
     ctx->cu_count = prop.multiProcessorCount;
     ctx->lds_max = prop.sharedMemPerBlock;
-    if (hip_stream) {
-        ctx->stream = (hipStream_t)hip_stream;
-    } else {
-        if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
-            delete ctx;
-            return BIMSIM_E_HIP;
-        }
-        ctx->own_stream = true;
-    }
+    // NULL selects the device's default stream: the stream the reference's
+    // kernels run on and the one torch's default stream work is ordered with.
+    ctx->stream = (hipStream_t)hip_stream;
     *out = ctx;
     return BIMSIM_OK;
 }
@@ -173,7 +167,6 @@ int bimsim_ctx_destroy(bimsim_ctx* ctx) {
         (void)hipEventDestroy(sp.b);
     }
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
-    if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return BIMSIM_OK;
 }

@@ -16,7 +16,6 @@
 struct bimsim_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    bool own_stream = false;
     std::string err;
     // stage timers (nearfieldStruct::t_Uf / t_Us): event pairs recorded on
     // the stream without waiting, resolved by bimsim_get_timings
@@ -50,6 +49,7 @@ enum ScratchSlot {
     kSlotFftB = 6,
     kSlotDet = 7,
     kSlotMisc = 8,
+    kSlotUfLut = 9,   // incident-field Bessel table
 };
 
 int set_error(bimsim_ctx* ctx, int code, const char* fmt, ...);

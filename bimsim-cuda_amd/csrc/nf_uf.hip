@@ -3,6 +3,7 @@
 // in reference operation order for every precision; FP32_FAST stores the
 // result as float pairs.  The Uf series is O(pixels x m), a small fraction of
 // the scattered-field work of any Monte-Carlo configuration.
+#include "geometry.h"
 #include "internal.h"
 #include "refmath.h"
 
@@ -144,9 +145,21 @@ extern "C" int bimsim_nf_uf(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int
     BIMSIM_CHECK_ARG(ctx, prec >= BIMSIM_FP64_PARITY && prec <= BIMSIM_FP32_FAST, "unknown precision");
     BIMSIM_CHECK_ARG(ctx, m >= 1 && m <= 100000, "incident order m must be in [1, 100000]");
     BIMSIM_CHECK_ARG(ctx, kmag > 0.0 && std::isfinite(kmag), "kmag must be positive");
-    if (mode == BIMSIM_UF_LUT)
-        return set_error(ctx, BIMSIM_E_UNSUPPORTED, "Uf look-up-table mode is not built");
-    BIMSIM_CHECK_ARG(ctx, mode == BIMSIM_UF_SERIES || mode == BIMSIM_UF_PLANE, "unknown Uf mode");
+    BIMSIM_CHECK_ARG(ctx, mode == BIMSIM_UF_SERIES || mode == BIMSIM_UF_PLANE || mode == BIMSIM_UF_LUT,
+                     "unknown Uf mode");
+    if (mode == BIMSIM_UF_LUT) {
+        // nearfieldStruct::scalarUfLut (nfScalarUfLut.cu:125-188): table over
+        // [d_min, d_max] from the focus with 2 max(uR, vR) samples
+        double dmin, dmax;
+        quad_dist(to_quad(*pos), mk(focus), &dmin, &dmax);
+        const int dR = 2 * std::max(uR, vR);
+        const size_t esz = prec == BIMSIM_FP32_FAST ? sizeof(float) : sizeof(double);
+        void* lut = nullptr;
+        BIMSIM_TRY(scratch_get(ctx, kSlotUfLut, esz * (size_t)dR * (m + 1), &lut));
+        BIMSIM_TRY(bimsim_uf_bessel_lut(ctx, m, kmag, dmin, dmax, dR, prec, lut));
+        return bimsim_nf_uf_lut(ctx, pos, uR, vR, focus, kdir, amp, cos_alpha, cos_beta, m, lut, dR,
+                                dmin, dmax, prec, Uf);
+    }
     if (!quirks && mode == BIMSIM_UF_SERIES)
         return set_error(ctx, BIMSIM_E_UNSUPPORTED,
                          "incident-field series without the reference recurrences is not built");

@@ -402,6 +402,166 @@ __device__ inline void cjyva(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyv
     for (int k = 1; k <= n; ++k) cyvp[k] = zsub(cyv[k - 1], zdiv(zsc(k + v0, cyv[k]), z));
 }
 
+// Real-argument J_{v0+k}, Y_{v0+k} and derivatives, k = 0..floor(v) (JYV,
+// reference bessjy.cpp:536-707).  Arrays hold floor(v)+2 entries; orders the
+// Miller start truncates stay 0.
+__device__ inline void jyv(double v, double x, double* jv, double* yv, double* djv, double* dyv) {
+    const double eps = 1e-15, euler = 0.5772156649015329;
+    const double x2 = x * x;
+    int n = (int)v;
+    const double v0 = v - n;
+    if (x < 1e-15) {
+        for (int k = 0; k <= n; ++k) {
+            jv[k] = 0.0;
+            yv[k] = -1e308;
+            djv[k] = 0.0;
+            dyv[k] = 1e308;
+        }
+        if (v0 == 0.0) {
+            jv[0] = 1.0;
+            djv[1] = 0.5;
+        } else {
+            djv[0] = 1e308;
+        }
+        return;
+    }
+    double bj0 = 0.0, bj1 = 0.0, by0 = 0.0, by1 = 0.0;
+    if (x <= 12.0) {
+        for (int l = 0; l < 2; ++l) {
+            const double vl = v0 + l;
+            double s = 1.0, t = 1.0;
+            for (int k = 1; k <= 40; ++k) {
+                t *= -0.25 * x2 / (k * (k + vl));
+                s += t;
+                if (fabs(t) < fabs(s) * eps) break;
+            }
+            const double val = s * (pow(0.5 * x, vl) / gamma_fn(1.0 + vl));
+            if (l == 0) bj0 = val;
+            else bj1 = val;
+        }
+    } else {
+        const int kz = x >= 50.0 ? 8 : (x >= 35.0 ? 10 : 11);
+        for (int l = 0; l < 2; ++l) {
+            const double mu = 4.0 * (l + v0) * (l + v0);
+            double p = 1.0, tp = 1.0;
+            for (int k = 1; k <= kz; ++k) {
+                tp *= (-0.78125e-2) * (mu - sq(4.0 * k - 3.0)) * (mu - sq(4.0 * k - 1.0)) /
+                      (k * (2.0 * k - 1.0) * x2);
+                p += tp;
+            }
+            double q = 1.0, tq = 1.0;
+            for (int k = 1; k <= kz; ++k) {
+                tq *= (-0.78125e-2) * (mu - sq(4.0 * k - 1.0)) * (mu - sq(4.0 * k + 1.0)) /
+                      (k * (2.0 * k + 1.0) * x2);
+                q += tq;
+            }
+            q *= 0.125 * (mu - 1.0) / x;
+            const double chi = x - (0.5 * (l + v0) + 0.25) * M_PI;
+            const double amp = sqrt(M_2_PI / x);
+            const double c = cos(chi), sn = sin(chi);
+            if (l == 0) {
+                bj0 = amp * (p * c - q * sn);
+                by0 = amp * (p * sn + q * c);
+            } else {
+                bj1 = amp * (p * c - q * sn);
+                by1 = amp * (p * sn + q * c);
+            }
+        }
+    }
+    jv[0] = bj0;
+    jv[1] = bj1;
+    djv[0] = v0 * jv[0] / x - jv[1];
+    djv[1] = -(1.0 + v0) * jv[1] / x + jv[0];
+    if (n >= 2 && n <= (int)(0.9 * x)) {
+        double f0 = bj0, f1 = bj1;
+        for (int k = 2; k <= n; ++k) {
+            const double f = 2.0 * (k + v0 - 1.0) * f1 / x - f0;
+            jv[k] = f;
+            f0 = f1;
+            f1 = f;
+        }
+    } else if (n >= 2) {
+        int m = msta1(x, 200);
+        if (m < n) {
+            for (int k = m + 1; k <= n; ++k) jv[k] = yv[k] = djv[k] = dyv[k] = 0.0;
+            n = m;
+        } else {
+            m = msta2(x, n, 15);
+        }
+        double f2 = 0.0, f1 = 1.0e-100, f = 0.0;
+        for (int k = m; k >= 0; --k) {
+            f = 2.0 * (v0 + k + 1.0) * f1 / x - f2;
+            if (k <= n) jv[k] = f;
+            f2 = f1;
+            f1 = f;
+        }
+        const double cs = fabs(bj0) > fabs(bj1) ? bj0 / f : bj1 / f2;
+        for (int k = 0; k <= n; ++k) jv[k] *= cs;
+    }
+    for (int k = 2; k <= n; ++k) djv[k] = -(k + v0) * jv[k] / x + jv[k - 1];
+    if (x <= 12.0) {
+        if (v0 != 0.0) {
+            double bu[2];
+            for (int l = 0; l < 2; ++l) {
+                const double vl = v0 + l;
+                double s = 1.0, t = 1.0;
+                for (int k = 1; k <= 40; ++k) {
+                    t *= -0.25 * x2 / (k * (k - vl));
+                    s += t;
+                    if (fabs(t) < fabs(s) * eps) break;
+                }
+                bu[l] = s * (pow(2.0 / x, vl) / gamma_fn(1.0 - vl));
+            }
+            const double pv0 = M_PI * v0, pv1 = M_PI * (1.0 + v0);
+            by0 = (bj0 * cos(pv0) - bu[0]) / sin(pv0);
+            by1 = (bj1 * cos(pv1) - bu[1]) / sin(pv1);
+        } else {
+            const double ec = log(0.5 * x) + euler;
+            double cs0 = 0.0, w0 = 0.0, r0 = 1.0;
+            for (int k = 1; k <= 30; ++k) {
+                w0 += 1.0 / k;
+                r0 *= -0.25 * x2 / (k * k);
+                cs0 += r0 * w0;
+            }
+            by0 = M_2_PI * (ec * bj0 - cs0);
+            double cs1 = 1.0, w1 = 0.0, r1 = 1.0;
+            for (int k = 1; k <= 30; ++k) {
+                w1 += 1.0 / k;
+                r1 *= -0.25 * x2 / (k * (k + 1));
+                cs1 += r1 * (2.0 * w1 + 1.0 / (k + 1.0));
+            }
+            by1 = M_2_PI * (ec * bj1 - 1.0 / x - 0.25 * x * cs1);
+        }
+    }
+    yv[0] = by0;
+    yv[1] = by1;
+    double g0 = by0, g1 = by1;
+    for (int k = 2; k <= n; ++k) {
+        const double g = 2.0 * (v0 + k - 1.0) * g1 / x - g0;
+        yv[k] = g;
+        g0 = g1;
+        g1 = g;
+    }
+    dyv[0] = v0 * yv[0] / x - yv[1];
+    for (int k = 1; k <= n; ++k) dyv[k] = -(k + v0) * yv[k] / x + yv[k - 1];
+}
+
+// Spherical j_l, y_l (real argument) for l = 0..nl (bessjyv_sph,
+// bessjy.cpp:709-728); arrays hold nl+2 entries.
+__device__ inline void sph_jy_real(int nl, double x, bool quirks, double* j, double* y, double* jp,
+                                   double* yp) {
+    for (int k = 0; k <= nl + 1; ++k) j[k] = y[k] = jp[k] = yp[k] = 0.0;
+    jyv(nl + 0.5, x, j, y, jp, yp);
+    const double pi = quirks ? 3.14159 : M_PI;
+    for (int k = 0; k <= nl; ++k) {
+        const double s = sqrt(pi / (x * 2.0));
+        j[k] = j[k] * s;
+        y[k] = y[k] * s;
+        jp[k] = -1.0 / (x * 2.0) * j[k] + jp[k] * s;
+        yp[k] = -1.0 / (x * 2.0) * y[k] + yp[k] * s;
+    }
+}
+
 // Spherical j_l, y_l, j'_l, y'_l for l = 0..nl (cbessjyva_sph,
 // cbessjy.cpp:718-737): scale by sqrt(pi/(2z)); the reference uses
 // pi = 3.14159 there.

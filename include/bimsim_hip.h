@@ -103,8 +103,10 @@ typedef struct {
 
 /* ---------------------------------------------------------------- context */
 
-/* hip_stream: a hipStream_t to run on, or NULL for a stream of the context's
- * own.  Replaces the implicit device/stream of main.cpp:46 (cudaSetDevice). */
+/* hip_stream: a hipStream_t to run on, or NULL for the device's default
+ * stream (where the reference launches everything).  All work of the context
+ * is ordered on that stream.  Replaces the implicit device/stream of
+ * main.cpp:46 (cudaSetDevice). */
 int bimsim_ctx_create(int device, void* hip_stream, bimsim_ctx** out);
 int bimsim_ctx_destroy(bimsim_ctx* ctx);
 int bimsim_sync(bimsim_ctx* ctx);
@@ -185,6 +187,56 @@ int bimsim_detector(bimsim_ctx* ctx, int prec, int quirks, void* U, void* Uf, in
                     int vR, double du, double dv, double NAin, double NAout,
                     double lambda, int padding, int Nd, int ss, void* D,
                     void* Di, void* Abs, int write_filtered);
+
+/* -------------------------------------------------- look-up-table paths */
+
+/* Radial j_l table of the incident field (nearfieldStruct::calcBesselLut,
+ * reference nearfield.cpp:222-283): lut[id*(m+1) + l] = j_l(kmag d) for
+ * d = dmin + id (dmax - dmin)/(dR - 1), NaN/inf replaced by 1 (l = 0 at
+ * d = 0) or 0.  Computed in FP64 on the GPU; stored as double, or float for
+ * BIMSIM_FP32_FAST. */
+int bimsim_uf_bessel_lut(bimsim_ctx* ctx, int m, double kmag, double dmin, double dmax,
+                         int dR, int prec, void* lut);
+
+/* Incident field from that table (gpuScalarUfLut, reference
+ * nfScalarUfLut.cu:11-123).  bimsim_nf_uf with BIMSIM_UF_LUT builds the
+ * table itself (dR = 2 max(uR, vR), nfScalarUfLut.cu:130-134). */
+int bimsim_nf_uf_lut(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int vR,
+                     const double focus[3], const double kdir[3], double amp,
+                     double cos_alpha, double cos_beta, int m, const void* lut, int dR,
+                     double dmin, double dmax, int prec, void* Uf);
+
+/* Table sizes of a sphere (sphere::calcUp, reference sphere.cpp:222-241):
+ * d_min/d_max from the slice, aR = 1 + 2R sqrt(a - d_min)/|diagonal|,
+ * dR = 2R (or dR_override > 0).  R = max(uR, vR).  Host only. */
+int bimsim_sphere_lut_dims(const bimsim_quad* pos, const bimsim_sphere* s, int R,
+                           int dR_override, int* aR, int* dR, double* dmin, double* dmax);
+
+/* Scattered / internal field tables of one sphere (sphere::calcHankelLut,
+ * calcBesselLut and the gpuScalarUsp / gpuScalarUip kernels, reference
+ * sphere.cpp:99-220, sphere.cu:4-149): Usp[itheta*dR + ir] and
+ * Uip[itheta*aR + ia] with theta = itheta * PI/(thetaR - 1).  Radial Bessel
+ * functions in FP64 on the GPU; tables complex128 (FP64 precisions) or
+ * complex64 (BIMSIM_FP32_FAST).  Uip may be NULL. */
+int bimsim_sphere_lut(bimsim_ctx* ctx, const bimsim_sphere* s, const void* A, const void* B,
+                      double kmag, double dmin, double dmax, int dR, int aR, int thetaR,
+                      int prec, void* Usp, void* Uip);
+
+/* U += scattered field of one sphere from its tables under nk waves
+ * (gpuScalarUpLut, reference nfScalarUpLut.cu:11-97; texture fetches as
+ * bilinear interpolation with exact weights and clamped edges). */
+int bimsim_nf_us_lut(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int vR,
+                     const double focus[3], double kmag, const bimsim_sphere* s,
+                     const void* Usp, int dR, const void* Uip, int aR, int thetaR,
+                     double dmin, double dmax, const double* waves, int nk, double subA,
+                     int prec, void* U);
+
+/* Rotation of the saved fields by deg about the slice centre
+ * (nearfieldStruct::Interpolate, reference nfInterpolate.cu:13-115): Uf with
+ * the float centre of gpuInterpolateUf, U with the integer centre of
+ * gpuInterpolateU; bilinear, zero outside.  Either pair may be NULL. */
+int bimsim_nf_interpolate(bimsim_ctx* ctx, int prec, double deg, int uR, int vR,
+                          const void* U_src, const void* Uf_src, void* U, void* Uf);
 
 #ifdef __cplusplus
 }

@@ -562,18 +562,23 @@ void nf_us_lut(const bimsim_quad& qd, int uR, int vR, const double* fd, T kmag,
             T u = (T)iu / (T)uR, v = (T)iv / (T)vR;
             v3<T> r = q.at(u, v) - ps;
             T d = r.len();
-            T di = ((d - lo) / (dmax - lo)) * (T)(dR - 1);
-            T ai = ((d - dmin) / (a - dmin)) * (T)(aR - 1);
+            // the reference declares these texture coordinates float
+            const float di = (float)(((d - lo) / (dmax - lo)) * (T)(dR - 1));
+            const float ai = (float)(((d - dmin) / (a - dmin)) * (T)(aR - 1));
             rc<T> sum;
             for (int w = 0; w < nk; ++w) {
                 r = r.norm();
                 T ct = K[w].dot(r);
                 if (ct < -1) ct = -1;
                 if (ct > 1) ct = 1;
-                T ti = (std::acos(ct) / (T)kPIf) * (T)(thetaR - 1);
+                // at the sphere centre r-hat is 0/0: the reference's texture
+                // coordinate is NaN there; every row of the Uip table's first
+                // column is A_0, so any angle gives the defined value
+                if (ct != ct) ct = 1;
+                const float ti = (float)((std::acos(ct) / (T)kPIf) * (T)(thetaR - 1));
                 rc<T> phase = cexp_(rc<T>(0, kmag * K[w].dot(c)));
-                rc<T> val = d < a ? bilinear_clamp(Uip, aR, thetaR, ai, ti)
-                                  : bilinear_clamp(Usp, dR, thetaR, di, ti);
+                rc<T> val = d < a ? bilinear_clamp(Uip, aR, thetaR, (T)ai, (T)ti)
+                                  : bilinear_clamp(Usp, dR, thetaR, (T)di, (T)ti);
                 sum = sum + scl(phase, (T)(1.0f / nk) * subA) * val;
             }
             const int i = iv * uR + iu;

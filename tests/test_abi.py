@@ -88,3 +88,15 @@ def test_context_without_device(pkg):
     h = ctypes.c_void_p()
     rc = pkg.lib().bimsim_ctx_create(0, None, ctypes.byref(h))
     assert rc == pkg.E_NODEVICE and not h.value
+
+
+def test_sphere_lut_dims_match_oracle(pkg, orc):
+    import oracle
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        size, z = float(rng.uniform(5, 60)), float(rng.uniform(-3, 3))
+        q, qo = pkg.Quad.xy(size, z), oracle.Quad.xy(size, z)
+        s = dict(p=tuple(rng.uniform(-10, 10, 3)), a=float(rng.uniform(0.5, 8)),
+                 n=complex(1.4, 0.01), Nl=10, coeff_off=0)
+        R = int(rng.integers(16, 300))
+        assert pkg.sphere_lut_dims(q, s, R) == orc.sphere_lut_dims(qo, s, R)
