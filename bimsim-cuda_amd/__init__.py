@@ -19,6 +19,8 @@ OK, E_INVALID, E_HIP, E_NOMEM, E_UNSUPPORTED, E_NODEVICE = 0, -1, -2, -3, -4, -5
 FP64_PARITY, FP64, FP32_FAST = 0, 1, 2
 QUIRKS_OFF, QUIRKS_REF = 0, 1
 UF_SERIES, UF_PLANE, UF_LUT = 0, 1, 2
+SLICE_INTENSITY, SLICE_MAGNITUDE, SLICE_REAL, SLICE_IMAG = 0, 1, 2, 3
+RATIO_ABSORBANCE, RATIO_TRANSMITTANCE = 0, 1
 US_SERIES, US_LUT = 0, 1
 RNG_GLIBC, RNG_SPLITMIX64 = 0, 1
 PI_F = 3.1415901184082031   # float(3.14159f), reference dataTypes.h:17
@@ -29,6 +31,9 @@ EXPORTS = (
     "bimsim_mc_sample_na", "bimsim_mie_coeffs", "bimsim_nf_uf", "bimsim_nf_us",
     "bimsim_nf_sum_uf", "bimsim_detector", "bimsim_uf_bessel_lut", "bimsim_nf_uf_lut",
     "bimsim_sphere_lut_dims", "bimsim_sphere_lut", "bimsim_nf_us_lut", "bimsim_nf_interpolate",
+    "bimsim_sphere_radial_lut", "bimsim_field_fft", "bimsim_field_bandpass",
+    "bimsim_field_to_scalar", "bimsim_field_scale", "bimsim_field_crop",
+    "bimsim_detector_resample", "bimsim_slice_ratio",
 )
 
 
@@ -123,6 +128,20 @@ def lib():
                                        c_int, c_double, c_int, c_void_p]
         L.bimsim_nf_interpolate.argtypes = [c_void_p, c_int, c_double, c_int, c_int, c_void_p,
                                             c_void_p, c_void_p, c_void_p]
+        L.bimsim_sphere_radial_lut.argtypes = [c_void_p, c_void_p, c_double, c_double, c_double,
+                                               c_int, c_int, c_void_p, c_void_p]
+        L.bimsim_field_fft.argtypes = [c_void_p, c_int, c_void_p, c_int, c_int, c_int]
+        L.bimsim_field_bandpass.argtypes = [c_void_p, c_int, c_int, c_void_p, c_int, c_int,
+                                            c_double, c_double, c_double, c_double, c_double]
+        L.bimsim_field_to_scalar.argtypes = [c_void_p, c_int, c_int, c_void_p, ctypes.c_longlong,
+                                             c_void_p]
+        L.bimsim_field_scale.argtypes = [c_void_p, c_int, c_void_p, ctypes.c_longlong, c_double]
+        L.bimsim_field_crop.argtypes = [c_void_p, c_int, c_void_p, c_int, c_int, c_int, c_int,
+                                        c_int, c_int, c_void_p]
+        L.bimsim_detector_resample.argtypes = [c_void_p, c_int, c_void_p, c_int, c_int, c_int,
+                                               c_double, c_int, c_void_p]
+        L.bimsim_slice_ratio.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p,
+                                         ctypes.c_longlong, c_void_p]
         _lib = L
     return _lib
 
@@ -254,6 +273,37 @@ class Context:
     def nf_interpolate(self, deg, uR, vR, U_src, Uf_src, U, Uf, prec=FP64_PARITY):
         self._check(self.L.bimsim_nf_interpolate(self.h, prec, deg, uR, vR, _ptr(U_src),
                                                  _ptr(Uf_src), _ptr(U), _ptr(Uf)))
+
+    def sphere_radial_lut(self, sphere, kmag, dmin, dmax, dR, aR, h, j):
+        S = sphere_array([sphere])
+        self._check(self.L.bimsim_sphere_radial_lut(self.h, S, kmag, dmin, dmax, dR, aR, _ptr(h),
+                                                    _ptr(j)))
+
+    def field_fft(self, F, uR, vR, inverse=False, prec=FP64_PARITY):
+        self._check(self.L.bimsim_field_fft(self.h, prec, _ptr(F), uR, vR, 1 if inverse else 0))
+
+    def field_bandpass(self, F, uR, vR, du, dv, NAin, NAout, lam, prec=FP64_PARITY,
+                       quirks=QUIRKS_REF):
+        self._check(self.L.bimsim_field_bandpass(self.h, prec, quirks, _ptr(F), uR, vR, du, dv,
+                                                 NAin, NAout, lam))
+
+    def field_to_scalar(self, F, op, S, prec=FP64_PARITY):
+        self._check(self.L.bimsim_field_to_scalar(self.h, prec, op, _ptr(F), F.numel(), _ptr(S)))
+
+    def field_scale(self, F, v, prec=FP64_PARITY):
+        self._check(self.L.bimsim_field_scale(self.h, prec, _ptr(F), F.numel(), v))
+
+    def field_crop(self, src, uR, vR, u, v, su, sv, dst, prec=FP64_PARITY):
+        self._check(self.L.bimsim_field_crop(self.h, prec, _ptr(src), uR, vR, u, v, su, sv,
+                                             _ptr(dst)))
+
+    def detector_resample(self, F, Du, Dv, ss, scale, accumulate, D, prec=FP64_PARITY):
+        self._check(self.L.bimsim_detector_resample(self.h, prec, _ptr(F), Du, Dv, ss, scale,
+                                                    1 if accumulate else 0, _ptr(D)))
+
+    def slice_ratio(self, op, D, Di, out, prec=FP64_PARITY):
+        self._check(self.L.bimsim_slice_ratio(self.h, prec, op, _ptr(D), _ptr(Di), D.numel(),
+                                              _ptr(out)))
 
     def detector(self, U, Uf, uR, vR, du, dv, NAin, NAout, lam, padding, Nd, ss, D, Di,
                  Abs=None, prec=FP64_PARITY, quirks=QUIRKS_REF, write_filtered=True):

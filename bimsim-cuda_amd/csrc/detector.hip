@@ -648,6 +648,34 @@ int detector_impl(bimsim_ctx* ctx, void* Uin, void* Ufin, int uR, int vR, double
     return BIMSIM_OK;
 }
 
+// fieldslice::toAngularSpectrum / fromAngularSpectrum (fieldslice.cpp:
+// 25-94): in-place forward (e^{-i}) or inverse (e^{+i}) 2D FFT, the inverse
+// followed by the 1/(R0 R1) of ScaleField.
+template <class R>
+int field_fft_impl(bimsim_ctx* ctx, void* Fin, int uR, int vR, int inverse) {
+    using C = typename C2<R>::T;
+    C* F = (C*)Fin;
+    const double *twd_u, *twd_v;
+    const float *twf_u, *twf_v;
+    BIMSIM_TRY(twiddles_get(ctx, uR, &twd_u, &twf_u));
+    BIMSIM_TRY(twiddles_get(ctx, vR, &twd_v, &twf_v));
+    const bool dbl = std::is_same<R, double>::value;
+    const C* tw_u = dbl ? (const C*)twd_u : (const C*)twf_u;
+    const C* tw_v = dbl ? (const C*)twd_v : (const C*)twf_v;
+    PostOp none{};
+    if (!inverse) {
+        BIMSIM_TRY(run_fft_pass<R>(ctx, F, vR, uR, uR, 1, tw_u, 0, none));
+        BIMSIM_TRY(run_fft_pass<R>(ctx, F, uR, vR, 1, uR, tw_v, 0, none));
+    } else {
+        PostOp sc{};
+        sc.kind = 2;
+        sc.scale = 1.0 / ((unsigned)uR * (unsigned)vR);
+        BIMSIM_TRY(run_fft_pass<R>(ctx, F, uR, vR, 1, uR, tw_v, 1, none));
+        BIMSIM_TRY(run_fft_pass<R>(ctx, F, vR, uR, uR, 1, tw_u, 1, sc));
+    }
+    return BIMSIM_OK;
+}
+
 }  // namespace
 }  // namespace bimsim
 
@@ -676,4 +704,13 @@ extern "C" int bimsim_detector(bimsim_ctx* ctx, int prec, int quirks, void* U, v
     if (rc != BIMSIM_OK) return rc;
     timer_end(ctx, tmr);
     return BIMSIM_OK;
+}
+
+extern "C" int bimsim_field_fft(bimsim_ctx* ctx, int prec, void* F, int uR, int vR, int inverse) {
+    if (!ctx) return BIMSIM_E_INVALID;
+    BIMSIM_CHECK_ARG(ctx, F && uR > 0 && vR > 0, "bad FFT arguments");
+    BIMSIM_CHECK_ARG(ctx, prec >= BIMSIM_FP64_PARITY && prec <= BIMSIM_FP32_FAST, "unknown precision");
+    BIMSIM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    if (prec == BIMSIM_FP32_FAST) return field_fft_impl<float>(ctx, F, uR, vR, inverse);
+    return field_fft_impl<double>(ctx, F, uR, vR, inverse);
 }

@@ -375,6 +375,32 @@ int bimsim_sphere_lut(bimsim_ctx* ctx, const bimsim_sphere* s, const void* A, co
     return BIMSIM_OK;
 }
 
+int bimsim_sphere_radial_lut(bimsim_ctx* ctx, const bimsim_sphere* s, double kmag, double dmin,
+                             double dmax, int dR, int aR, void* h, void* j) {
+    if (!ctx) return BIMSIM_E_INVALID;
+    BIMSIM_CHECK_ARG(ctx, s && (h || j), "null pointer argument");
+    BIMSIM_CHECK_ARG(ctx, s->Nl >= 1 && s->Nl <= 4096 && (!h || dR >= 2) && (!j || aR >= 2),
+                     "bad radial table sizes");
+    BIMSIM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    const int nl = s->Nl;
+    const int wstride = nl + 2;
+    const int nmax = std::max(h ? dR : 0, j ? aR : 0);
+    void* work = nullptr;
+    BIMSIM_TRY(scratch_get(ctx, kSlotMisc, sizeof(zj::zc) * 4 * (size_t)wstride * nmax, &work));
+    if (h) {
+        const double lo = std::max(s->a, dmin);
+        hipLaunchKernelGGL(hankel_lut_kernel, dim3((dR + 63) / 64), dim3(64), 0, ctx->stream, nl,
+                           kmag, lo, (dmax - lo) / (dR - 1), dR, (double*)work, wstride, (double2*)h);
+        BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    }
+    if (j) {
+        hipLaunchKernelGGL(bessel_lut_kernel, dim3((aR + 63) / 64), dim3(64), 0, ctx->stream, nl,
+                           kmag, s->n[0], s->n[1], s->a, aR, (zj::zc*)work, wstride, (double2*)j);
+        BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    }
+    return BIMSIM_OK;
+}
+
 int bimsim_nf_us_lut(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int vR,
                      const double focus[3], double kmag, const bimsim_sphere* s, const void* Usp,
                      int dR, const void* Uip, int aR, int thetaR, double dmin, double dmax,

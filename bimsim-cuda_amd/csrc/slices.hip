@@ -1,0 +1,237 @@
+// Element-wise operations on field slices (complex, row-major, u fastest) and
+// scalar slices (real): the fieldslice / scalarslice / microscopeStruct
+// building blocks that the host API exposes one by one.
+//   field -> scalar       fieldslice::Mag / Intensity / Real / Imag
+//                         (fieldslice.cu:6-27, 110-138, 156-230)
+//   scale                 fieldslice::ScaleField (fieldslice.cu:141-153, 243-254)
+//   crop                  fieldslice::crop (fieldslice.cu:319-362)
+//   detector resample     fieldslice::IntegrateAndResample / ResampleAndSave
+//                         (fieldslice.cu:30-108, 205-229)
+//   absorbance etc.       microscopeStruct::getAbsorbance / getTransmittance
+//                         (microscope.cu:722-791)
+//   objective band-pass   bandpass kernel (microscope.cu:495-526)
+// Every kernel is a grid-stride-free one-thread-per-element map; these are
+// HBM-bound and small next to the near-field series.  Built with
+// -ffp-contract=off so that the FP64 forms round like the reference's ops.
+#include "internal.h"
+
+namespace bimsim {
+namespace {
+
+constexpr int kThreads = 256;
+
+template <class R> struct Cplx;
+template <> struct Cplx<float> { using T = float2; };
+template <> struct Cplx<double> { using T = double2; };
+
+inline dim3 grid1(size_t n) { return dim3((unsigned)((n + kThreads - 1) / kThreads)); }
+
+// rts::complex::abs: sqrt(r*r + i*i)
+template <class R>
+__device__ inline R cabs2(typename Cplx<R>::T z) {
+    return z.x * z.x + z.y * z.y;
+}
+
+template <class R>
+__global__ void field_to_scalar_kernel(const typename Cplx<R>::T* __restrict__ F, size_t n, int op,
+                                       R* __restrict__ S) {
+    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const typename Cplx<R>::T z = F[i];
+    R v;
+    switch (op) {
+        case BIMSIM_SLICE_INTENSITY: {
+            const R m = sqrt(cabs2<R>(z));
+            v = m * m;
+            break;
+        }
+        case BIMSIM_SLICE_MAGNITUDE: {
+            const R m = sqrt(cabs2<R>(z));
+            v = sqrt(m * m);
+            break;
+        }
+        case BIMSIM_SLICE_REAL: v = z.x; break;
+        default: v = z.y; break;
+    }
+    S[i] = v;
+}
+
+template <class R>
+__global__ void field_scale_kernel(typename Cplx<R>::T* __restrict__ F, size_t n, R s) {
+    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    typename Cplx<R>::T z = F[i];
+    z.x *= s;
+    z.y *= s;
+    F[i] = z;
+}
+
+template <class C>
+__global__ void crop_kernel(const C* __restrict__ src, int uR, int u, int v, int su, int sv,
+                            C* __restrict__ dst) {
+    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= (size_t)su * sv) return;
+    const int iv = (int)(i / su), iu = (int)(i - (size_t)iv * su);
+    dst[i] = src[(size_t)(v + iv) * uR + (u + iu)];
+}
+
+// resample_intensity (accumulate = 1: D += scale*I/(ss*ss)) and
+// resample_intensity_without_integrating (accumulate = 0: D = I/(ss*ss)).
+// The field row stride is Du*ss (the detector-resolution field).
+template <class R>
+__global__ void resample_kernel(const typename Cplx<R>::T* __restrict__ F, int Du, int Dv, int ss,
+                                R scale, int accumulate, R* __restrict__ D) {
+    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= (size_t)Du * Dv) return;
+    const int iv = (int)(i / Du), iu = (int)(i - (size_t)iv * Du);
+    R I = 0;
+    for (int ix = 0; ix < ss; ++ix)
+        for (int iy = 0; iy < ss; ++iy) {
+            const size_t fi = (size_t)(iv * ss + iy) * ss * Du + (size_t)iu * ss + ix;
+            const R m = sqrt(cabs2<R>(F[fi]));
+            I += m * m;
+        }
+    const R den = (R)(ss * ss);
+    if (accumulate) D[i] += scale * I / den;
+    else D[i] = I / den;
+}
+
+template <class R>
+__global__ void ratio_kernel(const R* __restrict__ D, const R* __restrict__ Di, size_t n, int op,
+                             R* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const R t = D[i] / Di[i];
+    out[i] = op == BIMSIM_RATIO_ABSORBANCE ? -log10(t) : t;
+}
+
+// bandpass (microscope.cu:495-526): zero the samples whose spatial frequency
+// lies outside [NAin/lambda, NAout/lambda]; frequencies in double.
+template <class R>
+__global__ void bandpass_kernel(typename Cplx<R>::T* __restrict__ F, int uR, int vR, double du,
+                                double dv, double fmin, double fmax, int quirks) {
+    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= (size_t)uR * vR) return;
+    const int iv = (int)(i / uR), iu = (int)(i - (size_t)iv * uR);
+    double u, v;
+    if (iu <= uR / 2) u = (double)iu * du;
+    else u = -(double)(quirks ? uR - 1 - iu : uR - iu) * du;
+    if (iv <= vR / 2) v = (double)iv * dv;
+    else v = -(double)(quirks ? vR - 1 - iv : vR - iv) * dv;
+    const double fmag = sqrt(u * u + v * v);
+    if (fmag < fmin || fmag > fmax) {
+        typename Cplx<R>::T z;
+        z.x = 0;
+        z.y = 0;
+        F[i] = z;
+    }
+}
+
+bool prec_ok(int prec) { return prec >= BIMSIM_FP64_PARITY && prec <= BIMSIM_FP32_FAST; }
+
+}  // namespace
+}  // namespace bimsim
+
+using namespace bimsim;
+
+extern "C" {
+
+int bimsim_field_to_scalar(bimsim_ctx* ctx, int prec, int op, const void* F, long long n, void* S) {
+    if (!ctx) return BIMSIM_E_INVALID;
+    BIMSIM_CHECK_ARG(ctx, F && S && n > 0 && prec_ok(prec), "bad field-to-scalar arguments");
+    BIMSIM_CHECK_ARG(ctx, op >= BIMSIM_SLICE_INTENSITY && op <= BIMSIM_SLICE_IMAG, "unknown op");
+    BIMSIM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    if (prec == BIMSIM_FP32_FAST)
+        hipLaunchKernelGGL(field_to_scalar_kernel<float>, grid1(n), dim3(kThreads), 0, ctx->stream,
+                           (const float2*)F, (size_t)n, op, (float*)S);
+    else
+        hipLaunchKernelGGL(field_to_scalar_kernel<double>, grid1(n), dim3(kThreads), 0,
+                           ctx->stream, (const double2*)F, (size_t)n, op, (double*)S);
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return BIMSIM_OK;
+}
+
+int bimsim_field_scale(bimsim_ctx* ctx, int prec, void* F, long long n, double v) {
+    if (!ctx) return BIMSIM_E_INVALID;
+    BIMSIM_CHECK_ARG(ctx, F && n > 0 && prec_ok(prec), "bad field-scale arguments");
+    BIMSIM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    if (prec == BIMSIM_FP32_FAST)
+        hipLaunchKernelGGL(field_scale_kernel<float>, grid1(n), dim3(kThreads), 0, ctx->stream,
+                           (float2*)F, (size_t)n, (float)v);
+    else
+        hipLaunchKernelGGL(field_scale_kernel<double>, grid1(n), dim3(kThreads), 0, ctx->stream,
+                           (double2*)F, (size_t)n, v);
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return BIMSIM_OK;
+}
+
+int bimsim_field_crop(bimsim_ctx* ctx, int prec, const void* src, int uR, int vR, int u, int v,
+                      int su, int sv, void* dst) {
+    if (!ctx) return BIMSIM_E_INVALID;
+    BIMSIM_CHECK_ARG(ctx, src && dst && prec_ok(prec), "bad crop arguments");
+    BIMSIM_CHECK_ARG(ctx, su > 0 && sv > 0 && u >= 0 && v >= 0 && u + su <= uR && v + sv <= vR,
+                     "crop window outside the field");
+    BIMSIM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    const size_t n = (size_t)su * sv;
+    if (prec == BIMSIM_FP32_FAST)
+        hipLaunchKernelGGL(crop_kernel<float2>, grid1(n), dim3(kThreads), 0, ctx->stream,
+                           (const float2*)src, uR, u, v, su, sv, (float2*)dst);
+    else
+        hipLaunchKernelGGL(crop_kernel<double2>, grid1(n), dim3(kThreads), 0, ctx->stream,
+                           (const double2*)src, uR, u, v, su, sv, (double2*)dst);
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return BIMSIM_OK;
+}
+
+int bimsim_detector_resample(bimsim_ctx* ctx, int prec, const void* F, int Du, int Dv, int ss,
+                             double scale, int accumulate, void* D) {
+    if (!ctx) return BIMSIM_E_INVALID;
+    BIMSIM_CHECK_ARG(ctx, F && D && Du > 0 && Dv > 0 && ss > 0 && prec_ok(prec),
+                     "bad detector-resample arguments");
+    BIMSIM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    const size_t n = (size_t)Du * Dv;
+    if (prec == BIMSIM_FP32_FAST)
+        hipLaunchKernelGGL(resample_kernel<float>, grid1(n), dim3(kThreads), 0, ctx->stream,
+                           (const float2*)F, Du, Dv, ss, (float)scale, accumulate, (float*)D);
+    else
+        hipLaunchKernelGGL(resample_kernel<double>, grid1(n), dim3(kThreads), 0, ctx->stream,
+                           (const double2*)F, Du, Dv, ss, scale, accumulate, (double*)D);
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return BIMSIM_OK;
+}
+
+int bimsim_slice_ratio(bimsim_ctx* ctx, int prec, int op, const void* D, const void* Di,
+                       long long n, void* out) {
+    if (!ctx) return BIMSIM_E_INVALID;
+    BIMSIM_CHECK_ARG(ctx, D && Di && out && n > 0 && prec_ok(prec), "bad ratio arguments");
+    BIMSIM_CHECK_ARG(ctx, op == BIMSIM_RATIO_ABSORBANCE || op == BIMSIM_RATIO_TRANSMITTANCE,
+                     "unknown op");
+    BIMSIM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    if (prec == BIMSIM_FP32_FAST)
+        hipLaunchKernelGGL(ratio_kernel<float>, grid1(n), dim3(kThreads), 0, ctx->stream,
+                           (const float*)D, (const float*)Di, (size_t)n, op, (float*)out);
+    else
+        hipLaunchKernelGGL(ratio_kernel<double>, grid1(n), dim3(kThreads), 0, ctx->stream,
+                           (const double*)D, (const double*)Di, (size_t)n, op, (double*)out);
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return BIMSIM_OK;
+}
+
+int bimsim_field_bandpass(bimsim_ctx* ctx, int prec, int quirks, void* F, int uR, int vR,
+                          double du, double dv, double NAin, double NAout, double lambda) {
+    if (!ctx) return BIMSIM_E_INVALID;
+    BIMSIM_CHECK_ARG(ctx, F && uR > 0 && vR > 0 && prec_ok(prec), "bad band-pass arguments");
+    BIMSIM_CHECK_ARG(ctx, lambda > 0.0, "lambda must be positive");
+    BIMSIM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    const size_t n = (size_t)uR * vR;
+    if (prec == BIMSIM_FP32_FAST)
+        hipLaunchKernelGGL(bandpass_kernel<float>, grid1(n), dim3(kThreads), 0, ctx->stream,
+                           (float2*)F, uR, vR, du, dv, NAin / lambda, NAout / lambda, quirks);
+    else
+        hipLaunchKernelGGL(bandpass_kernel<double>, grid1(n), dim3(kThreads), 0, ctx->stream,
+                           (double2*)F, uR, vR, du, dv, NAin / lambda, NAout / lambda, quirks);
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return BIMSIM_OK;
+}
+
+}  // extern "C"

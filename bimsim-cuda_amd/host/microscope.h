@@ -1,0 +1,89 @@
+// microscopeStruct (reference microscope.h:23-142, microscope.cu:495-857):
+// the near field of a padded slice, the objective band-pass, the crop to the
+// detector, the detector integration and the derived images.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "fieldslice.h"
+#include "nearfield.h"
+#include "scalarslice.h"
+
+struct sourcePoint {
+    bsPoint f;
+    ptype A;
+};
+
+struct sourcePointES {
+    bsPoint f;
+    ptype A;
+    ptype rotationAngle;
+};
+
+struct microscopeStruct {
+    int interpolate = -1;
+    int padding = 1;
+    int ss = 1;
+    float scale = 1.0f;
+
+    fieldslice Ufd;                 // cropped incident field at the detector
+    fieldslice Ud;                  // cropped total field at the detector
+    bool scalarSim = true;
+
+    scalarslice* D = nullptr;       // detector image of the total field
+    scalarslice* D_ps1 = nullptr;
+    scalarslice* Di_ps1 = nullptr;
+    ptype sumD = 0;
+    scalarslice* Di = nullptr;      // detector image of the incident field
+    ptype sumDi = 0;
+
+    ptype objective[2] = {0, 1};
+    bsRect pos;
+    float z_d = 0;
+
+    std::vector<sourcePoint> focalPoints;
+    std::vector<sourcePointES> focalPointsES;
+
+    // The band-pass, crop and integration of SimulateImaging run as one
+    // fused band-limited FFT pass (true) or as the reference's separate
+    // steps (false); both leave the same fields and images.
+    bool fused_detector = true;
+
+    microscopeStruct();
+    ~microscopeStruct();
+    microscopeStruct(const microscopeStruct&) = delete;
+    microscopeStruct& operator=(const microscopeStruct&) = delete;
+
+    nearfieldStruct nf;
+
+    void setZdPos(float d);
+
+    void init();
+    void destroy();
+
+    void SimulateScattering();
+    void SimulateInterpolation();
+    void SimulateImaging();
+    void Simulate();
+    void SimulateAndSaveTexture();
+    void SimulateWithInterpolation();
+    void saveToTexture();
+
+    void setNearfield();
+    void setRes(int x_res, int y_res, int pad, int supersampling);
+    void setPos(bsPoint pMin, bsPoint pMax, bsVector normal);
+
+    void applyBandpass();
+    void getFarField();
+    void integrateDetector();
+    void clearDetector();
+
+    scalarslice getAbsorbance();
+    // always 0, as in the reference (microscope.cu:749-762)
+    ptype getAbsorbanceSpectrum();
+    scalarslice getTransmittance();
+    scalarslice getIntensity();
+    scalarslice getIncidentFieldImage();
+    std::string toStr();
+};

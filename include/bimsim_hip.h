@@ -84,6 +84,20 @@ typedef enum {
     BIMSIM_RNG_SPLITMIX64 = 1   /* SplitMix64(seed) uniform doubles             */
 } bimsim_rng;
 
+/* Scalar views of a field slice (fieldslice::Intensity / Mag / Real / Imag). */
+typedef enum {
+    BIMSIM_SLICE_INTENSITY = 0,
+    BIMSIM_SLICE_MAGNITUDE = 1,
+    BIMSIM_SLICE_REAL = 2,
+    BIMSIM_SLICE_IMAG = 3
+} bimsim_slice_op;
+
+/* Detector ratios (microscopeStruct::getAbsorbance / getTransmittance). */
+typedef enum {
+    BIMSIM_RATIO_ABSORBANCE = 0,
+    BIMSIM_RATIO_TRANSMITTANCE = 1
+} bimsim_ratio_op;
+
 /* rts::quad<ptype,3> (reference rts/math/quad.h:30-33): corner A, edges X, Y.
  * Pixel (iu, iv) of a uR x vR slice sits at A + X*(iu/uR) + Y*(iv/vR). */
 typedef struct {
@@ -188,6 +202,48 @@ int bimsim_detector(bimsim_ctx* ctx, int prec, int quirks, void* U, void* Uf, in
                     double lambda, int padding, int Nd, int ss, void* D,
                     void* Di, void* Abs, int write_filtered);
 
+/* ------------------------------------------- slice building blocks
+ * The steps of microscopeStruct::SimulateImaging one at a time, for host
+ * code that calls them separately.  Fields are complex128 (FP64
+ * precisions) or complex64 (BIMSIM_FP32_FAST), scalar slices double or
+ * float, all row-major with u fastest. */
+
+/* In-place 2D FFT of a field (fieldslice::toAngularSpectrum, inverse = 0;
+ * fromAngularSpectrum incl. the 1/(uR vR) of ScaleField, inverse = 1;
+ * reference fieldslice.cpp:25-94).  uR, vR products of 2, 3 and 5. */
+int bimsim_field_fft(bimsim_ctx* ctx, int prec, void* F, int uR, int vR, int inverse);
+
+/* Objective band-pass mask on an angular spectrum (bandpass kernel,
+ * reference microscope.cu:495-526): zero where |f| < NAin/lambda or
+ * |f| > NAout/lambda. */
+int bimsim_field_bandpass(bimsim_ctx* ctx, int prec, int quirks, void* F, int uR, int vR,
+                          double du, double dv, double NAin, double NAout, double lambda);
+
+/* S[i] = op(F[i]) (fieldslice::Intensity / Mag / Real / Imag, reference
+ * fieldslice.cu:6-27,110-230); op = bimsim_slice_op. */
+int bimsim_field_to_scalar(bimsim_ctx* ctx, int prec, int op, const void* F, long long n,
+                           void* S);
+
+/* F *= v (fieldslice::ScaleField, reference fieldslice.cu:141-153). */
+int bimsim_field_scale(bimsim_ctx* ctx, int prec, void* F, long long n, double v);
+
+/* dst (su x sv) = src[v.., u..] (fieldslice::crop, reference
+ * fieldslice.cu:319-362). */
+int bimsim_field_crop(bimsim_ctx* ctx, int prec, const void* src, int uR, int vR, int u, int v,
+                      int su, int sv, void* dst);
+
+/* Detector pixels from a (Du ss) x (Dv ss) field: accumulate != 0 gives
+ * D += scale |F|^2 averaged over ss x ss (resample_intensity); 0 gives
+ * D = the average (resample_intensity_without_integrating); reference
+ * fieldslice.cu:30-108. */
+int bimsim_detector_resample(bimsim_ctx* ctx, int prec, const void* F, int Du, int Dv, int ss,
+                             double scale, int accumulate, void* D);
+
+/* out = -log10(D/Di) or D/Di (calc_absorbance / calc_transmittance,
+ * reference microscope.cu:722-791); op = bimsim_ratio_op. */
+int bimsim_slice_ratio(bimsim_ctx* ctx, int prec, int op, const void* D, const void* Di,
+                       long long n, void* out);
+
 /* -------------------------------------------------- look-up-table paths */
 
 /* Radial j_l table of the incident field (nearfieldStruct::calcBesselLut,
@@ -221,6 +277,14 @@ int bimsim_sphere_lut_dims(const bimsim_quad* pos, const bimsim_sphere* s, int R
 int bimsim_sphere_lut(bimsim_ctx* ctx, const bimsim_sphere* s, const void* A, const void* B,
                       double kmag, double dmin, double dmax, int dR, int aR, int thetaR,
                       int prec, void* Usp, void* Uip);
+
+/* The radial tables alone (sphere::calcHankelLut / calcBesselLut, reference
+ * sphere.cpp:99-203), device complex128, row-major with l fastest:
+ * h[ir*(Nl+1) + l] = (j_l + i y_l)(kmag r), r over [max(a, dmin), dmax] in
+ * dR samples; j[ia*(Nl+1) + l] = j_l(kmag n rho), rho over [0, a] in aR
+ * samples with rho = 0 -> delta_l0.  Either output may be NULL. */
+int bimsim_sphere_radial_lut(bimsim_ctx* ctx, const bimsim_sphere* s, double kmag, double dmin,
+                             double dmax, int dR, int aR, void* h, void* j);
 
 /* U += scattered field of one sphere from its tables under nk waves
  * (gpuScalarUpLut, reference nfScalarUpLut.cu:11-97; texture fetches as

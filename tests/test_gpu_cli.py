@@ -1,0 +1,135 @@
+"""The bimsim command line end to end on the GPU (C++ host API over the C
+ABI), against the oracle run through the same steps (waves -> Uf -> Mie ->
+Us (series or tables) -> sum -> detector).  The CLI writes float32 ENVI
+files, so the FP64 forms are compared at float32 output rounding (2e-6
+normwise); FP32 absorbance to 1e-4 where the incident image is bright."""
+import math
+import os
+import re
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "bimsim-cuda_amd", "bimsim")
+PI_F = 3.1415901184082031
+
+
+def f32(x):
+    return struct.unpack("f", struct.pack("f", x))[0]
+
+
+def normrel(a, b):
+    m = np.max(np.abs(b))
+    return float(np.max(np.abs(np.asarray(a) - b)) / (m if m > 0 else 1.0))
+
+
+def read_envi(path):
+    hdr = open(path + ".hdr").read()
+    get = lambda k: int(re.search(rf"^{k} = (\d+)", hdr, re.M).group(1))
+    s, l, b = get("samples"), get("lines"), get("bands")
+    assert get("data type") == 4 and "interleave = bsq" in hdr
+    wl = re.search(r"wavelength = \{\s*([^}]*)\}", hdr).group(1)
+    return (np.fromfile(path, dtype=np.float32).reshape(b, l, s),
+            [float(x) for x in wl.split(",")])
+
+
+CASE = dict(lam=2.5, size=30.0, res=32, pad=1, sphere=(1.0, -2.0, 0.0, 4.0), n=(1.4, 0.05),
+            cond=0.5, obj=0.5, nk=50, m=40)
+
+
+def run_cli(tmp_path, *extra):
+    c = CASE
+    args = [EXE, "--spheres", *c["sphere"], "--materials", *c["n"], "--lambda", c["lam"],
+            "--xy", c["size"], 0, "--resolution", c["res"], "--padding", c["pad"],
+            "--condenser", c["cond"], "--objective", c["obj"], "--samples", c["nk"],
+            "--field-order", c["m"], "--intensity", "I", "--incident", "I0",
+            "--absorbance", "A", "--transmittance", "T", *extra]
+    p = subprocess.run([str(a) for a in args], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    return {k: read_envi(str(tmp_path / k)) for k in ["I", "I0", "A", "T"]}
+
+
+def oracle_case(orc, lut):
+    import oracle
+    c = CASE
+    lam = f32(c["lam"])
+    kmag = 2.0 * PI_F / lam
+    N = c["res"] * (2 * c["pad"] + 1)
+    L = c["size"] * (2 * c["pad"] + 1)
+    q = oracle.Quad.xy(L, 0.0)
+    x, y, z, a = (f32(v) for v in c["sphere"])
+    n = complex(f32(c["n"][0]), f32(c["n"][1]))
+    Nl = orc.sphere_order(a, lam)
+    s = dict(p=(x, y, z), a=a, n=n, Nl=Nl, coeff_off=0)
+    cond, obj = f32(c["cond"]), f32(c["obj"])
+    ca, cb = 1.0, math.cos(math.asin(cond))
+    waves = orc.mc_sample_na(c["nk"], (0.0, 0.0, 1.0), 0.0, cond, 0, 0)
+    waves = np.asarray(waves, dtype=np.float64).reshape(-1, 3).astype(np.float32).astype(np.float64)
+    subA = (2.0 * PI_F) * ((1.0 - cb) - (1.0 - ca))
+    focus, kdir = (0.0, 0.0, 0.0), (0.0, 0.0, 1.0)
+    Uf = orc.nf_uf(q, N, N, focus, kdir, kmag, 1.0, ca, cb, c["m"])
+    A, B = orc.mie_coeffs(Nl, kmag, a, n)
+    if lut:
+        aR, dR, dmin, dmax = orc.sphere_lut_dims(q, s, N)
+        Usp = orc.lut_eval(Nl, B, orc.lut_hankel(Nl, kmag, a, dmin, dmax, dR), 1000)
+        Uip = orc.lut_eval(Nl, A, orc.lut_bessel(Nl, kmag, a, n, aR), 1000)
+        Us = orc.nf_us_lut(q, N, N, focus, kmag, s, Usp, Uip, dmin, dmax, waves, subA)
+    else:
+        Us = orc.nf_us(q, N, N, focus, kmag, [s], A, B, waves, subA)
+    U = orc.nf_sum_uf(q, N, N, [s], Uf, Us)
+    out = orc.detector(U, Uf, 1.0 / L, 1.0 / L, 0.0, obj, lam, c["pad"], c["res"], 1)
+    return out[2], out[3], out[4]
+
+
+@pytest.fixture(scope="module")
+def exe():
+    if not os.path.exists(EXE):
+        pytest.skip("bimsim not built")
+    return EXE
+
+
+@pytest.mark.parametrize("lut", [False, True])
+def test_cli_parity_matches_oracle(exe, orc, tmp_path, lut):
+    extra = ["--precision", "parity"] + ([] if lut else ["--recursive-us"])
+    out = run_cli(tmp_path, *extra)
+    D, Di, Ab = oracle_case(orc, lut)
+    assert normrel(out["I"][0][0], D) <= 2e-6
+    assert normrel(out["I0"][0][0], Di) <= 2e-6
+    assert normrel(out["T"][0][0], D / Di) <= 2e-6
+    bright = Di > 1e-3 * Di.max()
+    assert np.max(np.abs(out["A"][0][0][bright] - Ab[bright])) <= 2e-6
+    assert out["A"][1] == [f32(CASE["lam"])]
+
+
+@pytest.mark.parametrize("lut", [False, True])
+def test_cli_fp32_absorbance(exe, orc, tmp_path, lut):
+    out = run_cli(tmp_path, *([] if lut else ["--recursive-us"]))
+    D, Di, Ab = oracle_case(orc, lut)
+    bright = Di > 1e-3 * Di.max()
+    assert np.max(np.abs(out["A"][0][0][bright] - Ab[bright])) <= 1e-4
+
+
+def test_cli_fp64_and_granular_imaging_agree(exe, orc, tmp_path):
+    a = run_cli(tmp_path, "--precision", "fp64", "--recursive-us")
+    D, Di, _ = oracle_case(orc, False)
+    assert normrel(a["I"][0][0], D) <= 2e-6 and normrel(a["I0"][0][0], Di) <= 2e-6
+
+
+def test_cli_append_and_images(exe, tmp_path):
+    run_cli(tmp_path, "--recursive-us", "--near-field", "n.bmp", "--far-field", "f.bmp")
+    out = run_cli(tmp_path, "--recursive-us", "--append", "--lambda", 3.0)
+    D, wl = out["I"]
+    assert D.shape == (2, CASE["res"], CASE["res"])
+    assert wl == [f32(2.5), f32(3.0)]
+    for name, side in [("n.bmp", CASE["res"] * 3), ("f.bmp", CASE["res"])]:
+        data = (tmp_path / name).read_bytes()
+        assert data[:2] == b"BM"
+        w, h = struct.unpack("<ii", data[18:26])
+        assert (w, h) == (side, side)
+        assert len(data) == 54 + ((3 * side + 3) & ~3) * side

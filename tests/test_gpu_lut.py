@@ -168,3 +168,18 @@ def test_rotation_resample(pkg, ctx, orc, prec, N, deg):
     tol = 1e-12 if prec == 0 else 1e-5
     assert normrel(host(Uo).reshape(N, N), want_U) <= tol
     assert normrel(host(Ufo).reshape(N, N), want_Uf) <= tol
+
+
+def test_radial_tables_match_oracle(pkg, ctx, orc, W):
+    import bimsim_amd.pipeline as P
+    inst, s = _sphere_case(pkg, W)
+    q = P.quad_of(inst)
+    aR, dR, dmin, dmax = pkg.sphere_lut_dims(q, s, inst["N"])
+    nl = s["Nl"]
+    h = torch.empty(dR * (nl + 1), dtype=torch.complex128, device="cuda")
+    j = torch.empty(aR * (nl + 1), dtype=torch.complex128, device="cuda")
+    ctx.sphere_radial_lut(dict(s, coeff_off=0), inst["kmag"], dmin, dmax, dR, aR, h, j)
+    want_h = orc.lut_hankel(nl, inst["kmag"], s["a"], dmin, dmax, dR)
+    want_j = orc.lut_bessel(nl, inst["kmag"], s["a"], s["n"], aR)
+    assert normrel(host(h).reshape(dR, nl + 1), want_h) <= 1e-12
+    assert normrel(host(j).reshape(aR, nl + 1), want_j) <= 1e-12
