@@ -33,7 +33,8 @@ EXPORTS = (
     "bimsim_sphere_lut_dims", "bimsim_sphere_lut", "bimsim_nf_us_lut", "bimsim_nf_interpolate",
     "bimsim_sphere_radial_lut", "bimsim_field_fft", "bimsim_field_bandpass",
     "bimsim_field_to_scalar", "bimsim_field_scale", "bimsim_field_crop",
-    "bimsim_detector_resample", "bimsim_slice_ratio",
+    "bimsim_detector_resample", "bimsim_slice_ratio", "bimsim_batch_create",
+    "bimsim_batch_launch", "bimsim_batch_destroy", "bimsim_batch_run",
 )
 
 
@@ -61,6 +62,16 @@ class Sphere(ctypes.Structure):
     """bimsim_sphere."""
     _fields_ = [("p", c_double * 3), ("a", c_double), ("n", c_double * 2),
                 ("Nl", c_int), ("coeff_off", c_int)]
+
+
+class Instance(ctypes.Structure):
+    """bimsim_instance: one absorbance image of a batch."""
+    _fields_ = [("pos", Quad), ("N", c_int), ("m", c_int), ("focus", c_double * 3),
+                ("kdir", c_double * 3), ("lambda_", c_double), ("amp", c_double),
+                ("condenser", c_double * 2), ("objective", c_double * 2),
+                ("plane_wave", c_int), ("sphere_begin", c_int), ("ns", c_int),
+                ("wave_begin", c_int), ("nk", c_int), ("padding", c_int), ("Nd", c_int),
+                ("ss", c_int), ("image_off", ctypes.c_longlong)]
 
 
 def sphere_array(spheres):
@@ -142,6 +153,13 @@ def lib():
                                                c_double, c_int, c_void_p]
         L.bimsim_slice_ratio.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p,
                                          ctypes.c_longlong, c_void_p]
+        L.bimsim_batch_create.argtypes = [c_void_p, ctypes.POINTER(Instance), c_int, c_void_p,
+                                          c_int, c_void_p, c_int, c_int, c_void_p, c_void_p,
+                                          c_void_p, c_int, ctypes.POINTER(c_void_p)]
+        L.bimsim_batch_launch.argtypes = [c_void_p]
+        L.bimsim_batch_destroy.argtypes = [c_void_p]
+        L.bimsim_batch_run.argtypes = [c_void_p, ctypes.POINTER(Instance), c_int, c_void_p, c_int,
+                                       c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]
         _lib = L
     return _lib
 
@@ -305,8 +323,51 @@ class Context:
         self._check(self.L.bimsim_slice_ratio(self.h, prec, op, _ptr(D), _ptr(Di), D.numel(),
                                               _ptr(out)))
 
+    def batch(self, instances, spheres, waves, D, Di, Abs=None, prec=FP32_FAST,
+              quirks=QUIRKS_REF, use_graph=False):
+        """A bimsim_batch over `instances` (a ctypes Instance array); runs the
+        pass once.  Keep the returned Batch (and the tensors) alive while
+        launching it again."""
+        S = sphere_array(spheres)
+        h = c_void_p()
+        self._check(self.L.bimsim_batch_create(self.h, instances, len(instances), S, len(spheres),
+                                               _ptr(waves), prec, quirks, _ptr(D), _ptr(Di),
+                                               _ptr(Abs), 1 if use_graph else 0, ctypes.byref(h)))
+        return Batch(self, h, (instances, S, waves, D, Di, Abs))
+
+    def batch_run(self, instances, spheres, waves, D, Di, Abs=None, prec=FP32_FAST,
+                  quirks=QUIRKS_REF):
+        S = sphere_array(spheres)
+        self._check(self.L.bimsim_batch_run(self.h, instances, len(instances), S, len(spheres),
+                                            _ptr(waves), prec, quirks, _ptr(D), _ptr(Di),
+                                            _ptr(Abs)))
+
     def detector(self, U, Uf, uR, vR, du, dv, NAin, NAout, lam, padding, Nd, ss, D, Di,
                  Abs=None, prec=FP64_PARITY, quirks=QUIRKS_REF, write_filtered=True):
         self._check(self.L.bimsim_detector(self.h, prec, quirks, _ptr(U), _ptr(Uf), uR, vR, du,
                                            dv, NAin, NAout, lam, padding, Nd, ss, _ptr(D),
                                            _ptr(Di), _ptr(Abs), 1 if write_filtered else 0))
+
+
+class Batch:
+    """A created bimsim_batch; launch() replays the pass."""
+
+    def __init__(self, ctx, h, keep):
+        self.ctx = ctx
+        self.h = h
+        self._keep = keep
+
+    def launch(self):
+        self.ctx._check(self.ctx.L.bimsim_batch_launch(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.L.bimsim_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

@@ -1,0 +1,41 @@
+// Device tables and launchers of the batched pipeline (many instances of
+// slice, spheres, wavelength and waves in one pass).
+#pragma once
+
+#include "internal.h"
+
+namespace bimsim {
+
+// Incident field of one instance.
+struct UfInst {
+    QuadArg q;
+    d3 f, k;
+    double kmag, amp, ca, cb;
+    int m, uR, vR, plane;
+    long long off;   // element offset of the instance's field
+};
+
+// Uf + Us sum of one instance.
+struct SumInst {
+    QuadArg q;
+    int uR, vR, s_begin, ns;
+    long long off;
+};
+
+// Mie coefficients of n spheres, sphere i at wavenumber kmag[i]; writes
+// A/B at each sphere's coeff_off.  work: n * 8 * (lmax + 2) complex doubles.
+int mie_batch_launch(bimsim_ctx* ctx, const SphereDev* d_sph, const double* d_kmag, int n,
+                     int lmax, int quirks, void* work, double2* A, double2* B);
+
+// Incident fields of n instances into Uf (complex64 when f32, else complex128).
+int uf_batch_launch(bimsim_ctx* ctx, const UfInst* d_inst, int n, int max_npix, int f32, void* Uf);
+
+// U = Uf or U += Uf outside the spheres, for n instances; d_sph holds
+// (x, y, z, a) per sphere.
+int sum_batch_launch(bimsim_ctx* ctx, const SumInst* d_inst, int n, int max_npix,
+                     const double4* d_sph, int f32, const void* Uf, void* U);
+
+// Zeroes `bytes` (a multiple of 4) at p with a kernel (graph-safe).
+int zero_launch(bimsim_ctx* ctx, void* p, size_t bytes);
+
+}  // namespace bimsim

@@ -1,0 +1,338 @@
+// Batches of independent absorbance images: the microscopeStruct::Simulate
+// sequence (reference microscope.cu:849-857, nearfield.cpp:194-218) for many
+// instances at once, optionally captured into one HIP graph.
+//
+//   Mie coefficients   one launch, one thread per (instance, sphere)
+//   incident fields    one launch over (pixel blocks, instances)
+//   scattered fields   FP32: coefficient kernel + one persistent work-queue
+//                      launch per order bucket (us_series.hip); FP64 forms:
+//                      the per-instance entry point
+//   Uf + Us            one launch over (pixel blocks, instances)
+//   detector           per instance (its own FFT size)
+#include <vector>
+
+#include "batch.h"
+#include "internal.h"
+#include "us_series.h"
+
+using namespace bimsim;
+
+struct bimsim_batch {
+    bimsim_ctx* ctx = nullptr;
+    int prec = 0, quirks = 1, n = 0;
+    bool f32 = false;
+    std::vector<bimsim_instance> inst;
+    std::vector<std::vector<bimsim_sphere>> inst_spheres;   // per instance, packed coeff_off
+    std::vector<double> kmag, subA, du, dv;
+    const double* waves = nullptr;
+    void *D = nullptr, *Di = nullptr, *Abs = nullptr;
+    size_t image_elems = 0;
+    int n_sph = 0, lmax = 0, max_npix = 0;
+    long long ncoef = 0, field_elems = 0;
+    std::vector<long long> field_off;
+    // device
+    SphereDev* d_sph = nullptr;
+    double* d_kmag = nullptr;
+    void* d_work = nullptr;
+    double2 *d_A = nullptr, *d_B = nullptr;
+    UfInst* d_uf = nullptr;
+    SumInst* d_sum = nullptr;
+    double4* d_sph4 = nullptr;
+    void *d_Uf = nullptr, *d_U = nullptr;
+    double* d_kdirs = nullptr;      // plane-wave directions, one row per instance
+    SeriesPlan plan_mc, plan_pw;    // FP32 series: Monte-Carlo / plane-wave instances
+    bool have_mc = false, have_pw = false;
+    hipStream_t capture_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+template <class T>
+int dev_upload(bimsim_ctx* ctx, T** dst, const std::vector<T>& v) {
+    if (v.empty()) return BIMSIM_OK;
+    if (hipMalloc((void**)dst, sizeof(T) * v.size()) != hipSuccess)
+        return set_error(ctx, BIMSIM_E_NOMEM, "out of device memory for batch tables");
+    BIMSIM_HIP_CHECK(ctx, hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    return BIMSIM_OK;
+}
+
+int dev_alloc(bimsim_ctx* ctx, void** dst, size_t bytes) {
+    if (bytes == 0) return BIMSIM_OK;
+    if (hipMalloc(dst, bytes) != hipSuccess)
+        return set_error(ctx, BIMSIM_E_NOMEM, "out of device memory for batch buffers (%zu bytes)",
+                         bytes);
+    return BIMSIM_OK;
+}
+
+void release(bimsim_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
+    series_plan_destroy(&b->plan_mc);
+    series_plan_destroy(&b->plan_pw);
+    void* ptrs[] = {b->d_sph, b->d_kmag, b->d_work, b->d_A, b->d_B, b->d_uf,
+                    b->d_sum, b->d_sph4, b->d_Uf, b->d_U, b->d_kdirs};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+}
+
+// One pass of the batch on ctx->stream.
+int enqueue(bimsim_batch* b) {
+    bimsim_ctx* ctx = b->ctx;
+    const size_t fb = b->f32 ? sizeof(float2) : sizeof(double2);
+    const size_t rb = b->f32 ? sizeof(float) : sizeof(double);
+    BIMSIM_TRY(mie_batch_launch(ctx, b->d_sph, b->d_kmag, b->n_sph, b->lmax, b->quirks, b->d_work,
+                                b->d_A, b->d_B));
+    BIMSIM_TRY(uf_batch_launch(ctx, b->d_uf, b->n, b->max_npix, b->f32, b->d_Uf));
+    if (b->f32) {
+        if (b->have_mc)
+            BIMSIM_TRY(series_plan_launch(ctx, b->plan_mc, b->waves, b->d_A, b->d_B, (float2*)b->d_U));
+        if (b->have_pw)
+            BIMSIM_TRY(series_plan_launch(ctx, b->plan_pw, b->d_kdirs, b->d_A, b->d_B, (float2*)b->d_U));
+    } else {
+        for (int i = 0; i < b->n; ++i) {
+            const bimsim_instance& I = b->inst[i];
+            if (I.ns == 0) continue;
+            const double* w = I.plane_wave ? b->d_kdirs + 3 * (size_t)i : b->waves + 3 * (size_t)I.wave_begin;
+            const int nk = I.plane_wave ? 1 : I.nk;
+            BIMSIM_TRY(bimsim_nf_us(ctx, &I.pos, I.N, I.N, I.focus, b->kmag[i], b->inst_spheres[i].data(),
+                                    I.ns, b->d_A, b->d_B, w, nk, b->subA[i], b->prec, b->quirks,
+                                    (char*)b->d_U + fb * b->field_off[i], 0));
+        }
+    }
+    BIMSIM_TRY(sum_batch_launch(ctx, b->d_sum, b->n, b->max_npix, b->d_sph4, b->f32, b->d_Uf, b->d_U));
+    BIMSIM_TRY(zero_launch(ctx, b->D, rb * b->image_elems));
+    BIMSIM_TRY(zero_launch(ctx, b->Di, rb * b->image_elems));
+    for (int i = 0; i < b->n; ++i) {
+        const bimsim_instance& I = b->inst[i];
+        void* Ab = b->Abs ? (char*)b->Abs + rb * I.image_off : nullptr;
+        BIMSIM_TRY(bimsim_detector(ctx, b->prec, b->quirks, (char*)b->d_U + fb * b->field_off[i],
+                                   (char*)b->d_Uf + fb * b->field_off[i], I.N, I.N, b->du[i], b->dv[i],
+                                   I.objective[0], I.objective[1], I.lambda, I.padding, I.Nd, I.ss,
+                                   (char*)b->D + rb * I.image_off, (char*)b->Di + rb * I.image_off,
+                                   Ab, 0));
+    }
+    return BIMSIM_OK;
+}
+
+double len3(const double* v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphere* spheres,
+          int n_spheres) {
+    bimsim_ctx* ctx = b->ctx;
+    b->n = n;
+    b->inst.assign(inst, inst + n);
+    b->f32 = b->prec == BIMSIM_FP32_FAST;
+    std::vector<SphereDev> sph_dev;
+    std::vector<double> sph_k;
+    std::vector<double4> sph4;
+    std::vector<UfInst> uf(n);
+    std::vector<SumInst> sum(n);
+    std::vector<double> kdirs(3 * (size_t)n);
+    SeriesBatch mc, pw;
+    b->inst_spheres.resize(n);
+    b->kmag.resize(n);
+    b->subA.resize(n);
+    b->du.resize(n);
+    b->dv.resize(n);
+    b->field_off.resize(n);
+    long long coef = 0, field = 0;
+    for (int i = 0; i < n; ++i) {
+        const bimsim_instance& I = inst[i];
+        BIMSIM_CHECK_ARG(ctx, I.N > 0 && (long long)I.N * I.N <= (1LL << 28), "instance N out of range");
+        BIMSIM_CHECK_ARG(ctx, I.lambda > 0.0 && std::isfinite(I.lambda), "instance lambda must be positive");
+        BIMSIM_CHECK_ARG(ctx, I.m >= 1 && I.m <= 100000, "instance m out of range");
+        BIMSIM_CHECK_ARG(ctx, I.ns >= 0 && I.sphere_begin >= 0 && I.sphere_begin + I.ns <= n_spheres,
+                         "instance sphere range outside the sphere array");
+        BIMSIM_CHECK_ARG(ctx, I.plane_wave || (I.nk >= 1 && I.wave_begin >= 0), "instance needs waves");
+        BIMSIM_CHECK_ARG(ctx, I.Nd > 0 && I.ss > 0 && I.padding >= 0 &&
+                                  (long long)(2 * I.padding + 1) * I.Nd * I.ss <= I.N && I.image_off >= 0,
+                         "instance detector does not fit its near field");
+        const double lx = len3(I.pos.X), ly = len3(I.pos.Y);
+        BIMSIM_CHECK_ARG(ctx, lx > 0.0 && ly > 0.0, "instance slice has a zero edge");
+        b->kmag[i] = 2.0 * (double)kPIf / I.lambda;
+        const double ca = std::cos(std::asin(I.condenser[0])), cb = std::cos(std::asin(I.condenser[1]));
+        b->subA[i] = I.plane_wave ? I.amp : 2.0 * (double)kPIf * I.amp * ((1.0 - cb) - (1.0 - ca));
+        b->du[i] = 1.0 / lx;
+        b->dv[i] = 1.0 / ly;
+        b->image_elems = std::max(b->image_elems, (size_t)(I.image_off + (long long)I.Nd * I.Nd));
+        const int npix = I.N * I.N;
+        b->max_npix = std::max(b->max_npix, npix);
+        b->field_off[i] = field;
+        const QuadArg q = to_quad(I.pos);
+        uf[i] = UfInst{q, mk(I.focus), mk(I.kdir), b->kmag[i], I.amp, ca, cb, I.m, I.N, I.N,
+                       I.plane_wave ? 1 : 0, field};
+        sum[i] = SumInst{q, I.N, I.N, (int)sph_dev.size(), I.ns, field};
+        for (int c = 0; c < 3; ++c) kdirs[3 * (size_t)i + c] = I.kdir[c];
+        // spheres of this instance, coefficients packed
+        SeriesBatch& sb = I.plane_wave ? pw : mc;
+        SeriesInst si{};
+        si.q = q;
+        si.f = mk(I.focus);
+        si.kmag = b->kmag[i];
+        const int nk = I.plane_wave ? 1 : I.nk;
+        si.sA = (double)(1.0f / (float)nk) * b->subA[i];
+        si.uR = si.vR = I.N;
+        si.s_begin = (int)sb.sph.size();
+        si.ns = I.ns;
+        si.nk = nk;
+        si.accumulate = 0;
+        si.wave_off = I.plane_wave ? i : I.wave_begin;
+        si.u_off = field;
+        for (int s = 0; s < I.ns; ++s) {
+            bimsim_sphere S = spheres[I.sphere_begin + s];
+            BIMSIM_CHECK_ARG(ctx, S.a > 0.0, "sphere radius must be positive");
+            if (S.Nl <= 0) S.Nl = bimsim_sphere_order(S.a, I.lambda);
+            BIMSIM_CHECK_ARG(ctx, S.Nl >= 2 && S.Nl <= 4096, "sphere order Nl must be in [2, 4096]");
+            S.coeff_off = (int)coef;
+            coef += S.Nl + 1;
+            b->lmax = std::max(b->lmax, S.Nl);
+            b->inst_spheres[i].push_back(S);
+            SphereDev d;
+            for (int c = 0; c < 3; ++c) d.p[c] = S.p[c];
+            d.a = S.a;
+            d.n[0] = S.n[0];
+            d.n[1] = S.n[1];
+            d.Nl = S.Nl;
+            d.coeff_off = S.coeff_off;
+            sph_dev.push_back(d);
+            sph_k.push_back(b->kmag[i]);
+            sph4.push_back(make_double4(S.p[0], S.p[1], S.p[2], S.a));
+            SeriesSph ss{};
+            ss.s = d;
+            ss.inst = (int)sb.inst.size();
+            sb.sph.push_back(ss);
+        }
+        if (I.ns > 0) sb.inst.push_back(si);
+        field += npix;
+    }
+    b->n_sph = (int)sph_dev.size();
+    b->ncoef = coef;
+    b->field_elems = field;
+    if (b->f32 && b->lmax + 1 > 112)
+        return set_error(ctx, BIMSIM_E_UNSUPPORTED, "FP32 series supports Nl <= 111 (got %d)", b->lmax);
+    const size_t fb = b->f32 ? sizeof(float2) : sizeof(double2);
+    BIMSIM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    BIMSIM_TRY(dev_upload(ctx, &b->d_sph, sph_dev));
+    BIMSIM_TRY(dev_upload(ctx, &b->d_kmag, sph_k));
+    BIMSIM_TRY(dev_upload(ctx, &b->d_uf, uf));
+    BIMSIM_TRY(dev_upload(ctx, &b->d_sum, sum));
+    BIMSIM_TRY(dev_upload(ctx, &b->d_sph4, sph4));
+    BIMSIM_TRY(dev_upload(ctx, &b->d_kdirs, kdirs));
+    BIMSIM_TRY(dev_alloc(ctx, &b->d_work, sizeof(double2) * 8 * (size_t)(b->lmax + 2) * std::max(1, b->n_sph)));
+    BIMSIM_TRY(dev_alloc(ctx, (void**)&b->d_A, sizeof(double2) * (size_t)std::max<long long>(1, coef)));
+    BIMSIM_TRY(dev_alloc(ctx, (void**)&b->d_B, sizeof(double2) * (size_t)std::max<long long>(1, coef)));
+    BIMSIM_TRY(dev_alloc(ctx, &b->d_Uf, fb * (size_t)field));
+    BIMSIM_TRY(dev_alloc(ctx, &b->d_U, fb * (size_t)field));
+    if (b->f32) {
+        const size_t budget = (size_t)8 << 30;
+        if (!mc.inst.empty()) {
+            BIMSIM_TRY(series_plan_create(ctx, mc, budget, &b->plan_mc));
+            b->have_mc = true;
+        }
+        if (!pw.inst.empty()) {
+            BIMSIM_TRY(series_plan_create(ctx, pw, budget, &b->plan_pw));
+            b->have_pw = true;
+        }
+    }
+    return BIMSIM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bimsim_batch_create(bimsim_ctx* ctx, const bimsim_instance* inst, int n,
+                        const bimsim_sphere* spheres, int n_spheres, const double* waves, int prec,
+                        int quirks, void* D, void* Di, void* Abs, int use_graph, bimsim_batch** out) {
+    if (!ctx || !out) return BIMSIM_E_INVALID;
+    *out = nullptr;
+    BIMSIM_CHECK_ARG(ctx, inst && n >= 1 && D && Di && n_spheres >= 0 && (n_spheres == 0 || spheres),
+                     "bad batch arguments");
+    BIMSIM_CHECK_ARG(ctx, prec >= BIMSIM_FP64_PARITY && prec <= BIMSIM_FP32_FAST, "unknown precision");
+    if (!quirks)
+        return set_error(ctx, BIMSIM_E_UNSUPPORTED, "batches run with the reference recurrences only");
+    bool need_waves = false;
+    for (int i = 0; i < n; ++i) need_waves |= !inst[i].plane_wave && inst[i].ns > 0;
+    BIMSIM_CHECK_ARG(ctx, waves || !need_waves, "waves must be given");
+    bimsim_batch* b = new bimsim_batch();
+    b->ctx = ctx;
+    b->prec = prec;
+    b->quirks = quirks;
+    b->waves = waves;
+    b->D = D;
+    b->Di = Di;
+    b->Abs = Abs;
+    int rc = build(b, inst, n, spheres, n_spheres);
+    // one eager pass: results, and every scratch buffer and table the pass
+    // needs is in place before a capture
+    if (rc == BIMSIM_OK) rc = enqueue(b);
+    if (rc == BIMSIM_OK && hipStreamSynchronize(ctx->stream) != hipSuccess)
+        rc = set_error(ctx, BIMSIM_E_HIP, "batch pass failed");
+    if (rc == BIMSIM_OK && use_graph) {
+        // capture on a stream of our own (the default stream cannot be captured)
+        const hipStream_t saved = ctx->stream;
+        const bool timing = ctx->timing;
+        if (hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking) != hipSuccess) {
+            rc = set_error(ctx, BIMSIM_E_HIP, "cannot create the capture stream");
+        } else {
+            ctx->stream = b->capture_stream;
+            ctx->timing = false;
+            if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                rc = set_error(ctx, BIMSIM_E_HIP, "hipStreamBeginCapture failed");
+            } else {
+                rc = enqueue(b);
+                hipGraph_t g = nullptr;
+                const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+                if (rc == BIMSIM_OK && e != hipSuccess)
+                    rc = set_error(ctx, BIMSIM_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+                b->graph = g;
+                if (rc == BIMSIM_OK && hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess)
+                    rc = set_error(ctx, BIMSIM_E_HIP, "hipGraphInstantiate failed");
+            }
+            ctx->stream = saved;
+            ctx->timing = timing;
+        }
+    }
+    if (rc != BIMSIM_OK) {
+        (void)hipDeviceSynchronize();
+        release(b);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return BIMSIM_OK;
+}
+
+int bimsim_batch_launch(bimsim_batch* b) {
+    if (!b) return BIMSIM_E_INVALID;
+    bimsim_ctx* ctx = b->ctx;
+    BIMSIM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    if (b->exec) {
+        BIMSIM_HIP_CHECK(ctx, hipGraphLaunch(b->exec, ctx->stream));
+        return BIMSIM_OK;
+    }
+    return enqueue(b);
+}
+
+int bimsim_batch_destroy(bimsim_batch* b) {
+    if (!b) return BIMSIM_E_INVALID;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    release(b);
+    delete b;
+    return BIMSIM_OK;
+}
+
+int bimsim_batch_run(bimsim_ctx* ctx, const bimsim_instance* inst, int n,
+                     const bimsim_sphere* spheres, int n_spheres, const double* waves, int prec,
+                     int quirks, void* D, void* Di, void* Abs) {
+    bimsim_batch* b = nullptr;
+    BIMSIM_TRY(bimsim_batch_create(ctx, inst, n, spheres, n_spheres, waves, prec, quirks, D, Di, Abs,
+                                   0, &b));
+    return bimsim_batch_destroy(b);
+}
+
+}  // extern "C"

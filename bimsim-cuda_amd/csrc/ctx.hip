@@ -5,6 +5,7 @@
 #include <cstring>
 
 #include "internal.h"
+#include "us_series.h"
 
 namespace bimsim {
 
@@ -155,6 +156,7 @@ int bimsim_ctx_destroy(bimsim_ctx* ctx) {
     if (!ctx) return BIMSIM_E_INVALID;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    series_cache_release(ctx);
     for (auto& kv : ctx->scratch)
         if (kv.second.first) (void)hipFree(kv.second.first);
     for (auto& kv : ctx->int_tables) (void)hipFree(kv.second);

@@ -35,6 +35,8 @@ struct bimsim_ctx {
     std::map<std::vector<int>, int*> int_tables;
     int cu_count = 256;
     size_t lds_max = 65536;
+    // last plan of the FP32 series (opaque, owned by us_series.hip)
+    void* series_cache = nullptr;
 };
 
 namespace bimsim {

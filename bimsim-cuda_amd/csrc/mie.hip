@@ -1,6 +1,7 @@
 // FP64 Mie coefficients on the GPU: sphere::calcCoeff (reference
 // sphere.cpp:21-97) with the complex spherical Bessel functions of
 // zj_device.h.  One thread per sphere.  Built with -ffp-contract=off.
+#include "batch.h"
 #include "internal.h"
 #include "zj_device.h"
 
@@ -15,13 +16,10 @@ struct SphereBatch {
 
 using zj::zc;
 
-__global__ void mie_coeff_kernel(SphereBatch batch, int ns, double kmag, int quirks, zc* work,
-                                 int wstride, double2* A, double2* B) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ns) return;
-    const SphereDev S = batch.s[t];
+// Coefficients of one sphere at wavenumber kmag (work: 8 arrays of wstride).
+__device__ void mie_one(const SphereDev& S, double kmag, int quirks, zc* w, int wstride,
+                        double2* A, double2* B) {
     const int nl = S.Nl;
-    zc* w = work + (size_t)t * 8 * wstride;
     zc *j1 = w, *y1 = w + wstride, *jp1 = w + 2 * wstride, *yp1 = w + 3 * wstride;
     zc *j2 = w + 4 * wstride, *y2 = w + 5 * wstride, *jp2 = w + 6 * wstride, *yp2 = w + 7 * wstride;
     const zc nc = {S.n[0], S.n[1]};
@@ -46,7 +44,32 @@ __global__ void mie_coeff_kernel(SphereBatch batch, int ns, double kmag, int qui
     }
 }
 
+__global__ void mie_coeff_kernel(SphereBatch batch, int ns, double kmag, int quirks, zc* work,
+                                 int wstride, double2* A, double2* B) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ns) return;
+    mie_one(batch.s[t], kmag, quirks, work + (size_t)t * 8 * wstride, wstride, A, B);
+}
+
+// One thread per sphere of a batch, sphere t at wavenumber kmag[t].
+__global__ void mie_batch_kernel(const SphereDev* __restrict__ sph, const double* __restrict__ kmag,
+                                 int n, int quirks, zc* work, int wstride, double2* A, double2* B) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    mie_one(sph[t], kmag[t], quirks, work + (size_t)t * 8 * wstride, wstride, A, B);
+}
+
 }  // namespace
+
+int mie_batch_launch(bimsim_ctx* ctx, const SphereDev* d_sph, const double* d_kmag, int n, int lmax,
+                     int quirks, void* work, double2* A, double2* B) {
+    if (n <= 0) return BIMSIM_OK;
+    hipLaunchKernelGGL(mie_batch_kernel, dim3((n + 63) / 64), dim3(64), 0, ctx->stream, d_sph,
+                       d_kmag, n, quirks, (zc*)work, lmax + 2, A, B);
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return BIMSIM_OK;
+}
+
 }  // namespace bimsim
 
 using namespace bimsim;

@@ -5,6 +5,7 @@
 // the scattered-field work of any Monte-Carlo configuration.
 #include "geometry.h"
 #include "internal.h"
+#include "batch.h"
 #include "refmath.h"
 
 namespace bimsim {
@@ -26,23 +27,15 @@ __device__ inline void store<float2>(float2* U, int i, dc v) {
     U[i] = make_float2((float)v.r, (float)v.i);
 }
 
-// gpuScalarUf, nfScalarUf.cu:46-175
-template <class Out>
-__global__ __launch_bounds__(kThreads) void uf_series_kernel(QuadArg q, int uR, int vR, d3 f,
-                                                             d3 k, double kmag, double amp,
-                                                             double ca, double cb, int m,
-                                                             Out* __restrict__ Uf) {
-    const int pix = blockIdx.x * kThreads + threadIdx.x;
-    if (pix >= uR * vR) return;
+// gpuScalarUf, nfScalarUf.cu:46-175: the field at pixel pix
+__device__ dc uf_series_value(const QuadArg& q, int uR, int vR, int pix, d3 f, d3 k, double kmag,
+                              double amp, double ca, double cb, int m) {
     int iu, iv;
     pixel_of(pix, uR, iu, iv);
     const d3 p = quad_at(q, (double)iu / (double)uR, (double)iv / (double)vR);
     d3 r = sub(p, f);
     const double d = len(r);
-    if (d < (double)kEpsFloat) {
-        store(Uf, pix, dc{amp * 2.0 * (double)kPIf * (ca - cb), 0.0});
-        return;
-    }
+    if (d < (double)kEpsFloat) return dc{amp * 2.0 * (double)kPIf * (ca - cb), 0.0};
     r = norm(r);
     const dc im = {0.0, 1.0};
     dc il = {1.0, 0.0};
@@ -79,22 +72,86 @@ __global__ __launch_bounds__(kThreads) void uf_series_kernel(QuadArg q, int uR, 
         sum = cadd(sum, cscl(cscl(cscl(il, jl), Pl), D));
         il = cmul(il, im);
     }
-    store(Uf, pix, cscl(cscl(cscl(sum, 2.0), (double)kPIf), amp));
+    return cscl(cscl(cscl(sum, 2.0), (double)kPIf), amp);
 }
 
-// gpuScalarUfp, nfScalarUf.cu:9-43
+// gpuScalarUfp, nfScalarUf.cu:9-43: the plane wave at pixel pix
+__device__ inline dc uf_plane_value(const QuadArg& q, int uR, int vR, int pix, d3 f, d3 k,
+                                    double kmag, double amp) {
+    int iu, iv;
+    pixel_of(pix, uR, iu, iv);
+    const d3 p = quad_at(q, (double)iu / (double)uR, (double)iv / (double)vR);
+    const d3 r = sub(p, f);
+    const double kr = kmag * dot(k, r);
+    return cscl(cexp_i(kr), amp);
+}
+
+template <class Out>
+__global__ __launch_bounds__(kThreads) void uf_series_kernel(QuadArg q, int uR, int vR, d3 f,
+                                                             d3 k, double kmag, double amp,
+                                                             double ca, double cb, int m,
+                                                             Out* __restrict__ Uf) {
+    const int pix = blockIdx.x * kThreads + threadIdx.x;
+    if (pix >= uR * vR) return;
+    store(Uf, pix, uf_series_value(q, uR, vR, pix, f, k, kmag, amp, ca, cb, m));
+}
+
 template <class Out>
 __global__ __launch_bounds__(kThreads) void uf_plane_kernel(QuadArg q, int uR, int vR, d3 f, d3 k,
                                                             double kmag, double amp,
                                                             Out* __restrict__ Uf) {
     const int pix = blockIdx.x * kThreads + threadIdx.x;
     if (pix >= uR * vR) return;
-    int iu, iv;
-    pixel_of(pix, uR, iu, iv);
-    const d3 p = quad_at(q, (double)iu / (double)uR, (double)iv / (double)vR);
-    const d3 r = sub(p, f);
-    const double kr = kmag * dot(k, r);
-    store(Uf, pix, cscl(cexp_i(kr), amp));
+    store(Uf, pix, uf_plane_value(q, uR, vR, pix, f, k, kmag, amp));
+}
+
+// Incident fields of a batch of instances.  Grid: (pixel blocks, instances).
+template <class Out>
+__global__ __launch_bounds__(kThreads) void uf_batch_kernel(const UfInst* __restrict__ insts, int n,
+                                                            Out* __restrict__ Uf) {
+    for (int i = blockIdx.y; i < n; i += gridDim.y) {
+        const UfInst& I = insts[i];
+        const int npix = I.uR * I.vR;
+        for (int pix = blockIdx.x * kThreads + threadIdx.x; pix < npix; pix += gridDim.x * kThreads) {
+            const dc v = I.plane ? uf_plane_value(I.q, I.uR, I.vR, pix, I.f, I.k, I.kmag, I.amp)
+                                 : uf_series_value(I.q, I.uR, I.vR, pix, I.f, I.k, I.kmag, I.amp,
+                                                   I.ca, I.cb, I.m);
+            store(Uf + I.off, pix, v);
+        }
+    }
+}
+
+// U = Uf (no spheres) or U += Uf outside every sphere (d < a is inside,
+// nfSumUf.cu:31-48) for a batch of instances.  Spheres as (x, y, z, a).
+template <class C>
+__global__ __launch_bounds__(kThreads) void sum_batch_kernel(const SumInst* __restrict__ insts, int n,
+                                                             const double4* __restrict__ sph,
+                                                             const C* __restrict__ Uf,
+                                                             C* __restrict__ U) {
+    for (int i = blockIdx.y; i < n; i += gridDim.y) {
+        const SumInst& I = insts[i];
+        const int npix = I.uR * I.vR;
+        for (int pix = blockIdx.x * kThreads + threadIdx.x; pix < npix; pix += gridDim.x * kThreads) {
+            const C b = Uf[I.off + pix];
+            if (I.ns == 0) {
+                U[I.off + pix] = b;
+                continue;
+            }
+            int iu, iv;
+            pixel_of(pix, I.uR, iu, iv);
+            const d3 p = quad_at(I.q, (double)iu / (double)I.uR, (double)iv / (double)I.vR);
+            bool in = false;
+            for (int s = I.s_begin; s < I.s_begin + I.ns && !in; ++s) {
+                const double4 S = sph[s];
+                if (len(sub(p, d3{S.x, S.y, S.z})) < S.w) in = true;
+            }
+            if (in) continue;
+            C a = U[I.off + pix];
+            a.x = a.x + b.x;
+            a.y = a.y + b.y;
+            U[I.off + pix] = a;
+        }
+    }
 }
 
 struct MaskBatch {
@@ -130,6 +187,32 @@ __global__ void add_outside_kernel(int npix, const unsigned char* __restrict__ m
 }
 
 }  // namespace
+
+int uf_batch_launch(bimsim_ctx* ctx, const UfInst* d_inst, int n, int max_npix, int f32, void* Uf) {
+    const dim3 grid((max_npix + kThreads - 1) / kThreads, std::min(n, 65535));
+    if (f32)
+        hipLaunchKernelGGL(uf_batch_kernel<float2>, grid, dim3(kThreads), 0, ctx->stream, d_inst, n,
+                           (float2*)Uf);
+    else
+        hipLaunchKernelGGL(uf_batch_kernel<double2>, grid, dim3(kThreads), 0, ctx->stream, d_inst,
+                           n, (double2*)Uf);
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return BIMSIM_OK;
+}
+
+int sum_batch_launch(bimsim_ctx* ctx, const SumInst* d_inst, int n, int max_npix,
+                     const double4* d_sph, int f32, const void* Uf, void* U) {
+    const dim3 grid((max_npix + kThreads - 1) / kThreads, std::min(n, 65535));
+    if (f32)
+        hipLaunchKernelGGL(sum_batch_kernel<float2>, grid, dim3(kThreads), 0, ctx->stream, d_inst, n,
+                           d_sph, (const float2*)Uf, (float2*)U);
+    else
+        hipLaunchKernelGGL(sum_batch_kernel<double2>, grid, dim3(kThreads), 0, ctx->stream, d_inst,
+                           n, d_sph, (const double2*)Uf, (double2*)U);
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return BIMSIM_OK;
+}
+
 }  // namespace bimsim
 
 using namespace bimsim;

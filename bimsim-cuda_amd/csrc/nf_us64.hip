@@ -5,16 +5,15 @@
 //   us_parity_kernel   the reference structure (precompute + optPc_gpuScalarUsp,
 //                      nfScalarUs.cu:17-48,449-491): per pixel, per wave, the
 //                      full calc_Us / calc_Ui recurrences in double.
-//   us_coef_kernel     per (pixel, sphere) wave-independent products
+//   us_coef64_kernel   per (pixel, sphere) wave-independent products
 //                      c_l = B_l h_l(kd) or A_l j_l(knd) (the only part of
-//                      calc_Us/calc_Ui that does not depend on the wave);
-//                      written in FP64 for the hoisted FP64 form, or scaled
-//                      by s_l and rounded to float for the FP32 series.
+//                      calc_Us/calc_Ui that does not depend on the wave).
 //   us_series64_kernel the hoisted FP64 form: same per-term operations as
 //                      the parity kernel, bit for bit.
 //   wave tables        per (sphere, wave) phase weights in FP64.
-// The FP32 series itself lives in nf_us32.hip.
+// The FP32 form lives in us_series.hip.
 #include "nf_us.h"
+#include "us_series.h"
 #include "refmath.h"
 
 namespace bimsim {
@@ -216,57 +215,6 @@ __global__ __launch_bounds__(kThreads) void us_series64_kernel(
     U[pix] = make_double2(acc.r, acc.i);
 }
 
-// FP32 series inputs for a batch of spheres: c'_l = s_l c_l as float2 in
-// cp[coff + l*npix + pix] (zero beyond Nl up to the 8-order block), r-hat in
-// rh[roff + pix].  Grid: (pixel blocks, spheres).
-__global__ __launch_bounds__(kThreads) void us_coef32_kernel(
-    QuadArg q, int uR, int vR, double kmag, UsSphereBatch batch, SeriesMeta meta,
-    const double2* __restrict__ A, const double2* __restrict__ B, float2* __restrict__ cp,
-    float4* __restrict__ rh) {
-    const int npix = uR * vR;
-    const int pix = blockIdx.x * kThreads + threadIdx.x;
-    const int s = blockIdx.y;
-    if (pix >= npix) return;
-    const SphereDev& S = batch.s[s];
-    int iu, iv;
-    pixel_of(pix, uR, iu, iv);
-    const d3 p = quad_at(q, (double)iu / (double)uR, (double)iv / (double)vR);
-    d3 r = sub(p, mk(S.p));
-    const double d = len(r);
-    r = norm(r);
-    float2* out = cp + meta.coff[s] + pix;
-    double sl = 1.0;   // s_l = (2l)! / (4^l (l!)^2)
-    const int k = pixel_coefficients(d, kmag, S, A + S.coeff_off, B + S.coeff_off,
-                                     [&](int l, dc c) {
-                                         if (l > 0) sl = sl * (2.0 * l - 1.0) / (2.0 * l);
-                                         out[(size_t)l * npix] = make_float2((float)(c.r * sl), (float)(c.i * sl));
-                                     });
-    for (int l = S.Nl + 1; l < 8 * meta.nb[s]; ++l) out[(size_t)l * npix] = make_float2(0.f, 0.f);
-    if (k == 2) r = d3{0.0, 0.0, 0.0};   // the series then reduces to c'_0 = A_0
-    rh[meta.roff[s] + pix] = make_float4((float)r.x, (float)r.y, (float)r.z, 0.f);
-}
-
-// FP32 wave records of a batch: grid (wave blocks, spheres).
-__global__ void wave32_kernel(d3 f, double kmag, double sA, UsSphereBatch batch, SeriesMeta meta,
-                              const double* __restrict__ waves, int nk, int nk_pad,
-                              WaveF* __restrict__ wt) {
-    const int w = blockIdx.x * blockDim.x + threadIdx.x;
-    const int s = blockIdx.y;
-    if (w >= nk_pad) return;
-    WaveF e = {};
-    if (w < nk) {
-        const d3 k = mk(waves + 3 * w);
-        const d3 c = sub(mk(batch.s[s].p), f);
-        const dc ph = cscl(cexp_i(kmag * dot(k, c)), sA);
-        e.kx2 = (float)(2.0 * k.x);
-        e.ky2 = (float)(2.0 * k.y);
-        e.kz2 = (float)(2.0 * k.z);
-        e.pr = (float)ph.r;
-        e.pi = (float)ph.i;
-    }
-    wt[meta.woff[s] + w] = e;
-}
-
 __global__ void clear_c64(double2* U, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) U[i] = make_double2(0.0, 0.0);
@@ -285,12 +233,6 @@ SphereDev to_dev(const bimsim_sphere& S) {
     d.Nl = S.Nl;
     d.coeff_off = S.coeff_off;
     return d;
-}
-
-int series_capacity(int lmax_plus1) {
-    for (int L : kSeriesBuckets)
-        if (L >= lmax_plus1) return L;
-    return -1;
 }
 
 }  // namespace
@@ -368,39 +310,30 @@ extern "C" int bimsim_nf_us(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int
             BIMSIM_HIP_CHECK(ctx, hipGetLastError());
         }
     } else {
-        const int L = series_capacity(lmax + 1);
-        if (L < 0)
-            return set_error(ctx, BIMSIM_E_UNSUPPORTED,
-                             "FP32 series supports Nl <= %d (got %d)", kSeriesMaxL - 1, lmax);
-        const int nk_pad = (nk + 1) / 2 * 2;
-        for (int s0 = 0; s0 < ns; s0 += kUsBatch) {
-            const int cnt = std::min(kUsBatch, ns - s0);
-            UsSphereBatch b{};
-            SeriesMeta meta{};
-            long long coff = 0;
-            for (int i = 0; i < cnt; ++i) {
-                b.s[i] = to_dev(spheres[s0 + i]);
-                meta.nb[i] = (spheres[s0 + i].Nl + 1 + 7) / 8;
-                meta.nl1[i] = spheres[s0 + i].Nl + 1;
-                meta.coff[i] = coff;
-                coff += (long long)meta.nb[i] * 8 * npix;
-                meta.roff[i] = (long long)i * npix;
-                meta.woff[i] = (long long)i * nk_pad;
-            }
-            void *cp = nullptr, *rh = nullptr, *wt = nullptr;
-            BIMSIM_TRY(scratch_get(ctx, kSlotCoef, sizeof(float2) * (size_t)coff, &cp));
-            BIMSIM_TRY(scratch_get(ctx, kSlotRhat, sizeof(float4) * (size_t)cnt * npix, &rh));
-            BIMSIM_TRY(scratch_get(ctx, kSlotWaves, sizeof(WaveF) * (size_t)cnt * nk_pad, &wt));
-            hipLaunchKernelGGL(wave32_kernel, dim3((nk_pad + 255) / 256, cnt), dim3(256), 0,
-                               ctx->stream, f, kmag, sA, b, meta, waves, nk, nk_pad, (WaveF*)wt);
-            hipLaunchKernelGGL(us_coef32_kernel, dim3(pix_blocks, cnt), dim3(kThreads), 0,
-                               ctx->stream, q, uR, vR, kmag, b, meta, (const double2*)A,
-                               (const double2*)B, (float2*)cp, (float4*)rh);
-            BIMSIM_HIP_CHECK(ctx, hipGetLastError());
-            BIMSIM_TRY(launch_us_series_f32(ctx, L, (const float2*)cp, (const float4*)rh,
-                                            (const WaveF*)wt, meta, cnt, nk_pad, npix, (float2*)U,
-                                            (s0 > 0 || accumulate) ? 1 : 0, s0 == 0));
+        // FP32 series: a batch of one instance
+        SeriesBatch batch;
+        SeriesInst I{};
+        I.q = q;
+        I.f = f;
+        I.kmag = kmag;
+        I.sA = sA;
+        I.uR = uR;
+        I.vR = vR;
+        I.s_begin = 0;
+        I.ns = ns;
+        I.nk = nk;
+        I.accumulate = accumulate ? 1 : 0;
+        batch.inst.push_back(I);
+        for (int s = 0; s < ns; ++s) {
+            SeriesSph S{};
+            S.s = to_dev(spheres[s]);
+            S.inst = 0;
+            batch.sph.push_back(S);
         }
+        const SeriesPlan* plan = nullptr;
+        BIMSIM_TRY(series_cached_plan(ctx, batch, &plan));
+        BIMSIM_TRY(series_plan_launch(ctx, *plan, waves, (const double2*)A, (const double2*)B,
+                                     (float2*)U));
     }
     timer_end(ctx, tmr);
     return BIMSIM_OK;

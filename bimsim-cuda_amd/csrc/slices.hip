@@ -13,6 +13,7 @@
 // Every kernel is a grid-stride-free one-thread-per-element map; these are
 // HBM-bound and small next to the near-field series.  Built with
 // -ffp-contract=off so that the FP64 forms round like the reference's ops.
+#include "batch.h"
 #include "internal.h"
 
 namespace bimsim {
@@ -129,7 +130,23 @@ __global__ void bandpass_kernel(typename Cplx<R>::T* __restrict__ F, int uR, int
 
 bool prec_ok(int prec) { return prec >= BIMSIM_FP64_PARITY && prec <= BIMSIM_FP32_FAST; }
 
+__global__ void zero_words(unsigned* __restrict__ p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads)
+        p[i] = 0u;
+}
+
 }  // namespace
+
+int zero_launch(bimsim_ctx* ctx, void* p, size_t bytes) {
+    const size_t n = bytes / 4;
+    if (n == 0) return BIMSIM_OK;
+    const size_t blocks = std::min<size_t>((n + kThreads - 1) / kThreads, 65535);
+    hipLaunchKernelGGL(zero_words, dim3((unsigned)blocks), dim3(kThreads), 0, ctx->stream,
+                       (unsigned*)p, n);
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return BIMSIM_OK;
+}
+
 }  // namespace bimsim
 
 using namespace bimsim;

@@ -1,0 +1,102 @@
+// Scattered-field series in the FP32 form for one or many instances
+// (slice, spheres, waves).  Two phases per chunk of instances:
+//   1. a coefficient kernel writes, per (sphere, pixel), the wave-independent
+//      products c'_l = s_l B_l h_l(kd) (or s_l A_l j_l(knd) inside) computed
+//      in FP64 with the reference recurrences, and r-hat;
+//   2. one persistent work-queue launch per order bucket runs the FP32
+//      rescaled-Legendre series: work items are (instance, 256-pixel block,
+//      wave split) and every workgroup takes items from a counter until the
+//      queue is empty.
+// Instances are grouped into chunks whose coefficient storage fits a memory
+// budget; the same buffers serve every chunk.
+#pragma once
+
+#include <vector>
+
+#include "nf_us.h"
+
+namespace bimsim {
+
+// One instance as the kernels see it (device table).
+struct SeriesInst {
+    QuadArg q;
+    d3 f;               // focus
+    double kmag;
+    double sA;          // (1.0f/nk) * subA
+    int uR, vR;
+    int s_begin, ns;    // spheres [s_begin, s_begin + ns) of the sphere table
+    int nk, nk_pad;     // waves, padded to a multiple of 2
+    int wchunk, nsplit; // waves per split, number of splits
+    int accumulate;     // U (+)= result
+    int pad0;
+    long long wave_off; // first wave (rows of 3 doubles) of the instance
+    long long u_off;    // element offset of the instance's field in U
+    long long part_off; // element offset of its split partial sums
+};
+
+struct SeriesSph {
+    SphereDev s;
+    int inst;           // owning instance
+    int nb;             // blocks of 8 orders: ceil((Nl + 1) / 8)
+    long long woff;     // offset of its WaveF records
+    long long coff;     // offset of c'[l][pix] in the chunk's coefficient buffer
+    long long roff;     // offset of r-hat[pix] in the chunk's r-hat buffer
+};
+
+struct SeriesItem {
+    int inst, pix0, split, pad;
+};
+
+// Host description of a batch: the caller fills every SeriesInst field up
+// to accumulate / wave_off / u_off, and SeriesSph.s / inst.
+struct SeriesBatch {
+    std::vector<SeriesInst> inst;
+    std::vector<SeriesSph> sph;
+};
+
+struct SeriesPlan {
+    int n_inst = 0, n_sph = 0;
+    int nk_max = 0, max_npix = 0;
+    long long n_wave_rec = 0, n_part = 0;
+    long long coef_elems = 0, rhat_elems = 0;   // per chunk (maximum)
+    struct Bucket {
+        int L, item_begin, item_count, counter;
+    };
+    struct Chunk {
+        int sph_begin, sph_count;
+        std::vector<Bucket> buckets;
+    };
+    std::vector<Chunk> chunks;
+    int n_counters = 0;
+    std::vector<int> split_insts;
+    SeriesInst* d_inst = nullptr;
+    SeriesSph* d_sph = nullptr;
+    SeriesItem* d_items = nullptr;
+    int* d_split_insts = nullptr;
+    int* d_counters = nullptr;
+    WaveF* d_wt = nullptr;
+    float2* d_part = nullptr;
+    float2* d_coef = nullptr;
+    float4* d_rhat = nullptr;
+};
+
+// Chooses splits and chunks, builds and uploads the tables and allocates
+// the plan's buffers (coefficient storage at most `coef_budget` bytes per
+// chunk unless one instance needs more).  Synchronises the stream.
+int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, SeriesPlan* plan);
+void series_plan_destroy(SeriesPlan* plan);
+
+// Launches the plan on the context stream; allocation- and
+// synchronisation-free, so it can be captured into a graph.
+int series_plan_launch(bimsim_ctx* ctx, const SeriesPlan& plan, const double* waves,
+                       const double2* A, const double2* B, float2* U);
+
+// Order capacity for Nl + 1 orders, or -1.
+int series_capacity(int nl1);
+
+// Plan for `batch`, reused while the batch tables are unchanged (the
+// context keeps its last plan; the key is the tables' bytes).
+int series_cached_plan(bimsim_ctx* ctx, SeriesBatch& batch, const SeriesPlan** out);
+void series_cache_release(bimsim_ctx* ctx);
+
+}  // namespace bimsim

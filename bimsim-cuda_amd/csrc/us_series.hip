@@ -1,0 +1,566 @@
+// FP32 scattered-field series (nfScalarUs hot loop, reference
+// nfScalarUs.cu:449-491 with calc_Us/calc_Ui :50-95,201-245), evaluated as
+//
+//   U(p) += sum_s sum_w phi_{s,w} sum_l c'_{s,l}(p) Q_l(x_{s,w}(p))
+//
+// with c'_l = s_l * B_l h_l(kd) (or s_l * A_l j_l(knd) inside the sphere),
+// computed once per (pixel, sphere) in FP64 with the reference recurrences,
+// and the rescaled Legendre polynomials Q_l = P_l / s_l obeying
+// Q_l = 2x Q_{l-1} - g_l Q_{l-2}, s_l = (2l)!/(4^l (l!)^2).  Per term: one
+// multiply and three FMAs with g_l as an instruction literal (the order loop
+// is unrolled, c'_l live in VGPRs).  All lanes of a wavefront walk the same
+// sphere, wave and order, so wave records are scalar loads.  Waves are
+// summed in chunks of 8 whose sums enter a Kahan accumulator.
+//
+// Built with -ffp-contract=off so that the FP64 recurrences round like the
+// reference's; every FP32 FMA of the series is written out with fmaf.
+#include <algorithm>
+#include <cstring>
+
+#include "refmath.h"
+#include "us_series.h"
+
+namespace bimsim {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kW = 2;   // waves in flight per thread
+constexpr int kBuckets[] = {16, 24, 32, 40, 48, 64, 80, 96, 112};
+
+__device__ inline dc ld2(const double2* p, int i) {
+    const double2 v = p[i];
+    return dc{v.x, v.y};
+}
+
+// Phase 1: c'_l(p) (orders padded with zeros to 8 nb) and r-hat(p) for
+// every sphere of a chunk.  Grid: (pixel blocks, spheres of the chunk).
+__global__ __launch_bounds__(kThreads) void series_coef_kernel(
+    const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph, int sph_begin,
+    int sph_count, const double2* __restrict__ A, const double2* __restrict__ B,
+    float2* __restrict__ cp, float4* __restrict__ rh) {
+    for (int si = blockIdx.y; si < sph_count; si += gridDim.y) {
+        const SeriesSph& SS = sph[sph_begin + si];
+        const SeriesInst& I = insts[SS.inst];
+        const SphereDev S = SS.s;
+        const int npix = I.uR * I.vR;
+        for (int pix = blockIdx.x * kThreads + threadIdx.x; pix < npix; pix += gridDim.x * kThreads) {
+            const int iv = pix / I.uR, iu = pix - iv * I.uR;
+            const d3 p = quad_at(I.q, (double)iu / (double)I.uR, (double)iv / (double)I.vR);
+            d3 r = sub(p, mk(S.p));
+            const double d = len(r);
+            r = norm(r);
+            float2* out = cp + SS.coff + pix;
+            const double2* As = A + S.coeff_off;
+            const double2* Bs = B + S.coeff_off;
+            const int Nl = S.Nl;
+            double sl = 1.0;   // s_l = (2l)! / (4^l (l!)^2)
+            auto put = [&](int l, dc c) {
+                if (l > 0) sl = sl * (2.0 * l - 1.0) / (2.0 * l);
+                out[(size_t)l * npix] = make_float2((float)(c.r * sl), (float)(c.i * sl));
+            };
+            bool centre = false;
+            if (d <= S.a) {
+                const dc knd = cscl(dc{S.n[0], S.n[1]}, I.kmag * d);
+                if (knd.r < (double)kEpsFloat) {
+                    // calc_Ui returns A_0 for every wave: the series reduces to c'_0
+                    put(0, ld2(As, 0));
+                    for (int l = 1; l <= Nl; ++l) put(l, dc{0.0, 0.0});
+                    centre = true;
+                } else {
+                    dc j[2];
+                    ref::init_jc(knd, j);
+                    put(0, cmul(ld2(As, 0), j[0]));
+                    if (Nl >= 1) put(1, cmul(ld2(As, 1), j[1]));
+                    for (int l = 2; l <= Nl; ++l) {
+                        ref::shift_jc(l, knd, j);
+                        put(l, cmul(ld2(As, l), j[1]));
+                    }
+                }
+            } else {
+                const double kd = I.kmag * d;
+                double j[2], y[2];
+                ref::init_j(kd, j);
+                ref::init_y(kd, y);
+                put(0, cmul(ld2(Bs, 0), dc{j[0], y[0]}));
+                if (Nl >= 1) put(1, cmul(ld2(Bs, 1), dc{j[1], y[1]}));
+                for (int l = 2; l <= Nl; ++l) {
+                    ref::shift_j(l, kd, j);
+                    ref::shift_y(l, kd, y);
+                    put(l, cmul(ld2(Bs, l), dc{j[1], y[1]}));
+                }
+            }
+            for (int l = Nl + 1; l < 8 * SS.nb; ++l) out[(size_t)l * npix] = make_float2(0.f, 0.f);
+            if (centre) r = d3{0.0, 0.0, 0.0};
+            rh[SS.roff + pix] = make_float4((float)r.x, (float)r.y, (float)r.z, 0.f);
+        }
+    }
+}
+
+// One step of the series for order l on kW waves (l >= 2).
+template <int l>
+__device__ __forceinline__ void series_step(const float* cr, const float* ci, const float* x2,
+                                            float* q0, float* q1, float* sr, float* si) {
+    const float g = -(float)legq_g(l);
+#pragma unroll
+    for (int u = 0; u < kW; ++u) {
+        const float qn = fmaf(g, q0[u], x2[u] * q1[u]);
+        q0[u] = q1[u];
+        q1[u] = qn;
+        sr[u] = fmaf(cr[l], qn, sr[u]);
+        si[u] = fmaf(ci[l], qn, si[u]);
+    }
+}
+
+template <int l0, int j>
+__device__ __forceinline__ void block_step(const float* cr, const float* ci, const float* x2,
+                                           float* q0, float* q1, float* sr, float* si) {
+    if constexpr (l0 + j >= 2) series_step<l0 + j>(cr, ci, x2, q0, q1, sr, si);
+}
+
+// Orders in blocks of 8: blocks below nfull run whole, block nfull runs its
+// first rem orders, each behind a wavefront-uniform test.
+template <int L, int B = 0>
+__device__ __forceinline__ void series_blocks(const float* cr, const float* ci, const float* x2,
+                                              float* q0, float* q1, float* sr, float* si,
+                                              int nfull, int rem) {
+    if constexpr (B < L / 8) {
+        constexpr int l0 = 8 * B;
+        if (B < nfull) {
+            block_step<l0, 0>(cr, ci, x2, q0, q1, sr, si);
+            block_step<l0, 1>(cr, ci, x2, q0, q1, sr, si);
+            block_step<l0, 2>(cr, ci, x2, q0, q1, sr, si);
+            block_step<l0, 3>(cr, ci, x2, q0, q1, sr, si);
+            block_step<l0, 4>(cr, ci, x2, q0, q1, sr, si);
+            block_step<l0, 5>(cr, ci, x2, q0, q1, sr, si);
+            block_step<l0, 6>(cr, ci, x2, q0, q1, sr, si);
+            block_step<l0, 7>(cr, ci, x2, q0, q1, sr, si);
+            series_blocks<L, B + 1>(cr, ci, x2, q0, q1, sr, si, nfull, rem);
+        } else if (B == nfull) {
+            if (0 < rem) block_step<l0, 0>(cr, ci, x2, q0, q1, sr, si);
+            if (1 < rem) block_step<l0, 1>(cr, ci, x2, q0, q1, sr, si);
+            if (2 < rem) block_step<l0, 2>(cr, ci, x2, q0, q1, sr, si);
+            if (3 < rem) block_step<l0, 3>(cr, ci, x2, q0, q1, sr, si);
+            if (4 < rem) block_step<l0, 4>(cr, ci, x2, q0, q1, sr, si);
+            if (5 < rem) block_step<l0, 5>(cr, ci, x2, q0, q1, sr, si);
+            if (6 < rem) block_step<l0, 6>(cr, ci, x2, q0, q1, sr, si);
+        }
+    }
+}
+
+__device__ __forceinline__ void kahan(float& acc_r, float& acc_i, float& cmp_r, float& cmp_i,
+                                      float yr0, float yi0) {
+    const float yr = yr0 - cmp_r, yi = yi0 - cmp_i;
+    const float tr = acc_r + yr, ti = acc_i + yi;
+    cmp_r = (tr - acc_r) - yr;
+    cmp_i = (ti - acc_i) - yi;
+    acc_r = tr;
+    acc_i = ti;
+}
+
+// Phase 2: persistent work-queue kernel over the items of one bucket.
+template <int L>
+__global__ __launch_bounds__(kThreads) void series_f32_kernel(
+    const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph,
+    const SeriesItem* __restrict__ items, int n_items, const float2* __restrict__ cp,
+    const float4* __restrict__ rh, const WaveF* __restrict__ wt, float2* __restrict__ U,
+    float2* __restrict__ part, int* __restrict__ counter) {
+    __shared__ int s_item;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();
+        if (it >= n_items) break;
+        const SeriesItem item = items[it];
+        const SeriesInst& I = insts[item.inst];
+        const int npix = I.uR * I.vR;
+        const int pix = item.pix0 + (int)threadIdx.x;
+        if (pix >= npix) continue;
+        const int w_begin = item.split * I.wchunk;
+        const int w_end = min(I.nk_pad, w_begin + I.wchunk);
+        float acc_r = 0.f, acc_i = 0.f, cmp_r = 0.f, cmp_i = 0.f;
+        for (int s = I.s_begin; s < I.s_begin + I.ns; ++s) {
+            const SeriesSph& SS = sph[s];
+            const int nb = SS.nb;
+            const int nl1 = SS.s.Nl + 1;
+            const int nfull = nl1 >> 3, rem = nl1 & 7;
+            float cr[L], ci[L];
+            const float2* c = cp + SS.coff + pix;
+#pragma unroll
+            for (int b = 0; b < L / 8; ++b) {
+                if (b < nb) {
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const float2 v = c[(size_t)(8 * b + j) * npix];
+                        cr[8 * b + j] = v.x;
+                        ci[8 * b + j] = v.y;
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) cr[8 * b + j] = ci[8 * b + j] = 0.f;
+                }
+            }
+            const float4 r = rh[SS.roff + pix];
+            const WaveF* W = wt + SS.woff;
+            if (w_begin >= w_end) continue;
+            float ch_r = 0.f, ch_i = 0.f;
+            int in_chunk = 0;
+            WaveF nxt[kW];
+#pragma unroll
+            for (int u = 0; u < kW; ++u) nxt[u] = W[w_begin + u];
+            for (int w = w_begin; w < w_end; w += kW) {
+                float x2[kW], q0[kW], q1[kW], sr[kW], si[kW], pr[kW], pi[kW];
+#pragma unroll
+                for (int u = 0; u < kW; ++u) {
+                    const WaveF e = nxt[u];
+                    x2[u] = fmaf(e.kz2, r.z, fmaf(e.ky2, r.y, e.kx2 * r.x));
+                    pr[u] = e.pr;
+                    pi[u] = e.pi;
+                    q0[u] = 1.f;
+                    q1[u] = x2[u];
+                    sr[u] = fmaf(cr[1], x2[u], cr[0]);
+                    si[u] = fmaf(ci[1], x2[u], ci[0]);
+                }
+                // prefetch the next records (the last iteration re-reads its own)
+                const int wn = w + kW < w_end ? w + kW : w;
+#pragma unroll
+                for (int u = 0; u < kW; ++u) nxt[u] = W[wn + u];
+                series_blocks<L>(cr, ci, x2, q0, q1, sr, si, nfull, rem);
+#pragma unroll
+                for (int u = 0; u < kW; ++u) {
+                    ch_r = fmaf(pr[u], sr[u], ch_r);
+                    ch_r = fmaf(-pi[u], si[u], ch_r);
+                    ch_i = fmaf(pr[u], si[u], ch_i);
+                    ch_i = fmaf(pi[u], sr[u], ch_i);
+                }
+                in_chunk += kW;
+                if (in_chunk >= 8) {
+                    kahan(acc_r, acc_i, cmp_r, cmp_i, ch_r, ch_i);
+                    ch_r = ch_i = 0.f;
+                    in_chunk = 0;
+                }
+            }
+            kahan(acc_r, acc_i, cmp_r, cmp_i, ch_r, ch_i);
+        }
+        if (I.nsplit == 1) {
+            float2* o = U + I.u_off + pix;
+            float2 v = make_float2(0.f, 0.f);
+            if (I.accumulate) v = *o;
+            *o = make_float2(v.x + acc_r, v.y + acc_i);
+        } else {
+            part[I.part_off + (long long)item.split * npix + pix] = make_float2(acc_r, acc_i);
+        }
+    }
+}
+
+// WaveF records: 2k and (1/nk) subA exp(i kmag k.c), c = p_s - f (FP64).
+__global__ void series_waves_kernel(const SeriesInst* __restrict__ insts,
+                                    const SeriesSph* __restrict__ sph, int n_sph,
+                                    const double* __restrict__ waves, WaveF* __restrict__ wt) {
+    for (int s = blockIdx.y; s < n_sph; s += gridDim.y) {
+        const SeriesSph& SS = sph[s];
+        const SeriesInst& I = insts[SS.inst];
+        for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < I.nk_pad;
+             w += gridDim.x * blockDim.x) {
+            WaveF e = {};
+            if (w < I.nk) {
+                const d3 k = mk(waves + 3 * (I.wave_off + w));
+                const d3 c = sub(mk(SS.s.p), I.f);
+                const dc ph = cscl(cexp_i(I.kmag * dot(k, c)), I.sA);
+                e.kx2 = (float)(2.0 * k.x);
+                e.ky2 = (float)(2.0 * k.y);
+                e.kz2 = (float)(2.0 * k.z);
+                e.pr = (float)ph.r;
+                e.pi = (float)ph.i;
+            }
+            wt[SS.woff + w] = e;
+        }
+    }
+}
+
+// U (+)= sum over splits, in split order, for the listed instances.
+__global__ void series_reduce_kernel(const SeriesInst* __restrict__ insts,
+                                     const int* __restrict__ list, int n,
+                                     const float2* __restrict__ part, float2* __restrict__ U) {
+    for (int k = blockIdx.y; k < n; k += gridDim.y) {
+        const SeriesInst& I = insts[list[k]];
+        const int npix = I.uR * I.vR;
+        for (int pix = blockIdx.x * blockDim.x + threadIdx.x; pix < npix;
+             pix += gridDim.x * blockDim.x) {
+            float sr = 0.f, si = 0.f;
+            for (int t = 0; t < I.nsplit; ++t) {
+                const float2 v = part[I.part_off + (long long)t * npix + pix];
+                sr += v.x;
+                si += v.y;
+            }
+            float2* o = U + I.u_off + pix;
+            float2 v = make_float2(0.f, 0.f);
+            if (I.accumulate) v = *o;
+            *o = make_float2(v.x + sr, v.y + si);
+        }
+    }
+}
+
+__global__ void zero_ints(int* p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+template <int L>
+void launch_bucket(dim3 grid, hipStream_t st, const SeriesPlan& P, const SeriesPlan::Bucket& K,
+                   float2* U) {
+    hipLaunchKernelGGL(series_f32_kernel<L>, grid, dim3(kThreads), 0, st, P.d_inst, P.d_sph,
+                       P.d_items + K.item_begin, K.item_count, P.d_coef, P.d_rhat, P.d_wt, U,
+                       P.d_part, P.d_counters + K.counter);
+}
+
+template <class T>
+int upload(bimsim_ctx* ctx, T** dst, const std::vector<T>& v) {
+    if (v.empty()) return BIMSIM_OK;
+    if (hipMalloc((void**)dst, sizeof(T) * v.size()) != hipSuccess)
+        return set_error(ctx, BIMSIM_E_NOMEM, "out of device memory for the series tables");
+    BIMSIM_HIP_CHECK(ctx, hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    return BIMSIM_OK;
+}
+
+template <class T>
+int alloc(bimsim_ctx* ctx, T** dst, long long n) {
+    if (n <= 0) return BIMSIM_OK;
+    if (hipMalloc((void**)dst, sizeof(T) * (size_t)n) != hipSuccess)
+        return set_error(ctx, BIMSIM_E_NOMEM, "out of device memory for the series buffers");
+    return BIMSIM_OK;
+}
+
+struct SeriesCache {
+    std::vector<char> key;
+    SeriesPlan plan;
+};
+
+}  // namespace
+
+int series_capacity(int nl1) {
+    for (int L : kBuckets)
+        if (L >= nl1) return L;
+    return -1;
+}
+
+void series_plan_destroy(SeriesPlan* P) {
+    (void)hipFree(P->d_inst);
+    (void)hipFree(P->d_sph);
+    (void)hipFree(P->d_items);
+    (void)hipFree(P->d_split_insts);
+    (void)hipFree(P->d_counters);
+    (void)hipFree(P->d_wt);
+    (void)hipFree(P->d_part);
+    (void)hipFree(P->d_coef);
+    (void)hipFree(P->d_rhat);
+    *P = SeriesPlan();
+}
+
+int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, SeriesPlan* P) {
+    *P = SeriesPlan();
+    const int ni = (int)batch.inst.size();
+    P->n_inst = ni;
+    P->n_sph = (int)batch.sph.size();
+    std::vector<int> bucket(ni);
+    std::vector<double> cost(ni);
+    long long blocks_all = 0;
+    int next_sph = 0;
+    for (int i = 0; i < ni; ++i) {
+        SeriesInst& I = batch.inst[i];
+        if (I.s_begin != next_sph || I.ns < 0 || I.s_begin + I.ns > P->n_sph)
+            return set_error(ctx, BIMSIM_E_INVALID, "instance spheres must be consecutive ranges");
+        next_sph += I.ns;
+        int nl1 = 1;
+        for (int s = I.s_begin; s < I.s_begin + I.ns; ++s) {
+            SeriesSph& S = batch.sph[s];
+            if (S.inst != i) return set_error(ctx, BIMSIM_E_INVALID, "sphere/instance mismatch");
+            S.nb = (S.s.Nl + 1 + 7) / 8;
+            nl1 = std::max(nl1, S.s.Nl + 1);
+        }
+        bucket[i] = series_capacity(nl1);
+        if (bucket[i] < 0)
+            return set_error(ctx, BIMSIM_E_UNSUPPORTED, "FP32 series supports Nl <= %d (got %d)",
+                             kBuckets[8] - 1, nl1 - 1);
+        I.nk_pad = (I.nk + kW - 1) / kW * kW;
+        const int npix = I.uR * I.vR;
+        blocks_all += (npix + kThreads - 1) / kThreads;
+        cost[i] = (double)npix * I.ns * I.nk_pad * bucket[i];
+        P->nk_max = std::max(P->nk_max, I.nk_pad);
+        P->max_npix = std::max(P->max_npix, npix);
+    }
+    if (next_sph != P->n_sph) return set_error(ctx, BIMSIM_E_INVALID, "unused spheres in batch");
+    // split the waves until the queue holds several items per CU (each
+    // split keeps at least 32 waves)
+    const long long target = 4LL * ctx->cu_count;
+    int mult = 1;
+    for (;;) {
+        bool can = false;
+        for (int i = 0; i < ni; ++i)
+            if (batch.inst[i].nk_pad / (2 * mult) >= 32) can = true;
+        if (blocks_all * mult >= target || !can) break;
+        mult *= 2;
+    }
+    long long woff = 0, poff = 0;
+    for (int s = 0; s < P->n_sph; ++s) {
+        batch.sph[s].woff = woff;
+        woff += batch.inst[batch.sph[s].inst].nk_pad;
+    }
+    P->n_wave_rec = woff;
+    for (int i = 0; i < ni; ++i) {
+        SeriesInst& I = batch.inst[i];
+        int nsplit = 1;
+        while (nsplit < mult && I.nk_pad / (2 * nsplit) >= 32) nsplit *= 2;
+        int wchunk = (I.nk_pad + nsplit - 1) / nsplit;
+        wchunk = std::max(kW, (wchunk + kW - 1) / kW * kW);
+        I.wchunk = wchunk;
+        I.nsplit = std::max(1, (I.nk_pad + wchunk - 1) / wchunk);
+        I.part_off = 0;
+        if (I.nsplit > 1) {
+            I.part_off = poff;
+            poff += (long long)I.nsplit * I.uR * I.vR;
+            P->split_insts.push_back(i);
+        }
+    }
+    P->n_part = poff;
+    // chunks of whole instances within the coefficient budget
+    std::vector<SeriesItem> items;
+    int i0 = 0;
+    while (i0 < ni) {
+        long long celems = 0, relems = 0;
+        int i1 = i0;
+        for (; i1 < ni; ++i1) {
+            const SeriesInst& I = batch.inst[i1];
+            const long long npix = (long long)I.uR * I.vR;
+            long long c = 0;
+            for (int s = I.s_begin; s < I.s_begin + I.ns; ++s) c += 8LL * batch.sph[s].nb * npix;
+            if (i1 > i0 && (size_t)(celems + c) * sizeof(float2) > coef_budget) break;
+            for (int s = I.s_begin; s < I.s_begin + I.ns; ++s) {
+                batch.sph[s].coff = celems;
+                celems += 8LL * batch.sph[s].nb * npix;
+                batch.sph[s].roff = relems;
+                relems += npix;
+            }
+        }
+        P->coef_elems = std::max(P->coef_elems, celems);
+        P->rhat_elems = std::max(P->rhat_elems, relems);
+        SeriesPlan::Chunk ch;
+        ch.sph_begin = batch.inst[i0].s_begin;
+        ch.sph_count = batch.inst[i1 - 1].s_begin + batch.inst[i1 - 1].ns - ch.sph_begin;
+        std::vector<int> order;
+        for (int i = i0; i < i1; ++i) order.push_back(i);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+        for (int L : kBuckets) {
+            const int begin = (int)items.size();
+            for (int i : order) {
+                if (bucket[i] != L) continue;
+                const SeriesInst& I = batch.inst[i];
+                const int npix = I.uR * I.vR;
+                for (int t = 0; t < I.nsplit; ++t)
+                    for (int p0 = 0; p0 < npix; p0 += kThreads) items.push_back(SeriesItem{i, p0, t, 0});
+            }
+            const int count = (int)items.size() - begin;
+            if (count > 0) ch.buckets.push_back(SeriesPlan::Bucket{L, begin, count, P->n_counters++});
+        }
+        P->chunks.push_back(ch);
+        i0 = i1;
+    }
+    BIMSIM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    BIMSIM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    int rc;
+    if ((rc = upload(ctx, &P->d_inst, batch.inst)) != BIMSIM_OK ||
+        (rc = upload(ctx, &P->d_sph, batch.sph)) != BIMSIM_OK ||
+        (rc = upload(ctx, &P->d_items, items)) != BIMSIM_OK ||
+        (rc = upload(ctx, &P->d_split_insts, P->split_insts)) != BIMSIM_OK ||
+        (rc = alloc(ctx, &P->d_counters, std::max(1, P->n_counters))) != BIMSIM_OK ||
+        (rc = alloc(ctx, &P->d_wt, std::max<long long>(1, P->n_wave_rec))) != BIMSIM_OK ||
+        (rc = alloc(ctx, &P->d_part, P->n_part)) != BIMSIM_OK ||
+        (rc = alloc(ctx, &P->d_coef, std::max<long long>(1, P->coef_elems))) != BIMSIM_OK ||
+        (rc = alloc(ctx, &P->d_rhat, std::max<long long>(1, P->rhat_elems))) != BIMSIM_OK) {
+        series_plan_destroy(P);
+        return rc;
+    }
+    return BIMSIM_OK;
+}
+
+int series_plan_launch(bimsim_ctx* ctx, const SeriesPlan& P, const double* waves, const double2* A,
+                       const double2* B, float2* U) {
+    if (P.n_inst == 0 || P.n_sph == 0) return BIMSIM_OK;
+    const hipStream_t st = ctx->stream;
+    hipLaunchKernelGGL(zero_ints, dim3(1), dim3(256), 0, st, P.d_counters, std::max(1, P.n_counters));
+    const dim3 wgrid((P.nk_max + 255) / 256, std::min(P.n_sph, 65535));
+    hipLaunchKernelGGL(series_waves_kernel, wgrid, dim3(256), 0, st, P.d_inst, P.d_sph, P.n_sph,
+                       waves, P.d_wt);
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    for (const SeriesPlan::Chunk& ch : P.chunks) {
+        const dim3 cgrid((P.max_npix + kThreads - 1) / kThreads, std::min(ch.sph_count, 65535));
+        hipLaunchKernelGGL(series_coef_kernel, cgrid, dim3(kThreads), 0, st, P.d_inst, P.d_sph,
+                           ch.sph_begin, ch.sph_count, A, B, P.d_coef, P.d_rhat);
+        BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+        const int tmr = timer_begin(ctx, kStageSeries);
+        for (const SeriesPlan::Bucket& K : ch.buckets) {
+            // resident workgroups: the queue keeps them busy until it drains
+            const dim3 grid(std::min(K.item_count, 8 * ctx->cu_count));
+            switch (K.L) {
+#define BIMSIM_CASE(LL) \
+    case LL: launch_bucket<LL>(grid, st, P, K, U); break;
+                BIMSIM_CASE(16)
+                BIMSIM_CASE(24)
+                BIMSIM_CASE(32)
+                BIMSIM_CASE(40)
+                BIMSIM_CASE(48)
+                BIMSIM_CASE(64)
+                BIMSIM_CASE(80)
+                BIMSIM_CASE(96)
+                BIMSIM_CASE(112)
+#undef BIMSIM_CASE
+                default:
+                    return set_error(ctx, BIMSIM_E_UNSUPPORTED, "no FP32 series kernel for %d orders", K.L);
+            }
+            BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+        }
+        timer_end(ctx, tmr);
+    }
+    if (!P.split_insts.empty()) {
+        const dim3 rgrid((P.max_npix + 255) / 256, std::min((int)P.split_insts.size(), 65535));
+        hipLaunchKernelGGL(series_reduce_kernel, rgrid, dim3(256), 0, st, P.d_inst, P.d_split_insts,
+                           (int)P.split_insts.size(), P.d_part, U);
+        BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    }
+    return BIMSIM_OK;
+}
+
+int series_cached_plan(bimsim_ctx* ctx, SeriesBatch& batch, const SeriesPlan** out) {
+    std::vector<char> key(sizeof(SeriesInst) * batch.inst.size() + sizeof(SeriesSph) * batch.sph.size());
+    if (!batch.inst.empty())
+        std::memcpy(key.data(), batch.inst.data(), sizeof(SeriesInst) * batch.inst.size());
+    if (!batch.sph.empty())
+        std::memcpy(key.data() + sizeof(SeriesInst) * batch.inst.size(), batch.sph.data(),
+                    sizeof(SeriesSph) * batch.sph.size());
+    SeriesCache* c = (SeriesCache*)ctx->series_cache;
+    if (c && c->key == key) {
+        *out = &c->plan;
+        return BIMSIM_OK;
+    }
+    if (c) {
+        BIMSIM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        series_plan_destroy(&c->plan);
+    } else {
+        c = new SeriesCache();
+        ctx->series_cache = c;
+    }
+    c->key.clear();
+    BIMSIM_TRY(series_plan_create(ctx, batch, (size_t)8 << 30, &c->plan));
+    c->key = key;
+    *out = &c->plan;
+    return BIMSIM_OK;
+}
+
+void series_cache_release(bimsim_ctx* ctx) {
+    SeriesCache* c = (SeriesCache*)ctx->series_cache;
+    if (!c) return;
+    series_plan_destroy(&c->plan);
+    delete c;
+    ctx->series_cache = nullptr;
+}
+
+}  // namespace bimsim

@@ -2,9 +2,10 @@
 nearfield.cpp:194-218): incident field, Mie coefficients, scattered field,
 Uf + Us sum, detector.  Buffers are torch device tensors; all compute is in
 libbimsim_hip.so on the context stream."""
+import numpy as np
 import torch
 
-from . import FP32_FAST, QUIRKS_REF, UF_PLANE, UF_SERIES, Quad
+from . import FP32_FAST, QUIRKS_REF, UF_PLANE, UF_SERIES, Instance, Quad
 
 
 def quad_of(inst):
@@ -42,8 +43,13 @@ def simulate(ctx, inst, waves, prec, buf=None, quirks=QUIRKS_REF, write_filtered
     ctx.nf_uf(q, N, N, inst["focus"], inst["kdir"], inst["kmag"], inst["amp"], inst["cos_alpha"],
               inst["cos_beta"], inst["m"], buf.Uf, mode=mode, prec=prec, quirks=quirks)
     ctx.mie_coeffs(inst["spheres"], inst["kmag"], buf.A, buf.B, quirks=quirks)
+    subA = inst["subA"]
+    if inst.get("plane_wave"):
+        # a single wave along k with amplitude A (nfScalarUs.cu:547-561)
+        waves = torch.tensor([list(inst["kdir"])], dtype=torch.float64, device=buf.U.device)
+        subA = inst["amp"]
     ctx.nf_us(q, N, N, inst["focus"], inst["kmag"], inst["spheres"], buf.A, buf.B, waves,
-              inst["subA"], buf.U, prec=prec, quirks=quirks)
+              subA, buf.U, prec=prec, quirks=quirks)
     ctx.nf_sum_uf(q, N, N, inst["spheres"], buf.Uf, buf.U, prec=prec)
     buf.D.zero_()
     buf.Di.zero_()
@@ -51,3 +57,46 @@ def simulate(ctx, inst, waves, prec, buf=None, quirks=QUIRKS_REF, write_filtered
                  inst["lam"], inst["padding"], inst["Nd"], inst["ss"], buf.D, buf.Di, buf.Abs,
                  prec=prec, quirks=quirks, write_filtered=write_filtered)
     return buf
+
+
+def batch_tables(insts, waves):
+    """bimsim_instance array, sphere list and stacked waves for a batch.
+    `waves[i]` is the (nk, 3) wave array of instance i (ignored for plane
+    waves).  Images are packed instance after instance."""
+    arr = (Instance * len(insts))()
+    spheres, rows = [], []
+    wb = 0
+    img = 0
+    for i, inst in enumerate(insts):
+        I = arr[i]
+        I.pos.A[:] = list(inst["A"])
+        I.pos.X[:] = list(inst["X"])
+        I.pos.Y[:] = list(inst["Y"])
+        I.N = inst["N"]
+        I.m = inst["m"]
+        I.focus[:] = list(inst["focus"])
+        I.kdir[:] = list(inst["kdir"])
+        I.lambda_ = inst["lam"]
+        I.amp = inst["amp"]
+        I.condenser[:] = list(inst["cond"])
+        I.objective[:] = list(inst["obj"])
+        I.plane_wave = 1 if inst.get("plane_wave") else 0
+        I.sphere_begin = len(spheres)
+        I.ns = len(inst["spheres"])
+        spheres.extend(inst["spheres"])
+        I.wave_begin = wb
+        if I.plane_wave:
+            I.nk = 1
+        else:
+            w = np.asarray(waves[i], dtype=np.float64).reshape(-1, 3)
+            I.nk = w.shape[0]
+            rows.append(w)
+            wb += w.shape[0]
+        I.padding = inst["padding"]
+        I.Nd = inst["Nd"]
+        I.ss = inst["ss"]
+        I.image_off = img
+        img += inst["Nd"] * inst["Nd"]
+    W = np.concatenate(rows) if rows else np.zeros((1, 3))
+    return arr, spheres, W, img
+

@@ -13,6 +13,18 @@ origin, k = +z, detector without padding and ss = 1):
          objective 0-0.5, Legendre Uf, direct Us series with 400 waves
   cfg2   3 spheres a~U(2,8), n~U(1.3,1.6)+iU(0,0.1), centres U(-15,15)^2,
          1000 waves over NA 0.2-0.62, lambda=3, N=512, L=60, objective 0-0.62
+  cfg3   50 non-overlapping spheres a~U(1,10), n as cfg2, centres
+         U(-100,100)^2, lambda=4, N=1024, L=200, 400 waves, condenser and
+         objective 0-0.5; scattered field from the sphere tables (radial
+         tables of 4096 samples, 1000 angles)
+  cfg4   spectral sweep: 256 wavenumbers 900-4000 cm^-1 (lambda = 1e4/nu), 20
+         non-overlapping spheres a~U(2,15), centres U(-50,50)^2, n(nu) =
+         1.5 + 0.08 (g/2)/((nu0 - nu) - i g/2), nu0 = 1650, g = 30; N=512,
+         L=100, condenser and objective 0.2-0.5, 400 waves
+  cfg5   ragged: single-sphere instances, a log-uniform [0.5, 25],
+         lambda~U(2.5,11), n~U(1.2,1.7)+iU(0,0.2), N from {256,384,512,768,
+         1024}, L = 4a; half of them a plane wave, the others 200-2000
+         Monte-Carlo waves; condenser and objective 0-0.5
 """
 import math
 import struct
@@ -69,6 +81,8 @@ def make_instance(spheres, lam, N, L, cond, obj, nk, wave_seed, amp=1.0, m=None,
         s["Nl"] = sphere_order(s["a"], lam)
         s["coeff_off"] = off
         off += s["Nl"] + 1
+    cond = (f32(cond[0]), f32(cond[1]))
+    obj = (f32(obj[0]), f32(obj[1]))
     ca = math.cos(math.asin(cond[0]))
     cb = math.cos(math.asin(cond[1]))
     half = L / 2.0
@@ -77,7 +91,7 @@ def make_instance(spheres, lam, N, L, cond, obj, nk, wave_seed, amp=1.0, m=None,
     inst = dict(
         lam=lam, kmag=kmag, N=N, L=float(L), A=(-half, -half, 0.0), X=(float(L), 0.0, 0.0),
         Y=(0.0, float(L), 0.0), focus=(0.0, 0.0, 0.0), kdir=(0.0, 0.0, 1.0), amp=amp,
-        cond=(f32(cond[0]), f32(cond[1])), obj=(f32(obj[0]), f32(obj[1])), cos_alpha=ca,
+        cond=cond, obj=obj, cos_alpha=ca,
         cos_beta=cb, m=m, spheres=spheres, ncoef=off, nk=nk, wave_seed=wave_seed,
         plane_wave=plane_wave, padding=0, Nd=N, ss=1, du=1.0 / L, dv=1.0 / L,
     )
@@ -112,6 +126,73 @@ def config(cid, N=None, nk=None):
         return make_instance(sph, 3.0, N or 512, 60.0, (0.2, 0.62), (0.0, 0.62), nk or 1000,
                              SEED_BASE + 0x100 + cid)
     raise ValueError(f"unknown configuration {cid}")
+
+
+def _non_overlapping(g, count, a_lo, a_hi, half):
+    """Rejection sampling of `count` spheres (radius, centre) with
+    |c_i - c_j| >= a_i + a_j."""
+    out = []
+    tries = 0
+    while len(out) < count:
+        tries += 1
+        if tries > 200000:
+            raise RuntimeError("cannot place non-overlapping spheres")
+        a = g.uniform(a_lo, a_hi)
+        cx = g.uniform(-half, half)
+        cy = g.uniform(-half, half)
+        if all(math.hypot(cx - o[1], cy - o[2]) >= a + o[0] for o in out):
+            out.append((a, cx, cy))
+    return out
+
+
+def lorentz_index(nu, nu0=1650.0, gamma=30.0, strength=0.08, n0=1.5):
+    """n(nu) = n0 + strength (gamma/2) / ((nu0 - nu) - i gamma/2): imaginary
+    peak `strength` at nu0, real part Kramers-Kronig consistent."""
+    return n0 + strength * (gamma / 2.0) / complex(nu0 - nu, -gamma / 2.0)
+
+
+def config3(N=1024, ns=50, nk=400, L=200.0):
+    g = SplitMix64(SEED_BASE + 3)
+    sph = []
+    for a, cx, cy in _non_overlapping(g, ns, 1.0, 10.0, L / 2.0):
+        nr, ni = g.uniform(1.3, 1.6), g.uniform(0.0, 0.1)
+        sph.append(dict(p=(cx, cy, 0.0), a=a, n=complex(nr, ni)))
+    inst = make_instance(sph, 4.0, N, L, (0.0, 0.5), (0.0, 0.5), nk, SEED_BASE + 0x103)
+    inst["lut"] = dict(dR=4096, thetaR=1000)
+    return inst
+
+
+def config4(n_nu=256, N=512, ns=20, nk=400, L=100.0, nu_lo=900.0, nu_hi=4000.0, stride=1):
+    """The spectral sweep as a list of instances (every `stride`-th
+    wavenumber of the n_nu-point grid)."""
+    g = SplitMix64(SEED_BASE + 4)
+    geo = _non_overlapping(g, ns, 2.0, 15.0, L / 2.0)
+    out = []
+    for i in range(0, n_nu, stride):
+        nu = nu_lo + i * (nu_hi - nu_lo) / (n_nu - 1)
+        lam = 1e4 / nu
+        n = lorentz_index(nu)
+        sph = [dict(p=(cx, cy, 0.0), a=a, n=n) for a, cx, cy in geo]
+        inst = make_instance(sph, lam, N, L, (0.2, 0.5), (0.2, 0.5), nk, SEED_BASE + 0x104)
+        inst["nu"] = nu
+        out.append(inst)
+    return out
+
+
+def config5(count=1024, sizes=(256, 384, 512, 768, 1024), nk_range=(200, 2000)):
+    g = SplitMix64(SEED_BASE + 5)
+    out = []
+    for i in range(count):
+        a = math.exp(g.uniform(math.log(0.5), math.log(25.0)))
+        lam = g.uniform(2.5, 11.0)
+        n = complex(g.uniform(1.2, 1.7), g.uniform(0.0, 0.2))
+        N = sizes[min(len(sizes) - 1, int(g.uniform(0, len(sizes))))]
+        plane = g.uniform() < 0.5
+        nk = 1 if plane else nk_range[0] + int(g.uniform(0, nk_range[1] - nk_range[0] + 1))
+        inst = make_instance([dict(p=(0.0, 0.0, 0.0), a=a, n=n)], lam, N, 4.0 * a, (0.0, 0.5),
+                             (0.0, 0.5), nk, SEED_BASE + 0x1000 + i, plane_wave=plane)
+        out.append(inst)
+    return out
 
 
 def terms(inst):

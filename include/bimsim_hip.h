@@ -302,6 +302,58 @@ int bimsim_nf_us_lut(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int vR,
 int bimsim_nf_interpolate(bimsim_ctx* ctx, int prec, double deg, int uR, int vR,
                           const void* U_src, const void* Uf_src, void* U, void* Uf);
 
+/* ---------------------------------------------------------------- batches
+ * Many independent absorbance images (spectral sweeps, ragged sets of
+ * spheres and grids) in one pass: every instance is a microscopeStruct::
+ * Simulate (reference microscope.cu:849-857) with its own slice, spheres,
+ * wavelength, optics and waves.  The Mie coefficients of all spheres, the
+ * incident fields of all instances and the Uf + Us sums run as one launch
+ * each; the FP32 scattered-field series runs as one persistent work-queue
+ * launch per order bucket over (instance, pixel block, wave split) items.
+ * With use_graph the whole pass is captured once into a HIP graph and
+ * bimsim_batch_launch replays it. */
+
+typedef struct {
+    bimsim_quad pos;        /* near-field slice (nearfieldStruct::pos)          */
+    int    N;               /* samples per side of the near field (uR = vR = N) */
+    int    m;               /* incident-field order (--field-order)             */
+    double focus[3];
+    double kdir[3];         /* incident direction                               */
+    double lambda;          /* wavelength; kmag = 2*PI/lambda, PI = 3.14159f    */
+    double amp;             /* incident amplitude A                             */
+    double condenser[2];    /* NAin, NAout                                      */
+    double objective[2];    /* NAin, NAout                                      */
+    int    plane_wave;      /* 1: plane wave along kdir (Uf and a single wave)  */
+    int    sphere_begin;    /* spheres [sphere_begin, sphere_begin + ns)        */
+    int    ns;
+    int    wave_begin;      /* Monte-Carlo waves [wave_begin, wave_begin + nk)  */
+    int    nk;
+    int    padding;         /* detector (microscopeStruct::padding, ss, Nd)     */
+    int    Nd;
+    int    ss;
+    long long image_off;    /* element offset of the instance's Nd x Nd images  */
+} bimsim_instance;
+
+typedef struct bimsim_batch bimsim_batch;
+
+/* spheres: host array; n is the refractive index at the owning instance's
+ * wavelength, Nl <= 0 asks for sphere::calcNl, coeff_off is ignored.
+ * waves: device, rows of 3 doubles.  D, Di (and Abs unless NULL): device
+ * images, float for BIMSIM_FP32_FAST and double otherwise; they are
+ * overwritten.  The device buffers must stay valid while the batch lives.
+ * Creation runs the pass once (results in D/Di/Abs) and synchronises. */
+int bimsim_batch_create(bimsim_ctx* ctx, const bimsim_instance* inst, int n,
+                        const bimsim_sphere* spheres, int n_spheres,
+                        const double* waves, int prec, int quirks, void* D,
+                        void* Di, void* Abs, int use_graph, bimsim_batch** out);
+/* Runs the pass again (asynchronously on the context stream). */
+int bimsim_batch_launch(bimsim_batch* batch);
+int bimsim_batch_destroy(bimsim_batch* batch);
+/* Create, run once and destroy. */
+int bimsim_batch_run(bimsim_ctx* ctx, const bimsim_instance* inst, int n,
+                     const bimsim_sphere* spheres, int n_spheres, const double* waves,
+                     int prec, int quirks, void* D, void* Di, void* Abs);
+
 #ifdef __cplusplus
 }
 #endif
