@@ -7,7 +7,13 @@ Uf + Us, detector FFT / band-pass / intensity / -log10) with inputs resident
 in device memory.  The workload is the 3-sphere, 1000-wave, 512 x 512
 Monte-Carlo focusing configuration (cfg2 in bimsim-cuda_amd/workloads.py).
 
-usage: python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu]
+A second, reported-beside measurement ("spectral") times the batched path on
+a spectral sweep: every 16th wavenumber of the 256-point 900-4000 cm^-1
+grid (16 images), 20 spheres, 400 waves, 512 x 512 (cfg4 in
+bimsim-cuda_amd/workloads.py), as one graph-captured batch in FP32 against
+the plain FP64 form on two of its images.
+
+usage: python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--no-spectral]
 Prints one JSON line.
 """
 import argparse
@@ -83,6 +89,59 @@ def cpu_baseline(pkg, W, inst, waves, A, B, budget_s=12.0):
     }
 
 
+def spectral(pkg, P, W, ctx, torch, stride=16, steps=3, fp64_images=2):
+    """Batched FP32 spectral imaging (one captured batch) against the plain
+    FP64 form, images/s and absorbance error on the same instances."""
+    insts = W.config4(stride=stride)
+    waves = [W.mc_waves(i, pkg.mc_sample_na) for i in insts]
+    arr, spheres, Wv, n_img = P.batch_tables(insts, waves)
+    wd = torch.as_tensor(Wv.ravel()).cuda()
+    D = torch.empty(n_img, dtype=torch.float32, device="cuda")
+    Di = torch.empty_like(D)
+    Ab = torch.empty_like(D)
+    log(f"bench: spectral batch of {len(insts)} images, creating the graph")
+    b = ctx.batch(arr, spheres, wd, D, Di, Ab, prec=pkg.FP32_FAST, use_graph=True)
+    b.launch()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.launch()
+    torch.cuda.synchronize()
+    t32 = (time.perf_counter() - t0) / steps
+    Ab32 = Ab.cpu().numpy().reshape(len(insts), -1)
+    b.close()
+    # plain FP64 form on evenly spaced instances of the same sweep
+    picks = [round(k * (len(insts) - 1) / max(1, fp64_images - 1)) for k in range(fp64_images)]
+    t64 = 0.0
+    err = 0.0
+    for k in picks:
+        w = torch.as_tensor(waves[k]).cuda()
+        P.simulate(ctx, insts[k], w, pkg.FP64_PARITY)   # warm
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        buf = P.simulate(ctx, insts[k], w, pkg.FP64_PARITY)
+        torch.cuda.synchronize()
+        t64 += time.perf_counter() - t0
+        Di64 = buf.Di.cpu().numpy()
+        bright = Di64 > 1e-3 * Di64.max()
+        err = max(err, float(np.max(np.abs(Ab32[k][bright] - buf.Abs.cpu().numpy()[bright]))))
+    t64 /= len(picks)
+    terms = sum(W.terms(i) for i in insts)
+    log(f"bench: spectral FP32 batch {t32 * 1e3:.1f} ms ({len(insts) / t32:.1f} images/s), "
+        f"FP64 plain {t64 * 1e3:.1f} ms/image")
+    return {
+        "workload": f"cfg4 sweep: {len(insts)} wavenumbers (every {stride}th of 256, 900-4000 "
+                    f"cm^-1), 20 spheres a~U(2,15), 400 waves, 512x512 over 100 um, graph-captured "
+                    f"batch",
+        "images_per_s_fp32_batch": len(insts) / t32,
+        "terms_per_s_fp32_batch": terms / t32,
+        "images_per_s_fp64_plain": 1.0 / t64,
+        "speedup_vs_fp64_plain": t64 * len(insts) / t32,
+        "absorbance_max_abs_err": err,
+        "fp64_images_compared": picks,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +149,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-spectral", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -201,6 +261,8 @@ def main():
             "launches": ser_n,
         },
     }
+    if not args.no_spectral:
+        out["spectral"] = spectral(pkg, P, W, ctx, torch)
     if not args.no_cpu:
         log("bench: CPU baseline")
         A = b64.A.cpu().numpy()

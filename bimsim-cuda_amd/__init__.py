@@ -34,7 +34,7 @@ EXPORTS = (
     "bimsim_sphere_radial_lut", "bimsim_field_fft", "bimsim_field_bandpass",
     "bimsim_field_to_scalar", "bimsim_field_scale", "bimsim_field_crop",
     "bimsim_detector_resample", "bimsim_slice_ratio", "bimsim_batch_create",
-    "bimsim_batch_launch", "bimsim_batch_destroy", "bimsim_batch_run",
+    "bimsim_batch_launch", "bimsim_batch_destroy", "bimsim_batch_run", "bimsim_set_autotune",
 )
 
 
@@ -103,6 +103,7 @@ def lib():
         for name in ("bimsim_ctx_destroy", "bimsim_sync"):
             getattr(L, name).argtypes = [c_void_p]
         L.bimsim_set_timing.argtypes = [c_void_p, c_int]
+        L.bimsim_set_autotune.argtypes = [c_void_p, c_int]
         L.bimsim_get_timings.argtypes = [c_void_p, ctypes.POINTER(ctypes.c_float),
                                          ctypes.POINTER(c_int)]
         L.bimsim_sphere_order.argtypes = [c_double, c_double]
@@ -226,6 +227,9 @@ class Context:
 
     def sync(self):
         self._check(self.L.bimsim_sync(self.h))
+
+    def set_autotune(self, on):
+        self._check(self.L.bimsim_set_autotune(self.h, 1 if on else 0))
 
     def set_timing(self, on):
         self._check(self.L.bimsim_set_timing(self.h, 1 if on else 0))

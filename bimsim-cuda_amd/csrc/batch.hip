@@ -227,13 +227,16 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
     BIMSIM_TRY(dev_alloc(ctx, &b->d_Uf, fb * (size_t)field));
     BIMSIM_TRY(dev_alloc(ctx, &b->d_U, fb * (size_t)field));
     if (b->f32) {
+        // the series plans are timed on real inputs: coefficients first
         const size_t budget = (size_t)8 << 30;
+        BIMSIM_TRY(mie_batch_launch(ctx, b->d_sph, b->d_kmag, b->n_sph, b->lmax, b->quirks,
+                                    b->d_work, b->d_A, b->d_B));
         if (!mc.inst.empty()) {
-            BIMSIM_TRY(series_plan_create(ctx, mc, budget, &b->plan_mc));
+            BIMSIM_TRY(series_plan_create_tuned(ctx, mc, budget, b->waves, b->d_A, b->d_B, &b->plan_mc));
             b->have_mc = true;
         }
         if (!pw.inst.empty()) {
-            BIMSIM_TRY(series_plan_create(ctx, pw, budget, &b->plan_pw));
+            BIMSIM_TRY(series_plan_create_tuned(ctx, pw, budget, b->d_kdirs, b->d_A, b->d_B, &b->plan_pw));
             b->have_pw = true;
         }
     }

@@ -183,6 +183,12 @@ const char* bimsim_last_error(const bimsim_ctx* ctx) {
     return ctx ? ctx->err.c_str() : "null context";
 }
 
+int bimsim_set_autotune(bimsim_ctx* ctx, int enable) {
+    if (!ctx) return BIMSIM_E_INVALID;
+    ctx->autotune = enable != 0;
+    return BIMSIM_OK;
+}
+
 int bimsim_set_timing(bimsim_ctx* ctx, int enable) {
     if (!ctx) return BIMSIM_E_INVALID;
     ctx->timing = enable != 0;

@@ -37,6 +37,9 @@ struct bimsim_ctx {
     size_t lds_max = 65536;
     // last plan of the FP32 series (opaque, owned by us_series.hip)
     void* series_cache = nullptr;
+    // series launch shapes chosen by timing, per batch shape
+    bool autotune = true;
+    std::map<std::string, std::pair<int, int>> tune;
 };
 
 namespace bimsim {

@@ -331,7 +331,7 @@ extern "C" int bimsim_nf_us(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int
             batch.sph.push_back(S);
         }
         const SeriesPlan* plan = nullptr;
-        BIMSIM_TRY(series_cached_plan(ctx, batch, &plan));
+        BIMSIM_TRY(series_cached_plan(ctx, batch, waves, (const double2*)A, (const double2*)B, &plan));
         BIMSIM_TRY(series_plan_launch(ctx, *plan, waves, (const double2*)A, (const double2*)B,
                                      (float2*)U));
     }

@@ -56,6 +56,8 @@ struct SeriesBatch {
 
 struct SeriesPlan {
     int n_inst = 0, n_sph = 0;
+    int split_mult = 1;       // waves split into up to this many items per block
+    int groups_per_cu = 8;    // resident workgroups of the persistent launch
     int nk_max = 0, max_npix = 0;
     long long n_wave_rec = 0, n_part = 0;
     long long coef_elems = 0, rhat_elems = 0;   // per chunk (maximum)
@@ -83,7 +85,18 @@ struct SeriesPlan {
 // Chooses splits and chunks, builds and uploads the tables and allocates
 // the plan's buffers (coefficient storage at most `coef_budget` bytes per
 // chunk unless one instance needs more).  Synchronises the stream.
-int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, SeriesPlan* plan);
+// split_mult = 0 picks the split from the queue length; groups_per_cu sets
+// the persistent grid.
+int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, SeriesPlan* plan,
+                       int split_mult = 0, int groups_per_cu = 8);
+
+// As series_plan_create, but when the queue is short (a few items per CU)
+// the split and grid are chosen by timing the candidates on these inputs
+// (into a temporary field); the choice is kept in the context per batch
+// shape.  A, B and waves must be ready on the stream.
+int series_plan_create_tuned(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget,
+                             const double* waves, const double2* A, const double2* B,
+                             SeriesPlan* plan);
 void series_plan_destroy(SeriesPlan* plan);
 
 // Launches the plan on the context stream; allocation- and
@@ -96,7 +109,8 @@ int series_capacity(int nl1);
 
 // Plan for `batch`, reused while the batch tables are unchanged (the
 // context keeps its last plan; the key is the tables' bytes).
-int series_cached_plan(bimsim_ctx* ctx, SeriesBatch& batch, const SeriesPlan** out);
+int series_cached_plan(bimsim_ctx* ctx, SeriesBatch& batch, const double* waves, const double2* A,
+                       const double2* B, const SeriesPlan** out);
 void series_cache_release(bimsim_ctx* ctx);
 
 }  // namespace bimsim

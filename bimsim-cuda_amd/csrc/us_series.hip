@@ -15,7 +15,9 @@
 // Built with -ffp-contract=off so that the FP64 recurrences round like the
 // reference's; every FP32 FMA of the series is written out with fmaf.
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
+#include <string>
 
 #include "refmath.h"
 #include "us_series.h"
@@ -356,8 +358,10 @@ void series_plan_destroy(SeriesPlan* P) {
     *P = SeriesPlan();
 }
 
-int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, SeriesPlan* P) {
+int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, SeriesPlan* P,
+                       int split_mult, int groups_per_cu) {
     *P = SeriesPlan();
+    P->groups_per_cu = std::max(1, groups_per_cu);
     const int ni = (int)batch.inst.size();
     P->n_inst = ni;
     P->n_sph = (int)batch.sph.size();
@@ -393,13 +397,18 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
     // split keeps at least 32 waves)
     const long long target = 4LL * ctx->cu_count;
     int mult = 1;
-    for (;;) {
-        bool can = false;
-        for (int i = 0; i < ni; ++i)
-            if (batch.inst[i].nk_pad / (2 * mult) >= 32) can = true;
-        if (blocks_all * mult >= target || !can) break;
-        mult *= 2;
+    if (split_mult > 0) {
+        mult = split_mult;
+    } else {
+        for (;;) {
+            bool can = false;
+            for (int i = 0; i < ni; ++i)
+                if (batch.inst[i].nk_pad / (2 * mult) >= 32) can = true;
+            if (blocks_all * mult >= target || !can) break;
+            mult *= 2;
+        }
     }
+    P->split_mult = mult;
     long long woff = 0, poff = 0;
     for (int s = 0; s < P->n_sph; ++s) {
         batch.sph[s].woff = woff;
@@ -499,7 +508,7 @@ int series_plan_launch(bimsim_ctx* ctx, const SeriesPlan& P, const double* waves
         const int tmr = timer_begin(ctx, kStageSeries);
         for (const SeriesPlan::Bucket& K : ch.buckets) {
             // resident workgroups: the queue keeps them busy until it drains
-            const dim3 grid(std::min(K.item_count, 8 * ctx->cu_count));
+            const dim3 grid(std::min(K.item_count, P.groups_per_cu * ctx->cu_count));
             switch (K.L) {
 #define BIMSIM_CASE(LL) \
     case LL: launch_bucket<LL>(grid, st, P, K, U); break;
@@ -529,7 +538,103 @@ int series_plan_launch(bimsim_ctx* ctx, const SeriesPlan& P, const double* waves
     return BIMSIM_OK;
 }
 
-int series_cached_plan(bimsim_ctx* ctx, SeriesBatch& batch, const SeriesPlan** out) {
+namespace {
+
+// Shape of a batch for the tuning cache: per instance (bucket, pixels,
+// spheres, waves) and the total.
+std::string shape_key(const SeriesBatch& batch) {
+    std::string k;
+    char buf[96];
+    for (const SeriesInst& I : batch.inst) {
+        int nl1 = 1;
+        for (int s = I.s_begin; s < I.s_begin + I.ns; ++s) nl1 = std::max(nl1, batch.sph[s].s.Nl + 1);
+        std::snprintf(buf, sizeof buf, "%d:%d:%d:%d;", series_capacity(nl1), I.uR * I.vR, I.ns, I.nk);
+        k += buf;
+    }
+    return k;
+}
+
+}  // namespace
+
+int series_plan_create_tuned(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget,
+                             const double* waves, const double2* A, const double2* B, SeriesPlan* P) {
+    // the heuristic plan first: long queues keep it
+    SeriesBatch work = batch;
+    BIMSIM_TRY(series_plan_create(ctx, work, coef_budget, P));
+    long long items = 0;
+    for (const SeriesPlan::Chunk& ch : P->chunks)
+        for (const SeriesPlan::Bucket& K : ch.buckets) items += K.item_count;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    (void)hipStreamIsCapturing(ctx->stream, &cap);
+    if (!ctx->autotune || items > 16LL * ctx->cu_count || cap != hipStreamCaptureStatusNone) {
+        batch = work;
+        return BIMSIM_OK;
+    }
+    const std::string key = shape_key(batch);
+    auto hit = ctx->tune.find(key);
+    int best_mult = P->split_mult, best_gpc = P->groups_per_cu;
+    if (hit == ctx->tune.end()) {
+        // time the candidates into a temporary field
+        long long field = 0;
+        for (const SeriesInst& I : batch.inst) field = std::max(field, I.u_off + (long long)I.uR * I.vR);
+        float2* tmp = nullptr;
+        if (hipMalloc((void**)&tmp, sizeof(float2) * (size_t)std::max<long long>(1, field)) != hipSuccess)
+            return set_error(ctx, BIMSIM_E_NOMEM, "out of device memory for tuning");
+        hipEvent_t e0, e1;
+        (void)hipEventCreate(&e0);
+        (void)hipEventCreate(&e1);
+        const bool timing = ctx->timing;
+        ctx->timing = false;
+        float best = 1e30f;
+        int rc = BIMSIM_OK;
+        for (int mult : {1, 2, 4, 8, 16}) {
+            for (int gpc : {2, 4, 8}) {
+                SeriesBatch cand = batch;
+                for (SeriesInst& I : cand.inst) I.accumulate = 0;
+                SeriesPlan T;
+                rc = series_plan_create(ctx, cand, coef_budget, &T, mult, gpc);
+                if (rc != BIMSIM_OK) break;
+                if (mult > 1 && T.split_insts.empty()) {   // no instance splits further
+                    series_plan_destroy(&T);
+                    continue;
+                }
+                rc = series_plan_launch(ctx, T, waves, A, B, tmp);   // warm
+                (void)hipEventRecord(e0, ctx->stream);
+                for (int r = 0; r < 2 && rc == BIMSIM_OK; ++r) rc = series_plan_launch(ctx, T, waves, A, B, tmp);
+                (void)hipEventRecord(e1, ctx->stream);
+                (void)hipEventSynchronize(e1);
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                series_plan_destroy(&T);
+                if (rc != BIMSIM_OK) break;
+                if (ms < best) {
+                    best = ms;
+                    best_mult = mult;
+                    best_gpc = gpc;
+                }
+            }
+            if (rc != BIMSIM_OK) break;
+        }
+        ctx->timing = timing;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipFree(tmp);
+        BIMSIM_TRY(rc);
+        ctx->tune[key] = std::make_pair(best_mult, best_gpc);
+    } else {
+        best_mult = hit->second.first;
+        best_gpc = hit->second.second;
+    }
+    if (best_mult == P->split_mult && best_gpc == P->groups_per_cu) {
+        batch = work;
+        return BIMSIM_OK;
+    }
+    series_plan_destroy(P);
+    return series_plan_create(ctx, batch, coef_budget, P, best_mult, best_gpc);
+}
+
+int series_cached_plan(bimsim_ctx* ctx, SeriesBatch& batch, const double* waves, const double2* A,
+                       const double2* B, const SeriesPlan** out) {
     std::vector<char> key(sizeof(SeriesInst) * batch.inst.size() + sizeof(SeriesSph) * batch.sph.size());
     if (!batch.inst.empty())
         std::memcpy(key.data(), batch.inst.data(), sizeof(SeriesInst) * batch.inst.size());
@@ -549,7 +654,7 @@ int series_cached_plan(bimsim_ctx* ctx, SeriesBatch& batch, const SeriesPlan** o
         ctx->series_cache = c;
     }
     c->key.clear();
-    BIMSIM_TRY(series_plan_create(ctx, batch, (size_t)8 << 30, &c->plan));
+    BIMSIM_TRY(series_plan_create_tuned(ctx, batch, (size_t)8 << 30, waves, A, B, &c->plan));
     c->key = key;
     *out = &c->plan;
     return BIMSIM_OK;

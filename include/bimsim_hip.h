@@ -135,6 +135,12 @@ int bimsim_abi_version(void);
  * ms[0] incident field, ms[1] scattered field (whole bimsim_nf_us),
  * ms[2] detector, ms[3] scattered-field series kernel launches only. */
 int bimsim_set_timing(bimsim_ctx* ctx, int enable);
+
+/* Launch-shape tuning of the FP32 scattered-field series (on by default):
+ * when a call's work queue is short, the first call of each shape times the
+ * candidate wave splits and grid sizes on its own inputs and keeps the
+ * fastest for the context.  No counterpart in the reference. */
+int bimsim_set_autotune(bimsim_ctx* ctx, int enable);
 int bimsim_get_timings(bimsim_ctx* ctx, float* ms, int* counts);
 
 /* ------------------------------------------------------------ host helpers */
