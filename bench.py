@@ -28,7 +28,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 METRIC = "Mie series terms/s + absorbance images/s, batched FP32 vs plain FP64, 1 GPU"
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector, AMD published
-PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "us_series_f32_cfg2_traffic.json")
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "series_f32_cfg2_traffic.md")
 
 
 def load_pkg():
@@ -150,6 +150,9 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-spectral", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one device-to-device copy of 256 MiB (the reference "
+                         "dispatch for scaling HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -161,6 +164,15 @@ def main():
     pkg = load_pkg()
     import bimsim_amd.pipeline as P
     import bimsim_amd.workloads as W
+
+    if args.calibrate_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("bench: calibration copy of 256 MiB done")
+        del src, dst
 
     inst = W.config(2)
     waves_h = W.mc_waves(inst, pkg.mc_sample_na)
@@ -215,10 +227,13 @@ def main():
     U64 = b64.U.cpu().numpy()
     field_err = float(np.max(np.abs(U32 - U64)) / np.max(np.abs(U64)))
 
+    # HBM bytes per launch of the series kernel from the committed PMC summary
     traffic = None
     if os.path.exists(PROFILE_TRAFFIC):
         with open(PROFILE_TRAFFIC) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch ="):
+                    traffic = int(line.split("=")[1])
 
     achieved = series_flops / (avg_series_ms * 1e-3) / 1e12 if avg_series_ms > 0 else None
     out = {

@@ -464,8 +464,10 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
                 if (bucket[i] != L) continue;
                 const SeriesInst& I = batch.inst[i];
                 const int npix = I.uR * I.vR;
-                for (int t = 0; t < I.nsplit; ++t)
-                    for (int p0 = 0; p0 < npix; p0 += kThreads) items.push_back(SeriesItem{i, p0, t, 0});
+                // the splits of a pixel block follow each other, so that they
+                // run side by side and share its coefficients in L2
+                for (int p0 = 0; p0 < npix; p0 += kThreads)
+                    for (int t = 0; t < I.nsplit; ++t) items.push_back(SeriesItem{i, p0, t, 0});
             }
             const int count = (int)items.size() - begin;
             if (count > 0) ch.buckets.push_back(SeriesPlan::Bucket{L, begin, count, P->n_counters++});
