@@ -89,6 +89,33 @@ def cpu_baseline(pkg, W, inst, waves, A, B, budget_s=12.0):
     }
 
 
+def harmonic_flops(Nl):
+    """FLOPs of the harmonic series kernel per (pixel, sphere): per pair
+    (m, l >= m) a Legendre step (4, none on the diagonal), the record
+    rotation by e^{i m phi} (6) and the accumulation (4); per m the next
+    diagonal value and phase (8); per l the product with c'_l (8)."""
+    n = Nl + 1
+    return 14 * n * (n + 1) // 2 + 12 * n + 8
+
+
+def direct_flops_per(Nl, nk):
+    """FLOPs of the direct series kernel per (pixel, sphere): per (wave, l)
+    the Legendre step and complex accumulation (8), per wave the phase and
+    cos(theta) (13)."""
+    return 8 * (Nl + 1) * nk + 13 * nk
+
+
+def series_kernel_ms(ctx, P, pkg, inst, waves, buf, steps):
+    ctx.set_timing(True)
+    ctx.timings()
+    for _ in range(steps):
+        P.simulate(ctx, inst, waves, pkg.FP32_FAST, buf)
+    tm = ctx.timings()
+    ctx.set_timing(False)
+    ms, n = tm["us_series"]
+    return ms / max(n, 1), n
+
+
 def spectral(pkg, P, W, ctx, torch, stride=16, steps=3, fp64_images=2):
     """Batched FP32 spectral imaging (one captured batch) against the plain
     FP64 form, images/s and absorbance error on the same instances."""
@@ -182,8 +209,11 @@ def main():
     b64 = P.Buffers(inst, pkg.FP64_PARITY)
     terms = W.terms(inst)
     n2 = inst["N"] ** 2
-    series_flops = n2 * sum(8 * (s["Nl"] + 1) * inst["nk"] + 13 * inst["nk"]
-                            for s in inst["spheres"])
+    series_flops = n2 * sum(harmonic_flops(s["Nl"]) for s in inst["spheres"])
+    direct_flops = n2 * sum(direct_flops_per(s["Nl"], inst["nk"]) for s in inst["spheres"])
+    # the headline runs the harmonic form, the choice the library makes for
+    # Monte-Carlo illumination (every instance with >= 16 waves)
+    ctx.set_series_algorithm(pkg.SERIES_HARMONIC)
 
     log(f"bench: workload cfg2 N={inst['N']} nk={inst['nk']} "
         f"Nl={[s['Nl'] for s in inst['spheres']]} m={inst['m']} terms/image={terms:.3e}")
@@ -192,19 +222,24 @@ def main():
     torch.cuda.synchronize()
     log("bench: warmup done")
 
-    ctx.set_timing(True)
-    ctx.timings()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         P.simulate(ctx, inst, waves, pkg.FP32_FAST, b32)
     torch.cuda.synchronize()
     t32 = (time.perf_counter() - t0) / args.steps
-    tm = ctx.timings()
-    ctx.set_timing(False)
-    ser_ms, ser_n = tm["us_series"]
-    avg_series_ms = ser_ms / max(ser_n, 1)
-    log(f"bench: FP32 {t32 * 1e3:.3f} ms/image, series kernel {avg_series_ms:.3f} ms/launch")
+    # the series kernel's own duration: HIP events on the library's stream
+    # around its launches, in a separate pass so the headline carries no events
+    avg_series_ms, ser_n = series_kernel_ms(ctx, P, pkg, inst, waves, b32, args.steps)
+    log(f"bench: FP32 {t32 * 1e3:.3f} ms/image, harmonic series kernel {avg_series_ms:.3f} ms/launch")
+    # the direct form beside it (same image, fewer steps)
+    ctx.set_series_algorithm(pkg.SERIES_DIRECT)
+    P.simulate(ctx, inst, waves, pkg.FP32_FAST, b32)
+    direct_ms, _ = series_kernel_ms(ctx, P, pkg, inst, waves, b32, max(1, args.steps // 3))
+    ctx.set_series_algorithm(pkg.SERIES_HARMONIC)
+    P.simulate(ctx, inst, waves, pkg.FP32_FAST, b32)
+    torch.cuda.synchronize()
+    log(f"bench: direct series kernel {direct_ms:.3f} ms/launch")
 
     # plain FP64 form (reference structure), fewer steps
     P.simulate(ctx, inst, waves, pkg.FP64_PARITY, b64)
@@ -270,10 +305,16 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS if achieved else None,
             "traffic": traffic,
-            "kernel": "us_series_f32",
+            "kernel": "series_harm_kernel<32> (harmonic FP32 scattered-field series)",
             "flops_per_launch": series_flops,
             "avg_launch_ms": avg_series_ms,
             "launches": ser_n,
+        },
+        "series_direct_form": {
+            "kernel": "series_f32_kernel<32>",
+            "avg_launch_ms": direct_ms,
+            "flops_per_launch": direct_flops,
+            "achieved_tflops": direct_flops / (direct_ms * 1e-3) / 1e12 if direct_ms > 0 else None,
         },
     }
     if not args.no_spectral:

@@ -21,6 +21,7 @@ QUIRKS_OFF, QUIRKS_REF = 0, 1
 UF_SERIES, UF_PLANE, UF_LUT = 0, 1, 2
 SLICE_INTENSITY, SLICE_MAGNITUDE, SLICE_REAL, SLICE_IMAG = 0, 1, 2, 3
 RATIO_ABSORBANCE, RATIO_TRANSMITTANCE = 0, 1
+SERIES_AUTO, SERIES_DIRECT, SERIES_HARMONIC = 0, 1, 2
 US_SERIES, US_LUT = 0, 1
 RNG_GLIBC, RNG_SPLITMIX64 = 0, 1
 PI_F = 3.1415901184082031   # float(3.14159f), reference dataTypes.h:17
@@ -35,6 +36,7 @@ EXPORTS = (
     "bimsim_field_to_scalar", "bimsim_field_scale", "bimsim_field_crop",
     "bimsim_detector_resample", "bimsim_slice_ratio", "bimsim_batch_create",
     "bimsim_batch_launch", "bimsim_batch_destroy", "bimsim_batch_run", "bimsim_set_autotune",
+    "bimsim_set_series_algorithm",
 )
 
 
@@ -104,6 +106,7 @@ def lib():
             getattr(L, name).argtypes = [c_void_p]
         L.bimsim_set_timing.argtypes = [c_void_p, c_int]
         L.bimsim_set_autotune.argtypes = [c_void_p, c_int]
+        L.bimsim_set_series_algorithm.argtypes = [c_void_p, c_int]
         L.bimsim_get_timings.argtypes = [c_void_p, ctypes.POINTER(ctypes.c_float),
                                          ctypes.POINTER(c_int)]
         L.bimsim_sphere_order.argtypes = [c_double, c_double]
@@ -227,6 +230,10 @@ class Context:
 
     def sync(self):
         self._check(self.L.bimsim_sync(self.h))
+
+    def set_series_algorithm(self, algo):
+        """SERIES_AUTO, SERIES_DIRECT or SERIES_HARMONIC (FP32 scattered field)."""
+        self._check(self.L.bimsim_set_series_algorithm(self.h, algo))
 
     def set_autotune(self, on):
         self._check(self.L.bimsim_set_autotune(self.h, 1 if on else 0))

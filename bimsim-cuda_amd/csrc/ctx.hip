@@ -183,6 +183,14 @@ const char* bimsim_last_error(const bimsim_ctx* ctx) {
     return ctx ? ctx->err.c_str() : "null context";
 }
 
+int bimsim_set_series_algorithm(bimsim_ctx* ctx, int algo) {
+    if (!ctx) return BIMSIM_E_INVALID;
+    BIMSIM_CHECK_ARG(ctx, algo >= BIMSIM_SERIES_AUTO && algo <= BIMSIM_SERIES_HARMONIC,
+                     "unknown series algorithm");
+    ctx->series_algo = algo;
+    return BIMSIM_OK;
+}
+
 int bimsim_set_autotune(bimsim_ctx* ctx, int enable) {
     if (!ctx) return BIMSIM_E_INVALID;
     ctx->autotune = enable != 0;

@@ -39,6 +39,7 @@ struct bimsim_ctx {
     void* series_cache = nullptr;
     // series launch shapes chosen by timing, per batch shape
     bool autotune = true;
+    int series_algo = 0;   // FP32 series: 0 auto, 1 direct, 2 harmonic
     std::map<std::string, std::pair<int, int>> tune;
 };
 

@@ -28,19 +28,24 @@ struct SeriesInst {
     int nk, nk_pad;     // waves, padded to a multiple of 2
     int wchunk, nsplit; // waves per split, number of splits
     int accumulate;     // U (+)= result
-    int pad0;
+    int lh;             // harmonic form: orders of the instance's Legendre table
     long long wave_off; // first wave (rows of 3 doubles) of the instance
     long long u_off;    // element offset of the instance's field in U
     long long part_off; // element offset of its split partial sums
+    long long pl_off;   // harmonic form: its normalised Legendre table
+    long long e_off;    // harmonic form: its azimuth phase table
 };
 
 struct SeriesSph {
     SphereDev s;
     int inst;           // owning instance
     int nb;             // blocks of 8 orders: ceil((Nl + 1) / 8)
-    long long woff;     // offset of its WaveF records
+    long long woff;     // offset of its WaveF records (and wave weights)
     long long coff;     // offset of c'[l][pix] in the chunk's coefficient buffer
     long long roff;     // offset of r-hat[pix] in the chunk's r-hat buffer
+    long long toff;     // harmonic form: offset of its harmonic records
+    int nchunk;         // harmonic form: wave chunks of its harmonic sums
+    int tl;             // harmonic form: record stride (its instance's order bucket)
 };
 
 struct SeriesItem {
@@ -54,8 +59,12 @@ struct SeriesBatch {
     std::vector<SeriesSph> sph;
 };
 
+// Algorithms of the FP32 series.
+enum SeriesAlgo { kSeriesAuto = 0, kSeriesDirect = 1, kSeriesHarmonic = 2 };
+
 struct SeriesPlan {
     int n_inst = 0, n_sph = 0;
+    bool harmonic = false;    // addition-theorem form
     int split_mult = 1;       // waves split into up to this many items per block
     int groups_per_cu = 8;    // resident workgroups of the persistent launch
     int nk_max = 0, max_npix = 0;
@@ -80,6 +89,18 @@ struct SeriesPlan {
     float2* d_part = nullptr;
     float2* d_coef = nullptr;
     float4* d_rhat = nullptr;
+    // harmonic form
+    long long n_pl = 0, n_e = 0, n_rec = 0, n_tpart = 0, n_z = 0;
+    int max_lh = 0, max_tl = 0;
+    int2* d_zmap = nullptr;       // (sphere, chunk) of every partial harmonic sum
+    long long* d_zpart = nullptr; // offset of each partial sum's (m, l) block
+    long long* d_zfirst = nullptr;// first partial sum of each sphere
+    double* d_pl = nullptr;       // normalised Legendre values per (m, wave, l)
+    double2* d_e = nullptr;       // exp(i m phi_k) per (m, wave)
+    double2* d_phi = nullptr;     // wave weights (1/nk) subA exp(i k k.c) per sphere
+    double2* d_tpart = nullptr;   // partial harmonic sums per (sphere, chunk, m, l)
+    float4* d_rec = nullptr;      // harmonic records per (sphere, m, l)
+    float2* d_ab = nullptr;       // Legendre recurrence constants per (m, l)
 };
 
 // Chooses splits and chunks, builds and uploads the tables and allocates

@@ -303,6 +303,229 @@ __global__ void series_reduce_kernel(const SeriesInst* __restrict__ insts,
     }
 }
 
+// ----------------------------------------------------------- harmonic form
+// By the addition theorem P_l(k.r) = 4 pi/(2l+1) sum_m Y_lm(k) Y*_lm(r), the
+// wave sum moves onto the sphere:
+//   S(p) = sum_l c_l(p) sum_{m=0..l} Pn_lm(cos th_r) G_lm(phi_r),
+//   G_lm = e^{-i m phi_r} T+_lm + e^{i m phi_r} T-_lm  (m > 0),  G_l0 = T_l0,
+//   T+-_lm = 4 pi/(2l+1) sum_w phi_w Pn_lm(cos th_w) e^{+-i m phi_w},
+// Pn_lm the orthonormal associated Legendre functions (the sign convention
+// cancels in each product).  T is formed once per sphere in FP64 and kept as
+// U1 = T+ + T-, U2 = (Im T+ - Im T-, Re T- - Re T+) (U1 = T, U2 = 0 at
+// m = 0), divided by s_l so that the stored c'_l = s_l c_l apply.
+constexpr int kAB = 113;            // stride of the recurrence-constant table (m, l <= 112)
+
+// Orthonormal Pn_lm(x) for l = m .. lh-1 of one wave into out[l - m]
+// (FP64): Pn_00 = 1/sqrt(4 pi), Pn_mm = sqrt((2m+1)/(2m)) s Pn_{m-1,m-1},
+// Pn_lm = a_lm (x Pn_{l-1,m} - b_lm Pn_{l-2,m}).
+__device__ inline void legendre_column(int m, int lh, double x, double* out, long long stride) {
+    const double st = sqrt(fmax(0.0, 1.0 - x * x));
+    double pmm = 0.28209479177387814;   // 1/sqrt(4 pi)
+    for (int k = 1; k <= m; ++k) pmm = sqrt((2.0 * k + 1.0) / (2.0 * k)) * st * pmm;
+    double p0 = 0.0, p1 = pmm;
+    out[0] = pmm;
+    for (int l = m + 1; l < lh; ++l) {
+        const double a = sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
+        const double b = sqrt((((double)l - 1.0) * (l - 1.0) - (double)m * m) /
+                              (4.0 * (l - 1.0) * (l - 1.0) - 1.0));
+        const double p = a * (x * p1 - b * p0);
+        p0 = p1;
+        p1 = p;
+        out[(long long)(l - m) * stride] = p;
+    }
+}
+
+// Offset of column m in an instance's triangular Legendre table.
+__host__ __device__ inline long long tri_off(int m, int lh) {
+    return (long long)m * lh - (long long)m * (m - 1) / 2;
+}
+
+// Pn_lm(cos th_w) in pl[pl_off + (tri_off(m) + (l - m)) * nk + w] and
+// e^{i m phi_w} in e[e_off + m * nk + w].  Grid: (wave blocks, instances, m).
+__global__ void harm_legendre_kernel(const SeriesInst* __restrict__ insts, int n_inst,
+                                     const double* __restrict__ waves, double* __restrict__ pl,
+                                     double2* __restrict__ e) {
+    const int m = blockIdx.z;
+    for (int i = blockIdx.y; i < n_inst; i += gridDim.y) {
+        const SeriesInst& I = insts[i];
+        if (m >= I.lh) continue;
+        for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < I.nk; w += gridDim.x * blockDim.x) {
+            const d3 k = mk(waves + 3 * (I.wave_off + w));
+            legendre_column(m, I.lh, k.z, pl + I.pl_off + tri_off(m, I.lh) * I.nk + w, I.nk);
+            const double phi = atan2(k.y, k.x);
+            e[I.e_off + (long long)m * I.nk + w] = make_double2(cos(m * phi), sin(m * phi));
+        }
+    }
+}
+
+// Wave weights (FP64): phi_w = (1/nk) subA exp(i kmag k.c).
+__global__ void harm_phase_kernel(const SeriesInst* __restrict__ insts,
+                                  const SeriesSph* __restrict__ sph, int n_sph,
+                                  const double* __restrict__ waves, double2* __restrict__ phi) {
+    for (int s = blockIdx.y; s < n_sph; s += gridDim.y) {
+        const SeriesSph& SS = sph[s];
+        const SeriesInst& I = insts[SS.inst];
+        for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < I.nk; w += gridDim.x * blockDim.x) {
+            const d3 k = mk(waves + 3 * (I.wave_off + w));
+            const d3 c = sub(mk(SS.s.p), I.f);
+            const dc ph = cscl(cexp_i(I.kmag * dot(k, c)), I.sA);
+            phi[SS.woff + w] = make_double2(ph.r, ph.i);
+        }
+    }
+}
+
+// Partial sums over one chunk of waves: thread l of block (m, z), z an entry
+// of zmap = (sphere, chunk), writes (T+, T-) before the 4 pi/(2l+1) factor
+// at part[(z * tl + m) * tl + l] (tl = the sphere's record stride).
+__global__ void harm_tsum_kernel(const SeriesInst* __restrict__ insts,
+                                 const SeriesSph* __restrict__ sph, const int2* __restrict__ zmap,
+                                 long long n_z, const long long* __restrict__ zpart,
+                                 const double* __restrict__ pl, const double2* __restrict__ e,
+                                 const double2* __restrict__ phi, double2* __restrict__ part) {
+    const int l = threadIdx.x;
+    const int m = blockIdx.x;
+    for (long long z = blockIdx.y; z < n_z; z += gridDim.y) {
+        const int2 sc = zmap[z];
+        const SeriesSph& SS = sph[sc.x];
+        const SeriesInst& I = insts[SS.inst];
+        const int nl1 = SS.s.Nl + 1;
+        if (m >= nl1 || l < m || l >= nl1) continue;
+        const int chunk_waves = (I.nk + SS.nchunk - 1) / SS.nchunk;
+        const int w0 = sc.y * chunk_waves, w1 = min(I.nk, w0 + chunk_waves);
+        dc tp = {0.0, 0.0}, tm = {0.0, 0.0};
+        const double* P = pl + I.pl_off + (tri_off(m, I.lh) + (l - m)) * I.nk;
+        const double2* E = e + I.e_off + (long long)m * I.nk;
+        const double2* F = phi + SS.woff;
+        for (int w = w0; w < w1; ++w) {
+            const double p = P[w];
+            const double2 ew = E[w];
+            const double2 fw = F[w];
+            const dc a = cscl(dc{fw.x, fw.y}, p);
+            tp = cadd(tp, cmul(a, dc{ew.x, ew.y}));
+            tm = cadd(tm, cmul(a, dc{ew.x, -ew.y}));
+        }
+        const int tl = SS.tl;
+        double2* o = part + (zpart[z] + (long long)m * tl + l) * 2;
+        o[0] = make_double2(tp.r, tp.i);
+        o[1] = make_double2(tm.r, tm.i);
+    }
+}
+
+// Records per (sphere, m, l): the chunk sums, 4 pi/(2l+1) / s_l, U1 and U2,
+// at rec[toff + m * tl + l].  Grid: (m, spheres), threads l.
+__global__ void harm_records_kernel(const SeriesSph* __restrict__ sph, int n_sph,
+                                    const long long* __restrict__ zfirst,
+                                    const long long* __restrict__ zpart,
+                                    const double2* __restrict__ part, float4* __restrict__ rec) {
+    const int l = threadIdx.x;
+    const int m = blockIdx.x;
+    for (int s = blockIdx.y; s < n_sph; s += gridDim.y) {
+        const SeriesSph& SS = sph[s];
+        const int tl = SS.tl;
+        if (l >= tl || m >= tl) continue;
+        float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+        const int nl1 = SS.s.Nl + 1;
+        if (m < nl1 && l >= m && l < nl1) {
+            dc tp = {0.0, 0.0}, tm = {0.0, 0.0};
+            for (int c = 0; c < SS.nchunk; ++c) {
+                const double2* o = part + (zpart[zfirst[s] + c] + (long long)m * tl + l) * 2;
+                tp = cadd(tp, dc{o[0].x, o[0].y});
+                tm = cadd(tm, dc{o[1].x, o[1].y});
+            }
+            double sl = 1.0;
+            for (int j = 1; j <= l; ++j) sl = sl * (2.0 * j - 1.0) / (2.0 * j);
+            const double f = 4.0 * M_PI / (2.0 * l + 1.0) / sl;
+            if (m == 0)
+                r = make_float4((float)(f * tp.r), (float)(f * tp.i), 0.f, 0.f);
+            else
+                r = make_float4((float)(f * (tp.r + tm.r)), (float)(f * (tp.i + tm.i)),
+                                (float)(f * (tp.i - tm.i)), (float)(f * (tm.r - tp.r)));
+        }
+        rec[SS.toff + (long long)m * tl + l] = r;
+    }
+}
+
+// Per-pixel harmonic sum, persistent over (instance, pixel block) items.
+template <int L>
+__global__ __launch_bounds__(kThreads) void series_harm_kernel(
+    const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph,
+    const SeriesItem* __restrict__ items, int n_items, const float2* __restrict__ cp,
+    const float4* __restrict__ rh, const float4* __restrict__ rec, const float2* __restrict__ ab,
+    float2* __restrict__ U, int* __restrict__ counter) {
+    __shared__ int s_item;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();
+        if (it >= n_items) break;
+        const SeriesItem item = items[it];
+        const SeriesInst& I = insts[item.inst];
+        const int npix = I.uR * I.vR;
+        const int pix = item.pix0 + (int)threadIdx.x;
+        if (pix >= npix) continue;
+        float acc_r = 0.f, acc_i = 0.f;
+        for (int s = I.s_begin; s < I.s_begin + I.ns; ++s) {
+            const SeriesSph& SS = sph[s];
+            const int nl1 = SS.s.Nl + 1;
+            const float4 r = rh[SS.roff + pix];
+            const float x = r.z;
+            const float st = sqrtf(fmaxf(0.f, fmaf(-x, x, 1.f)));
+            float z1r = 1.f, z1i = 0.f;
+            if (st > 1e-30f) {
+                z1r = r.x / st;
+                z1i = r.y / st;
+            }
+            float ar[L], ai[L];
+#pragma unroll
+            for (int l = 0; l < L; ++l) ar[l] = ai[l] = 0.f;
+            const float4* R = rec + SS.toff;   // stride L: the instance's bucket
+            float zr = 1.f, zi = 0.f;
+            float pmm = 0.28209479177387814f;
+            for (int m = 0; m < nl1; ++m) {
+                float p0 = 0.f, p1 = pmm;
+                const float4* Rm = R + m * L;
+                const float2* ABm = ab + m * kAB;
+#pragma unroll
+                for (int l = 0; l < L; ++l) {
+                    if (l >= m && l < nl1) {
+                        float p = p1;
+                        if (l > m) {
+                            const float2 c = ABm[l];
+                            p = c.x * fmaf(x, p1, -(c.y * p0));
+                            p0 = p1;
+                            p1 = p;
+                        }
+                        const float4 u = Rm[l];
+                        const float gr = fmaf(zi, u.z, zr * u.x);
+                        const float gi = fmaf(zi, u.w, zr * u.y);
+                        ar[l] = fmaf(p, gr, ar[l]);
+                        ai[l] = fmaf(p, gi, ai[l]);
+                    }
+                }
+                // the next diagonal term and e^{i (m+1) phi}
+                pmm = ab[(m + 1) * kAB + (m + 1)].x * st * pmm;
+                const float t = zr * z1r - zi * z1i;
+                zi = fmaf(zr, z1i, zi * z1r);
+                zr = t;
+            }
+            const float2* c = cp + SS.coff + pix;
+#pragma unroll
+            for (int l = 0; l < L; ++l) {
+                if (l < nl1) {
+                    const float2 v = c[(size_t)l * npix];
+                    acc_r = fmaf(v.x, ar[l], fmaf(-v.y, ai[l], acc_r));
+                    acc_i = fmaf(v.x, ai[l], fmaf(v.y, ar[l], acc_i));
+                }
+            }
+        }
+        float2* o = U + I.u_off + pix;
+        float2 v = make_float2(0.f, 0.f);
+        if (I.accumulate) v = *o;
+        *o = make_float2(v.x + acc_r, v.y + acc_i);
+    }
+}
+
 __global__ void zero_ints(int* p, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
@@ -313,6 +536,14 @@ void launch_bucket(dim3 grid, hipStream_t st, const SeriesPlan& P, const SeriesP
     hipLaunchKernelGGL(series_f32_kernel<L>, grid, dim3(kThreads), 0, st, P.d_inst, P.d_sph,
                        P.d_items + K.item_begin, K.item_count, P.d_coef, P.d_rhat, P.d_wt, U,
                        P.d_part, P.d_counters + K.counter);
+}
+
+template <int L>
+void launch_harm(dim3 grid, hipStream_t st, const SeriesPlan& P, const SeriesPlan::Bucket& K,
+                 float2* U) {
+    hipLaunchKernelGGL(series_harm_kernel<L>, grid, dim3(kThreads), 0, st, P.d_inst, P.d_sph,
+                       P.d_items + K.item_begin, K.item_count, P.d_coef, P.d_rhat, P.d_rec, P.d_ab,
+                       U, P.d_counters + K.counter);
 }
 
 template <class T>
@@ -355,6 +586,9 @@ void series_plan_destroy(SeriesPlan* P) {
     (void)hipFree(P->d_part);
     (void)hipFree(P->d_coef);
     (void)hipFree(P->d_rhat);
+    void* harm[] = {P->d_pl, P->d_e, P->d_phi, P->d_tpart, P->d_rec, P->d_ab, P->d_zmap,
+                    P->d_zpart, P->d_zfirst};
+    for (void* q : harm) (void)hipFree(q);
     *P = SeriesPlan();
 }
 
@@ -393,6 +627,13 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
         P->max_npix = std::max(P->max_npix, npix);
     }
     if (next_sph != P->n_sph) return set_error(ctx, BIMSIM_E_INVALID, "unused spheres in batch");
+    // the harmonic form replaces the wave loop by (Nl+1)(Nl+2)/2 terms per
+    // pixel and sphere: chosen unless some instance has very few waves
+    int min_nk = 1 << 30;
+    for (const SeriesInst& I : batch.inst) min_nk = std::min(min_nk, I.nk);
+    P->harmonic = ctx->series_algo == kSeriesHarmonic ||
+                  (ctx->series_algo == kSeriesAuto && min_nk >= 16);
+    if (P->harmonic) split_mult = 1;
     // split the waves until the queue holds several items per CU (each
     // split keeps at least 32 waves)
     const long long target = 4LL * ctx->cu_count;
@@ -475,9 +716,73 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
         P->chunks.push_back(ch);
         i0 = i1;
     }
+    std::vector<int2> zmap;
+    std::vector<long long> zpart, zfirst;
+    std::vector<float2> ab;
+    if (P->harmonic) {
+        long long npl = 0, ne = 0, nrec = 0, npairs = 0;
+        for (int i = 0; i < ni; ++i) {
+            SeriesInst& I = batch.inst[i];
+            int lh = 1;
+            for (int t = I.s_begin; t < I.s_begin + I.ns; ++t) lh = std::max(lh, batch.sph[t].s.Nl + 1);
+            I.lh = lh;
+            I.pl_off = npl;
+            npl += (long long)lh * (lh + 1) / 2 * I.nk;
+            I.e_off = ne;
+            ne += (long long)lh * I.nk;
+            P->max_lh = std::max(P->max_lh, lh);
+        }
+        long long sum_tri = 0;
+        for (int t = 0; t < P->n_sph; ++t) sum_tri += (long long)bucket[batch.sph[t].inst] * bucket[batch.sph[t].inst] / 2;
+        for (int t = 0; t < P->n_sph; ++t) {
+            SeriesSph& S = batch.sph[t];
+            const SeriesInst& I = batch.inst[S.inst];
+            S.tl = bucket[S.inst];
+            P->max_tl = std::max(P->max_tl, S.tl);
+            S.toff = nrec;
+            nrec += (long long)S.tl * S.tl;
+            // enough partial sums to occupy the GPU, at least 32 waves each
+            const long long want = ((1LL << 20) + sum_tri - 1) / std::max<long long>(1, sum_tri);
+            S.nchunk = (int)std::max<long long>(1, std::min<long long>(want, (I.nk + 31) / 32));
+            zfirst.push_back((long long)zmap.size());
+            for (int c = 0; c < S.nchunk; ++c) {
+                zmap.push_back(make_int2(t, c));
+                zpart.push_back(npairs);
+                npairs += (long long)S.tl * S.tl;
+            }
+        }
+        P->n_pl = npl;
+        P->n_e = ne;
+        P->n_rec = nrec;
+        P->n_tpart = 2 * npairs;
+        P->n_z = (long long)zmap.size();
+        ab.assign((size_t)kAB * kAB, make_float2(0.f, 0.f));
+        for (int m = 0; m < kAB; ++m) {
+            if (m >= 1) ab[(size_t)m * kAB + m] = make_float2((float)std::sqrt((2.0 * m + 1.0) / (2.0 * m)), 0.f);
+            for (int l = m + 1; l < kAB; ++l) {
+                const double a = std::sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
+                const double b = std::sqrt(std::max(0.0, ((l - 1.0) * (l - 1.0) - (double)m * m) /
+                                                             (4.0 * (l - 1.0) * (l - 1.0) - 1.0)));
+                ab[(size_t)m * kAB + l] = make_float2((float)a, (float)b);
+            }
+        }
+    }
     BIMSIM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     BIMSIM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     int rc;
+    if (P->harmonic &&
+        ((rc = upload(ctx, &P->d_zmap, zmap)) != BIMSIM_OK ||
+         (rc = upload(ctx, &P->d_zpart, zpart)) != BIMSIM_OK ||
+         (rc = upload(ctx, &P->d_zfirst, zfirst)) != BIMSIM_OK ||
+         (rc = upload(ctx, &P->d_ab, ab)) != BIMSIM_OK ||
+         (rc = alloc(ctx, &P->d_pl, P->n_pl)) != BIMSIM_OK ||
+         (rc = alloc(ctx, &P->d_e, P->n_e)) != BIMSIM_OK ||
+         (rc = alloc(ctx, &P->d_phi, std::max<long long>(1, P->n_wave_rec))) != BIMSIM_OK ||
+         (rc = alloc(ctx, &P->d_tpart, P->n_tpart)) != BIMSIM_OK ||
+         (rc = alloc(ctx, &P->d_rec, P->n_rec)) != BIMSIM_OK)) {
+        series_plan_destroy(P);
+        return rc;
+    }
     if ((rc = upload(ctx, &P->d_inst, batch.inst)) != BIMSIM_OK ||
         (rc = upload(ctx, &P->d_sph, batch.sph)) != BIMSIM_OK ||
         (rc = upload(ctx, &P->d_items, items)) != BIMSIM_OK ||
@@ -499,8 +804,21 @@ int series_plan_launch(bimsim_ctx* ctx, const SeriesPlan& P, const double* waves
     const hipStream_t st = ctx->stream;
     hipLaunchKernelGGL(zero_ints, dim3(1), dim3(256), 0, st, P.d_counters, std::max(1, P.n_counters));
     const dim3 wgrid((P.nk_max + 255) / 256, std::min(P.n_sph, 65535));
-    hipLaunchKernelGGL(series_waves_kernel, wgrid, dim3(256), 0, st, P.d_inst, P.d_sph, P.n_sph,
-                       waves, P.d_wt);
+    if (P.harmonic) {
+        // per-wave Legendre values and phases, then the per-sphere sums T
+        hipLaunchKernelGGL(harm_legendre_kernel, dim3((P.nk_max + 63) / 64, std::min(P.n_inst, 65535), P.max_lh),
+                           dim3(64), 0, st, P.d_inst, P.n_inst, waves, P.d_pl, P.d_e);
+        hipLaunchKernelGGL(harm_phase_kernel, wgrid, dim3(256), 0, st, P.d_inst, P.d_sph, P.n_sph, waves,
+                           P.d_phi);
+        hipLaunchKernelGGL(harm_tsum_kernel, dim3(P.max_lh, (unsigned)std::min<long long>(P.n_z, 65535)),
+                           dim3(128), 0, st, P.d_inst, P.d_sph, P.d_zmap, P.n_z, P.d_zpart, P.d_pl,
+                           P.d_e, P.d_phi, P.d_tpart);
+        hipLaunchKernelGGL(harm_records_kernel, dim3(P.max_tl, std::min(P.n_sph, 65535)), dim3(128), 0,
+                           st, P.d_sph, P.n_sph, P.d_zfirst, P.d_zpart, P.d_tpart, P.d_rec);
+    } else {
+        hipLaunchKernelGGL(series_waves_kernel, wgrid, dim3(256), 0, st, P.d_inst, P.d_sph, P.n_sph,
+                           waves, P.d_wt);
+    }
     BIMSIM_HIP_CHECK(ctx, hipGetLastError());
     for (const SeriesPlan::Chunk& ch : P.chunks) {
         const dim3 cgrid((P.max_npix + kThreads - 1) / kThreads, std::min(ch.sph_count, 65535));
@@ -513,7 +831,10 @@ int series_plan_launch(bimsim_ctx* ctx, const SeriesPlan& P, const double* waves
             const dim3 grid(std::min(K.item_count, P.groups_per_cu * ctx->cu_count));
             switch (K.L) {
 #define BIMSIM_CASE(LL) \
-    case LL: launch_bucket<LL>(grid, st, P, K, U); break;
+    case LL:                                                \
+        if (P.harmonic) launch_harm<LL>(grid, st, P, K, U); \
+        else launch_bucket<LL>(grid, st, P, K, U);          \
+        break;
                 BIMSIM_CASE(16)
                 BIMSIM_CASE(24)
                 BIMSIM_CASE(32)
@@ -544,8 +865,8 @@ namespace {
 
 // Shape of a batch for the tuning cache: per instance (bucket, pixels,
 // spheres, waves) and the total.
-std::string shape_key(const SeriesBatch& batch) {
-    std::string k;
+std::string shape_key(const bimsim_ctx* ctx, const SeriesBatch& batch) {
+    std::string k = "algo" + std::to_string(ctx->series_algo) + ";";
     char buf[96];
     for (const SeriesInst& I : batch.inst) {
         int nl1 = 1;
@@ -572,7 +893,7 @@ int series_plan_create_tuned(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_bu
         batch = work;
         return BIMSIM_OK;
     }
-    const std::string key = shape_key(batch);
+    const std::string key = shape_key(ctx, batch);
     auto hit = ctx->tune.find(key);
     int best_mult = P->split_mult, best_gpc = P->groups_per_cu;
     if (hit == ctx->tune.end()) {
@@ -637,7 +958,9 @@ int series_plan_create_tuned(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_bu
 
 int series_cached_plan(bimsim_ctx* ctx, SeriesBatch& batch, const double* waves, const double2* A,
                        const double2* B, const SeriesPlan** out) {
-    std::vector<char> key(sizeof(SeriesInst) * batch.inst.size() + sizeof(SeriesSph) * batch.sph.size());
+    std::vector<char> key(sizeof(SeriesInst) * batch.inst.size() + sizeof(SeriesSph) * batch.sph.size() +
+                          sizeof(int));
+    std::memcpy(key.data() + key.size() - sizeof(int), &ctx->series_algo, sizeof(int));
     if (!batch.inst.empty())
         std::memcpy(key.data(), batch.inst.data(), sizeof(SeriesInst) * batch.inst.size());
     if (!batch.sph.empty())

@@ -141,6 +141,23 @@ int bimsim_set_timing(bimsim_ctx* ctx, int enable);
  * candidate wave splits and grid sizes on its own inputs and keeps the
  * fastest for the context.  No counterpart in the reference. */
 int bimsim_set_autotune(bimsim_ctx* ctx, int enable);
+
+/* Algorithm of the FP32 scattered-field series (BIMSIM_FP32_FAST):
+ *   BIMSIM_SERIES_DIRECT    per pixel, wave and order, as the reference
+ *                           (rescaled Legendre recurrence per wave);
+ *   BIMSIM_SERIES_HARMONIC  the addition theorem P_l(k.r) = 4 pi/(2l+1)
+ *                           sum_m Y_lm(k) Y*_lm(r) moves the wave sum onto
+ *                           each sphere: (Nl+1)(Nl+2)/2 terms per pixel and
+ *                           sphere instead of nk (Nl+1);
+ *   BIMSIM_SERIES_AUTO      harmonic when every instance has >= 16 waves.
+ * Both sum the same series; they differ by rounding.  No counterpart in the
+ * reference. */
+typedef enum {
+    BIMSIM_SERIES_AUTO = 0,
+    BIMSIM_SERIES_DIRECT = 1,
+    BIMSIM_SERIES_HARMONIC = 2
+} bimsim_series_algo;
+int bimsim_set_series_algorithm(bimsim_ctx* ctx, int algo);
 int bimsim_get_timings(bimsim_ctx* ctx, float* ms, int* counts);
 
 /* ------------------------------------------------------------ host helpers */

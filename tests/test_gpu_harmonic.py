@@ -1,0 +1,94 @@
+"""The harmonic (addition-theorem) form of the FP32 scattered-field series
+against the FP64 oracle and against the direct FP32 form, on geometries that
+exercise every branch: slices through and off the sphere centres, pixels at
+a centre, tilted illumination (waves far from the z axis), orders up to the
+largest bucket, several spheres.  Tolerance as for the direct form: field
+<= 2e-5 normwise."""
+import math
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def normrel(a, b):
+    m = np.max(np.abs(b))
+    return float(np.max(np.abs(np.asarray(a) - b)) / (m if m > 0 else 1.0))
+
+
+@pytest.fixture(scope="module")
+def W(pkg):
+    import bimsim_amd.workloads as W
+    return W
+
+
+def cases(W):
+    out = {}
+    out["centre"] = W.config(1, N=40, nk=40)                    # pixel at the centre
+    out["three"] = W.config(2, N=48, nk=64)
+    t = W.make_instance([dict(p=(1.0, -2.0, -1.5), a=4.0, n=complex(1.45, 0.03)),
+                         dict(p=(-6.0, 5.0, 2.0), a=2.5, n=complex(1.6, 0.0))], 2.0, 40, 30.0,
+                        (0.1, 0.7), (0.0, 0.7), 48, 21)
+    t["A"] = (-15.0, -15.0, 1.0)                                  # slice off the centres
+    th = 0.6                                                      # tilted illumination
+    t["kdir"] = (math.sin(th), 0.0, math.cos(th))
+    out["tilted"] = t
+    big = W.make_instance([dict(p=(0.0, 0.0, 0.0), a=9.0, n=complex(1.3, 0.01))], 1.2, 32, 30.0,
+                          (0.0, 0.5), (0.0, 0.5), 24, 5)
+    out["large_order"] = big                                      # Nl ~ 60
+    return out
+
+
+@pytest.mark.parametrize("name", ["centre", "three", "tilted", "large_order"])
+def test_harmonic_matches_oracle_and_direct(pkg, ctx, orc, W, name):
+    import bimsim_amd.pipeline as P
+    inst = cases(W)[name]
+    q = P.quad_of(inst)
+    N = inst["N"]
+    waves = W.mc_waves(inst, pkg.mc_sample_na)
+    A = torch.zeros(inst["ncoef"], dtype=torch.complex128, device="cuda")
+    B = torch.zeros_like(A)
+    ctx.mie_coeffs(inst["spheres"], inst["kmag"], A, B)
+    wd = torch.as_tensor(waves).cuda()
+    ref = orc.nf_us(q, N, N, inst["focus"], inst["kmag"], inst["spheres"], host(A), host(B), waves,
+                    inst["subA"]).ravel()
+    out = {}
+    for algo in (pkg.SERIES_HARMONIC, pkg.SERIES_DIRECT):
+        ctx.set_series_algorithm(algo)
+        U = torch.empty(N * N, dtype=torch.complex64, device="cuda")
+        ctx.nf_us(q, N, N, inst["focus"], inst["kmag"], inst["spheres"], A, B, wd, inst["subA"], U,
+                  prec=pkg.FP32_FAST)
+        out[algo] = host(U).astype(np.complex128)
+    ctx.set_series_algorithm(pkg.SERIES_AUTO)
+    assert normrel(out[pkg.SERIES_HARMONIC], ref) <= 2e-5
+    assert normrel(out[pkg.SERIES_DIRECT], ref) <= 2e-5
+    assert normrel(out[pkg.SERIES_HARMONIC], out[pkg.SERIES_DIRECT]) <= 2e-5
+    if name == "large_order":
+        assert max(s["Nl"] for s in inst["spheres"]) >= 49   # the 64-order bucket
+
+
+def test_harmonic_accumulates(pkg, ctx, orc, W):
+    import bimsim_amd.pipeline as P
+    inst = W.config(2, N=32, nk=32)
+    q = P.quad_of(inst)
+    N = inst["N"]
+    waves = torch.as_tensor(W.mc_waves(inst, pkg.mc_sample_na)).cuda()
+    A = torch.zeros(inst["ncoef"], dtype=torch.complex128, device="cuda")
+    B = torch.zeros_like(A)
+    ctx.mie_coeffs(inst["spheres"], inst["kmag"], A, B)
+    ctx.set_series_algorithm(pkg.SERIES_HARMONIC)
+    U = torch.empty(N * N, dtype=torch.complex64, device="cuda")
+    args = (q, N, N, inst["focus"], inst["kmag"], inst["spheres"], A, B, waves, inst["subA"], U)
+    ctx.nf_us(*args, prec=pkg.FP32_FAST)
+    once = host(U).copy()
+    ctx.nf_us(*args, prec=pkg.FP32_FAST, accumulate=True)
+    ctx.set_series_algorithm(pkg.SERIES_AUTO)
+    assert normrel(host(U), 2 * once) <= 1e-6
