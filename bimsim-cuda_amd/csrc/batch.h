@@ -13,6 +13,7 @@ struct UfInst {
     double kmag, amp, ca, cb;
     int m, uR, vR, plane;
     long long off;   // element offset of the instance's field
+    long long tab;   // complex64 fields: offset of its uf_fast_table, else -1
 };
 
 // Uf + Us sum of one instance.
@@ -27,8 +28,16 @@ struct SumInst {
 int mie_batch_launch(bimsim_ctx* ctx, const SphereDev* d_sph, const double* d_kmag, int n,
                      int lmax, int quirks, void* work, double2* A, double2* B);
 
-// Incident fields of n instances into Uf (complex64 when f32, else complex128).
-int uf_batch_launch(bimsim_ctx* ctx, const UfInst* d_inst, int n, int max_npix, int f32, void* Uf);
+// Pixel-invariant table of the incident-field series for the complex64
+// form: (i^l sign) * D_l for l = 0..m, then 1/l for l = 0..m (entry 0
+// unused); D_l from the same FP64 Legendre recurrences as the per-pixel
+// reference form, so the values are identical.  Appends 2 (m + 1) doubles.
+void uf_fast_table(double ca, double cb, int m, std::vector<double>* out);
+
+// Incident fields of n instances into Uf (complex64 when f32, else
+// complex128); tabs holds the instances' uf_fast_table rows (f32 only).
+int uf_batch_launch(bimsim_ctx* ctx, const UfInst* d_inst, int n, int max_npix, int f32,
+                    const double* tabs, void* Uf);
 
 // U = Uf or U += Uf outside the spheres, for n instances; d_sph holds
 // (x, y, z, a) per sphere.

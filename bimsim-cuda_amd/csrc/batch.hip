@@ -36,6 +36,7 @@ struct bimsim_batch {
     void* d_work = nullptr;
     double2 *d_A = nullptr, *d_B = nullptr;
     UfInst* d_uf = nullptr;
+    double* d_uftab = nullptr;      // incident-field tables of complex64 batches
     SumInst* d_sum = nullptr;
     double4* d_sph4 = nullptr;
     void *d_Uf = nullptr, *d_U = nullptr;
@@ -72,7 +73,7 @@ void release(bimsim_batch* b) {
     if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
     series_plan_destroy(&b->plan_mc);
     series_plan_destroy(&b->plan_pw);
-    void* ptrs[] = {b->d_sph, b->d_kmag, b->d_work, b->d_A, b->d_B, b->d_uf,
+    void* ptrs[] = {b->d_sph, b->d_kmag, b->d_work, b->d_A, b->d_B, b->d_uf, b->d_uftab,
                     b->d_sum, b->d_sph4, b->d_Uf, b->d_U, b->d_kdirs};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -85,7 +86,7 @@ int enqueue(bimsim_batch* b) {
     const size_t rb = b->f32 ? sizeof(float) : sizeof(double);
     BIMSIM_TRY(mie_batch_launch(ctx, b->d_sph, b->d_kmag, b->n_sph, b->lmax, b->quirks, b->d_work,
                                 b->d_A, b->d_B));
-    BIMSIM_TRY(uf_batch_launch(ctx, b->d_uf, b->n, b->max_npix, b->f32, b->d_Uf));
+    BIMSIM_TRY(uf_batch_launch(ctx, b->d_uf, b->n, b->max_npix, b->f32, b->d_uftab, b->d_Uf));
     if (b->f32) {
         if (b->have_mc)
             BIMSIM_TRY(series_plan_launch(ctx, b->plan_mc, b->waves, b->d_A, b->d_B, (float2*)b->d_U));
@@ -129,6 +130,7 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
     std::vector<double> sph_k;
     std::vector<double4> sph4;
     std::vector<UfInst> uf(n);
+    std::vector<double> uftab;
     std::vector<SumInst> sum(n);
     std::vector<double> kdirs(3 * (size_t)n);
     SeriesBatch mc, pw;
@@ -162,8 +164,13 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
         b->max_npix = std::max(b->max_npix, npix);
         b->field_off[i] = field;
         const QuadArg q = to_quad(I.pos);
+        long long tab = -1;
+        if (b->f32 && !I.plane_wave) {
+            tab = (long long)uftab.size();
+            uf_fast_table(ca, cb, I.m, &uftab);
+        }
         uf[i] = UfInst{q, mk(I.focus), mk(I.kdir), b->kmag[i], I.amp, ca, cb, I.m, I.N, I.N,
-                       I.plane_wave ? 1 : 0, field};
+                       I.plane_wave ? 1 : 0, field, tab};
         sum[i] = SumInst{q, I.N, I.N, (int)sph_dev.size(), I.ns, field};
         for (int c = 0; c < 3; ++c) kdirs[3 * (size_t)i + c] = I.kdir[c];
         // spheres of this instance, coefficients packed
@@ -218,6 +225,7 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
     BIMSIM_TRY(dev_upload(ctx, &b->d_sph, sph_dev));
     BIMSIM_TRY(dev_upload(ctx, &b->d_kmag, sph_k));
     BIMSIM_TRY(dev_upload(ctx, &b->d_uf, uf));
+    BIMSIM_TRY(dev_upload(ctx, &b->d_uftab, uftab));
     BIMSIM_TRY(dev_upload(ctx, &b->d_sum, sum));
     BIMSIM_TRY(dev_upload(ctx, &b->d_sph4, sph4));
     BIMSIM_TRY(dev_upload(ctx, &b->d_kdirs, kdirs));

@@ -43,23 +43,37 @@ int scratch_get(bimsim_ctx* ctx, int slot, size_t bytes, void** out) {
     return BIMSIM_OK;
 }
 
-int int_table_get(bimsim_ctx* ctx, const std::vector<int>& v, const int** out) {
-    auto it = ctx->int_tables.find(v);
-    if (it == ctx->int_tables.end()) {
-        int* d = nullptr;
-        if (hipMalloc(&d, sizeof(int) * std::max<size_t>(v.size(), 1)) != hipSuccess) {
+namespace {
+
+template <class T>
+int table_get(bimsim_ctx* ctx, std::map<std::vector<T>, T*>& tables, const std::vector<T>& v,
+              const T** out) {
+    auto it = tables.find(v);
+    if (it == tables.end()) {
+        T* d = nullptr;
+        if (hipMalloc(&d, sizeof(T) * std::max<size_t>(v.size(), 1)) != hipSuccess) {
             (void)hipGetLastError();
             return set_error(ctx, BIMSIM_E_NOMEM, "table allocation failed");
         }
         if (!v.empty()) {
-            BIMSIM_HIP_CHECK(ctx, hipMemcpyAsync(d, v.data(), sizeof(int) * v.size(),
+            BIMSIM_HIP_CHECK(ctx, hipMemcpyAsync(d, v.data(), sizeof(T) * v.size(),
                                                  hipMemcpyHostToDevice, ctx->stream));
             BIMSIM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
         }
-        it = ctx->int_tables.emplace(v, d).first;
+        it = tables.emplace(v, d).first;
     }
     *out = it->second;
     return BIMSIM_OK;
+}
+
+}  // namespace
+
+int int_table_get(bimsim_ctx* ctx, const std::vector<int>& v, const int** out) {
+    return table_get(ctx, ctx->int_tables, v, out);
+}
+
+int dbl_table_get(bimsim_ctx* ctx, const std::vector<double>& v, const double** out) {
+    return table_get(ctx, ctx->dbl_tables, v, out);
 }
 
 int twiddles_get(bimsim_ctx* ctx, int n, const double** d, const float** f) {
@@ -160,6 +174,7 @@ int bimsim_ctx_destroy(bimsim_ctx* ctx) {
     for (auto& kv : ctx->scratch)
         if (kv.second.first) (void)hipFree(kv.second.first);
     for (auto& kv : ctx->int_tables) (void)hipFree(kv.second);
+    for (auto& kv : ctx->dbl_tables) (void)hipFree(kv.second);
     for (auto& kv : ctx->twiddles) {
         (void)hipFree(kv.second.first);
         (void)hipFree(kv.second.second);

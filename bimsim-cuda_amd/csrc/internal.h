@@ -33,6 +33,7 @@ struct bimsim_ctx {
     // device copies of small host tables, keyed by content (e.g. the
     // detector's pass-band column list)
     std::map<std::vector<int>, int*> int_tables;
+    std::map<std::vector<double>, double*> dbl_tables;
     int cu_count = 256;
     size_t lds_max = 65536;
     // last plan of the FP32 series (opaque, owned by us_series.hip)
@@ -65,6 +66,9 @@ int scratch_get(bimsim_ctx* ctx, int slot, size_t bytes, void** out);
 
 // Device copy of an int table (cached by content; uploaded once).
 int int_table_get(bimsim_ctx* ctx, const std::vector<int>& v, const int** out);
+
+// Device copy of a double table (cached by content; uploaded once).
+int dbl_table_get(bimsim_ctx* ctx, const std::vector<double>& v, const double** out);
 
 // Twiddle table for length n on the device (double and float copies).
 int twiddles_get(bimsim_ctx* ctx, int n, const double** d, const float** f);

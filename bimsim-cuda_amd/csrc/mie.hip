@@ -1,6 +1,11 @@
 // FP64 Mie coefficients on the GPU: sphere::calcCoeff (reference
 // sphere.cpp:21-97) with the complex spherical Bessel functions of
-// zj_device.h.  One thread per sphere.  Built with -ffp-contract=off.
+// zj_device.h.  Built with -ffp-contract=off.
+//
+// The work is a few long chains of dependent FP64 steps per sphere, so the
+// kernels spread it over lanes: a block of 64 lanes takes 32 spheres, lane
+// 2s computes the Bessel set of ka and lane 2s+1 that of kna for sphere s;
+// then every lane takes (sphere, order) pairs of the coefficient formula.
 #include "batch.h"
 #include "internal.h"
 #include "zj_device.h"
@@ -8,7 +13,7 @@
 namespace bimsim {
 namespace {
 
-constexpr int kSphPerLaunch = 32;
+constexpr int kSphPerLaunch = 32;   // spheres per block (two lanes each)
 
 struct SphereBatch {
     SphereDev s[kSphPerLaunch];
@@ -16,47 +21,83 @@ struct SphereBatch {
 
 using zj::zc;
 
-// Coefficients of one sphere at wavenumber kmag (work: 8 arrays of wstride).
-__device__ void mie_one(const SphereDev& S, double kmag, int quirks, zc* w, int wstride,
-                        double2* A, double2* B) {
-    const int nl = S.Nl;
-    zc *j1 = w, *y1 = w + wstride, *jp1 = w + 2 * wstride, *yp1 = w + 3 * wstride;
-    zc *j2 = w + 4 * wstride, *y2 = w + 5 * wstride, *jp2 = w + 6 * wstride, *yp2 = w + 7 * wstride;
+// Bessel set of one argument of sphere S (kind 0: ka, 1: kna) into the
+// sphere's work (8 arrays of wstride): j, y, j', y' of ka, then of kna.
+__device__ void mie_bessel(const SphereDev& S, double kmag, int kind, int quirks, zc* w,
+                           int wstride) {
+    zc* o = w + 4 * kind * wstride;
     const zc nc = {S.n[0], S.n[1]};
-    const zc ka = {kmag * S.a, 0.0};
-    const zc kna = zj::zsc(S.a, zj::zsc(kmag, nc));
-    zj::sph_jy_cplx(nl, ka, quirks != 0, j1, y1, jp1, yp1);
-    zj::sph_jy_cplx(nl, kna, quirks != 0, j2, y2, jp2, yp2);
+    const zc z = kind == 0 ? zc{kmag * S.a, 0.0} : zj::zsc(S.a, zj::zsc(kmag, nc));
+    zj::sph_jy_cplx(S.Nl, z, quirks != 0, o, o + wstride, o + 2 * wstride, o + 3 * wstride);
+}
+
+// A_l and B_l of sphere S from its Bessel sets.
+__device__ void mie_order(const SphereDev& S, int l, const zc* w, int wstride, double2* A,
+                          double2* B) {
+    const zc *j1 = w, *y1 = w + wstride, *jp1 = w + 2 * wstride, *yp1 = w + 3 * wstride;
+    const zc *j2 = w + 4 * wstride, *jp2 = w + 6 * wstride;
+    const zc nc = {S.n[0], S.n[1]};
     const zc ci = {0.0, 1.0};
     // i^l by exact repeated squaring, as std::pow(complex, int)
     const zc ipow[4] = {{1.0, 0.0}, {0.0, 1.0}, {-1.0, 0.0}, {-0.0, -1.0}};
-    for (int l = 0; l <= nl; ++l) {
-        const zc h = zj::zadd(j1[l], zj::zmul(ci, y1[l]));
-        const zc hp = zj::zadd(jp1[l], zj::zmul(ci, yp1[l]));
-        const zc num_a = zj::zsub(zj::zmul(j1[l], hp), zj::zmul(jp1[l], h));
-        const zc den = zj::zsub(zj::zmul(j2[l], hp), zj::zmul(zj::zmul(h, jp2[l]), nc));
-        const zc f = zj::zsc(2.0 * l + 1.0, ipow[l & 3]);
-        const zc An = zj::zmul(f, zj::zdiv(num_a, den));
-        const zc num_b = zj::zsub(zj::zmul(zj::zmul(j1[l], jp2[l]), nc), zj::zmul(j2[l], jp1[l]));
-        const zc Bn = zj::zmul(f, zj::zdiv(num_b, den));
-        A[S.coeff_off + l] = make_double2(An.r, An.i);
-        B[S.coeff_off + l] = make_double2(Bn.r, Bn.i);
+    const zc h = zj::zadd(j1[l], zj::zmul(ci, y1[l]));
+    const zc hp = zj::zadd(jp1[l], zj::zmul(ci, yp1[l]));
+    const zc num_a = zj::zsub(zj::zmul(j1[l], hp), zj::zmul(jp1[l], h));
+    const zc den = zj::zsub(zj::zmul(j2[l], hp), zj::zmul(zj::zmul(h, jp2[l]), nc));
+    const zc f = zj::zsc(2.0 * l + 1.0, ipow[l & 3]);
+    const zc An = zj::zmul(f, zj::zdiv(num_a, den));
+    const zc num_b = zj::zsub(zj::zmul(zj::zmul(j1[l], jp2[l]), nc), zj::zmul(j2[l], jp1[l]));
+    const zc Bn = zj::zmul(f, zj::zdiv(num_b, den));
+    A[S.coeff_off + l] = make_double2(An.r, An.i);
+    B[S.coeff_off + l] = make_double2(Bn.r, Bn.i);
+}
+
+// Coefficients of the block's cnt spheres (sh, wavenumbers kh, in LDS);
+// work holds 8 * wstride complex values per sphere of the block.
+__device__ void mie_block(const SphereDev* sh, const double* kh, int cnt, int quirks, zc* work,
+                          int wstride, double2* A, double2* B) {
+    const int t = (int)threadIdx.x;
+    const int s = t >> 1;
+    if (s < cnt) mie_bessel(sh[s], kh[s], t & 1, quirks, work + (size_t)s * 8 * wstride, wstride);
+    __syncthreads();
+    int lm = 0;
+    for (int i = 0; i < cnt; ++i) lm = max(lm, sh[i].Nl);
+    const int per = lm + 1;
+    for (int idx = t; idx < cnt * per; idx += (int)blockDim.x) {
+        const int i = idx / per, l = idx - i * per;
+        if (l <= sh[i].Nl) mie_order(sh[i], l, work + (size_t)i * 8 * wstride, wstride, A, B);
     }
 }
 
-__global__ void mie_coeff_kernel(SphereBatch batch, int ns, double kmag, int quirks, zc* work,
-                                 int wstride, double2* A, double2* B) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ns) return;
-    mie_one(batch.s[t], kmag, quirks, work + (size_t)t * 8 * wstride, wstride, A, B);
+__global__ __launch_bounds__(2 * kSphPerLaunch) void mie_coeff_kernel(
+    SphereBatch batch, int ns, double kmag, int quirks, zc* work, int wstride, double2* A,
+    double2* B) {
+    __shared__ SphereDev sh[kSphPerLaunch];
+    __shared__ double kh[kSphPerLaunch];
+    const int t = (int)threadIdx.x;
+    if (t < ns) {
+        sh[t] = batch.s[t];
+        kh[t] = kmag;
+    }
+    __syncthreads();
+    mie_block(sh, kh, ns, quirks, work, wstride, A, B);
 }
 
-// One thread per sphere of a batch, sphere t at wavenumber kmag[t].
-__global__ void mie_batch_kernel(const SphereDev* __restrict__ sph, const double* __restrict__ kmag,
-                                 int n, int quirks, zc* work, int wstride, double2* A, double2* B) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    mie_one(sph[t], kmag[t], quirks, work + (size_t)t * 8 * wstride, wstride, A, B);
+// Spheres [32 b, 32 b + 32) of a batch, sphere t at wavenumber kmag[t].
+__global__ __launch_bounds__(2 * kSphPerLaunch) void mie_batch_kernel(
+    const SphereDev* __restrict__ sph, const double* __restrict__ kmag, int n, int quirks,
+    zc* work, int wstride, double2* A, double2* B) {
+    __shared__ SphereDev sh[kSphPerLaunch];
+    __shared__ double kh[kSphPerLaunch];
+    const int s0 = (int)blockIdx.x * kSphPerLaunch;
+    const int cnt = min(kSphPerLaunch, n - s0);
+    const int t = (int)threadIdx.x;
+    if (t < cnt) {
+        sh[t] = sph[s0 + t];
+        kh[t] = kmag[s0 + t];
+    }
+    __syncthreads();
+    mie_block(sh, kh, cnt, quirks, work + (size_t)s0 * 8 * wstride, wstride, A, B);
 }
 
 }  // namespace
@@ -64,8 +105,9 @@ __global__ void mie_batch_kernel(const SphereDev* __restrict__ sph, const double
 int mie_batch_launch(bimsim_ctx* ctx, const SphereDev* d_sph, const double* d_kmag, int n, int lmax,
                      int quirks, void* work, double2* A, double2* B) {
     if (n <= 0) return BIMSIM_OK;
-    hipLaunchKernelGGL(mie_batch_kernel, dim3((n + 63) / 64), dim3(64), 0, ctx->stream, d_sph,
-                       d_kmag, n, quirks, (zc*)work, lmax + 2, A, B);
+    hipLaunchKernelGGL(mie_batch_kernel, dim3((n + kSphPerLaunch - 1) / kSphPerLaunch),
+                       dim3(2 * kSphPerLaunch), 0, ctx->stream, d_sph, d_kmag, n, quirks, (zc*)work,
+                       lmax + 2, A, B);
     BIMSIM_HIP_CHECK(ctx, hipGetLastError());
     return BIMSIM_OK;
 }
@@ -107,8 +149,9 @@ extern "C" int bimsim_mie_coeffs(bimsim_ctx* ctx, const bimsim_sphere* spheres, 
             d.Nl = S.Nl;
             d.coeff_off = S.coeff_off;
         }
-        hipLaunchKernelGGL(mie_coeff_kernel, dim3(1), dim3(64), 0, ctx->stream, b, cnt, kmag,
-                           quirks, (zj::zc*)work, wstride, (double2*)A_out, (double2*)B_out);
+        hipLaunchKernelGGL(mie_coeff_kernel, dim3(1), dim3(2 * kSphPerLaunch), 0, ctx->stream, b,
+                           cnt, kmag, quirks, (zj::zc*)work, wstride, (double2*)A_out,
+                           (double2*)B_out);
         BIMSIM_HIP_CHECK(ctx, hipGetLastError());
     }
     return BIMSIM_OK;

@@ -1,8 +1,9 @@
 // Incident field Uf and the Uf + Us sum (reference nfScalarUf.cu:9-217,
-// nfSumUf.cu:6-117).  Built with -ffp-contract=off: the series runs in FP64
-// in reference operation order for every precision; FP32_FAST stores the
-// result as float pairs.  The Uf series is O(pixels x m), a small fraction of
-// the scattered-field work of any Monte-Carlo configuration.
+// nfSumUf.cu:6-117).  Built with -ffp-contract=off.  The FP64 forms run the
+// series in reference operation order.  FP32_FAST runs it in FP64 too, with
+// the pixel-invariant factors i^l D_l from a per-instance table and the
+// recurrences multiplying by reciprocals (explicit FMAs), and stores float
+// pairs.
 #include "geometry.h"
 #include "internal.h"
 #include "batch.h"
@@ -75,6 +76,51 @@ __device__ dc uf_series_value(const QuadArg& q, int uR, int vR, int pix, d3 f, d
     return cscl(cscl(cscl(sum, 2.0), (double)kPIf), amp);
 }
 
+// The series of uf_series_value for complex64 fields: (i^l sign) D_l and
+// 1/l from uf_fast_table (T), j_l and P_l by their upward recurrences
+// multiplying by 1/(kd) and 1/l with FMAs, the j_l threshold rule of
+// shift_j (spherical_bessel.h:109-122) unchanged.  Differs from the
+// reference order by rounding only.
+__device__ dc uf_fast_value(const QuadArg& q, int uR, int vR, int pix, d3 f, d3 k, double kmag,
+                            double amp, double ca, double cb, int m, const double* __restrict__ T) {
+    int iu, iv;
+    pixel_of(pix, uR, iu, iv);
+    const d3 p = quad_at(q, (double)iu / (double)uR, (double)iv / (double)vR);
+    d3 r = sub(p, f);
+    const double d = len(r);
+    if (d < (double)kEpsFloat) return dc{amp * 2.0 * (double)kPIf * (ca - cb), 0.0};
+    r = norm(r);
+    const double kd = kmag * d;
+    const double inv = 1.0 / kd;
+    double sn, cs;
+    sincos(kd, &sn, &cs);
+    double j0 = sn * inv;
+    double j1 = fma(j0, inv, -(cs * inv));
+    const double ct = dot(k, r);
+    const double* Ds = T;
+    const double* rl = T + (m + 1);
+    double P0 = 1.0, P1 = ct;
+    double ar = j0 * Ds[0];
+    double ai = (j1 * ct) * Ds[1];
+    auto step = [&](int l) {
+        double jn = fma((double)(2 * l - 1) * j1, inv, -j0);
+        if ((double)l > kd && (jn < kJMin || jn > kJMax)) jn = 0.0;
+        j0 = j1;
+        j1 = jn;
+        const double Pn = fma((double)(2 * l - 1) * ct, P1, -((double)(l - 1) * P0)) * rl[l];
+        P0 = P1;
+        P1 = Pn;
+        return (jn * Pn) * Ds[l];
+    };
+    int l = 2;
+    for (; l + 1 <= m; l += 2) {
+        ar += step(l);
+        ai += step(l + 1);
+    }
+    if (l <= m) ar += step(l);
+    return dc{((ar * 2.0) * (double)kPIf) * amp, ((ai * 2.0) * (double)kPIf) * amp};
+}
+
 // gpuScalarUfp, nfScalarUf.cu:9-43: the plane wave at pixel pix
 __device__ inline dc uf_plane_value(const QuadArg& q, int uR, int vR, int pix, d3 f, d3 k,
                                     double kmag, double amp) {
@@ -96,6 +142,16 @@ __global__ __launch_bounds__(kThreads) void uf_series_kernel(QuadArg q, int uR, 
     store(Uf, pix, uf_series_value(q, uR, vR, pix, f, k, kmag, amp, ca, cb, m));
 }
 
+__global__ __launch_bounds__(kThreads) void uf_fast_kernel(QuadArg q, int uR, int vR, d3 f, d3 k,
+                                                           double kmag, double amp, double ca,
+                                                           double cb, int m,
+                                                           const double* __restrict__ T,
+                                                           float2* __restrict__ Uf) {
+    const int pix = blockIdx.x * kThreads + threadIdx.x;
+    if (pix >= uR * vR) return;
+    store(Uf, pix, uf_fast_value(q, uR, vR, pix, f, k, kmag, amp, ca, cb, m, T));
+}
+
 template <class Out>
 __global__ __launch_bounds__(kThreads) void uf_plane_kernel(QuadArg q, int uR, int vR, d3 f, d3 k,
                                                             double kmag, double amp,
@@ -108,14 +164,20 @@ __global__ __launch_bounds__(kThreads) void uf_plane_kernel(QuadArg q, int uR, i
 // Incident fields of a batch of instances.  Grid: (pixel blocks, instances).
 template <class Out>
 __global__ __launch_bounds__(kThreads) void uf_batch_kernel(const UfInst* __restrict__ insts, int n,
+                                                            const double* __restrict__ tabs,
                                                             Out* __restrict__ Uf) {
     for (int i = blockIdx.y; i < n; i += gridDim.y) {
         const UfInst& I = insts[i];
         const int npix = I.uR * I.vR;
         for (int pix = blockIdx.x * kThreads + threadIdx.x; pix < npix; pix += gridDim.x * kThreads) {
-            const dc v = I.plane ? uf_plane_value(I.q, I.uR, I.vR, pix, I.f, I.k, I.kmag, I.amp)
-                                 : uf_series_value(I.q, I.uR, I.vR, pix, I.f, I.k, I.kmag, I.amp,
-                                                   I.ca, I.cb, I.m);
+            dc v;
+            if (I.plane)
+                v = uf_plane_value(I.q, I.uR, I.vR, pix, I.f, I.k, I.kmag, I.amp);
+            else if (I.tab >= 0)
+                v = uf_fast_value(I.q, I.uR, I.vR, pix, I.f, I.k, I.kmag, I.amp, I.ca, I.cb, I.m,
+                                  tabs + I.tab);
+            else
+                v = uf_series_value(I.q, I.uR, I.vR, pix, I.f, I.k, I.kmag, I.amp, I.ca, I.cb, I.m);
             store(Uf + I.off, pix, v);
         }
     }
@@ -188,14 +250,40 @@ __global__ void add_outside_kernel(int npix, const unsigned char* __restrict__ m
 
 }  // namespace
 
-int uf_batch_launch(bimsim_ctx* ctx, const UfInst* d_inst, int n, int max_npix, int f32, void* Uf) {
+void uf_fast_table(double ca, double cb, int m, std::vector<double>* out) {
+    const size_t base = out->size();
+    out->resize(base + 2 * (size_t)(m + 1), 0.0);
+    double* Ds = out->data() + base;
+    double* rl = Ds + (m + 1);
+    // P_{n}(x) from P_{n-1}, P_{n-2} as ref::shift_legendre (legendre.h:20)
+    auto leg = [](int n, double x, double& P0, double& P1) {
+        const double Pn = ((double)(2 * n - 1) * x * P1 - (double)(n - 1) * P0) / (double)n;
+        P0 = P1;
+        P1 = Pn;
+    };
+    double Pa[3] = {1.0, ca, 1.0}, Pb[3] = {1.0, cb, 1.0};
+    for (int l = 0; l <= m; ++l) {
+        if (l >= 1) {
+            Pa[2] = Pa[0];
+            leg(l + 1, ca, Pa[0], Pa[1]);
+            Pb[2] = Pb[0];
+            leg(l + 1, cb, Pb[0], Pb[1]);
+        }
+        const double D = ((Pa[1] - Pa[2]) - Pb[1]) + Pb[2];
+        Ds[l] = (l & 2) ? -D : D;   // i^l = 1, i, -1, -i
+        rl[l] = l > 0 ? 1.0 / (double)l : 0.0;
+    }
+}
+
+int uf_batch_launch(bimsim_ctx* ctx, const UfInst* d_inst, int n, int max_npix, int f32,
+                    const double* tabs, void* Uf) {
     const dim3 grid((max_npix + kThreads - 1) / kThreads, std::min(n, 65535));
     if (f32)
         hipLaunchKernelGGL(uf_batch_kernel<float2>, grid, dim3(kThreads), 0, ctx->stream, d_inst, n,
-                           (float2*)Uf);
+                           tabs, (float2*)Uf);
     else
         hipLaunchKernelGGL(uf_batch_kernel<double2>, grid, dim3(kThreads), 0, ctx->stream, d_inst,
-                           n, (double2*)Uf);
+                           n, tabs, (double2*)Uf);
     BIMSIM_HIP_CHECK(ctx, hipGetLastError());
     return BIMSIM_OK;
 }
@@ -261,10 +349,14 @@ extern "C" int bimsim_nf_uf(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int
             hipLaunchKernelGGL(uf_plane_kernel<double2>, grid, dim3(kThreads), 0, ctx->stream, q,
                                uR, vR, f, k, kmag, amp, (double2*)Uf);
     } else {
-        if (f32)
-            hipLaunchKernelGGL(uf_series_kernel<float2>, grid, dim3(kThreads), 0, ctx->stream, q,
-                               uR, vR, f, k, kmag, amp, cos_alpha, cos_beta, m, (float2*)Uf);
-        else
+        if (f32) {
+            std::vector<double> tab;
+            uf_fast_table(cos_alpha, cos_beta, m, &tab);
+            const double* T = nullptr;
+            BIMSIM_TRY(dbl_table_get(ctx, tab, &T));
+            hipLaunchKernelGGL(uf_fast_kernel, grid, dim3(kThreads), 0, ctx->stream, q, uR, vR, f, k,
+                               kmag, amp, cos_alpha, cos_beta, m, T, (float2*)Uf);
+        } else
             hipLaunchKernelGGL(uf_series_kernel<double2>, grid, dim3(kThreads), 0, ctx->stream, q,
                                uR, vR, f, k, kmag, amp, cos_alpha, cos_beta, m, (double2*)Uf);
     }

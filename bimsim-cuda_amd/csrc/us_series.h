@@ -100,7 +100,7 @@ struct SeriesPlan {
     double2* d_phi = nullptr;     // wave weights (1/nk) subA exp(i k k.c) per sphere
     double2* d_tpart = nullptr;   // partial harmonic sums per (sphere, chunk, m, l)
     float4* d_rec = nullptr;      // harmonic records per (sphere, m, l)
-    float2* d_ab = nullptr;       // Legendre recurrence constants per (m, l)
+    float* d_cst = nullptr;       // scaled Legendre recurrence constants per (m, l)
 };
 
 // Chooses splits and chunks, builds and uploads the tables and allocates

@@ -312,8 +312,10 @@ __global__ void series_reduce_kernel(const SeriesInst* __restrict__ insts,
 // Pn_lm the orthonormal associated Legendre functions (the sign convention
 // cancels in each product).  T is formed once per sphere in FP64 and kept as
 // U1 = T+ + T-, U2 = (Im T+ - Im T-, Re T- - Re T+) (U1 = T, U2 = 0 at
-// m = 0), divided by s_l so that the stored c'_l = s_l c_l apply.
-constexpr int kAB = 113;            // stride of the recurrence-constant table (m, l <= 112)
+// m = 0), times the scale K_lm of the pixel kernel's recurrence.  The
+// radial factors c_l are formed per pixel inside the pixel kernel, so this
+// form keeps no per-pixel coefficient buffer.
+constexpr int kCS = 112;   // row stride of the recurrence-constant table (m, l < 112)
 
 // Orthonormal Pn_lm(x) for l = m .. lh-1 of one wave into out[l - m]
 // (FP64): Pn_00 = 1/sqrt(4 pi), Pn_mm = sqrt((2m+1)/(2m)) s Pn_{m-1,m-1},
@@ -411,8 +413,9 @@ __global__ void harm_tsum_kernel(const SeriesInst* __restrict__ insts,
     }
 }
 
-// Records per (sphere, m, l): the chunk sums, 4 pi/(2l+1) / s_l, U1 and U2,
-// at rec[toff + m * tl + l].  Grid: (m, spheres), threads l.
+// Records per (sphere, m, l): the chunk sums times 4 pi/(2l+1) K_lm,
+// K_lm = prod_{j=m+1..l} a_jm / 2, as U1 and U2 at rec[toff + m * tl + l]
+// (zero outside m <= l <= Nl).  Grid: (m, spheres), threads l.
 __global__ void harm_records_kernel(const SeriesSph* __restrict__ sph, int n_sph,
                                     const long long* __restrict__ zfirst,
                                     const long long* __restrict__ zpart,
@@ -432,9 +435,10 @@ __global__ void harm_records_kernel(const SeriesSph* __restrict__ sph, int n_sph
                 tp = cadd(tp, dc{o[0].x, o[0].y});
                 tm = cadd(tm, dc{o[1].x, o[1].y});
             }
-            double sl = 1.0;
-            for (int j = 1; j <= l; ++j) sl = sl * (2.0 * j - 1.0) / (2.0 * j);
-            const double f = 4.0 * M_PI / (2.0 * l + 1.0) / sl;
+            double K = 1.0;
+            for (int j = m + 1; j <= l; ++j)
+                K *= 0.5 * sqrt((4.0 * j * j - 1.0) / ((double)j * j - (double)m * m));
+            const double f = 4.0 * M_PI / (2.0 * l + 1.0) * K;
             if (m == 0)
                 r = make_float4((float)(f * tp.r), (float)(f * tp.i), 0.f, 0.f);
             else
@@ -445,12 +449,123 @@ __global__ void harm_records_kernel(const SeriesSph* __restrict__ sph, int n_sph
     }
 }
 
-// Per-pixel harmonic sum, persistent over (instance, pixel block) items.
+// Trigonometric start values of the radial recurrences of one pixel and
+// sphere, evaluated before the order sums so that their temporaries do not
+// coincide with the sums' registers: sin, cos of kd outside the sphere;
+// sin, cos of Re knd and sinh, cosh of Im knd inside.
+struct RadialTrig {
+    double t0, t1, t2, t3;
+};
+
+__device__ __forceinline__ RadialTrig harm_trig(const SphereDev& S, double kmag, double d) {
+    RadialTrig T;
+    if (d <= S.a) {
+        const dc knd = cscl(dc{S.n[0], S.n[1]}, kmag * d);
+        T.t0 = sin(knd.r);
+        T.t1 = cos(knd.r);
+        T.t2 = sinh(knd.i);
+        T.t3 = cosh(knd.i);
+    } else {
+        sincos(kmag * d, &T.t0, &T.t1);
+        T.t2 = T.t3 = 0.0;
+    }
+    return T;
+}
+
+// Radial factors of one pixel and sphere applied to its order sums:
+// acc += sum_{l <= Nl} c_l (ar_l + i ai_l), c_l = B_l (j_l + i y_l)(kd)
+// outside the sphere, A_l j_l(knd) inside (calc_Us / calc_Ui,
+// nfScalarUs.cu:50-95,201-245), in FP64 with the reference's upward
+// recurrences and threshold rules (spherical_bessel.h:85-141); the steps
+// multiply by 1/(kd) or 1/(knd) where the reference divides.
 template <int L>
-__global__ __launch_bounds__(kThreads) void series_harm_kernel(
+__device__ __forceinline__ void harm_radial(const SphereDev& S, double kmag, double d,
+                                            const RadialTrig& T, const double2* __restrict__ A,
+                                            const double2* __restrict__ B, const float* ar,
+                                            const float* ai, float& acc_r, float& acc_i) {
+    const int nl1 = S.Nl + 1;
+    auto mac = [&](int l, double cr, double ci) {
+        const float fr = (float)cr, fi = (float)ci;
+        acc_r = fmaf(fr, ar[l], fmaf(-fi, ai[l], acc_r));
+        acc_i = fmaf(fr, ai[l], fmaf(fi, ar[l], acc_i));
+    };
+    if (d <= S.a) {
+        const double2* As = A + S.coeff_off;
+        const dc knd = cscl(dc{S.n[0], S.n[1]}, kmag * d);
+        if (knd.r < (double)kEpsFloat) {
+            // calc_Ui returns A_0 for every wave: only the l = 0 term
+            const double2 a0 = As[0];
+            mac(0, a0.x, a0.y);
+            return;
+        }
+        // init_jc / shift_jc (csin, ccos as csin_r / ccos_r)
+        const dc inv = cdiv_naive(dc{1.0, 0.0}, knd);
+        const dc sn = {T.t0 * T.t3, T.t1 * T.t2}, cs = {T.t1 * T.t3, -(T.t0 * T.t2)};
+        dc j0 = cmul(sn, inv);
+        dc j1 = csub(cmul(j0, inv), cmul(cs, inv));
+#pragma clang loop unroll(full)
+        for (int l = 0; l < L; ++l) {
+            if (l >= nl1) continue;
+            dc jl = l == 0 ? j0 : j1;
+            if (l >= 2) {
+                dc jn = csub(cmul(cscl(j1, (double)(2 * l - 1)), inv), j0);
+                if ((double)l > knd.r && (jn.r < kJMin || jn.r > kJMax)) jn = dc{0.0, 0.0};
+                j0 = j1;
+                j1 = jn;
+                jl = jn;
+            }
+            const dc c = cmul(ld2(As, l), jl);
+            mac(l, c.r, c.i);
+        }
+        return;
+    }
+    const double2* Bs = B + S.coeff_off;
+    const double kd = kmag * d;
+    const double inv = 1.0 / kd;
+    const double sn = T.t0, cs = T.t1;
+    double j0 = sn * inv, j1 = fma(j0, inv, -(cs * inv));
+    double y0 = -(cs * inv), y1 = fma(y0, inv, -(sn * inv));
+#pragma clang loop unroll(full)
+    for (int l = 0; l < L; ++l) {
+        if (l >= nl1) continue;
+        double jl = l == 0 ? j0 : j1, yl = l == 0 ? y0 : y1;
+        if (l >= 2) {
+            double jn = fma((double)(2 * l - 1) * j1, inv, -j0);
+            if ((double)l > kd && (jn < kJMin || jn > kJMax)) jn = 0.0;
+            j0 = j1;
+            j1 = jn;
+            double yn = fma((double)(2 * l - 1) * y1, inv, -y0);
+            if (yn < kYMin || ((double)l > kd && yn > 0.0)) {
+                yn = 0.0;
+                y1 = 0.0;
+            }
+            y0 = y1;
+            y1 = yn;
+            jl = jn;
+            yl = yn;
+        }
+        const double2 b = Bs[l];
+        mac(l, b.x * jl - b.y * yl, b.x * yl + b.y * jl);
+    }
+}
+
+// Per-pixel harmonic sum, persistent over (instance, pixel block) items.
+// Row m of the order sums runs the scaled recurrence
+//   q_mm = Pn_mm,  q_lm = 2x q_{l-1,m} - c_lm q_{l-2,m}
+// (Pn_lm = K_lm q_lm, K_lm = prod_{j=m+1..l} a_jm / 2, folded into the
+// records) in blocks of four orders: one scalar load of four records and
+// one of four constants per block, then straight-line FP32 code; only the
+// block holding the diagonal branches.  cst[m * kCS + l] holds c_lm
+// (l >= m + 2) and the diagonal factor sqrt((2m+1)/(2m)) at l = m.
+// Waves per SIMD the harmonic kernel is compiled for: the order sums take
+// 2L VGPRs, so small buckets can run four waves per SIMD.
+constexpr int harm_waves(int L) { return L <= 32 ? 4 : (L <= 48 ? 3 : (L <= 80 ? 2 : 1)); }
+
+template <int L>
+__global__ __launch_bounds__(kThreads, harm_waves(L)) void series_harm_kernel(
     const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph,
-    const SeriesItem* __restrict__ items, int n_items, const float2* __restrict__ cp,
-    const float4* __restrict__ rh, const float4* __restrict__ rec, const float2* __restrict__ ab,
+    const SeriesItem* __restrict__ items, int n_items, const double2* __restrict__ A,
+    const double2* __restrict__ B, const float4* __restrict__ rec, const float* __restrict__ cst,
     float2* __restrict__ U, int* __restrict__ counter) {
     __shared__ int s_item;
     for (;;) {
@@ -464,60 +579,67 @@ __global__ __launch_bounds__(kThreads) void series_harm_kernel(
         const int npix = I.uR * I.vR;
         const int pix = item.pix0 + (int)threadIdx.x;
         if (pix >= npix) continue;
+        const int iv = pix / I.uR, iu = pix - iv * I.uR;
+        const d3 p = quad_at(I.q, (double)iu / (double)I.uR, (double)iv / (double)I.vR);
         float acc_r = 0.f, acc_i = 0.f;
         for (int s = I.s_begin; s < I.s_begin + I.ns; ++s) {
             const SeriesSph& SS = sph[s];
-            const int nl1 = SS.s.Nl + 1;
-            const float4 r = rh[SS.roff + pix];
-            const float x = r.z;
+            const SphereDev S = SS.s;
+            const int nl1 = S.Nl + 1;
+            d3 r = sub(p, mk(S.p));
+            const double d = len(r);
+            // direction; the centre (knd below float epsilon) keeps only l = 0
+            const bool centre = d <= S.a && S.n[0] * (I.kmag * d) < (double)kEpsFloat;
+            r = centre ? d3{0.0, 0.0, 0.0} : norm(r);
+            const float x = (float)r.z;
             const float st = sqrtf(fmaxf(0.f, fmaf(-x, x, 1.f)));
             float z1r = 1.f, z1i = 0.f;
             if (st > 1e-30f) {
-                z1r = r.x / st;
-                z1i = r.y / st;
+                z1r = (float)r.x / st;
+                z1i = (float)r.y / st;
             }
+            const float x2 = 2.f * x;
+            const RadialTrig trig = harm_trig(S, I.kmag, d);
             float ar[L], ai[L];
 #pragma unroll
             for (int l = 0; l < L; ++l) ar[l] = ai[l] = 0.f;
-            const float4* R = rec + SS.toff;   // stride L: the instance's bucket
+            const float4* R = rec + SS.toff;   // row stride L: the instance's bucket
+            const int be = (nl1 + 3) >> 2;
             float zr = 1.f, zi = 0.f;
-            float pmm = 0.28209479177387814f;
+            float pmm = 0.28209479177387814f;   // 1/sqrt(4 pi)
+#pragma clang loop unroll(disable)
             for (int m = 0; m < nl1; ++m) {
-                float p0 = 0.f, p1 = pmm;
+                const float* Cm = cst + m * kCS;
                 const float4* Rm = R + m * L;
-                const float2* ABm = ab + m * kAB;
-#pragma unroll
-                for (int l = 0; l < L; ++l) {
-                    if (l >= m && l < nl1) {
-                        float p = p1;
-                        if (l > m) {
-                            const float2 c = ABm[l];
-                            p = c.x * fmaf(x, p1, -(c.y * p0));
-                            p0 = p1;
-                            p1 = p;
-                        }
-                        const float4 u = Rm[l];
-                        const float gr = fmaf(zi, u.z, zr * u.x);
-                        const float gi = fmaf(zi, u.w, zr * u.y);
-                        ar[l] = fmaf(p, gr, ar[l]);
-                        ai[l] = fmaf(p, gi, ai[l]);
+                const int bf = m >> 2;
+                float q0 = 0.f, q1 = 0.f;
+#pragma clang loop unroll(full)
+                for (int b = 0; b < L / 4; ++b) {
+                    if (b < bf || b >= be) continue;
+                    const float4 c4 = *reinterpret_cast<const float4*>(Cm + 4 * b);
+                    const float cc[4] = {c4.x, c4.y, c4.z, c4.w};
+                    float4 rr[4];
+#pragma clang loop unroll(full)
+                    for (int j = 0; j < 4; ++j) rr[j] = Rm[4 * b + j];
+#pragma clang loop unroll(full)
+                    for (int j = 0; j < 4; ++j) {
+                        // the diagonal value enters at l = m; below it q stays 0
+                        const float e = 4 * b + j == m ? pmm : 0.f;
+                        const float q = fmaf(x2, q1, fmaf(-cc[j], q0, e));
+                        q0 = q1;
+                        q1 = q;
+                        const float t1 = q * zr, t2 = q * zi;
+                        ar[4 * b + j] = fmaf(t2, rr[j].z, fmaf(t1, rr[j].x, ar[4 * b + j]));
+                        ai[4 * b + j] = fmaf(t2, rr[j].w, fmaf(t1, rr[j].y, ai[4 * b + j]));
                     }
                 }
-                // the next diagonal term and e^{i (m+1) phi}
-                pmm = ab[(m + 1) * kAB + (m + 1)].x * st * pmm;
+                // the next row's diagonal value and phase
+                pmm = Cm[kCS + m + 1] * st * pmm;
                 const float t = zr * z1r - zi * z1i;
                 zi = fmaf(zr, z1i, zi * z1r);
                 zr = t;
             }
-            const float2* c = cp + SS.coff + pix;
-#pragma unroll
-            for (int l = 0; l < L; ++l) {
-                if (l < nl1) {
-                    const float2 v = c[(size_t)l * npix];
-                    acc_r = fmaf(v.x, ar[l], fmaf(-v.y, ai[l], acc_r));
-                    acc_i = fmaf(v.x, ai[l], fmaf(v.y, ar[l], acc_i));
-                }
-            }
+            harm_radial<L>(S, I.kmag, d, trig, A, B, ar, ai, acc_r, acc_i);
         }
         float2* o = U + I.u_off + pix;
         float2 v = make_float2(0.f, 0.f);
@@ -540,10 +662,10 @@ void launch_bucket(dim3 grid, hipStream_t st, const SeriesPlan& P, const SeriesP
 
 template <int L>
 void launch_harm(dim3 grid, hipStream_t st, const SeriesPlan& P, const SeriesPlan::Bucket& K,
-                 float2* U) {
+                 const double2* A, const double2* B, float2* U) {
     hipLaunchKernelGGL(series_harm_kernel<L>, grid, dim3(kThreads), 0, st, P.d_inst, P.d_sph,
-                       P.d_items + K.item_begin, K.item_count, P.d_coef, P.d_rhat, P.d_rec, P.d_ab,
-                       U, P.d_counters + K.counter);
+                       P.d_items + K.item_begin, K.item_count, A, B, P.d_rec, P.d_cst, U,
+                       P.d_counters + K.counter);
 }
 
 template <class T>
@@ -586,7 +708,7 @@ void series_plan_destroy(SeriesPlan* P) {
     (void)hipFree(P->d_part);
     (void)hipFree(P->d_coef);
     (void)hipFree(P->d_rhat);
-    void* harm[] = {P->d_pl, P->d_e, P->d_phi, P->d_tpart, P->d_rec, P->d_ab, P->d_zmap,
+    void* harm[] = {P->d_pl, P->d_e, P->d_phi, P->d_tpart, P->d_rec, P->d_cst, P->d_zmap,
                     P->d_zpart, P->d_zfirst};
     for (void* q : harm) (void)hipFree(q);
     *P = SeriesPlan();
@@ -681,6 +803,8 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
         for (; i1 < ni; ++i1) {
             const SeriesInst& I = batch.inst[i1];
             const long long npix = (long long)I.uR * I.vR;
+            // the harmonic form forms its coefficients in the pixel kernel
+            if (P->harmonic) continue;
             long long c = 0;
             for (int s = I.s_begin; s < I.s_begin + I.ns; ++s) c += 8LL * batch.sph[s].nb * npix;
             if (i1 > i0 && (size_t)(celems + c) * sizeof(float2) > coef_budget) break;
@@ -718,7 +842,7 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
     }
     std::vector<int2> zmap;
     std::vector<long long> zpart, zfirst;
-    std::vector<float2> ab;
+    std::vector<float> cst;
     if (P->harmonic) {
         long long npl = 0, ne = 0, nrec = 0, npairs = 0;
         for (int i = 0; i < ni; ++i) {
@@ -756,14 +880,20 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
         P->n_rec = nrec;
         P->n_tpart = 2 * npairs;
         P->n_z = (long long)zmap.size();
-        ab.assign((size_t)kAB * kAB, make_float2(0.f, 0.f));
-        for (int m = 0; m < kAB; ++m) {
-            if (m >= 1) ab[(size_t)m * kAB + m] = make_float2((float)std::sqrt((2.0 * m + 1.0) / (2.0 * m)), 0.f);
-            for (int l = m + 1; l < kAB; ++l) {
-                const double a = std::sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
-                const double b = std::sqrt(std::max(0.0, ((l - 1.0) * (l - 1.0) - (double)m * m) /
-                                                             (4.0 * (l - 1.0) * (l - 1.0) - 1.0)));
-                ab[(size_t)m * kAB + l] = make_float2((float)a, (float)b);
+        // c_lm = 4 b_lm / a_{l-1,m} (l >= m + 2) and the diagonal factors,
+        // a_lm = sqrt((4l^2-1)/(l^2-m^2)), b_lm = sqrt(((l-1)^2-m^2)/(4(l-1)^2-1))
+        auto a_ = [](int l, int m) {
+            return std::sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
+        };
+        // (one row more: the kernel reads the diagonal factor of row m + 1)
+        cst.assign((size_t)(kCS + 2) * kCS, 0.f);
+        for (int m = 0; m <= kCS; ++m) {
+            if (m >= 1) cst[(size_t)m * kCS + m] = (float)std::sqrt((2.0 * m + 1.0) / (2.0 * m));
+            if (m == kCS) break;
+            for (int l = m + 2; l < kCS; ++l) {
+                const double b = std::sqrt(((l - 1.0) * (l - 1.0) - (double)m * m) /
+                                           (4.0 * (l - 1.0) * (l - 1.0) - 1.0));
+                cst[(size_t)m * kCS + l] = (float)(4.0 * b / a_(l - 1, m));
             }
         }
     }
@@ -774,7 +904,7 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
         ((rc = upload(ctx, &P->d_zmap, zmap)) != BIMSIM_OK ||
          (rc = upload(ctx, &P->d_zpart, zpart)) != BIMSIM_OK ||
          (rc = upload(ctx, &P->d_zfirst, zfirst)) != BIMSIM_OK ||
-         (rc = upload(ctx, &P->d_ab, ab)) != BIMSIM_OK ||
+         (rc = upload(ctx, &P->d_cst, cst)) != BIMSIM_OK ||
          (rc = alloc(ctx, &P->d_pl, P->n_pl)) != BIMSIM_OK ||
          (rc = alloc(ctx, &P->d_e, P->n_e)) != BIMSIM_OK ||
          (rc = alloc(ctx, &P->d_phi, std::max<long long>(1, P->n_wave_rec))) != BIMSIM_OK ||
@@ -821,10 +951,12 @@ int series_plan_launch(bimsim_ctx* ctx, const SeriesPlan& P, const double* waves
     }
     BIMSIM_HIP_CHECK(ctx, hipGetLastError());
     for (const SeriesPlan::Chunk& ch : P.chunks) {
-        const dim3 cgrid((P.max_npix + kThreads - 1) / kThreads, std::min(ch.sph_count, 65535));
-        hipLaunchKernelGGL(series_coef_kernel, cgrid, dim3(kThreads), 0, st, P.d_inst, P.d_sph,
-                           ch.sph_begin, ch.sph_count, A, B, P.d_coef, P.d_rhat);
-        BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+        if (!P.harmonic) {
+            const dim3 cgrid((P.max_npix + kThreads - 1) / kThreads, std::min(ch.sph_count, 65535));
+            hipLaunchKernelGGL(series_coef_kernel, cgrid, dim3(kThreads), 0, st, P.d_inst, P.d_sph,
+                               ch.sph_begin, ch.sph_count, A, B, P.d_coef, P.d_rhat);
+            BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+        }
         const int tmr = timer_begin(ctx, kStageSeries);
         for (const SeriesPlan::Bucket& K : ch.buckets) {
             // resident workgroups: the queue keeps them busy until it drains
@@ -832,7 +964,7 @@ int series_plan_launch(bimsim_ctx* ctx, const SeriesPlan& P, const double* waves
             switch (K.L) {
 #define BIMSIM_CASE(LL) \
     case LL:                                                \
-        if (P.harmonic) launch_harm<LL>(grid, st, P, K, U); \
+        if (P.harmonic) launch_harm<LL>(grid, st, P, K, A, B, U); \
         else launch_bucket<LL>(grid, st, P, K, U);          \
         break;
                 BIMSIM_CASE(16)

@@ -8,7 +8,10 @@
 // The complex helpers follow the C99/glibc formulas that the reference's
 // std::complex<double> calls reach on Linux: Smith's quotient (libgcc
 // __divdc3), csqrt, clog via hypot/atan2, ccos/csin via cosh/sinh, and
-// pow(z, v) = polar(exp(v log|z|), v arg z) (libstdc++).
+// pow(z, v) = polar(exp(v log|z|), v arg z) (libstdc++).  The order
+// recurrences of CJYVA multiply by 1/z (one Smith quotient per call) where
+// the reference divides by z at every step: a rounding-level difference
+// that takes a long chain of dependent divisions off a device thread.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -305,10 +308,11 @@ __device__ inline void cjyva(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyv
     }
     cjv[0] = j0;
     cjv[1] = j1;
+    const zc zr = rdiv(1.0, z);
     if (n >= 2 && n <= (int)(0.25 * a0)) {
         zc f0 = j0, f1 = j1;
         for (int k = 2; k <= n; ++k) {
-            const zc f = zsub(zdiv(zsc(2.0 * (k + v0 - 1.0), f1), z), f0);
+            const zc f = zsub(zmul(zsc(2.0 * (k + v0 - 1.0), f1), zr), f0);
             cjv[k] = f;
             f0 = f1;
             f1 = f;
@@ -323,7 +327,7 @@ __device__ inline void cjyva(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyv
         }
         zc f2 = zero, f1 = {1.0e-100, 0.0}, f = zero;
         for (int k = m; k >= 0; --k) {
-            f = zsub(zdiv(zsc(2.0 * (v0 + k + 1.0), f1), z), f2);
+            f = zsub(zmul(zsc(2.0 * (v0 + k + 1.0), f1), zr), f2);
             if (k <= n) cjv[k] = f;
             f2 = f1;
             f1 = f;
@@ -331,8 +335,8 @@ __device__ inline void cjyva(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyv
         const zc cs = zabs(j0) > zabs(j1) ? zdiv(j0, f) : zdiv(j1, f2);
         for (int k = 0; k <= n; ++k) cjv[k] = zmul(cjv[k], cs);
     }
-    cjvp[0] = zsub(zdiv(zsc(v0, cjv[0]), z), cjv[1]);
-    for (int k = 1; k <= n; ++k) cjvp[k] = zadd(zdiv(zsc(-(k + v0), cjv[k]), z), cjv[k - 1]);
+    cjvp[0] = zsub(zmul(zsc(v0, cjv[0]), zr), cjv[1]);
+    for (int k = 1; k <= n; ++k) cjvp[k] = zadd(zmul(zsc(-(k + v0), cjv[k]), zr), cjv[k - 1]);
     cyv[0] = y0;
     cyv[1] = y1;
     const double ya0 = zabs(y0);
@@ -340,7 +344,7 @@ __device__ inline void cjyva(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyv
     {
         zc g0 = y0, g1 = y1;
         for (int k = 2; k <= n; ++k) {
-            const zc g = zsub(zdiv(zsc(2.0 * (v0 + k - 1.0), g1), z), g0);
+            const zc g = zsub(zmul(zsc(2.0 * (v0 + k - 1.0), g1), zr), g0);
             const double yak = zabs(g), ya1 = zabs(g0);
             if (yak < ya0 && yak < ya1) lb = k;
             cyv[k] = g;
@@ -357,7 +361,7 @@ __device__ inline void cjyva(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyv
             zc h0 = zero, h1 = zero, h2 = one;
             lb0 = lb;
             for (int k = lb; k >= 1; --k) {
-                h0 = zsub(zdiv(zsc(2.0 * (k + v0), h1), z), h2);
+                h0 = zsub(zmul(zsc(2.0 * (k + v0), h1), zr), h2);
                 h2 = h1;
                 h1 = h0;
             }
@@ -365,13 +369,13 @@ __device__ inline void cjyva(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyv
             h2 = zero;
             h1 = one;
             for (int k = lb; k >= 1; --k) {
-                h0 = zsub(zdiv(zsc(2.0 * (k + v0), h1), z), h2);
+                h0 = zsub(zmul(zsc(2.0 * (k + v0), h1), zr), h2);
                 h2 = h1;
                 h1 = h0;
             }
             const zc p11 = h0, p21 = h2;
             if (lb == n)
-                cjv[lb + 1] = zsub(zdiv(zsc(2.0 * (lb + v0), cjv[lb]), z), cjv[lb - 1]);
+                cjv[lb + 1] = zsub(zmul(zsc(2.0 * (lb + v0), cjv[lb]), zr), cjv[lb - 1]);
             if (zabs(cjv[0]) > zabs(cjv[1])) {
                 cyv[lb + 1] = zdiv(zsub(zmul(cjv[lb + 1], y0), zdiv(zsc(2.0, p11), piz)), cjv[0]);
                 cyv[lb] = zdiv(zadd(zmul(cjv[lb], y0), zdiv(zsc(2.0, p12), piz)), cjv[0]);
@@ -381,7 +385,7 @@ __device__ inline void cjyva(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyv
             }
             zc yl2 = cyv[lb + 1], yl1 = cyv[lb];
             for (int k = lb - 1; k >= 0; --k) {
-                const zc yk = zsub(zdiv(zsc(2.0 * (k + v0 + 1.0), yl1), z), yl2);
+                const zc yk = zsub(zmul(zsc(2.0 * (k + v0 + 1.0), yl1), zr), yl2);
                 cyv[k] = yk;
                 yl2 = yl1;
                 yl1 = yk;
@@ -389,7 +393,7 @@ __device__ inline void cjyva(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyv
             yl1 = cyv[lb];
             yl2 = cyv[lb + 1];
             for (int k = lb + 1; k < n; ++k) {
-                const zc yk = zsub(zdiv(zsc(2.0 * (k + v0), yl2), z), yl1);
+                const zc yk = zsub(zmul(zsc(2.0 * (k + v0), yl2), zr), yl1);
                 cyv[k + 1] = yk;
                 yl1 = yl2;
                 yl2 = yk;
@@ -398,8 +402,8 @@ __device__ inline void cjyva(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyv
                 if (zabs(cyv[k]) < zabs(cyv[k - 1])) lb = k;
         }
     }
-    cyvp[0] = zsub(zdiv(zsc(v0, cyv[0]), z), cyv[1]);
-    for (int k = 1; k <= n; ++k) cyvp[k] = zsub(cyv[k - 1], zdiv(zsc(k + v0, cyv[k]), z));
+    cyvp[0] = zsub(zmul(zsc(v0, cyv[0]), zr), cyv[1]);
+    for (int k = 1; k <= n; ++k) cyvp[k] = zsub(cyv[k - 1], zmul(zsc(k + v0, cyv[k]), zr));
 }
 
 // Real-argument J_{v0+k}, Y_{v0+k} and derivatives, k = 0..floor(v) (JYV,
@@ -570,11 +574,12 @@ __device__ inline void sph_jy_cplx(int nl, zc z, bool quirks, zc* j, zc* y, zc* 
     cjyva(nl + 0.5, z, j, y, jp, yp, quirks);
     const double pi = quirks ? 3.14159 : M_PI;
     const zc z2 = zsc(2.0, z);
+    // the reference evaluates these two per order; they do not depend on it
+    const zc s = zsqrt(rdiv(pi, z2));
+    const zc inv = rdiv(-1.0, z2);
     for (int k = 0; k <= nl; ++k) {
-        const zc s = zsqrt(rdiv(pi, z2));
         j[k] = zmul(j[k], s);
         y[k] = zmul(y[k], s);
-        const zc inv = rdiv(-1.0, z2);
         jp[k] = zadd(zmul(inv, j[k]), zmul(jp[k], s));
         yp[k] = zadd(zmul(inv, y[k]), zmul(yp[k], s));
     }
