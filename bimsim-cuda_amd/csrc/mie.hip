@@ -3,9 +3,11 @@
 // zj_device.h.  Built with -ffp-contract=off.
 //
 // The work is a few long chains of dependent FP64 steps per sphere, so the
-// kernels spread it over lanes: a block of 64 lanes takes 32 spheres, lane
-// 2s computes the Bessel set of ka and lane 2s+1 that of kna for sphere s;
-// then every lane takes (sphere, order) pairs of the coefficient formula.
+// kernels spread it over lanes and keep it out of global memory: a block
+// takes up to 32 spheres, lane 2s computes the Bessel set of ka and lane
+// 2s+1 that of kna for sphere s into LDS (the recurrences reread their
+// arrays, so LDS latency instead of a global round trip per step), then
+// every lane takes (sphere, order) pairs of the coefficient formula.
 #include "batch.h"
 #include "internal.h"
 #include "zj_device.h"
@@ -13,7 +15,8 @@
 namespace bimsim {
 namespace {
 
-constexpr int kSphPerLaunch = 32;   // spheres per block (two lanes each)
+constexpr int kSphPerLaunch = 32;   // spheres per launch of the single-call entry
+constexpr size_t kMieLds = 61440;   // LDS budget of a block's Bessel arrays (plus 2 KB static)
 
 struct SphereBatch {
     SphereDev s[kSphPerLaunch];
@@ -21,17 +24,23 @@ struct SphereBatch {
 
 using zj::zc;
 
-// Bessel set of one argument of sphere S (kind 0: ka, 1: kna) into the
-// sphere's work (8 arrays of wstride): j, y, j', y' of ka, then of kna.
-__device__ void mie_bessel(const SphereDev& S, double kmag, int kind, int quirks, zc* w,
+// Spheres per block so that 2 lanes x 4 arrays x wstride complex values
+// per sphere fit the LDS budget.
+int mie_spheres_per_block(int wstride) {
+    const size_t per = 2 * 4 * sizeof(zc) * (size_t)wstride;
+    return (int)std::max<size_t>(1, std::min<size_t>(kSphPerLaunch, kMieLds / per));
+}
+
+// Bessel set of one argument of sphere S (kind 0: ka, 1: kna): j, y, j', y'
+// at o, o + wstride, o + 2 wstride, o + 3 wstride.
+__device__ void mie_bessel(const SphereDev& S, double kmag, int kind, int quirks, zc* o,
                            int wstride) {
-    zc* o = w + 4 * kind * wstride;
     const zc nc = {S.n[0], S.n[1]};
     const zc z = kind == 0 ? zc{kmag * S.a, 0.0} : zj::zsc(S.a, zj::zsc(kmag, nc));
     zj::sph_jy_cplx(S.Nl, z, quirks != 0, o, o + wstride, o + 2 * wstride, o + 3 * wstride);
 }
 
-// A_l and B_l of sphere S from its Bessel sets.
+// A_l and B_l of sphere S from its two Bessel sets (w: ka set, then kna).
 __device__ void mie_order(const SphereDev& S, int l, const zc* w, int wstride, double2* A,
                           double2* B) {
     const zc *j1 = w, *y1 = w + wstride, *jp1 = w + 2 * wstride, *yp1 = w + 3 * wstride;
@@ -53,51 +62,72 @@ __device__ void mie_order(const SphereDev& S, int l, const zc* w, int wstride, d
 }
 
 // Coefficients of the block's cnt spheres (sh, wavenumbers kh, in LDS);
-// work holds 8 * wstride complex values per sphere of the block.
-__device__ void mie_block(const SphereDev* sh, const double* kh, int cnt, int quirks, zc* work,
-                          int wstride, double2* A, double2* B) {
+// lds holds 8 wstride complex values per sphere.  Two lanes per sphere.
+__device__ __forceinline__ void mie_block(const SphereDev* sh, const double* kh, int cnt,
+                                          int quirks, zc* lds, int wstride, double2* A,
+                                          double2* B) {
     const int t = (int)threadIdx.x;
     const int s = t >> 1;
-    if (s < cnt) mie_bessel(sh[s], kh[s], t & 1, quirks, work + (size_t)s * 8 * wstride, wstride);
+    if (s < cnt)
+        mie_bessel(sh[s], kh[s], t & 1, quirks, lds + (size_t)(8 * s + 4 * (t & 1)) * wstride,
+                   wstride);
     __syncthreads();
     int lm = 0;
     for (int i = 0; i < cnt; ++i) lm = max(lm, sh[i].Nl);
     const int per = lm + 1;
     for (int idx = t; idx < cnt * per; idx += (int)blockDim.x) {
         const int i = idx / per, l = idx - i * per;
-        if (l <= sh[i].Nl) mie_order(sh[i], l, work + (size_t)i * 8 * wstride, wstride, A, B);
+        if (l <= sh[i].Nl) mie_order(sh[i], l, lds + (size_t)8 * i * wstride, wstride, A, B);
     }
 }
 
+// Spheres [b spb, b spb + spb) of the batch passed by value, at one kmag.
+// kGlobal: the Bessel arrays (8 wstride values per sphere) are in work,
+// for orders too high for LDS; otherwise in LDS (a separate instantiation,
+// so that its accesses compile to LDS instructions).
+template <bool kGlobal>
 __global__ __launch_bounds__(2 * kSphPerLaunch) void mie_coeff_kernel(
     SphereBatch batch, int ns, double kmag, int quirks, zc* work, int wstride, double2* A,
     double2* B) {
+    extern __shared__ zc lds[];
     __shared__ SphereDev sh[kSphPerLaunch];
     __shared__ double kh[kSphPerLaunch];
+    const int spb = (int)blockDim.x / 2;
+    const int s0 = (int)blockIdx.x * spb;
+    const int cnt = min(spb, ns - s0);
     const int t = (int)threadIdx.x;
-    if (t < ns) {
-        sh[t] = batch.s[t];
+    if (t < cnt) {
+        sh[t] = batch.s[s0 + t];
         kh[t] = kmag;
     }
     __syncthreads();
-    mie_block(sh, kh, ns, quirks, work, wstride, A, B);
+    if (kGlobal)
+        mie_block(sh, kh, cnt, quirks, work + (size_t)8 * s0 * wstride, wstride, A, B);
+    else
+        mie_block(sh, kh, cnt, quirks, lds, wstride, A, B);
 }
 
-// Spheres [32 b, 32 b + 32) of a batch, sphere t at wavenumber kmag[t].
+// Spheres [b spb, b spb + spb) of a batch, sphere t at wavenumber kmag[t].
+template <bool kGlobal>
 __global__ __launch_bounds__(2 * kSphPerLaunch) void mie_batch_kernel(
     const SphereDev* __restrict__ sph, const double* __restrict__ kmag, int n, int quirks,
     zc* work, int wstride, double2* A, double2* B) {
+    extern __shared__ zc lds[];
     __shared__ SphereDev sh[kSphPerLaunch];
     __shared__ double kh[kSphPerLaunch];
-    const int s0 = (int)blockIdx.x * kSphPerLaunch;
-    const int cnt = min(kSphPerLaunch, n - s0);
+    const int spb = (int)blockDim.x / 2;
+    const int s0 = (int)blockIdx.x * spb;
+    const int cnt = min(spb, n - s0);
     const int t = (int)threadIdx.x;
     if (t < cnt) {
         sh[t] = sph[s0 + t];
         kh[t] = kmag[s0 + t];
     }
     __syncthreads();
-    mie_block(sh, kh, cnt, quirks, work + (size_t)s0 * 8 * wstride, wstride, A, B);
+    if (kGlobal)
+        mie_block(sh, kh, cnt, quirks, work + (size_t)8 * s0 * wstride, wstride, A, B);
+    else
+        mie_block(sh, kh, cnt, quirks, lds, wstride, A, B);
 }
 
 }  // namespace
@@ -105,9 +135,20 @@ __global__ __launch_bounds__(2 * kSphPerLaunch) void mie_batch_kernel(
 int mie_batch_launch(bimsim_ctx* ctx, const SphereDev* d_sph, const double* d_kmag, int n, int lmax,
                      int quirks, void* work, double2* A, double2* B) {
     if (n <= 0) return BIMSIM_OK;
-    hipLaunchKernelGGL(mie_batch_kernel, dim3((n + kSphPerLaunch - 1) / kSphPerLaunch),
-                       dim3(2 * kSphPerLaunch), 0, ctx->stream, d_sph, d_kmag, n, quirks, (zc*)work,
-                       lmax + 2, A, B);
+    const int wstride = lmax + 2;
+    const int spb = mie_spheres_per_block(wstride);
+    size_t lds = sizeof(zc) * 8 * (size_t)wstride * spb;
+    zc* w = nullptr;
+    if (lds > std::min(kMieLds, ctx->lds_max)) {   // very high orders: global arrays
+        lds = 0;
+        w = (zc*)work;
+    }
+    if (w)
+        hipLaunchKernelGGL(mie_batch_kernel<true>, dim3((n + spb - 1) / spb), dim3(2 * spb), 0,
+                           ctx->stream, d_sph, d_kmag, n, quirks, w, wstride, A, B);
+    else
+        hipLaunchKernelGGL(mie_batch_kernel<false>, dim3((n + spb - 1) / spb), dim3(2 * spb), lds,
+                           ctx->stream, d_sph, d_kmag, n, quirks, w, wstride, A, B);
     BIMSIM_HIP_CHECK(ctx, hipGetLastError());
     return BIMSIM_OK;
 }
@@ -133,9 +174,14 @@ extern "C" int bimsim_mie_coeffs(bimsim_ctx* ctx, const bimsim_sphere* spheres, 
     }
     BIMSIM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     const int wstride = lmax + 2;
+    const int spb = mie_spheres_per_block(wstride);
+    size_t lds = sizeof(zj::zc) * 8 * (size_t)wstride * spb;
     void* work = nullptr;
-    BIMSIM_TRY(scratch_get(ctx, kSlotMisc,
-                           sizeof(zj::zc) * 8 * (size_t)wstride * kSphPerLaunch, &work));
+    if (lds > std::min(kMieLds, ctx->lds_max)) {   // very high orders: global arrays
+        lds = 0;
+        BIMSIM_TRY(scratch_get(ctx, kSlotMisc,
+                               sizeof(zj::zc) * 8 * (size_t)wstride * kSphPerLaunch, &work));
+    }
     for (int s0 = 0; s0 < ns; s0 += kSphPerLaunch) {
         const int cnt = std::min(kSphPerLaunch, ns - s0);
         SphereBatch b{};
@@ -149,9 +195,14 @@ extern "C" int bimsim_mie_coeffs(bimsim_ctx* ctx, const bimsim_sphere* spheres, 
             d.Nl = S.Nl;
             d.coeff_off = S.coeff_off;
         }
-        hipLaunchKernelGGL(mie_coeff_kernel, dim3(1), dim3(2 * kSphPerLaunch), 0, ctx->stream, b,
-                           cnt, kmag, quirks, (zj::zc*)work, wstride, (double2*)A_out,
-                           (double2*)B_out);
+        if (work)
+            hipLaunchKernelGGL(mie_coeff_kernel<true>, dim3((cnt + spb - 1) / spb), dim3(2 * spb), 0,
+                               ctx->stream, b, cnt, kmag, quirks, (zj::zc*)work, wstride,
+                               (double2*)A_out, (double2*)B_out);
+        else
+            hipLaunchKernelGGL(mie_coeff_kernel<false>, dim3((cnt + spb - 1) / spb), dim3(2 * spb),
+                               lds, ctx->stream, b, cnt, kmag, quirks, nullptr, wstride,
+                               (double2*)A_out, (double2*)B_out);
         BIMSIM_HIP_CHECK(ctx, hipGetLastError());
     }
     return BIMSIM_OK;

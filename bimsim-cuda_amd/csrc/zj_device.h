@@ -9,9 +9,10 @@
 // std::complex<double> calls reach on Linux: Smith's quotient (libgcc
 // __divdc3), csqrt, clog via hypot/atan2, ccos/csin via cosh/sinh, and
 // pow(z, v) = polar(exp(v log|z|), v arg z) (libstdc++).  The order
-// recurrences of CJYVA multiply by 1/z (one Smith quotient per call) where
-// the reference divides by z at every step: a rounding-level difference
-// that takes a long chain of dependent divisions off a device thread.
+// recurrences and the asymptotic series of CJYVA multiply by 1/z and 1/z^2
+// (one Smith quotient each per call) where the reference divides at every
+// step: a rounding-level difference that takes long chains of dependent
+// divisions off a device thread.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -226,18 +227,19 @@ __device__ inline void cjyva(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyv
         }
     } else {
         const int kz = a0 >= 50.0 ? 8 : (a0 >= 35.0 ? 10 : 11);
+        const zc z2r = rdiv(1.0, z2);   // the terms divide by k(2k -+ 1) z^2
         for (int l = 0; l < 2; ++l) {
             const double mu = 4.0 * (l + v0) * (l + v0);
             zc p = one, tp = one;
             for (int k = 1; k <= kz; ++k) {
                 zc num = zsc(mu - sq(4.0 * k - 1.0), zsc(mu - sq(4.0 * k - 3.0), zsc(-0.78125e-2, tp)));
-                tp = zdiv(num, zsc(k * (2.0 * k - 1.0), z2));
+                tp = zmul(zsc(1.0 / (k * (2.0 * k - 1.0)), num), z2r);
                 p = zadd(p, tp);
             }
             zc q = one, tq = one;
             for (int k = 1; k <= kz; ++k) {
                 zc num = zsc(mu - sq(4.0 * k + 1.0), zsc(mu - sq(4.0 * k - 1.0), zsc(-0.78125e-2, tq)));
-                tq = zdiv(num, zsc(k * (2.0 * k + 1.0), z2));
+                tq = zmul(zsc(1.0 / (k * (2.0 * k + 1.0)), num), z2r);
                 q = zadd(q, tq);
             }
             q = zmul(q, rdiv(0.125 * (mu - 1.0), z1));
@@ -342,14 +344,18 @@ __device__ inline void cjyva(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyv
     const double ya0 = zabs(y0);
     int lb = 0;
     {
+        // |y_{k-2}| and |y_{k-1}| carried along (one magnitude per order)
         zc g0 = y0, g1 = y1;
+        double a_g0 = ya0, a_g1 = zabs(y1);
         for (int k = 2; k <= n; ++k) {
             const zc g = zsub(zmul(zsc(2.0 * (v0 + k - 1.0), g1), zr), g0);
-            const double yak = zabs(g), ya1 = zabs(g0);
+            const double yak = zabs(g), ya1 = a_g0;
             if (yak < ya0 && yak < ya1) lb = k;
             cyv[k] = g;
             g0 = g1;
             g1 = g;
+            a_g0 = a_g1;
+            a_g1 = yak;
         }
     }
     int lb0 = 0;
@@ -398,8 +404,12 @@ __device__ inline void cjyva(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyv
                 yl1 = yl2;
                 yl2 = yk;
             }
-            for (int k = 2; k <= n; ++k)
-                if (zabs(cyv[k]) < zabs(cyv[k - 1])) lb = k;
+            double a_prev = zabs(cyv[1]);
+            for (int k = 2; k <= n; ++k) {
+                const double a_k = zabs(cyv[k]);
+                if (a_k < a_prev) lb = k;
+                a_prev = a_k;
+            }
         }
     }
     cyvp[0] = zsub(zmul(zsc(v0, cyv[0]), zr), cyv[1]);
@@ -578,10 +588,11 @@ __device__ inline void sph_jy_cplx(int nl, zc z, bool quirks, zc* j, zc* y, zc* 
     const zc s = zsqrt(rdiv(pi, z2));
     const zc inv = rdiv(-1.0, z2);
     for (int k = 0; k <= nl; ++k) {
-        j[k] = zmul(j[k], s);
-        y[k] = zmul(y[k], s);
-        jp[k] = zadd(zmul(inv, j[k]), zmul(jp[k], s));
-        yp[k] = zadd(zmul(inv, y[k]), zmul(yp[k], s));
+        const zc jk = zmul(j[k], s), yk = zmul(y[k], s), jpk = jp[k], ypk = yp[k];
+        j[k] = jk;
+        y[k] = yk;
+        jp[k] = zadd(zmul(inv, jk), zmul(jpk, s));
+        yp[k] = zadd(zmul(inv, yk), zmul(ypk, s));
     }
 }
 
