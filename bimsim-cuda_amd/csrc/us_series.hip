@@ -549,18 +549,20 @@ __device__ __forceinline__ void harm_radial(const SphereDev& S, double kmag, dou
     }
 }
 
+// Waves per SIMD the harmonic kernel is compiled for: the order sums take
+// 2L VGPRs, so small buckets can run four waves per SIMD.
+constexpr int harm_waves(int L) { return L <= 32 ? 4 : (L <= 48 ? 3 : (L <= 80 ? 2 : 1)); }
+
 // Per-pixel harmonic sum, persistent over (instance, pixel block) items.
 // Row m of the order sums runs the scaled recurrence
 //   q_mm = Pn_mm,  q_lm = 2x q_{l-1,m} - c_lm q_{l-2,m}
 // (Pn_lm = K_lm q_lm, K_lm = prod_{j=m+1..l} a_jm / 2, folded into the
 // records) in blocks of four orders: one scalar load of four records and
-// one of four constants per block, then straight-line FP32 code; only the
-// block holding the diagonal branches.  cst[m * kCS + l] holds c_lm
-// (l >= m + 2) and the diagonal factor sqrt((2m+1)/(2m)) at l = m.
-// Waves per SIMD the harmonic kernel is compiled for: the order sums take
-// 2L VGPRs, so small buckets can run four waves per SIMD.
-constexpr int harm_waves(int L) { return L <= 32 ? 4 : (L <= 48 ? 3 : (L <= 80 ? 2 : 1)); }
-
+// one of their constants per block, then straight-line FP32 code (blocks
+// wholly below the diagonal or past Nl are skipped).  cst[2 (m kCS + l)]
+// holds c_lm (l >= m + 2) and the diagonal factor sqrt((2m+1)/(2m)) at
+// l = m (where the recurrence does not read c); cst[2 (m kCS + l) + 1] is 1
+// at l = m, else 0, and injects the diagonal value (q is 0 below it).
 template <int L>
 __global__ __launch_bounds__(kThreads, harm_waves(L)) void series_harm_kernel(
     const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph,
@@ -607,37 +609,60 @@ __global__ __launch_bounds__(kThreads, harm_waves(L)) void series_harm_kernel(
             const int be = (nl1 + 3) >> 2;
             float zr = 1.f, zi = 0.f;
             float pmm = 0.28209479177387814f;   // 1/sqrt(4 pi)
+            // rows in pairs (m, m + 1), m even: both start in block m / 4, so
+            // one wait covers two rows' loads and the two recurrences
+            // interleave (a row m + 1 = Nl + 1 has zero records)
 #pragma clang loop unroll(disable)
-            for (int m = 0; m < nl1; ++m) {
-                const float* Cm = cst + m * kCS;
-                const float4* Rm = R + m * L;
+            for (int m = 0; m < nl1; m += 2) {
+                const float* Ca = cst + 2 * m * kCS;
+                const float* Cb = Ca + 2 * kCS;
+                const float4* Ra = R + m * L;
+                const float4* Rb = Ra + L;
+                const float pa = pmm, pb = Cb[2 * (m + 1)] * st * pmm;
+                const float zar = zr, zai = zi;
+                const float zbr = zr * z1r - zi * z1i, zbi = fmaf(zr, z1i, zi * z1r);
                 const int bf = m >> 2;
-                float q0 = 0.f, q1 = 0.f;
+                float qa0 = 0.f, qa1 = 0.f, qb0 = 0.f, qb1 = 0.f;
 #pragma clang loop unroll(full)
                 for (int b = 0; b < L / 4; ++b) {
                     if (b < bf || b >= be) continue;
-                    const float4 c4 = *reinterpret_cast<const float4*>(Cm + 4 * b);
-                    const float cc[4] = {c4.x, c4.y, c4.z, c4.w};
-                    float4 rr[4];
-#pragma clang loop unroll(full)
-                    for (int j = 0; j < 4; ++j) rr[j] = Rm[4 * b + j];
+                    const float4 ca0 = *reinterpret_cast<const float4*>(Ca + 8 * b);
+                    const float4 ca1 = *reinterpret_cast<const float4*>(Ca + 8 * b + 4);
+                    const float4 cb0 = *reinterpret_cast<const float4*>(Cb + 8 * b);
+                    const float4 cb1 = *reinterpret_cast<const float4*>(Cb + 8 * b + 4);
+                    const float cca[4] = {ca0.x, ca0.z, ca1.x, ca1.z};
+                    const float eea[4] = {ca0.y, ca0.w, ca1.y, ca1.w};
+                    const float ccb[4] = {cb0.x, cb0.z, cb1.x, cb1.z};
+                    const float eeb[4] = {cb0.y, cb0.w, cb1.y, cb1.w};
+                    float4 ra[4], rb[4];
 #pragma clang loop unroll(full)
                     for (int j = 0; j < 4; ++j) {
-                        // the diagonal value enters at l = m; below it q stays 0
-                        const float e = 4 * b + j == m ? pmm : 0.f;
-                        const float q = fmaf(x2, q1, fmaf(-cc[j], q0, e));
-                        q0 = q1;
-                        q1 = q;
-                        const float t1 = q * zr, t2 = q * zi;
-                        ar[4 * b + j] = fmaf(t2, rr[j].z, fmaf(t1, rr[j].x, ar[4 * b + j]));
-                        ai[4 * b + j] = fmaf(t2, rr[j].w, fmaf(t1, rr[j].y, ai[4 * b + j]));
+                        ra[j] = Ra[4 * b + j];
+                        rb[j] = Rb[4 * b + j];
+                    }
+#pragma clang loop unroll(full)
+                    for (int j = 0; j < 4; ++j) {
+                        // the diagonal value enters at l = m (ee = 1 there, else
+                        // 0); below it q stays 0
+                        const int l = 4 * b + j;
+                        const float qa = fmaf(x2, qa1, fmaf(-cca[j], qa0, eea[j] * pa));
+                        const float qb = fmaf(x2, qb1, fmaf(-ccb[j], qb0, eeb[j] * pb));
+                        qa0 = qa1;
+                        qa1 = qa;
+                        qb0 = qb1;
+                        qb1 = qb;
+                        const float ta1 = qa * zar, ta2 = qa * zai;
+                        const float tb1 = qb * zbr, tb2 = qb * zbi;
+                        ar[l] = fmaf(ta2, ra[j].z, fmaf(ta1, ra[j].x, ar[l]));
+                        ai[l] = fmaf(ta2, ra[j].w, fmaf(ta1, ra[j].y, ai[l]));
+                        ar[l] = fmaf(tb2, rb[j].z, fmaf(tb1, rb[j].x, ar[l]));
+                        ai[l] = fmaf(tb2, rb[j].w, fmaf(tb1, rb[j].y, ai[l]));
                     }
                 }
-                // the next row's diagonal value and phase
-                pmm = Cm[kCS + m + 1] * st * pmm;
-                const float t = zr * z1r - zi * z1i;
-                zi = fmaf(zr, z1i, zi * z1r);
-                zr = t;
+                // the next pair's diagonal value and phase
+                pmm = Cb[2 * (kCS + m + 2)] * st * pb;
+                zr = zbr * z1r - zbi * z1i;
+                zi = fmaf(zbr, z1i, zbi * z1r);
             }
             harm_radial<L>(S, I.kmag, d, trig, A, B, ar, ai, acc_r, acc_i);
         }
@@ -886,14 +911,15 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
             return std::sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
         };
         // (one row more: the kernel reads the diagonal factor of row m + 1)
-        cst.assign((size_t)(kCS + 2) * kCS, 0.f);
+        cst.assign((size_t)2 * (kCS + 2) * kCS, 0.f);
         for (int m = 0; m <= kCS; ++m) {
-            if (m >= 1) cst[(size_t)m * kCS + m] = (float)std::sqrt((2.0 * m + 1.0) / (2.0 * m));
+            if (m >= 1) cst[2 * ((size_t)m * kCS + m)] = (float)std::sqrt((2.0 * m + 1.0) / (2.0 * m));
             if (m == kCS) break;
+            cst[2 * ((size_t)m * kCS + m) + 1] = 1.f;
             for (int l = m + 2; l < kCS; ++l) {
                 const double b = std::sqrt(((l - 1.0) * (l - 1.0) - (double)m * m) /
                                            (4.0 * (l - 1.0) * (l - 1.0) - 1.0));
-                cst[(size_t)m * kCS + l] = (float)(4.0 * b / a_(l - 1, m));
+                cst[2 * ((size_t)m * kCS + l)] = (float)(4.0 * b / a_(l - 1, m));
             }
         }
     }
