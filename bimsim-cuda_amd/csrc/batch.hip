@@ -2,7 +2,8 @@
 // sequence (reference microscope.cu:849-857, nearfield.cpp:194-218) for many
 // instances at once, optionally captured into one HIP graph.
 //
-//   Mie coefficients   one launch, one thread per (instance, sphere)
+//   Mie coefficients   one launch, two lanes per (instance, sphere); for
+//                      FP32 on a side stream beside the next two rows
 //   incident fields    one launch over (pixel blocks, instances)
 //   scattered fields   FP32: coefficient kernel + one persistent work-queue
 //                      launch per order bucket (us_series.hip); FP64 forms:
@@ -44,6 +45,10 @@ struct bimsim_batch {
     SeriesPlan plan_mc, plan_pw;    // FP32 series: Monte-Carlo / plane-wave instances
     bool have_mc = false, have_pw = false;
     hipStream_t capture_stream = nullptr;
+    // FP32 passes compute the Mie coefficients on a stream of their own,
+    // beside the incident fields and wave tables (joined before the sums)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -71,6 +76,9 @@ void release(bimsim_batch* b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
+    if (b->side) (void)hipStreamDestroy(b->side);
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    if (b->ev_join) (void)hipEventDestroy(b->ev_join);
     series_plan_destroy(&b->plan_mc);
     series_plan_destroy(&b->plan_pw);
     void* ptrs[] = {b->d_sph, b->d_kmag, b->d_work, b->d_A, b->d_B, b->d_uf, b->d_uftab,
@@ -84,15 +92,29 @@ int enqueue(bimsim_batch* b) {
     bimsim_ctx* ctx = b->ctx;
     const size_t fb = b->f32 ? sizeof(float2) : sizeof(double2);
     const size_t rb = b->f32 ? sizeof(float) : sizeof(double);
-    BIMSIM_TRY(mie_batch_launch(ctx, b->d_sph, b->d_kmag, b->n_sph, b->lmax, b->quirks, b->d_work,
-                                b->d_A, b->d_B));
-    BIMSIM_TRY(uf_batch_launch(ctx, b->d_uf, b->n, b->max_npix, b->f32, b->d_uftab, b->d_Uf));
     if (b->f32) {
+        // fork: coefficients on the side stream, fields and wave tables here
+        BIMSIM_HIP_CHECK(ctx, hipEventRecord(b->ev_fork, ctx->stream));
+        BIMSIM_HIP_CHECK(ctx, hipStreamWaitEvent(b->side, b->ev_fork, 0));
+        const hipStream_t main = ctx->stream;
+        ctx->stream = b->side;
+        const int rc = mie_batch_launch(ctx, b->d_sph, b->d_kmag, b->n_sph, b->lmax, b->quirks,
+                                        b->d_work, b->d_A, b->d_B);
+        ctx->stream = main;
+        BIMSIM_TRY(rc);
+        BIMSIM_HIP_CHECK(ctx, hipEventRecord(b->ev_join, b->side));
+        BIMSIM_TRY(uf_batch_launch(ctx, b->d_uf, b->n, b->max_npix, b->f32, b->d_uftab, b->d_Uf));
+        if (b->have_mc) BIMSIM_TRY(series_plan_launch_waves(ctx, b->plan_mc, b->waves));
+        if (b->have_pw) BIMSIM_TRY(series_plan_launch_waves(ctx, b->plan_pw, b->d_kdirs));
+        BIMSIM_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, b->ev_join, 0));
         if (b->have_mc)
-            BIMSIM_TRY(series_plan_launch(ctx, b->plan_mc, b->waves, b->d_A, b->d_B, (float2*)b->d_U));
+            BIMSIM_TRY(series_plan_launch_sums(ctx, b->plan_mc, b->d_A, b->d_B, (float2*)b->d_U));
         if (b->have_pw)
-            BIMSIM_TRY(series_plan_launch(ctx, b->plan_pw, b->d_kdirs, b->d_A, b->d_B, (float2*)b->d_U));
+            BIMSIM_TRY(series_plan_launch_sums(ctx, b->plan_pw, b->d_A, b->d_B, (float2*)b->d_U));
     } else {
+        BIMSIM_TRY(mie_batch_launch(ctx, b->d_sph, b->d_kmag, b->n_sph, b->lmax, b->quirks,
+                                    b->d_work, b->d_A, b->d_B));
+        BIMSIM_TRY(uf_batch_launch(ctx, b->d_uf, b->n, b->max_npix, b->f32, b->d_uftab, b->d_Uf));
         for (int i = 0; i < b->n; ++i) {
             const bimsim_instance& I = b->inst[i];
             if (I.ns == 0) continue;
@@ -235,6 +257,10 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
     BIMSIM_TRY(dev_alloc(ctx, &b->d_Uf, fb * (size_t)field));
     BIMSIM_TRY(dev_alloc(ctx, &b->d_U, fb * (size_t)field));
     if (b->f32) {
+        if (hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess)
+            return set_error(ctx, BIMSIM_E_HIP, "cannot create the batch's side stream");
         // the series plans are timed on real inputs: coefficients first
         const size_t budget = (size_t)8 << 30;
         BIMSIM_TRY(mie_batch_launch(ctx, b->d_sph, b->d_kmag, b->n_sph, b->lmax, b->quirks,

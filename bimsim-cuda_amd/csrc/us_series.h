@@ -121,9 +121,14 @@ int series_plan_create_tuned(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_bu
 void series_plan_destroy(SeriesPlan* plan);
 
 // Launches the plan on the context stream; allocation- and
-// synchronisation-free, so it can be captured into a graph.
+// synchronisation-free, so it can be captured into a graph.  The two halves
+// can run apart: the wave tables (harmonic sums or wave records, from the
+// waves only) and the per-pixel sums (which read A and B).
 int series_plan_launch(bimsim_ctx* ctx, const SeriesPlan& plan, const double* waves,
                        const double2* A, const double2* B, float2* U);
+int series_plan_launch_waves(bimsim_ctx* ctx, const SeriesPlan& plan, const double* waves);
+int series_plan_launch_sums(bimsim_ctx* ctx, const SeriesPlan& plan, const double2* A,
+                            const double2* B, float2* U);
 
 // Order capacity for Nl + 1 orders, or -1.
 int series_capacity(int nl1);

@@ -956,6 +956,11 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
 
 int series_plan_launch(bimsim_ctx* ctx, const SeriesPlan& P, const double* waves, const double2* A,
                        const double2* B, float2* U) {
+    BIMSIM_TRY(series_plan_launch_waves(ctx, P, waves));
+    return series_plan_launch_sums(ctx, P, A, B, U);
+}
+
+int series_plan_launch_waves(bimsim_ctx* ctx, const SeriesPlan& P, const double* waves) {
     if (P.n_inst == 0 || P.n_sph == 0) return BIMSIM_OK;
     const hipStream_t st = ctx->stream;
     hipLaunchKernelGGL(zero_ints, dim3(1), dim3(256), 0, st, P.d_counters, std::max(1, P.n_counters));
@@ -976,6 +981,13 @@ int series_plan_launch(bimsim_ctx* ctx, const SeriesPlan& P, const double* waves
                            waves, P.d_wt);
     }
     BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return BIMSIM_OK;
+}
+
+int series_plan_launch_sums(bimsim_ctx* ctx, const SeriesPlan& P, const double2* A,
+                            const double2* B, float2* U) {
+    if (P.n_inst == 0 || P.n_sph == 0) return BIMSIM_OK;
+    const hipStream_t st = ctx->stream;
     for (const SeriesPlan::Chunk& ch : P.chunks) {
         if (!P.harmonic) {
             const dim3 cgrid((P.max_npix + kThreads - 1) / kThreads, std::min(ch.sph_count, 65535));
