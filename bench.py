@@ -31,7 +31,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 METRIC = "Mie series terms/s + absorbance images/s, batched FP32 vs plain FP64, 1 GPU"
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector, AMD published
-PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "series_f32_cfg2_traffic.md")
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "series_harm_cfg2_traffic.md")
 
 
 def load_pkg():

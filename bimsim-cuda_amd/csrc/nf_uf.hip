@@ -4,6 +4,8 @@
 // the pixel-invariant factors i^l D_l from a per-instance table and the
 // recurrences multiplying by reciprocals (explicit FMAs), and stores float
 // pairs.
+#include <type_traits>
+
 #include "geometry.h"
 #include "internal.h"
 #include "batch.h"
@@ -173,7 +175,7 @@ __global__ __launch_bounds__(kThreads) void uf_batch_kernel(const UfInst* __rest
             dc v;
             if (I.plane)
                 v = uf_plane_value(I.q, I.uR, I.vR, pix, I.f, I.k, I.kmag, I.amp);
-            else if (I.tab >= 0)
+            else if constexpr (std::is_same<Out, float2>::value)   // tables for every instance
                 v = uf_fast_value(I.q, I.uR, I.vR, pix, I.f, I.k, I.kmag, I.amp, I.ca, I.cb, I.m,
                                   tabs + I.tab);
             else
