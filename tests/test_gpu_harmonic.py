@@ -92,3 +92,28 @@ def test_harmonic_accumulates(pkg, ctx, orc, W):
     ctx.nf_us(*args, prec=pkg.FP32_FAST, accumulate=True)
     ctx.set_series_algorithm(pkg.SERIES_AUTO)
     assert normrel(host(U), 2 * once) <= 1e-6
+
+
+@pytest.mark.parametrize("a,bucket", [(9.0, 80), (11.0, 96), (13.5, 112)])
+def test_harmonic_high_order_buckets(pkg, ctx, orc, W, a, bucket):
+    """Spheres of order 74, 88 and 105 (buckets 80, 96, 112: the kernels
+    compiled for two and one waves per SIMD) against the oracle."""
+    import bimsim_amd.pipeline as P
+    inst = W.make_instance([dict(p=(3.0, -2.0, 0.5), a=a, n=complex(1.2, 0.01))], 1.0, 24, 40.0,
+                           (0.0, 0.5), (0.0, 0.5), 24, 11)
+    Nl = inst["spheres"][0]["Nl"]
+    assert bucket - 16 < Nl + 1 <= bucket
+    q = P.quad_of(inst)
+    N = inst["N"]
+    waves = W.mc_waves(inst, pkg.mc_sample_na)
+    A = torch.zeros(inst["ncoef"], dtype=torch.complex128, device="cuda")
+    B = torch.zeros_like(A)
+    ctx.mie_coeffs(inst["spheres"], inst["kmag"], A, B)
+    ref = orc.nf_us(q, N, N, inst["focus"], inst["kmag"], inst["spheres"], host(A), host(B), waves,
+                    inst["subA"]).ravel()
+    ctx.set_series_algorithm(pkg.SERIES_HARMONIC)
+    U = torch.empty(N * N, dtype=torch.complex64, device="cuda")
+    ctx.nf_us(q, N, N, inst["focus"], inst["kmag"], inst["spheres"], A, B,
+              torch.as_tensor(waves).cuda(), inst["subA"], U, prec=pkg.FP32_FAST)
+    ctx.set_series_algorithm(pkg.SERIES_AUTO)
+    assert normrel(host(U).astype(np.complex128), ref) <= 2e-5
