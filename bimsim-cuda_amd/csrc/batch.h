@@ -16,11 +16,13 @@ struct UfInst {
     long long tab;   // complex64 fields: offset of its uf_fast_table, else -1
 };
 
-// Uf + Us sum of one instance.
+// Uf + Us sum of one instance, and the clearing of its detector images.
 struct SumInst {
     QuadArg q;
     int uR, vR, s_begin, ns;
     long long off;
+    long long image_off;   // its Nd x Nd images in D and Di
+    int nd2;               // Nd * Nd (<= uR * vR)
 };
 
 // Mie coefficients of n spheres, sphere i at wavenumber kmag[i]; writes
@@ -40,9 +42,10 @@ int uf_batch_launch(bimsim_ctx* ctx, const UfInst* d_inst, int n, int max_npix, 
                     const double* tabs, void* Uf);
 
 // U = Uf or U += Uf outside the spheres, for n instances; d_sph holds
-// (x, y, z, a) per sphere.
+// (x, y, z, a) per sphere.  Also zeroes each instance's images in D and Di
+// (float when f32, else double), which the detector then accumulates into.
 int sum_batch_launch(bimsim_ctx* ctx, const SumInst* d_inst, int n, int max_npix,
-                     const double4* d_sph, int f32, const void* Uf, void* U);
+                     const double4* d_sph, int f32, const void* Uf, void* U, void* D, void* Di);
 
 // Zeroes `bytes` (a multiple of 4) at p with a kernel (graph-safe).
 int zero_launch(bimsim_ctx* ctx, void* p, size_t bytes);

@@ -125,9 +125,8 @@ int enqueue(bimsim_batch* b) {
                                     (char*)b->d_U + fb * b->field_off[i], 0));
         }
     }
-    BIMSIM_TRY(sum_batch_launch(ctx, b->d_sum, b->n, b->max_npix, b->d_sph4, b->f32, b->d_Uf, b->d_U));
-    BIMSIM_TRY(zero_launch(ctx, b->D, rb * b->image_elems));
-    BIMSIM_TRY(zero_launch(ctx, b->Di, rb * b->image_elems));
+    BIMSIM_TRY(sum_batch_launch(ctx, b->d_sum, b->n, b->max_npix, b->d_sph4, b->f32, b->d_Uf, b->d_U,
+                                b->D, b->Di));
     for (int i = 0; i < b->n; ++i) {
         const bimsim_instance& I = b->inst[i];
         void* Ab = b->Abs ? (char*)b->Abs + rb * I.image_off : nullptr;
@@ -193,7 +192,7 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
         }
         uf[i] = UfInst{q, mk(I.focus), mk(I.kdir), b->kmag[i], I.amp, ca, cb, I.m, I.N, I.N,
                        I.plane_wave ? 1 : 0, field, tab};
-        sum[i] = SumInst{q, I.N, I.N, (int)sph_dev.size(), I.ns, field};
+        sum[i] = SumInst{q, I.N, I.N, (int)sph_dev.size(), I.ns, field, I.image_off, I.Nd * I.Nd};
         for (int c = 0; c < 3; ++c) kdirs[3 * (size_t)i + c] = I.kdir[c];
         // spheres of this instance, coefficients packed
         SeriesBatch& sb = I.plane_wave ? pw : mc;

@@ -145,7 +145,7 @@ __global__ void bessel_lut_kernel(int nl, double kmag, double nr, double ni, dou
     // k*n*r as (k*n)*r with k*n scaling each component
     const zj::zc knr = {(kmag * nr) * r, (kmag * ni) * r};
     zj::zc* j = work + (size_t)ir * 4 * wstride;
-    zj::sph_jy_cplx(nl, knr, true, j, j + wstride, j + 2 * wstride, j + 3 * wstride);
+    zj::sph_jy_cplx(nl, knr, true, j, j + wstride, j + 2 * wstride, j + 3 * wstride, false);
     for (int l = 0; l <= nl; ++l) jl[(size_t)ir * (nl + 1) + l] = make_double2(j[l].r, j[l].i);
 }
 

@@ -32,12 +32,14 @@ int mie_spheres_per_block(int wstride) {
 }
 
 // Bessel set of one argument of sphere S (kind 0: ka, 1: kna): j, y, j', y'
-// at o, o + wstride, o + 2 wstride, o + 3 wstride.
+// at o, o + wstride, o + 2 wstride, o + 3 wstride (y, y' of kna stay 0).
 __device__ void mie_bessel(const SphereDev& S, double kmag, int kind, int quirks, zc* o,
                            int wstride) {
     const zc nc = {S.n[0], S.n[1]};
     const zc z = kind == 0 ? zc{kmag * S.a, 0.0} : zj::zsc(S.a, zj::zsc(kmag, nc));
-    zj::sph_jy_cplx(S.Nl, z, quirks != 0, o, o + wstride, o + 2 * wstride, o + 3 * wstride);
+    // the coefficients use y and y' of ka only
+    zj::sph_jy_cplx(S.Nl, z, quirks != 0, o, o + wstride, o + 2 * wstride, o + 3 * wstride,
+                    kind == 0);
 }
 
 // A_l and B_l of sphere S from its two Bessel sets (w: ka set, then kna).

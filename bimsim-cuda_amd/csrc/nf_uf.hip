@@ -187,15 +187,20 @@ __global__ __launch_bounds__(kThreads) void uf_batch_kernel(const UfInst* __rest
 
 // U = Uf (no spheres) or U += Uf outside every sphere (d < a is inside,
 // nfSumUf.cu:31-48) for a batch of instances.  Spheres as (x, y, z, a).
-template <class C>
+template <class C, class R>
 __global__ __launch_bounds__(kThreads) void sum_batch_kernel(const SumInst* __restrict__ insts, int n,
                                                              const double4* __restrict__ sph,
                                                              const C* __restrict__ Uf,
-                                                             C* __restrict__ U) {
+                                                             C* __restrict__ U, R* __restrict__ D,
+                                                             R* __restrict__ Di) {
     for (int i = blockIdx.y; i < n; i += gridDim.y) {
         const SumInst& I = insts[i];
         const int npix = I.uR * I.vR;
         for (int pix = blockIdx.x * kThreads + threadIdx.x; pix < npix; pix += gridDim.x * kThreads) {
+            if (pix < I.nd2) {
+                D[I.image_off + pix] = R(0);
+                Di[I.image_off + pix] = R(0);
+            }
             const C b = Uf[I.off + pix];
             if (I.ns == 0) {
                 U[I.off + pix] = b;
@@ -291,14 +296,15 @@ int uf_batch_launch(bimsim_ctx* ctx, const UfInst* d_inst, int n, int max_npix, 
 }
 
 int sum_batch_launch(bimsim_ctx* ctx, const SumInst* d_inst, int n, int max_npix,
-                     const double4* d_sph, int f32, const void* Uf, void* U) {
+                     const double4* d_sph, int f32, const void* Uf, void* U, void* D, void* Di) {
     const dim3 grid((max_npix + kThreads - 1) / kThreads, std::min(n, 65535));
     if (f32)
-        hipLaunchKernelGGL(sum_batch_kernel<float2>, grid, dim3(kThreads), 0, ctx->stream, d_inst, n,
-                           d_sph, (const float2*)Uf, (float2*)U);
+        hipLaunchKernelGGL((sum_batch_kernel<float2, float>), grid, dim3(kThreads), 0, ctx->stream,
+                           d_inst, n, d_sph, (const float2*)Uf, (float2*)U, (float*)D, (float*)Di);
     else
-        hipLaunchKernelGGL(sum_batch_kernel<double2>, grid, dim3(kThreads), 0, ctx->stream, d_inst,
-                           n, d_sph, (const double2*)Uf, (double2*)U);
+        hipLaunchKernelGGL((sum_batch_kernel<double2, double>), grid, dim3(kThreads), 0, ctx->stream,
+                           d_inst, n, d_sph, (const double2*)Uf, (double2*)U, (double*)D,
+                           (double*)Di);
     BIMSIM_HIP_CHECK(ctx, hipGetLastError());
     return BIMSIM_OK;
 }

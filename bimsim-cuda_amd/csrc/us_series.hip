@@ -258,7 +258,10 @@ __global__ __launch_bounds__(kThreads) void series_f32_kernel(
 // WaveF records: 2k and (1/nk) subA exp(i kmag k.c), c = p_s - f (FP64).
 __global__ void series_waves_kernel(const SeriesInst* __restrict__ insts,
                                     const SeriesSph* __restrict__ sph, int n_sph,
-                                    const double* __restrict__ waves, WaveF* __restrict__ wt) {
+                                    const double* __restrict__ waves, WaveF* __restrict__ wt,
+                                    int* __restrict__ counters, int n_counters) {
+    if (blockIdx.x == 0 && blockIdx.y == 0)   // the work-queue counters of the sums
+        for (int i = threadIdx.x; i < n_counters; i += blockDim.x) counters[i] = 0;
     for (int s = blockIdx.y; s < n_sph; s += gridDim.y) {
         const SeriesSph& SS = sph[s];
         const SeriesInst& I = insts[SS.inst];
@@ -342,36 +345,35 @@ __host__ __device__ inline long long tri_off(int m, int lh) {
     return (long long)m * lh - (long long)m * (m - 1) / 2;
 }
 
-// Pn_lm(cos th_w) in pl[pl_off + (tri_off(m) + (l - m)) * nk + w] and
-// e^{i m phi_w} in e[e_off + m * nk + w].  Grid: (wave blocks, instances, m).
-__global__ void harm_legendre_kernel(const SeriesInst* __restrict__ insts, int n_inst,
+// Pn_lm(cos th_w) in pl[pl_off + (tri_off(m) + (l - m)) * nk + w],
+// e^{i m phi_w} in e[e_off + m * nk + w] and, at m = 0, the wave weights
+// phi_w = (1/nk) subA exp(i kmag k.c) of every sphere of the instance in
+// phi[woff + w] (FP64); block 0 also clears the work-queue counters.
+// Grid: (wave blocks, instances, m).
+__global__ void harm_legendre_kernel(const SeriesInst* __restrict__ insts,
+                                     const SeriesSph* __restrict__ sph, int n_inst,
                                      const double* __restrict__ waves, double* __restrict__ pl,
-                                     double2* __restrict__ e) {
+                                     double2* __restrict__ e, double2* __restrict__ phi,
+                                     int* __restrict__ counters, int n_counters) {
     const int m = blockIdx.z;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && m == 0)
+        for (int i = threadIdx.x; i < n_counters; i += blockDim.x) counters[i] = 0;
     for (int i = blockIdx.y; i < n_inst; i += gridDim.y) {
         const SeriesInst& I = insts[i];
         if (m >= I.lh) continue;
         for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < I.nk; w += gridDim.x * blockDim.x) {
             const d3 k = mk(waves + 3 * (I.wave_off + w));
             legendre_column(m, I.lh, k.z, pl + I.pl_off + tri_off(m, I.lh) * I.nk + w, I.nk);
-            const double phi = atan2(k.y, k.x);
-            e[I.e_off + (long long)m * I.nk + w] = make_double2(cos(m * phi), sin(m * phi));
-        }
-    }
-}
-
-// Wave weights (FP64): phi_w = (1/nk) subA exp(i kmag k.c).
-__global__ void harm_phase_kernel(const SeriesInst* __restrict__ insts,
-                                  const SeriesSph* __restrict__ sph, int n_sph,
-                                  const double* __restrict__ waves, double2* __restrict__ phi) {
-    for (int s = blockIdx.y; s < n_sph; s += gridDim.y) {
-        const SeriesSph& SS = sph[s];
-        const SeriesInst& I = insts[SS.inst];
-        for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < I.nk; w += gridDim.x * blockDim.x) {
-            const d3 k = mk(waves + 3 * (I.wave_off + w));
-            const d3 c = sub(mk(SS.s.p), I.f);
-            const dc ph = cscl(cexp_i(I.kmag * dot(k, c)), I.sA);
-            phi[SS.woff + w] = make_double2(ph.r, ph.i);
+            const double az = atan2(k.y, k.x);
+            e[I.e_off + (long long)m * I.nk + w] = make_double2(cos(m * az), sin(m * az));
+            if (m == 0) {
+                for (int s = I.s_begin; s < I.s_begin + I.ns; ++s) {
+                    const SeriesSph& SS = sph[s];
+                    const d3 c = sub(mk(SS.s.p), I.f);
+                    const dc ph = cscl(cexp_i(I.kmag * dot(k, c)), I.sA);
+                    phi[SS.woff + w] = make_double2(ph.r, ph.i);
+                }
+            }
         }
     }
 }
@@ -673,10 +675,6 @@ __global__ __launch_bounds__(kThreads, harm_waves(L)) void series_harm_kernel(
     }
 }
 
-__global__ void zero_ints(int* p, int n) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
-}
-
 template <int L>
 void launch_bucket(dim3 grid, hipStream_t st, const SeriesPlan& P, const SeriesPlan::Bucket& K,
                    float2* U) {
@@ -963,14 +961,14 @@ int series_plan_launch(bimsim_ctx* ctx, const SeriesPlan& P, const double* waves
 int series_plan_launch_waves(bimsim_ctx* ctx, const SeriesPlan& P, const double* waves) {
     if (P.n_inst == 0 || P.n_sph == 0) return BIMSIM_OK;
     const hipStream_t st = ctx->stream;
-    hipLaunchKernelGGL(zero_ints, dim3(1), dim3(256), 0, st, P.d_counters, std::max(1, P.n_counters));
     const dim3 wgrid((P.nk_max + 255) / 256, std::min(P.n_sph, 65535));
     if (P.harmonic) {
-        // per-wave Legendre values and phases, then the per-sphere sums T
-        hipLaunchKernelGGL(harm_legendre_kernel, dim3((P.nk_max + 63) / 64, std::min(P.n_inst, 65535), P.max_lh),
-                           dim3(64), 0, st, P.d_inst, P.n_inst, waves, P.d_pl, P.d_e);
-        hipLaunchKernelGGL(harm_phase_kernel, wgrid, dim3(256), 0, st, P.d_inst, P.d_sph, P.n_sph, waves,
-                           P.d_phi);
+        // per-wave Legendre values, azimuth phases and wave weights, then the
+        // per-sphere sums T
+        hipLaunchKernelGGL(harm_legendre_kernel,
+                           dim3((P.nk_max + 63) / 64, std::min(P.n_inst, 65535), P.max_lh), dim3(64),
+                           0, st, P.d_inst, P.d_sph, P.n_inst, waves, P.d_pl, P.d_e, P.d_phi,
+                           P.d_counters, P.n_counters);
         hipLaunchKernelGGL(harm_tsum_kernel, dim3(P.max_lh, (unsigned)std::min<long long>(P.n_z, 65535)),
                            dim3(128), 0, st, P.d_inst, P.d_sph, P.d_zmap, P.n_z, P.d_zpart, P.d_pl,
                            P.d_e, P.d_phi, P.d_tpart);
@@ -978,7 +976,7 @@ int series_plan_launch_waves(bimsim_ctx* ctx, const SeriesPlan& P, const double*
                            st, P.d_sph, P.n_sph, P.d_zfirst, P.d_zpart, P.d_tpart, P.d_rec);
     } else {
         hipLaunchKernelGGL(series_waves_kernel, wgrid, dim3(256), 0, st, P.d_inst, P.d_sph, P.n_sph,
-                           waves, P.d_wt);
+                           waves, P.d_wt, P.d_counters, P.n_counters);
     }
     BIMSIM_HIP_CHECK(ctx, hipGetLastError());
     return BIMSIM_OK;

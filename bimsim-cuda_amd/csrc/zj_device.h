@@ -184,8 +184,11 @@ __device__ inline double sq(double v) { return v * v; }
 
 // Complex-argument J_{v0+k}, Y_{v0+k} and derivatives, k = 0..floor(v)
 // (CJYVA).  Arrays hold floor(v)+2 entries.  typo reproduces the reference's
-// y_{v0} asymptotic defect (cbessjy.cpp:541).
-__device__ inline void cjyva(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyvp, bool typo) {
+// y_{v0} asymptotic defect (cbessjy.cpp:541).  want_y = false stops after
+// J and J' (their values do not depend on the Y part); cyv, cyvp are then
+// left as they were.
+__device__ inline void cjyva(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyvp, bool typo,
+                             bool want_y = true) {
     const double eps = 1e-15, euler = 0.5772156649015329;
     const zc one = {1.0, 0.0}, zero = {0.0, 0.0};
     const double a0 = zabs(z);
@@ -257,7 +260,7 @@ __device__ inline void cjyva(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyv
             }
         }
     }
-    if (a0 <= 12.0) {
+    if (a0 <= 12.0 && want_y) {
         if (v0 != 0.0) {
             zc u[2];
             for (int l = 0; l < 2; ++l) {
@@ -339,6 +342,7 @@ __device__ inline void cjyva(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyv
     }
     cjvp[0] = zsub(zmul(zsc(v0, cjv[0]), zr), cjv[1]);
     for (int k = 1; k <= n; ++k) cjvp[k] = zadd(zmul(zsc(-(k + v0), cjv[k]), zr), cjv[k - 1]);
+    if (!want_y) return;
     cyv[0] = y0;
     cyv[1] = y1;
     const double ya0 = zabs(y0);
@@ -579,9 +583,10 @@ __device__ inline void sph_jy_real(int nl, double x, bool quirks, double* j, dou
 // Spherical j_l, y_l, j'_l, y'_l for l = 0..nl (cbessjyva_sph,
 // cbessjy.cpp:718-737): scale by sqrt(pi/(2z)); the reference uses
 // pi = 3.14159 there.
-__device__ inline void sph_jy_cplx(int nl, zc z, bool quirks, zc* j, zc* y, zc* jp, zc* yp) {
+__device__ inline void sph_jy_cplx(int nl, zc z, bool quirks, zc* j, zc* y, zc* jp, zc* yp,
+                                   bool want_y = true) {
     for (int k = 0; k <= nl + 1; ++k) j[k] = y[k] = jp[k] = yp[k] = zc{0.0, 0.0};
-    cjyva(nl + 0.5, z, j, y, jp, yp, quirks);
+    cjyva(nl + 0.5, z, j, y, jp, yp, quirks, want_y);
     const double pi = quirks ? 3.14159 : M_PI;
     const zc z2 = zsc(2.0, z);
     // the reference evaluates these two per order; they do not depend on it
