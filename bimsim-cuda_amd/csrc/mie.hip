@@ -4,10 +4,11 @@
 //
 // The work is a few long chains of dependent FP64 steps per sphere, so the
 // kernels spread it over lanes and keep it out of global memory: a block
-// takes up to 32 spheres, lane 2s computes the Bessel set of ka and lane
-// 2s+1 that of kna for sphere s into LDS (the recurrences reread their
-// arrays, so LDS latency instead of a global round trip per step), then
-// every lane takes (sphere, order) pairs of the coefficient formula.
+// takes up to 32 spheres and three lanes per sphere compute, into LDS, J of
+// ka, Y of ka and J of kna (the Y of kna is not used); the recurrences
+// reread their arrays, so LDS latency instead of a global round trip per
+// step.  Then every lane takes (sphere, order) pairs of the coefficient
+// formula.
 #include "batch.h"
 #include "internal.h"
 #include "zj_device.h"
@@ -26,20 +27,22 @@ using zj::zc;
 
 // Spheres per block so that 2 lanes x 4 arrays x wstride complex values
 // per sphere fit the LDS budget.
+constexpr int kLanes = 3;            // lanes per sphere
+
 int mie_spheres_per_block(int wstride) {
     const size_t per = 2 * 4 * sizeof(zc) * (size_t)wstride;
     return (int)std::max<size_t>(1, std::min<size_t>(kSphPerLaunch, kMieLds / per));
 }
 
-// Bessel set of one argument of sphere S (kind 0: ka, 1: kna): j, y, j', y'
-// at o, o + wstride, o + 2 wstride, o + 3 wstride (y, y' of kna stay 0).
-__device__ void mie_bessel(const SphereDev& S, double kmag, int kind, int quirks, zc* o,
+// One part of the Bessel sets of sphere S into its work w (j, y, j', y' of
+// ka, then of kna, wstride apart): role 0 J of ka, 1 Y of ka, 2 J of kna.
+__device__ void mie_bessel(const SphereDev& S, double kmag, int role, int quirks, zc* w,
                            int wstride) {
     const zc nc = {S.n[0], S.n[1]};
-    const zc z = kind == 0 ? zc{kmag * S.a, 0.0} : zj::zsc(S.a, zj::zsc(kmag, nc));
-    // the coefficients use y and y' of ka only
+    const zc z = role < 2 ? zc{kmag * S.a, 0.0} : zj::zsc(S.a, zj::zsc(kmag, nc));
+    zc* o = role < 2 ? w : w + 4 * wstride;
     zj::sph_jy_cplx(S.Nl, z, quirks != 0, o, o + wstride, o + 2 * wstride, o + 3 * wstride,
-                    kind == 0);
+                    role == 1, role != 1);
 }
 
 // A_l and B_l of sphere S from its two Bessel sets (w: ka set, then kna).
@@ -64,15 +67,14 @@ __device__ void mie_order(const SphereDev& S, int l, const zc* w, int wstride, d
 }
 
 // Coefficients of the block's cnt spheres (sh, wavenumbers kh, in LDS);
-// lds holds 8 wstride complex values per sphere.  Two lanes per sphere.
+// lds holds 8 wstride complex values per sphere.  Three lanes per sphere.
 __device__ __forceinline__ void mie_block(const SphereDev* sh, const double* kh, int cnt,
                                           int quirks, zc* lds, int wstride, double2* A,
                                           double2* B) {
     const int t = (int)threadIdx.x;
-    const int s = t >> 1;
+    const int s = t / kLanes;
     if (s < cnt)
-        mie_bessel(sh[s], kh[s], t & 1, quirks, lds + (size_t)(8 * s + 4 * (t & 1)) * wstride,
-                   wstride);
+        mie_bessel(sh[s], kh[s], t % kLanes, quirks, lds + (size_t)8 * s * wstride, wstride);
     __syncthreads();
     int lm = 0;
     for (int i = 0; i < cnt; ++i) lm = max(lm, sh[i].Nl);
@@ -88,13 +90,13 @@ __device__ __forceinline__ void mie_block(const SphereDev* sh, const double* kh,
 // for orders too high for LDS; otherwise in LDS (a separate instantiation,
 // so that its accesses compile to LDS instructions).
 template <bool kGlobal>
-__global__ __launch_bounds__(2 * kSphPerLaunch) void mie_coeff_kernel(
+__global__ __launch_bounds__(kLanes * kSphPerLaunch) void mie_coeff_kernel(
     SphereBatch batch, int ns, double kmag, int quirks, zc* work, int wstride, double2* A,
     double2* B) {
     extern __shared__ zc lds[];
     __shared__ SphereDev sh[kSphPerLaunch];
     __shared__ double kh[kSphPerLaunch];
-    const int spb = (int)blockDim.x / 2;
+    const int spb = (int)blockDim.x / kLanes;
     const int s0 = (int)blockIdx.x * spb;
     const int cnt = min(spb, ns - s0);
     const int t = (int)threadIdx.x;
@@ -111,13 +113,13 @@ __global__ __launch_bounds__(2 * kSphPerLaunch) void mie_coeff_kernel(
 
 // Spheres [b spb, b spb + spb) of a batch, sphere t at wavenumber kmag[t].
 template <bool kGlobal>
-__global__ __launch_bounds__(2 * kSphPerLaunch) void mie_batch_kernel(
+__global__ __launch_bounds__(kLanes * kSphPerLaunch) void mie_batch_kernel(
     const SphereDev* __restrict__ sph, const double* __restrict__ kmag, int n, int quirks,
     zc* work, int wstride, double2* A, double2* B) {
     extern __shared__ zc lds[];
     __shared__ SphereDev sh[kSphPerLaunch];
     __shared__ double kh[kSphPerLaunch];
-    const int spb = (int)blockDim.x / 2;
+    const int spb = (int)blockDim.x / kLanes;
     const int s0 = (int)blockIdx.x * spb;
     const int cnt = min(spb, n - s0);
     const int t = (int)threadIdx.x;
@@ -146,11 +148,11 @@ int mie_batch_launch(bimsim_ctx* ctx, const SphereDev* d_sph, const double* d_km
         w = (zc*)work;
     }
     if (w)
-        hipLaunchKernelGGL(mie_batch_kernel<true>, dim3((n + spb - 1) / spb), dim3(2 * spb), 0,
-                           ctx->stream, d_sph, d_kmag, n, quirks, w, wstride, A, B);
+        hipLaunchKernelGGL(mie_batch_kernel<true>, dim3((n + spb - 1) / spb), dim3(kLanes * spb),
+                           0, ctx->stream, d_sph, d_kmag, n, quirks, w, wstride, A, B);
     else
-        hipLaunchKernelGGL(mie_batch_kernel<false>, dim3((n + spb - 1) / spb), dim3(2 * spb), lds,
-                           ctx->stream, d_sph, d_kmag, n, quirks, w, wstride, A, B);
+        hipLaunchKernelGGL(mie_batch_kernel<false>, dim3((n + spb - 1) / spb), dim3(kLanes * spb),
+                           lds, ctx->stream, d_sph, d_kmag, n, quirks, w, wstride, A, B);
     BIMSIM_HIP_CHECK(ctx, hipGetLastError());
     return BIMSIM_OK;
 }
@@ -198,13 +200,13 @@ extern "C" int bimsim_mie_coeffs(bimsim_ctx* ctx, const bimsim_sphere* spheres, 
             d.coeff_off = S.coeff_off;
         }
         if (work)
-            hipLaunchKernelGGL(mie_coeff_kernel<true>, dim3((cnt + spb - 1) / spb), dim3(2 * spb), 0,
-                               ctx->stream, b, cnt, kmag, quirks, (zj::zc*)work, wstride,
-                               (double2*)A_out, (double2*)B_out);
+            hipLaunchKernelGGL(mie_coeff_kernel<true>, dim3((cnt + spb - 1) / spb),
+                               dim3(kLanes * spb), 0, ctx->stream, b, cnt, kmag, quirks,
+                               (zj::zc*)work, wstride, (double2*)A_out, (double2*)B_out);
         else
-            hipLaunchKernelGGL(mie_coeff_kernel<false>, dim3((cnt + spb - 1) / spb), dim3(2 * spb),
-                               lds, ctx->stream, b, cnt, kmag, quirks, nullptr, wstride,
-                               (double2*)A_out, (double2*)B_out);
+            hipLaunchKernelGGL(mie_coeff_kernel<false>, dim3((cnt + spb - 1) / spb),
+                               dim3(kLanes * spb), lds, ctx->stream, b, cnt, kmag, quirks, nullptr,
+                               wstride, (double2*)A_out, (double2*)B_out);
         BIMSIM_HIP_CHECK(ctx, hipGetLastError());
     }
     return BIMSIM_OK;

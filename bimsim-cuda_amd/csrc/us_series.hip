@@ -45,7 +45,8 @@ __global__ __launch_bounds__(kThreads) void series_coef_kernel(
         const SeriesInst& I = insts[SS.inst];
         const SphereDev S = SS.s;
         const int npix = I.uR * I.vR;
-        for (int pix = blockIdx.x * kThreads + threadIdx.x; pix < npix; pix += gridDim.x * kThreads) {
+        for (int pix = blockIdx.x * kThreads + threadIdx.x; pix < npix;
+             pix += gridDim.x * kThreads) {
             const int iv = pix / I.uR, iu = pix - iv * I.uR;
             const d3 p = quad_at(I.q, (double)iu / (double)I.uR, (double)iv / (double)I.vR);
             d3 r = sub(p, mk(S.p));
@@ -845,7 +846,8 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
         ch.sph_count = batch.inst[i1 - 1].s_begin + batch.inst[i1 - 1].ns - ch.sph_begin;
         std::vector<int> order;
         for (int i = i0; i < i1; ++i) order.push_back(i);
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return cost[a] > cost[b]; });
         for (int L : kBuckets) {
             const int begin = (int)items.size();
             for (int i : order) {
@@ -858,7 +860,8 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
                     for (int t = 0; t < I.nsplit; ++t) items.push_back(SeriesItem{i, p0, t, 0});
             }
             const int count = (int)items.size() - begin;
-            if (count > 0) ch.buckets.push_back(SeriesPlan::Bucket{L, begin, count, P->n_counters++});
+            if (count > 0)
+                ch.buckets.push_back(SeriesPlan::Bucket{L, begin, count, P->n_counters++});
         }
         P->chunks.push_back(ch);
         i0 = i1;
@@ -871,7 +874,8 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
         for (int i = 0; i < ni; ++i) {
             SeriesInst& I = batch.inst[i];
             int lh = 1;
-            for (int t = I.s_begin; t < I.s_begin + I.ns; ++t) lh = std::max(lh, batch.sph[t].s.Nl + 1);
+            for (int t = I.s_begin; t < I.s_begin + I.ns; ++t)
+                lh = std::max(lh, batch.sph[t].s.Nl + 1);
             I.lh = lh;
             I.pl_off = npl;
             npl += (long long)lh * (lh + 1) / 2 * I.nk;
@@ -880,7 +884,10 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
             P->max_lh = std::max(P->max_lh, lh);
         }
         long long sum_tri = 0;
-        for (int t = 0; t < P->n_sph; ++t) sum_tri += (long long)bucket[batch.sph[t].inst] * bucket[batch.sph[t].inst] / 2;
+        for (int t = 0; t < P->n_sph; ++t) {
+            const long long b = bucket[batch.sph[t].inst];
+            sum_tri += b * b / 2;
+        }
         for (int t = 0; t < P->n_sph; ++t) {
             SeriesSph& S = batch.sph[t];
             const SeriesInst& I = batch.inst[S.inst];
@@ -911,7 +918,8 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
         // (one row more: the kernel reads the diagonal factor of row m + 1)
         cst.assign((size_t)2 * (kCS + 2) * kCS, 0.f);
         for (int m = 0; m <= kCS; ++m) {
-            if (m >= 1) cst[2 * ((size_t)m * kCS + m)] = (float)std::sqrt((2.0 * m + 1.0) / (2.0 * m));
+            if (m >= 1)
+                cst[2 * ((size_t)m * kCS + m)] = (float)std::sqrt((2.0 * m + 1.0) / (2.0 * m));
             if (m == kCS) break;
             cst[2 * ((size_t)m * kCS + m) + 1] = 1.f;
             for (int l = m + 2; l < kCS; ++l) {

@@ -184,11 +184,15 @@ __device__ inline double sq(double v) { return v * v; }
 
 // Complex-argument J_{v0+k}, Y_{v0+k} and derivatives, k = 0..floor(v)
 // (CJYVA).  Arrays hold floor(v)+2 entries.  typo reproduces the reference's
-// y_{v0} asymptotic defect (cbessjy.cpp:541).  want_y = false stops after
-// J and J' (their values do not depend on the Y part); cyv, cyvp are then
-// left as they were.
+// y_{v0} asymptotic defect (cbessjy.cpp:541).
+//
+// The two halves can run apart (on different lanes): want_y = false stops
+// after J and J' (their values do not depend on the Y part) and leaves cyv,
+// cyvp untouched; want_j = false, for real z only (the complex-z Y
+// correction reads J), skips the Miller recurrence and leaves cjv, cjvp
+// untouched.  Each call writes only the arrays of the halves it computes.
 __device__ inline void cjyva(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyvp, bool typo,
-                             bool want_y = true) {
+                             bool want_y = true, bool want_j = true) {
     const double eps = 1e-15, euler = 0.5772156649015329;
     const zc one = {1.0, 0.0}, zero = {0.0, 0.0};
     const double a0 = zabs(z);
@@ -235,13 +239,15 @@ __device__ inline void cjyva(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyv
             const double mu = 4.0 * (l + v0) * (l + v0);
             zc p = one, tp = one;
             for (int k = 1; k <= kz; ++k) {
-                zc num = zsc(mu - sq(4.0 * k - 1.0), zsc(mu - sq(4.0 * k - 3.0), zsc(-0.78125e-2, tp)));
+                zc num = zsc(mu - sq(4.0 * k - 1.0),
+                             zsc(mu - sq(4.0 * k - 3.0), zsc(-0.78125e-2, tp)));
                 tp = zmul(zsc(1.0 / (k * (2.0 * k - 1.0)), num), z2r);
                 p = zadd(p, tp);
             }
             zc q = one, tq = one;
             for (int k = 1; k <= kz; ++k) {
-                zc num = zsc(mu - sq(4.0 * k + 1.0), zsc(mu - sq(4.0 * k - 1.0), zsc(-0.78125e-2, tq)));
+                zc num = zsc(mu - sq(4.0 * k + 1.0),
+                             zsc(mu - sq(4.0 * k - 1.0), zsc(-0.78125e-2, tq)));
                 tq = zmul(zsc(1.0 / (k * (2.0 * k + 1.0)), num), z2r);
                 q = zadd(q, tq);
             }
@@ -311,12 +317,14 @@ __device__ inline void cjyva(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyv
             j1 = zmul(j1, f1);
         }
     }
-    cjv[0] = j0;
-    cjv[1] = j1;
+    if (want_j) {
+        cjv[0] = j0;
+        cjv[1] = j1;
+    }
     const zc zr = rdiv(1.0, z);
     if (n >= 2 && n <= (int)(0.25 * a0)) {
         zc f0 = j0, f1 = j1;
-        for (int k = 2; k <= n; ++k) {
+        for (int k = 2; k <= n && want_j; ++k) {
             const zc f = zsub(zmul(zsc(2.0 * (k + v0 - 1.0), f1), zr), f0);
             cjv[k] = f;
             f0 = f1;
@@ -325,23 +333,30 @@ __device__ inline void cjyva(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyv
     } else if (n >= 2) {
         int m = msta1(a0, 200);
         if (m < n) {
-            for (int k = m + 1; k <= n; ++k) cjv[k] = cyv[k] = cjvp[k] = cyvp[k] = zero;
+            for (int k = m + 1; k <= n; ++k) {
+                if (want_j) cjv[k] = cjvp[k] = zero;
+                if (want_y) cyv[k] = cyvp[k] = zero;
+            }
             n = m;
-        } else {
+        } else if (want_j) {
             m = msta2(a0, n, 15);
         }
-        zc f2 = zero, f1 = {1.0e-100, 0.0}, f = zero;
-        for (int k = m; k >= 0; --k) {
-            f = zsub(zmul(zsc(2.0 * (v0 + k + 1.0), f1), zr), f2);
-            if (k <= n) cjv[k] = f;
-            f2 = f1;
-            f1 = f;
+        if (want_j) {
+            zc f2 = zero, f1 = {1.0e-100, 0.0}, f = zero;
+            for (int k = m; k >= 0; --k) {
+                f = zsub(zmul(zsc(2.0 * (v0 + k + 1.0), f1), zr), f2);
+                if (k <= n) cjv[k] = f;
+                f2 = f1;
+                f1 = f;
+            }
+            const zc cs = zabs(j0) > zabs(j1) ? zdiv(j0, f) : zdiv(j1, f2);
+            for (int k = 0; k <= n; ++k) cjv[k] = zmul(cjv[k], cs);
         }
-        const zc cs = zabs(j0) > zabs(j1) ? zdiv(j0, f) : zdiv(j1, f2);
-        for (int k = 0; k <= n; ++k) cjv[k] = zmul(cjv[k], cs);
     }
-    cjvp[0] = zsub(zmul(zsc(v0, cjv[0]), zr), cjv[1]);
-    for (int k = 1; k <= n; ++k) cjvp[k] = zadd(zmul(zsc(-(k + v0), cjv[k]), zr), cjv[k - 1]);
+    if (want_j) {
+        cjvp[0] = zsub(zmul(zsc(v0, cjv[0]), zr), cjv[1]);
+        for (int k = 1; k <= n; ++k) cjvp[k] = zadd(zmul(zsc(-(k + v0), cjv[k]), zr), cjv[k - 1]);
+    }
     if (!want_y) return;
     cyv[0] = y0;
     cyv[1] = y1;
@@ -583,21 +598,32 @@ __device__ inline void sph_jy_real(int nl, double x, bool quirks, double* j, dou
 // Spherical j_l, y_l, j'_l, y'_l for l = 0..nl (cbessjyva_sph,
 // cbessjy.cpp:718-737): scale by sqrt(pi/(2z)); the reference uses
 // pi = 3.14159 there.
+// want_j / want_y select the halves as in cjyva (want_j is forced for
+// complex z when Y is wanted); only the selected arrays are written.
 __device__ inline void sph_jy_cplx(int nl, zc z, bool quirks, zc* j, zc* y, zc* jp, zc* yp,
-                                   bool want_y = true) {
-    for (int k = 0; k <= nl + 1; ++k) j[k] = y[k] = jp[k] = yp[k] = zc{0.0, 0.0};
-    cjyva(nl + 0.5, z, j, y, jp, yp, quirks, want_y);
+                                   bool want_y = true, bool want_j = true) {
+    if (want_y && z.i != 0.0) want_j = true;
+    for (int k = 0; k <= nl + 1; ++k) {
+        if (want_j) j[k] = jp[k] = zc{0.0, 0.0};
+        if (want_y) y[k] = yp[k] = zc{0.0, 0.0};
+    }
+    cjyva(nl + 0.5, z, j, y, jp, yp, quirks, want_y, want_j);
     const double pi = quirks ? 3.14159 : M_PI;
     const zc z2 = zsc(2.0, z);
     // the reference evaluates these two per order; they do not depend on it
     const zc s = zsqrt(rdiv(pi, z2));
     const zc inv = rdiv(-1.0, z2);
     for (int k = 0; k <= nl; ++k) {
-        const zc jk = zmul(j[k], s), yk = zmul(y[k], s), jpk = jp[k], ypk = yp[k];
-        j[k] = jk;
-        y[k] = yk;
-        jp[k] = zadd(zmul(inv, jk), zmul(jpk, s));
-        yp[k] = zadd(zmul(inv, yk), zmul(ypk, s));
+        if (want_j) {
+            const zc jk = zmul(j[k], s), jpk = jp[k];
+            j[k] = jk;
+            jp[k] = zadd(zmul(inv, jk), zmul(jpk, s));
+        }
+        if (want_y) {
+            const zc yk = zmul(y[k], s), ypk = yp[k];
+            y[k] = yk;
+            yp[k] = zadd(zmul(inv, yk), zmul(ypk, s));
+        }
     }
 }
 
