@@ -360,9 +360,11 @@ void LoadParameters(const rts::arglist& args, microscopeStruct* S, fileoutStruct
     lImagePlane(args, S);
     lNearfield(args, S);
     lOutputParams(args, out);
+    // an extended source, or else the interpolation mode (arguments.h:576-587)
     if (args("extended-source"))
-        throw Fail("bimsim: --extended-source is not built in this version");
-    if (args("interpolate")) S->interpolate = args["interpolate"].as_int();
+        S->LoadExtendedSource(args["extended-source"].as_text());
+    else if (args("interpolate"))
+        S->interpolate = args["interpolate"].as_int();
 }
 
 void print_config(const microscopeStruct& S, const fileoutStruct& out) {
@@ -398,6 +400,12 @@ void print_config(const microscopeStruct& S, const fileoutStruct& out) {
                   << s.iMaterial << "\n";
     }
     std::cout << "interpolate=" << S.interpolate << "\n";
+    std::cout << "sources=" << S.focalPoints.size() << "\n";
+    for (size_t i = 0; i < S.focalPoints.size() && i < 8; ++i) {
+        const sourcePoint& p = S.focalPoints[i];
+        std::cout << "source" << i << "=" << p.f[0] << "," << p.f[1] << "," << p.f[2] << "," << p.A
+                  << "\n";
+    }
     std::cout << "files=" << out.intFile << "|" << out.incFile << "|" << out.absFile << "|"
               << out.transFile << "|" << out.nearFile << "|" << out.farFile << "\n";
     std::cout << "wavenumber=" << out.wavenumber << "\n";

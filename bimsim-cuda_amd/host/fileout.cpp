@@ -46,10 +46,16 @@ void fileoutStruct::saveDetector(microscopeStruct* scope) {
 }
 
 void fileoutStruct::Save(microscopeStruct* scope) {
-    if (!scope->focalPoints.empty() || scope->interpolate != -1)
+    if (!scope->focalPoints.empty()) {
+        // extended source: the summed detector images only (fileout.cu:199-214)
+        scope->SimulateExtendedSource();
+        saveDetector(scope);
+        return;
+    }
+    if (scope->interpolate != -1)
         throw std::runtime_error(
-            "bimsim: extended-source and interpolated simulations (--extended-source, "
-            "--interpolate 0..3) are not built; run a single point source");
+            "bimsim: interpolated extended-source simulations (--interpolate 0..3) are not "
+            "built; use --extended-source with a source image");
     scope->Simulate();
     saveNearField(&scope->nf);
     saveFarField(scope);

@@ -1,4 +1,5 @@
 // microscopeStruct over the C ABI (reference microscope.cu:495-857).
+#include <iostream>
 #include <sstream>
 
 #include "microscope.h"
@@ -135,6 +136,34 @@ void microscopeStruct::SimulateImaging() {
 void microscopeStruct::Simulate() {
     SimulateScattering();
     SimulateImaging();
+}
+
+void microscopeStruct::LoadExtendedSource(std::string filename) {
+    int Rx = 0, Ry = 0;
+    const std::vector<unsigned char> img = read_bmp_channel0(filename, &Rx, &Ry);
+    // x outer, y inner as the reference, so the detector sums run in its order
+    for (int x = 0; x < Rx; ++x)
+        for (int y = 0; y < Ry; ++y) {
+            const unsigned char s = img[(size_t)y * Rx + x];
+            if (s == 0) continue;
+            // the reference keeps u, v and the amplitude in float
+            const float u = (float)((ptype)x / (ptype)Rx), v = (float)((ptype)y / (ptype)Ry);
+            sourcePoint p;
+            p.f = pos(u, v);
+            p.A = (ptype)((float)s / 255.0f);
+            focalPoints.push_back(p);
+        }
+    if (settings().verbose) std::cout << "Number of point sources: " << focalPoints.size() << std::endl;
+}
+
+void microscopeStruct::SimulateExtendedSource() {
+    clearDetector();
+    scale = 1.0f;
+    for (const sourcePoint& p : focalPoints) {
+        nf.focus = p.f;
+        nf.A = p.A;
+        Simulate();   // the detector images accumulate
+    }
 }
 
 void microscopeStruct::SimulateAndSaveTexture() {

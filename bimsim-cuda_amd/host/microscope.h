@@ -70,6 +70,12 @@ struct microscopeStruct {
     void SimulateWithInterpolation();
     void saveToTexture();
 
+    // Extended sources (microscope.cu:873-914, 1258-1296): one point source
+    // per non-zero pixel of a source image, focus at pos(x / Rx, y / Ry) and
+    // amplitude value / 255; the detector images sum the sources' images.
+    void LoadExtendedSource(std::string filename);
+    void SimulateExtendedSource();
+
     void setNearfield();
     void setRes(int x_res, int y_res, int pad, int supersampling);
     void setPos(bsPoint pMin, bsPoint pMax, bsVector normal);

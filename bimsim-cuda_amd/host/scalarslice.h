@@ -4,6 +4,7 @@
 #pragma once
 
 #include <string>
+#include <vector>
 
 #include "dataTypes.h"
 
@@ -56,3 +57,9 @@ struct scalarslice {
 // (rts/envi/envi.h:196-343, envi_header.h:48-70,318-364).
 void envi_add_band(const std::string& filename, const float* band, int samples, int lines,
                    double wavelength, bool append);
+
+// Source images of the extended-source driver: an uncompressed BMP (8-bit
+// palette, 24- or 32-bit) read as the first stored channel of each pixel
+// (blue for colour files, the grey value for grey ones; the reference reads
+// channel 0 of its image class), row 0 at the top, *w x *h bytes.
+std::vector<unsigned char> read_bmp_channel0(const std::string& filename, int* w, int* h);

@@ -217,6 +217,51 @@ void write_bmp(const std::string& filename, const std::vector<unsigned char>& rg
 
 }  // namespace
 
+std::vector<unsigned char> read_bmp_channel0(const std::string& filename, int* w, int* h) {
+    std::ifstream f(filename, std::ios::binary);
+    if (!f) throw std::runtime_error("bimsim: cannot read source image " + filename);
+    std::vector<unsigned char> data((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    auto u16 = [&](size_t o) { return (unsigned)data[o] | ((unsigned)data[o + 1] << 8); };
+    auto u32 = [&](size_t o) { return u16(o) | (u16(o + 2) << 16); };
+    if (data.size() < 54 || data[0] != 'B' || data[1] != 'M')
+        throw std::runtime_error("bimsim: " + filename + " is not a BMP image (only BMP sources are read)");
+    const unsigned off = u32(10), hsize = u32(14);
+    const int width = (int)u32(18), height_raw = (int)u32(22);
+    const unsigned bpp = u16(28), comp = u32(30);
+    const bool top_down = height_raw < 0;
+    const int height = top_down ? -height_raw : height_raw;
+    if (width <= 0 || height <= 0 || width > (1 << 16) || height > (1 << 16))
+        throw std::runtime_error("bimsim: bad BMP size in " + filename);
+    if (!((bpp == 8 || bpp == 24) && comp == 0) && !(bpp == 32 && (comp == 0 || comp == 3)))
+        throw std::runtime_error("bimsim: unsupported BMP format in " + filename +
+                                 " (uncompressed 8-, 24- or 32-bit only)");
+    const size_t row = ((size_t)bpp * width + 31) / 32 * 4;
+    if ((size_t)off + row * height > data.size())
+        throw std::runtime_error("bimsim: truncated BMP " + filename);
+    std::vector<unsigned char> pal;
+    if (bpp == 8) {
+        unsigned n = u32(46);
+        if (n == 0) n = 256;
+        const size_t p0 = 14 + (size_t)hsize;
+        if (p0 + 4 * (size_t)n > off) throw std::runtime_error("bimsim: bad BMP palette in " + filename);
+        for (unsigned i = 0; i < n; ++i) pal.push_back(data[p0 + 4 * i]);   // blue
+        pal.resize(256, 0);
+    }
+    std::vector<unsigned char> out((size_t)width * height);
+    for (int y = 0; y < height; ++y) {
+        const size_t src = off + row * (size_t)(top_down ? y : height - 1 - y);
+        for (int x = 0; x < width; ++x) {
+            unsigned char v;
+            if (bpp == 8) v = pal[data[src + x]];
+            else v = data[src + (size_t)x * (bpp / 8)];
+            out[(size_t)y * width + x] = v;
+        }
+    }
+    *w = width;
+    *h = height;
+    return out;
+}
+
 void scalarslice::toImage(std::string filename, ptype vmin, ptype vmax, stim::colormapType cmap) {
     const size_t n = (size_t)R[0] * R[1];
     std::vector<double> v(n);

@@ -146,6 +146,50 @@ def test_wavenumber_k_vector_and_flags():
     assert c["lut_uf"] == "1" and int(c["precision"]) == 0
 
 
-def test_unbuilt_modes_are_errors_not_silent():
-    p = run("--print-config", "--extended-source", "img.bmp")
-    assert p.returncode != 0 and "not built" in p.stdout
+def test_missing_source_image_is_an_error(tmp_path):
+    p = run("--print-config", "--extended-source", "img.bmp", cwd=tmp_path)
+    assert p.returncode != 0 and "cannot read source image" in p.stdout
+
+
+def write_bmp(path, pixels, bpp=24):
+    """pixels[y][x] grey values, row 0 at the top; 24-bit (grey in all three
+    channels) or 8-bit with a grey palette."""
+    h, w = len(pixels), len(pixels[0])
+    row = (bpp * w + 31) // 32 * 4
+    pal = b"".join(bytes((i, i, i, 0)) for i in range(256)) if bpp == 8 else b""
+    off = 54 + len(pal)
+    body = b""
+    for y in range(h - 1, -1, -1):     # bottom-up
+        line = b"".join(bytes((v,)) if bpp == 8 else bytes((v, v, v)) for v in pixels[y])
+        body += line + b"\0" * (row - len(line))
+    hdr = b"BM" + struct.pack("<IHHI", off + len(body), 0, 0, off)
+    hdr += struct.pack("<IiiHHIIiiII", 40, w, h, 1, bpp, 0, len(body), 2835, 2835,
+                       256 if bpp == 8 else 0, 0)
+    path.write_bytes(hdr + pal + body)
+
+
+@pytest.mark.parametrize("bpp", [24, 8])
+def test_extended_source_points(tmp_path, bpp):
+    """One point source per non-zero pixel (microscope.cu:1258-1296): x outer,
+    y inner, focus pos(x/Rx, y/Ry) of the detector plane, amplitude v/255."""
+    px = [[0, 0, 255, 0],
+          [0, 128, 0, 0],
+          [64, 0, 0, 0]]                # 4 wide, 3 high; row 0 at the top
+    write_bmp(tmp_path / "src.bmp", px, bpp)
+    c = config("--xy", 40, 1.5, "--extended-source", "src.bmp", cwd=tmp_path)
+    assert int(c["sources"]) == 3
+    want = []
+    for x in range(4):
+        for y in range(3):
+            if px[y][x]:
+                u, v = f32(x / 4), f32(y / 3)
+                want.append((-20.0 + 40.0 * u, -20.0 + 40.0 * v, 1.5, f32(px[y][x] / 255)))
+    for i, w in enumerate(want):
+        got = vec(c[f"source{i}"])
+        assert all(abs(a - b) <= 1e-5 for a, b in zip(got, w)), (i, got, w)
+
+
+def test_extended_source_rejects_other_formats(tmp_path):
+    (tmp_path / "src.png").write_bytes(b"\x89PNG\r\n\x1a\n")
+    p = run("--print-config", "--extended-source", "src.png", cwd=tmp_path)
+    assert p.returncode != 0 and "BMP" in p.stdout + p.stderr

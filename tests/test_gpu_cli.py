@@ -55,7 +55,7 @@ def run_cli(tmp_path, *extra):
     return {k: read_envi(str(tmp_path / k)) for k in ["I", "I0", "A", "T"]}
 
 
-def oracle_case(orc, lut):
+def oracle_case(orc, lut, focus=(0.0, 0.0, 0.0), amp=1.0):
     import oracle
     c = CASE
     lam = f32(c["lam"])
@@ -71,9 +71,9 @@ def oracle_case(orc, lut):
     ca, cb = 1.0, math.cos(math.asin(cond))
     waves = orc.mc_sample_na(c["nk"], (0.0, 0.0, 1.0), 0.0, cond, 0, 0)
     waves = np.asarray(waves, dtype=np.float64).reshape(-1, 3).astype(np.float32).astype(np.float64)
-    subA = (2.0 * PI_F) * ((1.0 - cb) - (1.0 - ca))
-    focus, kdir = (0.0, 0.0, 0.0), (0.0, 0.0, 1.0)
-    Uf = orc.nf_uf(q, N, N, focus, kdir, kmag, 1.0, ca, cb, c["m"])
+    subA = (2.0 * PI_F) * amp * ((1.0 - cb) - (1.0 - ca))
+    kdir = (0.0, 0.0, 1.0)
+    Uf = orc.nf_uf(q, N, N, focus, kdir, kmag, amp, ca, cb, c["m"])
     A, B = orc.mie_coeffs(Nl, kmag, a, n)
     if lut:
         aR, dR, dmin, dmax = orc.sphere_lut_dims(q, s, N)
@@ -133,3 +133,32 @@ def test_cli_append_and_images(exe, tmp_path):
         w, h = struct.unpack("<ii", data[18:26])
         assert (w, h) == (side, side)
         assert len(data) == 54 + ((3 * side + 3) & ~3) * side
+
+
+def test_cli_extended_source_matches_oracle(exe, orc, tmp_path):
+    """--extended-source: one point source per non-zero pixel of a BMP
+    (focus pos(x/Rx, y/Ry) on the detector plane, amplitude v/255); the
+    detector images are the sums of the sources' images (microscope.cu:873-914)."""
+    from test_cli import write_bmp
+    px = [[0, 200, 0],
+          [0, 0, 90],
+          [255, 0, 0]]
+    write_bmp(tmp_path / "src.bmp", px)
+    out = run_cli(tmp_path, "--precision", "parity", "--recursive-us",
+                  "--extended-source", "src.bmp")
+    half = CASE["size"] / 2
+    D = Di = 0.0
+    for x in range(3):
+        for y in range(3):
+            if not px[y][x]:
+                continue
+            u, v = f32(x / 3), f32(y / 3)
+            f = (-half + CASE["size"] * u, -half + CASE["size"] * v, 0.0)
+            d, di, _ = oracle_case(orc, False, f, f32(px[y][x] / 255))
+            D = D + d
+            Di = Di + di
+    assert normrel(out["I"][0][0], D) <= 2e-6
+    assert normrel(out["I0"][0][0], Di) <= 2e-6
+    bright = Di > 1e-3 * Di.max()
+    Ab = -np.log10(D / Di)
+    assert np.max(np.abs(out["A"][0][0][bright] - Ab[bright])) <= 2e-6
