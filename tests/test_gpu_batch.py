@@ -175,3 +175,37 @@ def test_batch_matches_oracle(pkg, ctx, orc, W, shape):
             else:
                 assert err <= 1e-4, (shape, err)
         b.close()
+
+
+@pytest.mark.parametrize("prec", [0, 2])
+def test_extended_source_as_one_batch(pkg, ctx, W, prec):
+    """An extended source as one batch: point sources (focus, amplitude) as
+    instances sharing one detector image, which then holds the sum of the
+    sources' images, as SimulateExtendedSource accumulates them."""
+    base = W.config(2, N=48, nk=24)
+    insts = []
+    for f, amp in [((0.0, 0.0, 0.0), 1.0), ((3.0, -2.0, 0.0), 0.5), ((-4.0, 5.0, 0.0), 0.25)]:
+        i = dict(base)
+        i["focus"] = f
+        i["amp"] = amp
+        i["subA"] = (2.0 * W.PI_F) * amp * ((1.0 - i["cos_beta"]) - (1.0 - i["cos_alpha"]))
+        insts.append(i)
+    waves = waves_of(pkg, W, insts)
+    singles = [single(pkg, ctx, i, w, prec) for i, w in zip(insts, waves)]
+    import bimsim_amd.pipeline as P
+    arr, spheres, Wv, n_img = P.batch_tables(insts, waves)
+    for k in range(len(insts)):
+        arr[k].image_off = 0
+    n = insts[0]["Nd"] ** 2
+    rt = torch.float32 if prec == pkg.FP32_FAST else torch.float64
+    D = torch.empty(n, dtype=rt, device="cuda")
+    Di = torch.empty_like(D)
+    Ab = torch.empty_like(D)
+    b = ctx.batch(arr, spheres, torch.as_tensor(Wv.ravel()).cuda(), D, Di, Ab, prec=prec)
+    want_D = singles[0][0] + singles[1][0] + singles[2][0]
+    want_Di = singles[0][1] + singles[1][1] + singles[2][1]
+    tol = 1e-13 if prec == 0 else 2e-6
+    assert normrel(host(D), want_D) <= tol and normrel(host(Di), want_Di) <= tol
+    bright = want_Di > 1e-3 * want_Di.max()
+    assert np.max(np.abs(host(Ab)[bright] - (-np.log10(want_D / want_Di))[bright])) <= 100 * tol
+    b.close()
