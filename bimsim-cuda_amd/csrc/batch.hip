@@ -103,9 +103,9 @@ int enqueue(bimsim_batch* b) {
         ctx->stream = main;
         BIMSIM_TRY(rc);
         BIMSIM_HIP_CHECK(ctx, hipEventRecord(b->ev_join, b->side));
-        BIMSIM_TRY(uf_batch_launch(ctx, b->d_uf, b->n, b->max_npix, b->f32, b->d_uftab, b->d_Uf));
         if (b->have_mc) BIMSIM_TRY(series_plan_launch_waves(ctx, b->plan_mc, b->waves));
         if (b->have_pw) BIMSIM_TRY(series_plan_launch_waves(ctx, b->plan_pw, b->d_kdirs));
+        BIMSIM_TRY(uf_batch_launch(ctx, b->d_uf, b->n, b->max_npix, b->f32, b->d_uftab, b->d_Uf));
         BIMSIM_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, b->ev_join, 0));
         if (b->have_mc)
             BIMSIM_TRY(series_plan_launch_sums(ctx, b->plan_mc, b->d_A, b->d_B, (float2*)b->d_U));
