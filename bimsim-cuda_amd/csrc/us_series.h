@@ -101,6 +101,7 @@ struct SeriesPlan {
     double2* d_tpart = nullptr;   // partial harmonic sums per (sphere, chunk, m, l)
     float4* d_rec = nullptr;      // harmonic records per (sphere, m, l)
     float* d_cst = nullptr;       // scaled Legendre recurrence constants per (m, l)
+    double* d_h64 = nullptr;      // FP64 Legendre constants and scales of the wave sums
 };
 
 // Chooses splits and chunks, builds and uploads the tables and allocates

@@ -320,20 +320,27 @@ __global__ void series_reduce_kernel(const SeriesInst* __restrict__ insts,
 // radial factors c_l are formed per pixel inside the pixel kernel, so this
 // form keeps no per-pixel coefficient buffer.
 constexpr int kCS = 112;   // row stride of the recurrence-constant table (m, l < 112)
+// FP64 tables of the wave sums (host-built, so the device loops carry no
+// square roots): a_lm, b_lm at h64[2 (m kH + l)], sqrt((2k+1)/(2k)) at
+// h64[kHDiag + k], K_lm = prod_{j=m+1..l} a_jm / 2 at h64[kHK + m kH + l].
+constexpr int kH = 113;
+constexpr int kHDiag = 2 * kH * kH;
+constexpr int kHK = kHDiag + kH;
+constexpr int kHSize = kHK + kH * kH;
 
 // Orthonormal Pn_lm(x) for l = m .. lh-1 of one wave into out[l - m]
 // (FP64): Pn_00 = 1/sqrt(4 pi), Pn_mm = sqrt((2m+1)/(2m)) s Pn_{m-1,m-1},
-// Pn_lm = a_lm (x Pn_{l-1,m} - b_lm Pn_{l-2,m}).
-__device__ inline void legendre_column(int m, int lh, double x, double* out, long long stride) {
+// Pn_lm = a_lm (x Pn_{l-1,m} - b_lm Pn_{l-2,m}); constants from h64.
+__device__ inline void legendre_column(int m, int lh, double x, double* out, long long stride,
+                                       const double* __restrict__ h64) {
     const double st = sqrt(fmax(0.0, 1.0 - x * x));
     double pmm = 0.28209479177387814;   // 1/sqrt(4 pi)
-    for (int k = 1; k <= m; ++k) pmm = sqrt((2.0 * k + 1.0) / (2.0 * k)) * st * pmm;
+    for (int k = 1; k <= m; ++k) pmm = h64[kHDiag + k] * st * pmm;
     double p0 = 0.0, p1 = pmm;
     out[0] = pmm;
+    const double* ab = h64 + 2 * (m * kH);
     for (int l = m + 1; l < lh; ++l) {
-        const double a = sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
-        const double b = sqrt((((double)l - 1.0) * (l - 1.0) - (double)m * m) /
-                              (4.0 * (l - 1.0) * (l - 1.0) - 1.0));
+        const double a = ab[2 * l], b = ab[2 * l + 1];
         const double p = a * (x * p1 - b * p0);
         p0 = p1;
         p1 = p;
@@ -353,7 +360,8 @@ __host__ __device__ inline long long tri_off(int m, int lh) {
 // Grid: (wave blocks, instances, m).
 __global__ void harm_legendre_kernel(const SeriesInst* __restrict__ insts,
                                      const SeriesSph* __restrict__ sph, int n_inst,
-                                     const double* __restrict__ waves, double* __restrict__ pl,
+                                     const double* __restrict__ waves,
+                                     const double* __restrict__ h64, double* __restrict__ pl,
                                      double2* __restrict__ e, double2* __restrict__ phi,
                                      int* __restrict__ counters, int n_counters) {
     const int m = blockIdx.z;
@@ -364,7 +372,7 @@ __global__ void harm_legendre_kernel(const SeriesInst* __restrict__ insts,
         if (m >= I.lh) continue;
         for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < I.nk; w += gridDim.x * blockDim.x) {
             const d3 k = mk(waves + 3 * (I.wave_off + w));
-            legendre_column(m, I.lh, k.z, pl + I.pl_off + tri_off(m, I.lh) * I.nk + w, I.nk);
+            legendre_column(m, I.lh, k.z, pl + I.pl_off + tri_off(m, I.lh) * I.nk + w, I.nk, h64);
             const double az = atan2(k.y, k.x);
             e[I.e_off + (long long)m * I.nk + w] = make_double2(cos(m * az), sin(m * az));
             if (m == 0) {
@@ -422,7 +430,8 @@ __global__ void harm_tsum_kernel(const SeriesInst* __restrict__ insts,
 __global__ void harm_records_kernel(const SeriesSph* __restrict__ sph, int n_sph,
                                     const long long* __restrict__ zfirst,
                                     const long long* __restrict__ zpart,
-                                    const double2* __restrict__ part, float4* __restrict__ rec) {
+                                    const double2* __restrict__ part,
+                                    const double* __restrict__ h64, float4* __restrict__ rec) {
     const int l = threadIdx.x;
     const int m = blockIdx.x;
     for (int s = blockIdx.y; s < n_sph; s += gridDim.y) {
@@ -432,16 +441,17 @@ __global__ void harm_records_kernel(const SeriesSph* __restrict__ sph, int n_sph
         float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
         const int nl1 = SS.s.Nl + 1;
         if (m < nl1 && l >= m && l < nl1) {
+            // a sphere's chunks are consecutive blocks of tl^2 (T+, T-) pairs
             dc tp = {0.0, 0.0}, tm = {0.0, 0.0};
+            const double2* o = part + (zpart[zfirst[s]] + (long long)m * tl + l) * 2;
+            const long long cs = 2LL * tl * tl;
+#pragma unroll 8
             for (int c = 0; c < SS.nchunk; ++c) {
-                const double2* o = part + (zpart[zfirst[s] + c] + (long long)m * tl + l) * 2;
-                tp = cadd(tp, dc{o[0].x, o[0].y});
-                tm = cadd(tm, dc{o[1].x, o[1].y});
+                const double2 a = o[c * cs], b = o[c * cs + 1];
+                tp = cadd(tp, dc{a.x, a.y});
+                tm = cadd(tm, dc{b.x, b.y});
             }
-            double K = 1.0;
-            for (int j = m + 1; j <= l; ++j)
-                K *= 0.5 * sqrt((4.0 * j * j - 1.0) / ((double)j * j - (double)m * m));
-            const double f = 4.0 * M_PI / (2.0 * l + 1.0) * K;
+            const double f = 4.0 * M_PI / (2.0 * l + 1.0) * h64[kHK + m * kH + l];
             if (m == 0)
                 r = make_float4((float)(f * tp.r), (float)(f * tp.i), 0.f, 0.f);
             else
@@ -732,7 +742,8 @@ void series_plan_destroy(SeriesPlan* P) {
     (void)hipFree(P->d_part);
     (void)hipFree(P->d_coef);
     (void)hipFree(P->d_rhat);
-    void* harm[] = {P->d_pl, P->d_e, P->d_phi, P->d_tpart, P->d_rec, P->d_cst, P->d_zmap,
+    void* harm[] = {P->d_pl,   P->d_e,    P->d_phi,   P->d_tpart, P->d_rec, P->d_cst,
+                    P->d_h64, P->d_zmap,
                     P->d_zpart, P->d_zfirst};
     for (void* q : harm) (void)hipFree(q);
     *P = SeriesPlan();
@@ -869,6 +880,7 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
     std::vector<int2> zmap;
     std::vector<long long> zpart, zfirst;
     std::vector<float> cst;
+    std::vector<double> h64;
     if (P->harmonic) {
         long long npl = 0, ne = 0, nrec = 0, npairs = 0;
         for (int i = 0; i < ni; ++i) {
@@ -915,6 +927,21 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
         auto a_ = [](int l, int m) {
             return std::sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
         };
+        h64.assign((size_t)kHSize, 0.0);
+        for (int m = 0; m < kH; ++m) {
+            if (m >= 1) h64[kHDiag + m] = std::sqrt((2.0 * m + 1.0) / (2.0 * m));
+            double K = 1.0;
+            h64[kHK + (size_t)m * kH + m] = K;
+            for (int l = m + 1; l < kH; ++l) {
+                const double a = std::sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
+                const double b = std::sqrt((((double)l - 1.0) * (l - 1.0) - (double)m * m) /
+                                           (4.0 * (l - 1.0) * (l - 1.0) - 1.0));
+                h64[2 * ((size_t)m * kH + l)] = a;
+                h64[2 * ((size_t)m * kH + l) + 1] = b;
+                K *= 0.5 * std::sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
+                h64[kHK + (size_t)m * kH + l] = K;
+            }
+        }
         // (one row more: the kernel reads the diagonal factor of row m + 1)
         cst.assign((size_t)2 * (kCS + 2) * kCS, 0.f);
         for (int m = 0; m <= kCS; ++m) {
@@ -937,6 +964,7 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
          (rc = upload(ctx, &P->d_zpart, zpart)) != BIMSIM_OK ||
          (rc = upload(ctx, &P->d_zfirst, zfirst)) != BIMSIM_OK ||
          (rc = upload(ctx, &P->d_cst, cst)) != BIMSIM_OK ||
+         (rc = upload(ctx, &P->d_h64, h64)) != BIMSIM_OK ||
          (rc = alloc(ctx, &P->d_pl, P->n_pl)) != BIMSIM_OK ||
          (rc = alloc(ctx, &P->d_e, P->n_e)) != BIMSIM_OK ||
          (rc = alloc(ctx, &P->d_phi, std::max<long long>(1, P->n_wave_rec))) != BIMSIM_OK ||
@@ -973,15 +1001,16 @@ int series_plan_launch_waves(bimsim_ctx* ctx, const SeriesPlan& P, const double*
     if (P.harmonic) {
         // per-wave Legendre values, azimuth phases and wave weights, then the
         // per-sphere sums T
-        hipLaunchKernelGGL(harm_legendre_kernel,
-                           dim3((P.nk_max + 63) / 64, std::min(P.n_inst, 65535), P.max_lh), dim3(64),
-                           0, st, P.d_inst, P.d_sph, P.n_inst, waves, P.d_pl, P.d_e, P.d_phi,
-                           P.d_counters, P.n_counters);
-        hipLaunchKernelGGL(harm_tsum_kernel, dim3(P.max_lh, (unsigned)std::min<long long>(P.n_z, 65535)),
-                           dim3(128), 0, st, P.d_inst, P.d_sph, P.d_zmap, P.n_z, P.d_zpart, P.d_pl,
-                           P.d_e, P.d_phi, P.d_tpart);
-        hipLaunchKernelGGL(harm_records_kernel, dim3(P.max_tl, std::min(P.n_sph, 65535)), dim3(128), 0,
-                           st, P.d_sph, P.n_sph, P.d_zfirst, P.d_zpart, P.d_tpart, P.d_rec);
+        const dim3 lgrid((P.nk_max + 63) / 64, std::min(P.n_inst, 65535), P.max_lh);
+        hipLaunchKernelGGL(harm_legendre_kernel, lgrid, dim3(64), 0, st, P.d_inst, P.d_sph,
+                           P.n_inst, waves, P.d_h64, P.d_pl, P.d_e, P.d_phi, P.d_counters,
+                           P.n_counters);
+        const dim3 tgrid(P.max_lh, (unsigned)std::min<long long>(P.n_z, 65535));
+        hipLaunchKernelGGL(harm_tsum_kernel, tgrid, dim3(128), 0, st, P.d_inst, P.d_sph, P.d_zmap,
+                           P.n_z, P.d_zpart, P.d_pl, P.d_e, P.d_phi, P.d_tpart);
+        const dim3 rgrid(P.max_tl, std::min(P.n_sph, 65535));
+        hipLaunchKernelGGL(harm_records_kernel, rgrid, dim3(128), 0, st, P.d_sph, P.n_sph,
+                           P.d_zfirst, P.d_zpart, P.d_tpart, P.d_h64, P.d_rec);
     } else {
         hipLaunchKernelGGL(series_waves_kernel, wgrid, dim3(256), 0, st, P.d_inst, P.d_sph, P.n_sph,
                            waves, P.d_wt, P.d_counters, P.n_counters);
