@@ -16,6 +16,9 @@ grid (16 images), 20 spheres, 400 waves, 512 x 512 (cfg4 in
 bimsim-cuda_amd/workloads.py), as one graph-captured batch in FP32 against
 the plain FP64 form on two of its images.
 
+A third section ("lut_path") times the sphere-table path on one sphere of
+the 50-sphere 1024 x 1024 table workload (cfg3).
+
 usage: python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--no-spectral]
 Prints one JSON line.
 """
@@ -118,6 +121,51 @@ def series_kernel_ms(ctx, P, pkg, inst, waves, buf, steps):
     ctx.set_timing(False)
     ms, n = tm["us_series"]
     return ms / max(n, 1), n
+
+
+def lut_path(pkg, P, W, ctx, torch, steps=3):
+    """The sphere-table path (sphere::calcUp + gpuScalarUpLut) on one sphere of
+    the 50-sphere 1024 x 1024 table workload (cfg3: lambda 4 um, 200 um field,
+    400 waves, tables of 4096 radii x 1000 angles), FP32 tables and lookups."""
+    inst = W.make_instance([dict(p=(3.0, -2.0, 0.0), a=6.0, n=complex(1.45, 0.05))], 4.0, 1024,
+                           200.0, (0.0, 0.5), (0.0, 0.5), 400, 11)
+    s = dict(inst["spheres"][0], coeff_off=0)
+    q = P.quad_of(inst)
+    N, thetaR = inst["N"], 1000
+    aR, dR, dmin, dmax = pkg.sphere_lut_dims(q, s, N, 4096)
+    A = torch.zeros(s["Nl"] + 1, dtype=torch.complex128, device="cuda")
+    B = torch.zeros_like(A)
+    ctx.mie_coeffs([s], inst["kmag"], A, B)
+    waves = torch.as_tensor(W.mc_waves(inst, pkg.mc_sample_na)).cuda()
+    Usp = torch.empty(thetaR * dR, dtype=torch.complex64, device="cuda")
+    Uip = torch.empty(thetaR * aR, dtype=torch.complex64, device="cuda")
+    U = torch.zeros(N * N, dtype=torch.complex64, device="cuda")
+    t_tab = t_look = 0.0
+    for rep in range(steps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.sphere_lut(s, A, B, inst["kmag"], dmin, dmax, dR, aR, thetaR, Usp, Uip,
+                       prec=pkg.FP32_FAST)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ctx.nf_us_lut(q, N, N, inst["focus"], inst["kmag"], s, Usp, dR, Uip, aR, thetaR, dmin,
+                      dmax, waves, inst["subA"], U, prec=pkg.FP32_FAST)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if rep:   # the first pass warms up
+            t_tab += (t1 - t0) / steps
+            t_look += (t2 - t1) / steps
+    lookups = N * N * inst["nk"]
+    log(f"bench: sphere tables {t_tab * 1e3:.2f} ms, table lookups {t_look * 1e3:.2f} ms per sphere")
+    return {
+        "workload": "one sphere (a 6 um, Nl %d) of cfg3: 1024x1024 over 200 um, lambda 4 um, "
+                    "400 waves, tables 4096 radii x 1000 angles, FP32" % s["Nl"],
+        "table_ms": t_tab * 1e3,
+        "table_terms_per_s": dR * thetaR * (s["Nl"] + 1) / t_tab,
+        "lookup_ms": t_look * 1e3,
+        "lookups_per_s": lookups / t_look,
+        "image_ms_est_50_spheres": 50 * (t_tab + t_look) * 1e3,
+    }
 
 
 def spectral(pkg, P, W, ctx, torch, stride=16, steps=3, fp64_images=2):
@@ -348,6 +396,7 @@ def main():
     }
     if not args.no_spectral:
         out["spectral"] = spectral(pkg, P, W, ctx, torch)
+        out["lut_path"] = lut_path(pkg, P, W, ctx, torch)
     if not args.no_cpu:
         log("bench: CPU baseline")
         A = b64.A.cpu().numpy()
