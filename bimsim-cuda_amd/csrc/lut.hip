@@ -223,6 +223,104 @@ __global__ __launch_bounds__(kThreads) void uplut_kernel(
     st(U, pix, cadd(o, sum));
 }
 
+// FP32 form of gpuScalarUpLut (nfScalarUpLut.cu:11-97) for float tables of
+// width >= 2.  What depends on the pixel only (d, r-hat, the table row, the
+// radial column pair and weight) is computed once per pixel in FP64; what
+// depends on the wave only (direction and the phase (1/nk) subA
+// exp(i kmag k.c)) is computed once per workgroup in FP64 into LDS.  The
+// per-(pixel, wave) work is the reference's float arithmetic: the angle
+// acos(k.r-hat) and a bilinear fetch as two 16-byte loads of neighbouring
+// texels.  Clamp addressing is kept by clamping the coordinates into
+// [0, W-1] x [0, thetaR-1]: at a clamped edge the weight of the outer texel
+// is 0 or 1, which reproduces the edge texel.
+constexpr int kLutWaves = 1024;   // waves per LDS chunk
+constexpr int kXcds = 8;          // XCDs of the chip
+static_assert(kThreads == 256, "a workgroup is four waves over a 16 x 16 tile");
+
+typedef float f4_a8 __attribute__((ext_vector_type(4), aligned(8)));
+
+__global__ __launch_bounds__(kThreads) void uplut_f32_kernel(
+    QuadArg q, int uR, int vR, d3 f, double kmag, d3 ps, double a, const float2* __restrict__ Usp,
+    int dR, const float2* __restrict__ Uip, int aR, int thetaR, double dmin, double dmax,
+    const double* __restrict__ waves, int nk, double sA, float2* __restrict__ U) {
+    __shared__ float4 wk[kLutWaves];   // (kx, ky, kz, phase re)
+    __shared__ float wpi[kLutWaves];   // phase im
+    // Workgroups take 16 x 16 pixel tiles and each wave an 8 x 8 quarter, so
+    // that the lanes of a wave fetch nearby texels.  Workgroups are
+    // dispatched to the XCDs in turn; tiles are renumbered so that each XCD
+    // (and its L2) holds a contiguous band of the image.
+    const int tiles_u = (uR + 15) >> 4;
+    const int ntiles = tiles_u * ((vR + 15) >> 4);
+    const int xcd = blockIdx.x % kXcds, slot = blockIdx.x / kXcds;
+    const int tile = xcd * (ntiles / kXcds) + min(xcd, ntiles % kXcds) + slot;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int iu0 = (tile % tiles_u) * 16 + (wave & 1) * 8 + (lane & 7);
+    const int iv0 = (tile / tiles_u) * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const bool live = iu0 < uR && iv0 < vR;
+    const int iu = min(iu0, uR - 1), iv = min(iv0, vR - 1);
+    const d3 p = quad_at(q, (double)iu / (double)uR, (double)iv / (double)vR);
+    const d3 r = sub(p, ps);
+    const double d = len(r);
+    // r-hat is 0/0 at the sphere centre; the angle there is taken as 0 (as
+    // in the FP64 kernel) through a bias of the cosine with r-hat = 0
+    const d3 rn = norm(r);
+    const bool centre = rn.x != rn.x || rn.y != rn.y || rn.z != rn.z;
+    const float rx = centre ? 0.0f : (float)rn.x, ry = centre ? 0.0f : (float)rn.y;
+    const float rz = centre ? 0.0f : (float)rn.z, cbias = centre ? 1.0f : 0.0f;
+    const bool inside = d < a;
+    const double lo = fmax(a, dmin);
+    const float di = (float)(((d - lo) / (dmax - lo)) * (double)(dR - 1));
+    const float ai = (float)(((d - dmin) / (a - dmin)) * (double)(aR - 1));
+    const int W = inside ? aR : dR;
+    const float2* L = inside ? Uip : Usp;
+    float x = inside ? ai : di;
+    x = x >= 0.0f ? x : 0.0f;                    // also maps NaN to the first column
+    x = x <= (float)(W - 1) ? x : (float)(W - 1);
+    const int xa = min((int)floorf(x), W - 2);
+    const float wx = x - (float)xa;
+    const float* Lx = (const float*)(L + xa);
+    const int W2 = 2 * W;   // floats per table row
+    const float tscale = (float)(thetaR - 1) / kPIf;
+    const float tmax = (float)(thetaR - 1);
+    const d3 c = sub(ps, f);
+    float sr = 0.0f, si = 0.0f;
+    for (int w0 = 0; w0 < nk; w0 += kLutWaves) {
+        const int nw = min(kLutWaves, nk - w0);
+        __syncthreads();
+        for (int i = threadIdx.x; i < nw; i += kThreads) {
+            const d3 k = mk(waves + 3 * (size_t)(w0 + i));
+            const dc ph = cscl(cexp_i(kmag * dot(k, c)), sA);
+            wk[i] = make_float4((float)k.x, (float)k.y, (float)k.z, (float)ph.r);
+            wpi[i] = (float)ph.i;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int w = 0; w < nw; ++w) {
+            const float4 k = wk[w];
+            float ct = fmaf(k.z, rz, fmaf(k.y, ry, fmaf(k.x, rx, cbias)));
+            ct = fminf(fmaxf(ct, -1.0f), 1.0f);
+            float t = acosf(ct) * tscale;
+            t = t <= tmax ? t : tmax;
+            const int ya = min((int)t, thetaR - 2);
+            const float wy = t - (float)ya;
+            const float* row = Lx + __umul24(ya, W2);
+            const f4_a8 A = *(const f4_a8*)row;
+            const f4_a8 B = *(const f4_a8*)(row + W2);
+            const float ar = fmaf(wx, A.z - A.x, A.x), aim = fmaf(wx, A.w - A.y, A.y);
+            const float br = fmaf(wx, B.z - B.x, B.x), bim = fmaf(wx, B.w - B.y, B.y);
+            const float vr = fmaf(wy, br - ar, ar), vi = fmaf(wy, bim - aim, aim);
+            const float pr = k.w, pi = wpi[w];
+            sr = fmaf(pr, vr, fmaf(-pi, vi, sr));
+            si = fmaf(pr, vi, fmaf(pi, vr, si));
+        }
+    }
+    if (live) {
+        const size_t pix = (size_t)iv * uR + iu;
+        const float2 o = U[pix];
+        U[pix] = make_float2(o.x + sr, o.y + si);
+    }
+}
+
 // gpuInterpolateU (integer centre) / gpuInterpolateUf (float centre), zero border
 template <class C, class T>
 __global__ void rotate_kernel(const C* __restrict__ src, C* __restrict__ dst, int uR, int vR,
@@ -416,7 +514,14 @@ int bimsim_nf_us_lut(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int vR,
     const QuadArg q = to_quad(*pos);
     const double sA = (double)(1.0f / (float)nk) * subA;   // (1.0f/nk) * A
     const int tmr = timer_begin(ctx, kStageUs);
-    if (prec == BIMSIM_FP32_FAST)
+    // the FP32 kernel addresses rows with 24-bit products
+    const bool rows24 = (long long)thetaR * 2 * std::max(dR, aR) < (1LL << 24);
+    const dim3 tiles((unsigned)((uR + 15) / 16) * (unsigned)((vR + 15) / 16));
+    if (prec == BIMSIM_FP32_FAST && aR >= 2 && rows24)
+        hipLaunchKernelGGL(uplut_f32_kernel, tiles, dim3(kThreads), 0, ctx->stream, q, uR, vR,
+                           mk(focus), kmag, mk(s->p), s->a, (const float2*)Usp, dR,
+                           (const float2*)Uip, aR, thetaR, dmin, dmax, waves, nk, sA, (float2*)U);
+    else if (prec == BIMSIM_FP32_FAST)   // one-column inner table, or very large tables
         hipLaunchKernelGGL((uplut_kernel<float2, float2>), grid, dim3(kThreads), 0, ctx->stream, q,
                            uR, vR, mk(focus), kmag, mk(s->p), s->a, (const float2*)Usp, dR,
                            (const float2*)Uip, aR, thetaR, dmin, dmax, waves, nk, sA, (float2*)U);

@@ -121,6 +121,63 @@ def test_scattered_field_from_tables(pkg, ctx, orc, W, prec):
     assert normrel(host(U), 2 * ref) <= TOL[prec]
 
 
+def _fp32_lookup_cases(W):
+    out = {}
+    # a pixel at the sphere centre (r-hat = 0/0 there)
+    out["centre"] = W.make_instance([dict(p=(0.0, 0.0, 0.0), a=3.0, n=complex(1.4, 0.03))], 3.0,
+                                    24, 24.0, (0.0, 0.5), (0.0, 0.5), 16, 5)
+    # more waves than one workgroup stages at a time
+    out["many_waves"] = W.make_instance([dict(p=(2.0, 1.0, 0.5), a=4.0, n=complex(1.5, 0.0))],
+                                        3.0, 16, 24.0, (0.0, 0.5), (0.0, 0.5), 1500, 7)
+    # tilted illumination: cosines down to -1 (the last table row)
+    t = W.make_instance([dict(p=(-3.0, 2.0, -1.0), a=3.5, n=complex(1.3, 0.05))], 2.5, 24,
+                        24.0, (0.1, 0.7), (0.0, 0.7), 32, 9)
+    t["kdir"] = (math.sin(1.2), 0.0, math.cos(1.2))
+    out["tilted"] = t
+    # an inner table one column wide
+    out["one_column"] = W.make_instance([dict(p=(1.0, 1.0, 0.0), a=0.3, n=complex(1.4, 0.0))],
+                                        3.0, 16, 40.0, (0.0, 0.5), (0.0, 0.5), 12, 3)
+    return out
+
+
+@pytest.mark.parametrize("name", ["centre", "many_waves", "tilted", "one_column"])
+def test_fp32_lookup_edges(pkg, ctx, orc, W, name):
+    """FP32 lookups (tables stored as float) against the FP64 oracle on the
+    cases with their own code: the sphere centre, wave staging, angles at the
+    table's last row, and a one-column inner table."""
+    import bimsim_amd.pipeline as P
+    inst = _fp32_lookup_cases(W)[name]
+    s = inst["spheres"][0]
+    q = P.quad_of(inst)
+    N = inst["N"]
+    waves = W.mc_waves(inst, pkg.mc_sample_na)
+    aR, dR, dmin, dmax = pkg.sphere_lut_dims(q, s, N)
+    if name == "one_column":
+        assert aR == 1
+    else:
+        assert aR >= 2
+    thetaR = 240
+    A, B = orc.mie_coeffs(s["Nl"], inst["kmag"], s["a"], s["n"])
+    Usp = orc.lut_eval(s["Nl"], B, orc.lut_hankel(s["Nl"], inst["kmag"], s["a"], dmin, dmax, dR),
+                       thetaR)
+    Uip = orc.lut_eval(s["Nl"], A, orc.lut_bessel(s["Nl"], inst["kmag"], s["a"], s["n"], aR),
+                       thetaR)
+    ref = orc.nf_us_lut(q, N, N, inst["focus"], inst["kmag"], s, Usp, Uip, dmin, dmax, waves,
+                        inst["subA"]).ravel()
+    if name == "centre":
+        iu = iv = N // 2
+        assert abs(ref[iv * N + iu]) > 0
+    w = torch.as_tensor(np.ascontiguousarray(waves, dtype=np.float64).ravel()).cuda()
+    U = torch.zeros(N * N, dtype=torch.complex64, device="cuda")
+    ctx.nf_us_lut(q, N, N, inst["focus"], inst["kmag"], s,
+                  torch.as_tensor(Usp.ravel()).to(torch.complex64).cuda(), dR,
+                  torch.as_tensor(Uip.ravel()).to(torch.complex64).cuda(), aR, thetaR, dmin, dmax,
+                  w, inst["subA"], U, prec=pkg.FP32_FAST)
+    got = host(U)
+    assert np.all(np.isfinite(got))
+    assert normrel(got, ref) <= TOL[2]
+
+
 def test_table_path_end_to_end(pkg, ctx, orc, W):
     """Tables built on the GPU then looked up on the GPU, against the oracle
     doing both steps."""
