@@ -172,6 +172,48 @@ __global__ void lut_eval_kernel(int nl, const double2* __restrict__ C, const dou
     st(L, (size_t)it * nr + ir, acc);
 }
 
+// lut_eval_kernel for kEvalRows angle rows per workgroup, with the same
+// operations in the same order (identical results): the rows' Legendre
+// values are computed once into LDS, and C_l rad[ir][l] once per (ir, l)
+// rather than once per (angle, ir, l).
+constexpr int kEvalRows = 16;
+constexpr int kEvalMaxOrders = 512;   // LDS: kEvalRows doubles per order
+
+template <class Out>
+__global__ __launch_bounds__(kThreads) void lut_eval_rows_kernel(
+    int nl, const double2* __restrict__ C, const double2* __restrict__ rad, int nr, int thetaR,
+    double dtheta, Out* __restrict__ L) {
+    extern __shared__ double Pl[];   // [l][row], (nl + 1) kEvalRows doubles
+    const int it0 = blockIdx.y * kEvalRows;
+    if (threadIdx.x < kEvalRows) {
+        const int r = threadIdx.x;
+        const double ct = cos(dtheta * (it0 + r));
+        double P0 = 1.0, P1 = ct;
+        Pl[r] = P0;
+        Pl[kEvalRows + r] = P1;
+        for (int l = 2; l <= nl; ++l) {
+            ref::shift_legendre(l, ct, P0, P1);
+            Pl[l * kEvalRows + r] = P1;
+        }
+    }
+    __syncthreads();
+    const int ir = blockIdx.x * kThreads + threadIdx.x;
+    if (ir >= nr) return;
+    dc acc[kEvalRows];
+#pragma unroll
+    for (int r = 0; r < kEvalRows; ++r) acc[r] = dc{0.0, 0.0};
+    const double2* h = rad + (size_t)ir * (nl + 1);
+    for (int l = 0; l <= nl; ++l) {
+        const dc c = cmul(ld(C, l), ld(h, l));
+        const double* P = Pl + l * kEvalRows;
+#pragma unroll
+        for (int r = 0; r < kEvalRows; ++r) acc[r] = cadd(acc[r], cscl(c, P[r]));
+    }
+#pragma unroll
+    for (int r = 0; r < kEvalRows; ++r)
+        if (it0 + r < thetaR) st(L, (size_t)(it0 + r) * nr + ir, acc[r]);
+}
+
 // bilinear fetch with clamp addressing at data coordinates (x, y)
 template <class CT>
 __device__ inline dc fetch_clamp(const CT* L, int W, int H, float xf, float yf) {
@@ -367,6 +409,30 @@ void sphere_lut_dims_host(const QuadArg& q, const bimsim_sphere& s, int R, int d
     *dR = dR_override > 0 ? dR_override : 2 * R;
 }
 
+// L[it][ir] = sum_l C_l rad[ir][l] P_l(cos(it dtheta)) for nr radii
+static int lut_eval_launch(bimsim_ctx* ctx, bool f32, int nl, const double2* C,
+                           const double2* rad, int nr, int thetaR, double dtheta, void* L) {
+    const unsigned gx = (unsigned)((nr + kThreads - 1) / kThreads);
+    if (nl + 1 <= kEvalMaxOrders) {
+        const dim3 grid(gx, (unsigned)((thetaR + kEvalRows - 1) / kEvalRows));
+        const size_t lds = sizeof(double) * kEvalRows * (size_t)(nl + 1);
+        if (f32)
+            hipLaunchKernelGGL(lut_eval_rows_kernel<float2>, grid, dim3(kThreads), lds,
+                               ctx->stream, nl, C, rad, nr, thetaR, dtheta, (float2*)L);
+        else
+            hipLaunchKernelGGL(lut_eval_rows_kernel<double2>, grid, dim3(kThreads), lds,
+                               ctx->stream, nl, C, rad, nr, thetaR, dtheta, (double2*)L);
+    } else if (f32) {
+        hipLaunchKernelGGL(lut_eval_kernel<float2>, dim3(gx, thetaR), dim3(kThreads), 0,
+                           ctx->stream, nl, C, rad, nr, thetaR, dtheta, (float2*)L);
+    } else {
+        hipLaunchKernelGGL(lut_eval_kernel<double2>, dim3(gx, thetaR), dim3(kThreads), 0,
+                           ctx->stream, nl, C, rad, nr, thetaR, dtheta, (double2*)L);
+    }
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return BIMSIM_OK;
+}
+
 }  // namespace bimsim
 
 using namespace bimsim;
@@ -450,25 +516,11 @@ int bimsim_sphere_lut(bimsim_ctx* ctx, const bimsim_sphere* s, const void* A, co
     const double lo = std::max(s->a, dmin);
     hipLaunchKernelGGL(hankel_lut_kernel, dim3((dR + 63) / 64), dim3(64), 0, ctx->stream, nl, kmag,
                        lo, (dmax - lo) / (dR - 1), dR, (double*)work, wstride, (double2*)rad);
-    if (f32)
-        hipLaunchKernelGGL(lut_eval_kernel<float2>, dim3((dR + 255) / 256, thetaR), dim3(256), 0,
-                           ctx->stream, nl, Bd, (const double2*)rad, dR, thetaR, dtheta, (float2*)Usp);
-    else
-        hipLaunchKernelGGL(lut_eval_kernel<double2>, dim3((dR + 255) / 256, thetaR), dim3(256), 0,
-                           ctx->stream, nl, Bd, (const double2*)rad, dR, thetaR, dtheta, (double2*)Usp);
-    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    BIMSIM_TRY(lut_eval_launch(ctx, f32, nl, Bd, (const double2*)rad, dR, thetaR, dtheta, Usp));
     if (Uip) {
         hipLaunchKernelGGL(bessel_lut_kernel, dim3((aR + 63) / 64), dim3(64), 0, ctx->stream, nl,
                            kmag, s->n[0], s->n[1], s->a, aR, (zj::zc*)work, wstride, (double2*)rad);
-        if (f32)
-            hipLaunchKernelGGL(lut_eval_kernel<float2>, dim3((aR + 255) / 256, thetaR), dim3(256),
-                               0, ctx->stream, nl, Ad, (const double2*)rad, aR, thetaR, dtheta,
-                               (float2*)Uip);
-        else
-            hipLaunchKernelGGL(lut_eval_kernel<double2>, dim3((aR + 255) / 256, thetaR), dim3(256),
-                               0, ctx->stream, nl, Ad, (const double2*)rad, aR, thetaR, dtheta,
-                               (double2*)Uip);
-        BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+        BIMSIM_TRY(lut_eval_launch(ctx, f32, nl, Ad, (const double2*)rad, aR, thetaR, dtheta, Uip));
     }
     return BIMSIM_OK;
 }
