@@ -36,7 +36,8 @@ EXPORTS = (
     "bimsim_field_to_scalar", "bimsim_field_scale", "bimsim_field_crop",
     "bimsim_detector_resample", "bimsim_slice_ratio", "bimsim_batch_create",
     "bimsim_batch_launch", "bimsim_batch_destroy", "bimsim_batch_run", "bimsim_set_autotune",
-    "bimsim_set_series_algorithm",
+    "bimsim_set_series_algorithm", "bimsim_detector_rotate_add", "bimsim_detector_ring_sums",
+    "bimsim_detector_from_line",
 )
 
 
@@ -157,6 +158,12 @@ def lib():
                                                c_double, c_int, c_void_p]
         L.bimsim_slice_ratio.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p,
                                          ctypes.c_longlong, c_void_p]
+        L.bimsim_detector_rotate_add.argtypes = [c_void_p, c_int, c_double, c_int, c_int,
+                                                 c_double, c_void_p, c_void_p]
+        L.bimsim_detector_ring_sums.argtypes = [c_void_p, c_int, c_int, c_int, c_int, c_int,
+                                                c_void_p, c_void_p]
+        L.bimsim_detector_from_line.argtypes = [c_void_p, c_int, c_int, c_int, c_double,
+                                                c_void_p, c_int, c_void_p]
         L.bimsim_batch_create.argtypes = [c_void_p, ctypes.POINTER(Instance), c_int, c_void_p,
                                           c_int, c_void_p, c_int, c_int, c_void_p, c_void_p,
                                           c_void_p, c_int, ctypes.POINTER(c_void_p)]
@@ -329,6 +336,21 @@ class Context:
     def detector_resample(self, F, Du, Dv, ss, scale, accumulate, D, prec=FP64_PARITY):
         self._check(self.L.bimsim_detector_resample(self.h, prec, _ptr(F), Du, Dv, ss, scale,
                                                     1 if accumulate else 0, _ptr(D)))
+
+    def detector_rotate_add(self, deg, uR, vR, scale, src, D, prec=FP64_PARITY):
+        """D += scale * src rotated by deg (microscope.cu:89-132)."""
+        self._check(self.L.bimsim_detector_rotate_add(self.h, prec, deg, uR, vR, scale,
+                                                      _ptr(src), _ptr(D)))
+
+    def detector_ring_sums(self, uR, vR, n_samples, n_rings, src, line, prec=FP64_PARITY):
+        """line[r] += the ring sums of src (microscope.cu:138-240)."""
+        self._check(self.L.bimsim_detector_ring_sums(self.h, prec, uR, vR, n_samples, n_rings,
+                                                     _ptr(src), _ptr(line)))
+
+    def detector_from_line(self, uR, vR, scale, line, D, prec=FP64_PARITY):
+        """D = scale * line(|p - centre|) (microscope.cu:37-86)."""
+        self._check(self.L.bimsim_detector_from_line(self.h, prec, uR, vR, scale, _ptr(line),
+                                                     line.numel(), _ptr(D)))
 
     def slice_ratio(self, op, D, Di, out, prec=FP64_PARITY):
         self._check(self.L.bimsim_slice_ratio(self.h, prec, op, _ptr(D), _ptr(Di), D.numel(),

@@ -52,10 +52,18 @@ void fileoutStruct::Save(microscopeStruct* scope) {
         saveDetector(scope);
         return;
     }
-    if (scope->interpolate != -1)
-        throw std::runtime_error(
-            "bimsim: interpolated extended-source simulations (--interpolate 0..3) are not "
-            "built; use --extended-source with a source image");
+    // rings of point sources (fileout.cu:215-241)
+    switch (scope->interpolate) {
+        case 3: scope->SimulateExtendedSourceWithInterpolation(0, scope->nf.extSource); break;
+        case 1: scope->SimulateExtendedSourceInterpAtDetector(scope->nf.extSource); break;
+        case 0: scope->SimulateExtendedGausssianSourceNoInterp(scope->nf.extSource); break;
+        case 2: scope->SimulateExtendedSourceInterpLineAtDetector(scope->nf.extSource); break;
+        default: break;
+    }
+    if (scope->interpolate >= 0 && scope->interpolate <= 3) {
+        saveDetector(scope);
+        return;
+    }
     scope->Simulate();
     saveNearField(&scope->nf);
     saveFarField(scope);

@@ -1,6 +1,8 @@
 // microscopeStruct over the C ABI (reference microscope.cu:495-857).
+#include <cmath>
 #include <iostream>
 #include <sstream>
+#include <stdexcept>
 
 #include "microscope.h"
 
@@ -164,6 +166,156 @@ void microscopeStruct::SimulateExtendedSource() {
         nf.A = p.A;
         Simulate();   // the detector images accumulate
     }
+}
+
+// Ring of numPoints sources at `radius` (microscope.cu:1298-1325): the angle
+// steps by 2 PI/numPoints in float starting at one step, and each source
+// keeps its own angle as its rotation.
+void microscopeStruct::generatePointSources(int numPoints, float radius, float amplitude) {
+    focalPointsES.clear();
+    const float step = 2 * PI / (float)numPoints;
+    float theta = step;
+    for (int i = 0; i < numPoints; ++i) {
+        sourcePointES p;
+        p.rotationAngle = theta;
+        const float u = std::cos(theta) * radius, v = std::sin(theta) * radius;
+        p.f = bsPoint(u, v, 0);
+        p.A = amplitude;
+        focalPointsES.push_back(p);
+        theta += step;
+    }
+}
+
+void microscopeStruct::saveDetectorForOnePs() {
+    Ud.ResampleAndSave(D_ps1, ss);
+    Ufd.ResampleAndSave(Di_ps1, ss);
+}
+
+// The reference copies D_ps1 / Di_ps1 into textures; the samplers here read
+// the images themselves, which stay unchanged until the next
+// saveDetectorForOnePs.
+void microscopeStruct::SaveDetectorToTexture() {}
+
+// Likewise for the radial lines (read in place by InterpLineAtDetector).
+void microscopeStruct::SaveLineToTexture(int radiusLength) { line_len = radiusLength; }
+
+void microscopeStruct::InterpAtDetector() {
+    bimsim_ctx* ctx = context();
+    check(bimsim_detector_rotate_add(ctx, Di->prec, Di->rotationAngle, Di->R[0], Di->R[1], scale,
+                                     Di_ps1->S, Di->S),
+          "InterpAtDetector");
+    check(bimsim_detector_rotate_add(ctx, D->prec, D->rotationAngle, D->R[0], D->R[1], scale,
+                                     D_ps1->S, D->S),
+          "InterpAtDetector");
+}
+
+void microscopeStruct::InterpLineAtDetector() {
+    bimsim_ctx* ctx = context();
+    check(bimsim_detector_from_line(ctx, Di->prec, Di->R[0], Di->R[1], scale, Di->line, line_len,
+                                    Di->S),
+          "InterpLineAtDetector");
+    check(bimsim_detector_from_line(ctx, D->prec, D->R[0], D->R[1], scale, D->line, line_len, D->S),
+          "InterpLineAtDetector");
+}
+
+void microscopeStruct::InterpLineForAllPointSources(int numSamples, int numRings) {
+    if (!D->line || !Di->line)
+        throw std::runtime_error("bimsim: the radial lines exist only with --interpolate 2");
+    bimsim_ctx* ctx = context();
+    check(bimsim_detector_ring_sums(ctx, Di->prec, Di_ps1->R[0], Di_ps1->R[1], numSamples, numRings,
+                                    Di_ps1->S, Di->line),
+          "InterpLineForAllPointSources");
+    check(bimsim_detector_ring_sums(ctx, D->prec, D_ps1->R[0], D_ps1->R[1], numSamples, numRings,
+                                    D_ps1->S, D->line),
+          "InterpLineForAllPointSources");
+}
+
+// microscope.cu:1004-1095
+void microscopeStruct::SimulateExtendedSourceInterpAtDetector(int* pointsRings) {
+    clearDetector();
+    const int numRings = pointsRings[1];
+    scale = 1.0f;
+    Simulate();   // the source at the current focus
+    const int npts = pointsRings[0];
+    const float spacing = 1;
+    float radius = spacing;
+    for (float j = spacing; j < numRings; j += spacing) {
+        generatePointSources(npts, radius, 1);
+        nf.focus = focalPointsES[0].f;
+        nf.A = focalPointsES[0].A;
+        Simulate();
+        saveDetectorForOnePs();
+        SaveDetectorToTexture();
+        for (int i = 1; i < npts; ++i) {
+            nf.focus = focalPointsES[i].f;
+            nf.A = focalPointsES[i].A;
+            nf.rotationAngle = focalPointsES[i].rotationAngle;
+            D->rotationAngle = focalPointsES[i].rotationAngle;
+            Di->rotationAngle = focalPointsES[i].rotationAngle;
+            InterpAtDetector();
+        }
+        radius += spacing;
+    }
+}
+
+// microscope.cu:1097-1187.  The detector images are replaced by the radial
+// lines at the end, so the sources' own images do not reach them.
+void microscopeStruct::SimulateExtendedSourceInterpLineAtDetector(int* pointsRings) {
+    clearDetector();
+    const int numRings = pointsRings[1] - 1;
+    scale = 1.0f;
+    nf.A = 1;
+    Simulate();   // the source at the current focus
+    saveDetectorForOnePs();
+    SaveDetectorToTexture();
+    InterpLineForAllPointSources(pointsRings[0], pointsRings[1]);
+    for (int j = 1; j <= numRings; ++j) {
+        nf.focus = bsPoint((ptype)j, 0, 0);
+        nf.A = 1;
+        Simulate();
+        saveDetectorForOnePs();
+        SaveDetectorToTexture();
+        InterpLineForAllPointSources(pointsRings[0], pointsRings[1]);
+    }
+    SaveLineToTexture(pointsRings[1]);
+    InterpLineAtDetector();
+}
+
+// microscope.cu:1189-1256
+void microscopeStruct::SimulateExtendedSourceWithInterpolation(float, int* pointsRings) {
+    scale = 1;
+    clearDetector();
+    const int numRings = pointsRings[1];
+    Simulate();   // the source at the current focus
+    const float spacing = 1;
+    float radius = spacing;
+    for (int j = 1; j < numRings; ++j) {
+        const int npts = pointsRings[0];
+        generatePointSources(npts, radius, 1);
+        radius += spacing;
+        nf.focus = focalPointsES[0].f;
+        nf.A = focalPointsES[0].A;
+        SimulateAndSaveTexture();
+        for (int i = 1; i < npts; ++i) {
+            nf.focus = focalPointsES[i].f;
+            nf.A = focalPointsES[i].A;
+            nf.rotationAngle = focalPointsES[i].rotationAngle;
+            SimulateWithInterpolation();
+        }
+    }
+}
+
+// microscope.cu:1327-1410.  After the source at the current focus the
+// reference steps its ring counter, an int, by 0.5 from 0, so for one ring
+// or more it never leaves the loop; that case is an error here.
+void microscopeStruct::SimulateExtendedGausssianSourceNoInterp(int* pointsRings) {
+    clearDetector();
+    const int numRings = pointsRings[1];
+    Simulate();
+    if (numRings > 0)
+        throw std::runtime_error(
+            "bimsim: --interpolate 0 with one ring or more does not terminate in the reference "
+            "(its ring counter is an int stepped by 0.5); use --points-rings <sigma> 0");
 }
 
 void microscopeStruct::SimulateAndSaveTexture() {

@@ -44,6 +44,7 @@ struct microscopeStruct {
 
     std::vector<sourcePoint> focalPoints;
     std::vector<sourcePointES> focalPointsES;
+    int line_len = 0;               // samples of the radial lines (--interpolate 2)
 
     // The band-pass, crop and integration of SimulateImaging run as one
     // fused band-limited FFT pass (true) or as the reference's separate
@@ -75,6 +76,30 @@ struct microscopeStruct {
     // amplitude value / 255; the detector images sum the sources' images.
     void LoadExtendedSource(std::string filename);
     void SimulateExtendedSource();
+
+    // Extended sources from rings of point sources (microscope.cu:37-420,
+    // 1004-1465), selected by --interpolate with --points-rings
+    // (points per ring, rings); pointsRings is nf.extSource:
+    //   1: each ring's first source is simulated; the others add its detector
+    //      images rotated by their angles (SimulateExtendedSourceInterpAtDetector);
+    //   2: ring sums of each simulated source's detector images into a radial
+    //      line, then the detector images are that line as a function of the
+    //      distance from the centre (SimulateExtendedSourceInterpLineAtDetector);
+    //   3: each ring's first near field is rotated for the other sources and
+    //      imaged (SimulateExtendedSourceWithInterpolation);
+    //   0: the Gaussian ring source without interpolation
+    //      (SimulateExtendedGausssianSourceNoInterp).
+    void generatePointSources(int numPoints, float radius, float amplitude);
+    void saveDetectorForOnePs();
+    void SaveDetectorToTexture();
+    void SaveLineToTexture(int radiusLength);
+    void InterpAtDetector();
+    void InterpLineAtDetector();
+    void InterpLineForAllPointSources(int numSamples, int numRings);
+    void SimulateExtendedSourceInterpAtDetector(int* pointsRings);
+    void SimulateExtendedSourceInterpLineAtDetector(int* pointsRings);
+    void SimulateExtendedSourceWithInterpolation(float sphereRadius, int* pointsRings);
+    void SimulateExtendedGausssianSourceNoInterp(int* pointsRings);
 
     void setNearfield();
     void setRes(int x_res, int y_res, int pad, int supersampling);

@@ -325,6 +325,32 @@ int bimsim_nf_us_lut(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int vR,
 int bimsim_nf_interpolate(bimsim_ctx* ctx, int prec, double deg, int uR, int vR,
                           const void* U_src, const void* Uf_src, void* U, void* Uf);
 
+/* ------------------------------- extended sources, detector-side images
+ * Images are uR x vR real samples (float for FP32_FAST, double otherwise);
+ * the centre is ((int)(uR/2.0), (int)(vR/2.0)). */
+
+/* D += scale * src rotated by deg about the centre, bilinear with a zero
+ * border (InterpolateAndIntegrateUd / InterpolateAndIntegrateUfd, reference
+ * microscope.cu:89-132). */
+int bimsim_detector_rotate_add(bimsim_ctx* ctx, int prec, double deg, int uR, int vR,
+                               double scale, const void* src, void* D);
+
+/* line[r] += sum over n_samples points of the ring of radius r about the
+ * centre, for r = 0..n_rings-1: point t at angle (2 PI/n_samples)(t+1),
+ * (uc + r cos, vc - r sin), bilinear with a zero border; the sum is the
+ * reference's tree (step n_samples/2, halving to 1), which leaves out points
+ * when n_samples is not a power of two (simRing_Ud / simRing_Ufd, reference
+ * microscope.cu:138-240).  1 <= n_samples <= 1023. */
+int bimsim_detector_ring_sums(bimsim_ctx* ctx, int prec, int uR, int vR, int n_samples,
+                              int n_rings, const void* src, void* line);
+
+/* D = scale * line at the distance |(iu, iv) - centre| (float), linear
+ * between samples with a zero border outside [0, len)
+ * (InterpolateAndIntegrateUdFromLine / ...UfdFromLine, reference
+ * microscope.cu:37-86). */
+int bimsim_detector_from_line(bimsim_ctx* ctx, int prec, int uR, int vR, double scale,
+                              const void* line, int len, void* D);
+
 /* ---------------------------------------------------------------- batches
  * Many independent absorbance images (spectral sweeps, ragged sets of
  * spheres and grids) in one pass: every instance is a microscopeStruct::

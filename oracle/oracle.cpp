@@ -719,6 +719,75 @@ void interpolate_impl(const double* in, double* out, int uR, int vR, T deg, bool
         }
 }
 
+// ---------------------------------------- extended-source detector images
+// Real images sampled like the reference's float textures at the same
+// coordinates: bilinear (exact weights), zero outside.
+template <class T>
+T fetch_border_real(const double* S, int W, int H, float xf, float yf) {
+    const T x = (T)xf, y = (T)yf;
+    const T fx = std::floor(x), fy = std::floor(y);
+    const T wx = x - fx, wy = y - fy;
+    const int x0 = (int)fx, y0 = (int)fy;
+    auto at = [&](int xx, int yy) -> T {
+        if (xx < 0 || yy < 0 || xx >= W || yy >= H) return 0;
+        return (T)S[(size_t)yy * W + xx];
+    };
+    const T a = ((T)1 - wx) * at(x0, y0) + wx * at(x0 + 1, y0);
+    const T b = ((T)1 - wx) * at(x0, y0 + 1) + wx * at(x0 + 1, y0 + 1);
+    return ((T)1 - wy) * a + wy * b;
+}
+
+// InterpolateAndIntegrateUd / Ufd, microscope.cu:89-132
+template <class T>
+void det_rotate_add_impl(const double* src, double* D, int uR, int vR, T deg, T scale) {
+    const int uc = (int)(uR / 2.0), vc = (int)(vR / 2.0);
+    const T cs = std::cos(deg), sn = std::sin(deg);
+    for (int iv = 0; iv < vR; ++iv)
+        for (int iu = 0; iu < uR; ++iu) {
+            const float nx = (float)((T)uc + (T)(iu - uc) * cs - (T)(iv - vc) * sn);
+            const float ny = (float)((T)vc + (T)(iu - uc) * sn + (T)(iv - vc) * cs);
+            const size_t i = (size_t)iv * uR + iu;
+            D[i] = (double)((T)D[i] + scale * fetch_border_real<T>(src, uR, vR, nx, ny));
+        }
+}
+
+// simRing_Ud / simRing_Ufd, microscope.cu:138-240: one ring per radius, the
+// samples summed by the workgroup's tree (stride n/2 halving to 1)
+template <class T>
+void det_ring_sums_impl(const double* src, double* line, int uR, int vR, int ns, int nrings) {
+    const int uc = (int)(uR / 2.0), vc = (int)(vR / 2.0);
+    std::vector<T> s(ns);
+    for (int r = 0; r < nrings; ++r) {
+        for (int t = 0; t < ns; ++t) {
+            const T th = (T)((2.0 * (double)kPIf / ns) * (double)(t + 1));
+            const float nx = (float)((T)uc + (T)r * std::cos(th));
+            const float ny = (float)((T)vc - (T)r * std::sin(th));
+            s[t] = fetch_border_real<T>(src, uR, vR, nx, ny);
+        }
+        for (int step = ns / 2; step >= 1; step >>= 1)
+            for (int t = 0; t < step; ++t) s[t] = s[t] + s[t + step];
+        line[r] = (double)((T)line[r] + s[0]);
+    }
+}
+
+// InterpolateAndIntegrateUdFromLine / UfdFromLine, microscope.cu:37-86
+template <class T>
+void det_from_line_impl(const double* line, int len, double* D, int uR, int vR, T scale) {
+    const int uc = (int)(uR / 2.0), vc = (int)(vR / 2.0);
+    for (int iv = 0; iv < vR; ++iv)
+        for (int iu = 0; iu < uR; ++iu) {
+            const float du = (float)(iu - uc), dv = (float)(iv - vc);
+            const float r = std::sqrt(du * du + dv * dv);
+            const T x = (T)r;
+            const T fx = std::floor(x);
+            const T w = x - fx;
+            const int x0 = (int)fx;
+            const T a = (x0 >= 0 && x0 < len) ? (T)line[x0] : (T)0;
+            const T b = (x0 + 1 >= 0 && x0 + 1 < len) ? (T)line[x0 + 1] : (T)0;
+            D[(size_t)iv * uR + iu] = (double)(scale * (((T)1 - w) * a + w * b));
+        }
+}
+
 // ----------------------------------------------------------------- random
 // glibc random(3) TYPE_3 additive generator, as rand() after srand(seed).
 struct GlibcRand {
@@ -1052,6 +1121,28 @@ int orc_nf_interpolate(const double* Uin, double* Uout, int uR, int vR, double d
                        int centre_float, int fp32) {
     if (fp32) interpolate_impl<float>(Uin, Uout, uR, vR, (float)deg, centre_float != 0);
     else interpolate_impl<double>(Uin, Uout, uR, vR, deg, centre_float != 0);
+    return 0;
+}
+
+int orc_det_rotate_add(const double* src, double* D, int uR, int vR, double deg, double scale,
+                       int fp32) {
+    if (fp32) det_rotate_add_impl<float>(src, D, uR, vR, (float)deg, (float)scale);
+    else det_rotate_add_impl<double>(src, D, uR, vR, deg, scale);
+    return 0;
+}
+
+int orc_det_ring_sums(const double* src, double* line, int uR, int vR, int n_samples,
+                      int n_rings, int fp32) {
+    if (n_samples < 1) return -1;
+    if (fp32) det_ring_sums_impl<float>(src, line, uR, vR, n_samples, n_rings);
+    else det_ring_sums_impl<double>(src, line, uR, vR, n_samples, n_rings);
+    return 0;
+}
+
+int orc_det_from_line(const double* line, int len, double* D, int uR, int vR, double scale,
+                      int fp32) {
+    if (fp32) det_from_line_impl<float>(line, len, D, uR, vR, (float)scale);
+    else det_from_line_impl<double>(line, len, D, uR, vR, scale);
     return 0;
 }
 

@@ -84,6 +84,18 @@ int orc_nf_us_lut(const bimsim_quad* q, int uR, int vR, const double f[3], doubl
 int orc_nf_interpolate(const double* Uin, double* Uout, int uR, int vR, double deg,
                        int centre_float, int fp32);
 
+/* Extended-source detector images (microscope.cu:37-240); real uR x vR
+ * images, centre ((int)(uR/2.0), (int)(vR/2.0)), zero border:
+ * D += scale * src rotated by deg (InterpolateAndIntegrateUd, :89-132);
+ * line[r] += the reference's tree sum over a ring of n_samples points
+ * (simRing_Ud, :196-240); D = scale * line(|p - centre|) (:37-60). */
+int orc_det_rotate_add(const double* src, double* D, int uR, int vR, double deg, double scale,
+                       int fp32);
+int orc_det_ring_sums(const double* src, double* line, int uR, int vR, int n_samples,
+                      int n_rings, int fp32);
+int orc_det_from_line(const double* line, int len, double* D, int uR, int vR, double scale,
+                      int fp32);
+
 /* Detector (microscope.cu:495-526,608-662,722-791; fieldslice.cpp:25-94;
  * fieldslice.cu:71-153,319-362).  U and Uf are band-passed in place. */
 int orc_fft2(double* data, int uR, int vR, int inverse);

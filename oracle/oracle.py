@@ -214,6 +214,32 @@ class Oracle:
                                   c_double(deg), c_int(centre_float), c_int(fp32))
         return out.view(np.complex128).reshape(vR, uR)
 
+    # ---- extended-source detector images (real uR x vR arrays)
+    def det_rotate_add(self, src, D, deg, scale=1.0, fp32=0):
+        vR, uR = src.shape
+        out = np.ascontiguousarray(D, dtype=np.float64).copy()
+        self.L.orc_det_rotate_add(_p(np.ascontiguousarray(src, dtype=np.float64).ravel()),
+                                  _p(out.ravel()), c_int(uR), c_int(vR), c_double(deg),
+                                  c_double(scale), c_int(fp32))
+        return out
+
+    def det_ring_sums(self, src, line, n_samples, n_rings, fp32=0):
+        vR, uR = src.shape
+        out = np.ascontiguousarray(line, dtype=np.float64).copy()
+        assert out.size >= n_rings
+        rc = self.L.orc_det_ring_sums(_p(np.ascontiguousarray(src, dtype=np.float64).ravel()),
+                                      _p(out), c_int(uR), c_int(vR), c_int(n_samples),
+                                      c_int(n_rings), c_int(fp32))
+        assert rc == 0
+        return out
+
+    def det_from_line(self, line, uR, vR, scale=1.0, fp32=0):
+        line = np.ascontiguousarray(line, dtype=np.float64)
+        out = np.zeros((vR, uR))
+        self.L.orc_det_from_line(_p(line), c_int(line.size), _p(out.ravel()), c_int(uR),
+                                 c_int(vR), c_double(scale), c_int(fp32))
+        return out
+
     # ---- detector
     def fft2(self, U, inverse=False):
         vR, uR = U.shape

@@ -56,6 +56,21 @@ def run_cli(tmp_path, *extra):
 
 
 def oracle_case(orc, lut, focus=(0.0, 0.0, 0.0), amp=1.0):
+    """Detector images (D, Di, absorbance) of one source at `focus`."""
+    U, Uf = oracle_fields(orc, lut, focus, amp)
+    return oracle_detector(orc, U, Uf)
+
+
+def oracle_detector(orc, U, Uf):
+    c = CASE
+    L = c["size"] * (2 * c["pad"] + 1)
+    out = orc.detector(U, Uf, 1.0 / L, 1.0 / L, 0.0, f32(c["obj"]), f32(c["lam"]), c["pad"],
+                       c["res"], 1)
+    return out[2], out[3], out[4]
+
+
+def oracle_fields(orc, lut, focus=(0.0, 0.0, 0.0), amp=1.0):
+    """Near fields (U = Uf + Us outside the spheres, Uf) of one source."""
     import oracle
     c = CASE
     lam = f32(c["lam"])
@@ -82,9 +97,7 @@ def oracle_case(orc, lut, focus=(0.0, 0.0, 0.0), amp=1.0):
         Us = orc.nf_us_lut(q, N, N, focus, kmag, s, Usp, Uip, dmin, dmax, waves, subA)
     else:
         Us = orc.nf_us(q, N, N, focus, kmag, [s], A, B, waves, subA)
-    U = orc.nf_sum_uf(q, N, N, [s], Uf, Us)
-    out = orc.detector(U, Uf, 1.0 / L, 1.0 / L, 0.0, obj, lam, c["pad"], c["res"], 1)
-    return out[2], out[3], out[4]
+    return orc.nf_sum_uf(q, N, N, [s], Uf, Us), Uf
 
 
 @pytest.fixture(scope="module")
@@ -162,3 +175,102 @@ def test_cli_extended_source_matches_oracle(exe, orc, tmp_path):
     bright = Di > 1e-3 * Di.max()
     Ab = -np.log10(D / Di)
     assert np.max(np.abs(out["A"][0][0][bright] - Ab[bright])) <= 2e-6
+
+
+def ring_sources(npts, radius):
+    """generatePointSources (microscope.cu:1298-1325): float angles stepping
+    by 2 PI_f/npts from one step, foci (cos, sin) * radius."""
+    step = np.float32(2) * np.float32(PI_F) / np.float32(npts)
+    theta = step
+    out = []
+    for _ in range(npts):
+        u = np.float32(np.cos(theta)) * np.float32(radius)
+        v = np.float32(np.sin(theta)) * np.float32(radius)
+        out.append(((float(u), float(v), 0.0), float(theta)))
+        theta = np.float32(theta + step)
+    return out
+
+
+def check_images(out, D, Di):
+    assert normrel(out["I"][0][0], D) <= 2e-6
+    assert normrel(out["I0"][0][0], Di) <= 2e-6
+    bright = Di > 1e-3 * Di.max()
+    Ab = -np.log10(D / Di)
+    assert np.max(np.abs(out["A"][0][0][bright] - Ab[bright])) <= 2e-6
+
+
+PARITY = ("--precision", "parity", "--recursive-us")
+
+
+def test_cli_interpolate_at_detector(exe, orc, tmp_path):
+    """--interpolate 1: per ring, the first source's detector images rotated
+    by each other source's angle are added (microscope.cu:1004-1095)."""
+    npts, rings = 4, 3
+    out = run_cli(tmp_path, *PARITY, "--interpolate", 1, "--points-rings", npts, rings)
+    D, Di, _ = oracle_case(orc, False)
+    for radius in range(1, rings):
+        src = ring_sources(npts, radius)
+        d0, di0, _ = oracle_case(orc, False, src[0][0], 1.0)
+        D, Di = D + d0, Di + di0
+        for _, ang in src[1:]:
+            D = orc.det_rotate_add(d0, D, ang)
+            Di = orc.det_rotate_add(di0, Di, ang)
+    check_images(out, D, Di)
+
+
+@pytest.mark.parametrize("npts", [4, 6])
+def test_cli_interpolate_line(exe, orc, tmp_path, npts):
+    """--interpolate 2: ring sums of each source's detector images (sources
+    at the focus and at (j, 0, 0)) build radial lines, and the images are the
+    lines at each pixel's distance from the centre (microscope.cu:1097-1187);
+    six samples per ring sum four of them, as the reference's tree does."""
+    rings = 3
+    out = run_cli(tmp_path, *PARITY, "--interpolate", 2, "--points-rings", npts, rings)
+    line = np.zeros(rings)
+    linei = np.zeros(rings)
+    for j in range(0, rings):
+        d, di, _ = oracle_case(orc, False, (float(j), 0.0, 0.0), 1.0)
+        line = orc.det_ring_sums(d, line, npts, rings)
+        linei = orc.det_ring_sums(di, linei, npts, rings)
+    n = CASE["res"]
+    D = orc.det_from_line(line, n, n)
+    Di = orc.det_from_line(linei, n, n)
+    assert normrel(out["I"][0][0], D) <= 2e-6
+    assert normrel(out["I0"][0][0], Di) <= 2e-6
+
+
+def test_cli_interpolate_near_field(exe, orc, tmp_path):
+    """--interpolate 3: per ring, the first source's near fields rotated by
+    each other source's angle (U about the integer centre, Uf about the float
+    centre) go through the detector (microscope.cu:1189-1256)."""
+    npts, rings = 3, 3
+    out = run_cli(tmp_path, *PARITY, "--interpolate", 3, "--points-rings", npts, rings)
+    D, Di, _ = oracle_case(orc, False)
+    for radius in range(1, rings):
+        src = ring_sources(npts, radius)
+        U0, Uf0 = oracle_fields(orc, False, src[0][0], 1.0)
+        d, di, _ = oracle_detector(orc, U0, Uf0)
+        D, Di = D + d, Di + di
+        for _, ang in src[1:]:
+            U = orc.nf_interpolate(U0, ang, centre_float=0)
+            Uf = orc.nf_interpolate(Uf0, ang, centre_float=1)
+            d, di, _ = oracle_detector(orc, U, Uf)
+            D, Di = D + d, Di + di
+    check_images(out, D, Di)
+
+
+def test_cli_interpolate_gaussian_rings(exe, orc, tmp_path):
+    """--interpolate 0: with no rings the source at the focus alone; with one
+    ring or more the reference's loop never ends, which is an error here."""
+    out = run_cli(tmp_path, *PARITY, "--interpolate", 0, "--points-rings", 2, 0)
+    D, Di, _ = oracle_case(orc, False)
+    check_images(out, D, Di)
+    c = CASE
+    args = [exe, "--spheres", *c["sphere"], "--materials", *c["n"], "--lambda", c["lam"],
+            "--xy", c["size"], 0, "--resolution", c["res"], "--padding", c["pad"],
+            "--samples", c["nk"], "--field-order", c["m"], "--intensity", "I2", *PARITY,
+            "--interpolate", 0, "--points-rings", 2, 1]
+    p = subprocess.run([str(a) for a in args], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert p.returncode != 0 and "does not terminate" in (p.stdout + p.stderr)
+    assert not (tmp_path / "I2").exists()
