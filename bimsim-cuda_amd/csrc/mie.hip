@@ -4,11 +4,13 @@
 //
 // The work is a few long chains of dependent FP64 steps per sphere, so the
 // kernels spread it over lanes and keep it out of global memory: a block
-// takes up to 32 spheres and three lanes per sphere compute, into LDS, J of
-// ka, Y of ka and J of kna (the Y of kna is not used); the recurrences
-// reread their arrays, so LDS latency instead of a global round trip per
-// step.  Then every lane takes (sphere, order) pairs of the coefficient
-// formula.
+// takes up to 32 spheres and computes, into LDS, J of ka, Y of ka and J of
+// kna (the Y of kna is not used), one wave per part and one lane per sphere
+// in it, so that the three parts' different code paths run side by side on
+// different waves rather than one after another in one wave; the
+// recurrences reread their arrays, so LDS latency instead of a global round
+// trip per step.  Then every lane takes (sphere, order) pairs of the
+// coefficient formula.
 #include "batch.h"
 #include "internal.h"
 #include "zj_device.h"
@@ -25,9 +27,12 @@ struct SphereBatch {
 
 using zj::zc;
 
-// Spheres per block so that 2 lanes x 4 arrays x wstride complex values
+// Spheres per block so that 2 sets x 4 arrays x wstride complex values
 // per sphere fit the LDS budget.
-constexpr int kLanes = 3;            // lanes per sphere
+constexpr int kLanes = 3;            // parts per sphere: one wave each
+constexpr int kWave = 64;
+constexpr int kMieThreads = kLanes * kWave;
+static_assert(kSphPerLaunch <= kWave, "one lane per sphere in each part's wave");
 
 int mie_spheres_per_block(int wstride) {
     const size_t per = 2 * 4 * sizeof(zc) * (size_t)wstride;
@@ -67,14 +72,15 @@ __device__ void mie_order(const SphereDev& S, int l, const zc* w, int wstride, d
 }
 
 // Coefficients of the block's cnt spheres (sh, wavenumbers kh, in LDS);
-// lds holds 8 wstride complex values per sphere.  Three lanes per sphere.
+// lds holds 8 wstride complex values per sphere.  Wave r computes part r of
+// sphere (lane).
 __device__ __forceinline__ void mie_block(const SphereDev* sh, const double* kh, int cnt,
                                           int quirks, zc* lds, int wstride, double2* A,
                                           double2* B) {
     const int t = (int)threadIdx.x;
-    const int s = t / kLanes;
+    const int s = t % kWave;
     if (s < cnt)
-        mie_bessel(sh[s], kh[s], t % kLanes, quirks, lds + (size_t)8 * s * wstride, wstride);
+        mie_bessel(sh[s], kh[s], t / kWave, quirks, lds + (size_t)8 * s * wstride, wstride);
     __syncthreads();
     int lm = 0;
     for (int i = 0; i < cnt; ++i) lm = max(lm, sh[i].Nl);
@@ -90,13 +96,12 @@ __device__ __forceinline__ void mie_block(const SphereDev* sh, const double* kh,
 // for orders too high for LDS; otherwise in LDS (a separate instantiation,
 // so that its accesses compile to LDS instructions).
 template <bool kGlobal>
-__global__ __launch_bounds__(kLanes * kSphPerLaunch) void mie_coeff_kernel(
-    SphereBatch batch, int ns, double kmag, int quirks, zc* work, int wstride, double2* A,
-    double2* B) {
+__global__ __launch_bounds__(kMieThreads) void mie_coeff_kernel(
+    SphereBatch batch, int ns, int spb, double kmag, int quirks, zc* work, int wstride,
+    double2* A, double2* B) {
     extern __shared__ zc lds[];
     __shared__ SphereDev sh[kSphPerLaunch];
     __shared__ double kh[kSphPerLaunch];
-    const int spb = (int)blockDim.x / kLanes;
     const int s0 = (int)blockIdx.x * spb;
     const int cnt = min(spb, ns - s0);
     const int t = (int)threadIdx.x;
@@ -113,13 +118,12 @@ __global__ __launch_bounds__(kLanes * kSphPerLaunch) void mie_coeff_kernel(
 
 // Spheres [b spb, b spb + spb) of a batch, sphere t at wavenumber kmag[t].
 template <bool kGlobal>
-__global__ __launch_bounds__(kLanes * kSphPerLaunch) void mie_batch_kernel(
-    const SphereDev* __restrict__ sph, const double* __restrict__ kmag, int n, int quirks,
-    zc* work, int wstride, double2* A, double2* B) {
+__global__ __launch_bounds__(kMieThreads) void mie_batch_kernel(
+    const SphereDev* __restrict__ sph, const double* __restrict__ kmag, int n, int spb,
+    int quirks, zc* work, int wstride, double2* A, double2* B) {
     extern __shared__ zc lds[];
     __shared__ SphereDev sh[kSphPerLaunch];
     __shared__ double kh[kSphPerLaunch];
-    const int spb = (int)blockDim.x / kLanes;
     const int s0 = (int)blockIdx.x * spb;
     const int cnt = min(spb, n - s0);
     const int t = (int)threadIdx.x;
@@ -148,11 +152,11 @@ int mie_batch_launch(bimsim_ctx* ctx, const SphereDev* d_sph, const double* d_km
         w = (zc*)work;
     }
     if (w)
-        hipLaunchKernelGGL(mie_batch_kernel<true>, dim3((n + spb - 1) / spb), dim3(kLanes * spb),
-                           0, ctx->stream, d_sph, d_kmag, n, quirks, w, wstride, A, B);
+        hipLaunchKernelGGL(mie_batch_kernel<true>, dim3((n + spb - 1) / spb), dim3(kMieThreads),
+                           0, ctx->stream, d_sph, d_kmag, n, spb, quirks, w, wstride, A, B);
     else
-        hipLaunchKernelGGL(mie_batch_kernel<false>, dim3((n + spb - 1) / spb), dim3(kLanes * spb),
-                           lds, ctx->stream, d_sph, d_kmag, n, quirks, w, wstride, A, B);
+        hipLaunchKernelGGL(mie_batch_kernel<false>, dim3((n + spb - 1) / spb), dim3(kMieThreads),
+                           lds, ctx->stream, d_sph, d_kmag, n, spb, quirks, w, wstride, A, B);
     BIMSIM_HIP_CHECK(ctx, hipGetLastError());
     return BIMSIM_OK;
 }
@@ -201,12 +205,12 @@ extern "C" int bimsim_mie_coeffs(bimsim_ctx* ctx, const bimsim_sphere* spheres, 
         }
         if (work)
             hipLaunchKernelGGL(mie_coeff_kernel<true>, dim3((cnt + spb - 1) / spb),
-                               dim3(kLanes * spb), 0, ctx->stream, b, cnt, kmag, quirks,
+                               dim3(kMieThreads), 0, ctx->stream, b, cnt, spb, kmag, quirks,
                                (zj::zc*)work, wstride, (double2*)A_out, (double2*)B_out);
         else
             hipLaunchKernelGGL(mie_coeff_kernel<false>, dim3((cnt + spb - 1) / spb),
-                               dim3(kLanes * spb), lds, ctx->stream, b, cnt, kmag, quirks, nullptr,
-                               wstride, (double2*)A_out, (double2*)B_out);
+                               dim3(kMieThreads), lds, ctx->stream, b, cnt, spb, kmag, quirks,
+                               nullptr, wstride, (double2*)A_out, (double2*)B_out);
         BIMSIM_HIP_CHECK(ctx, hipGetLastError());
     }
     return BIMSIM_OK;

@@ -46,6 +46,8 @@ struct SeriesSph {
     long long toff;     // harmonic form: offset of its harmonic records
     int nchunk;         // harmonic form: wave chunks of its harmonic sums
     int tl;             // harmonic form: record stride (its instance's order bucket)
+    int planar;         // harmonic form: the slice contains the centre and is normal
+                        // to z, so every pixel has x = r-hat.z = 0
 };
 
 struct SeriesItem {

@@ -403,6 +403,7 @@ __global__ void harm_tsum_kernel(const SeriesInst* __restrict__ insts,
         const SeriesInst& I = insts[SS.inst];
         const int nl1 = SS.s.Nl + 1;
         if (m >= nl1 || l < m || l >= nl1) continue;
+        if (SS.planar && ((l + m) & 1)) continue;   // Pn_lm(0) = 0: not read
         const int chunk_waves = (I.nk + SS.nchunk - 1) / SS.nchunk;
         const int w0 = sc.y * chunk_waves, w1 = min(I.nk, w0 + chunk_waves);
         dc tp = {0.0, 0.0}, tm = {0.0, 0.0};
@@ -426,18 +427,22 @@ __global__ void harm_tsum_kernel(const SeriesInst* __restrict__ insts,
 
 // Records per (sphere, m, l): the chunk sums times 4 pi/(2l+1) K_lm,
 // K_lm = prod_{j=m+1..l} a_jm / 2, as U1 and U2 at rec[toff + m * tl + l]
-// (zero outside m <= l <= Nl).  Grid: (m, spheres), threads l.
+// (zero outside m <= l <= Nl).  Planar spheres (x = 0 at every pixel) fold
+// Pn_lm(0) in instead of K_lm and keep only l + m even, row m holding
+// l = 2k + (m & 1) at rec[toff + m * (tl/2) + k].  Grid: (m, spheres),
+// threads l (or k).
 __global__ void harm_records_kernel(const SeriesSph* __restrict__ sph, int n_sph,
                                     const long long* __restrict__ zfirst,
                                     const long long* __restrict__ zpart,
                                     const double2* __restrict__ part,
                                     const double* __restrict__ h64, float4* __restrict__ rec) {
-    const int l = threadIdx.x;
     const int m = blockIdx.x;
     for (int s = blockIdx.y; s < n_sph; s += gridDim.y) {
         const SeriesSph& SS = sph[s];
         const int tl = SS.tl;
-        if (l >= tl || m >= tl) continue;
+        // planar spheres: row m holds l = 2k + (m & 1), k < tl / 2
+        const int l = SS.planar ? 2 * (int)threadIdx.x + (m & 1) : (int)threadIdx.x;
+        if ((SS.planar ? (int)threadIdx.x >= tl / 2 : l >= tl) || m >= tl) continue;
         float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
         const int nl1 = SS.s.Nl + 1;
         if (m < nl1 && l >= m && l < nl1) {
@@ -451,14 +456,27 @@ __global__ void harm_records_kernel(const SeriesSph* __restrict__ sph, int n_sph
                 tp = cadd(tp, dc{a.x, a.y});
                 tm = cadd(tm, dc{b.x, b.y});
             }
-            const double f = 4.0 * M_PI / (2.0 * l + 1.0) * h64[kHK + m * kH + l];
+            double f;
+            if (SS.planar) {
+                // Pn_lm(0): Pn_mm(0) = Pn_00 prod_k sqrt((2k+1)/(2k)),
+                // Pn_lm(0) = -a_lm b_lm Pn_{l-2,m}(0), folded into the record
+                double pn = 0.28209479177387814;
+                for (int k = 1; k <= m; ++k) pn *= h64[kHDiag + k];
+                const double* ab = h64 + 2 * (m * kH);
+                for (int j = m + 2; j <= l; j += 2) pn = -(ab[2 * j] * ab[2 * j + 1]) * pn;
+                f = 4.0 * M_PI / (2.0 * l + 1.0) * pn;
+            } else {
+                f = 4.0 * M_PI / (2.0 * l + 1.0) * h64[kHK + m * kH + l];
+            }
             if (m == 0)
                 r = make_float4((float)(f * tp.r), (float)(f * tp.i), 0.f, 0.f);
             else
                 r = make_float4((float)(f * (tp.r + tm.r)), (float)(f * (tp.i + tm.i)),
                                 (float)(f * (tp.i - tm.i)), (float)(f * (tm.r - tp.r)));
         }
-        rec[SS.toff + (long long)m * tl + l] = r;
+        const long long at = SS.planar ? (long long)m * (tl / 2) + threadIdx.x
+                                       : (long long)m * tl + l;
+        rec[SS.toff + at] = r;
     }
 }
 
@@ -558,7 +576,50 @@ __device__ __forceinline__ void harm_radial(const SphereDev& S, double kmag, dou
             yl = yn;
         }
         const double2 b = Bs[l];
-        mac(l, b.x * jl - b.y * yl, b.x * yl + b.y * jl);
+        mac(l, fma(b.x, jl, -(b.y * yl)), fma(b.x, yl, b.y * jl));
+    }
+}
+
+// Order sums of a planar sphere (x = 0 at every pixel): the Legendre
+// values are in the records, so row m only rotates them by e^{i m phi}:
+//   ar_l + i ai_l += cos(m phi) U1_lm + sin(m phi) U2_lm,  l = m, m+2, ...
+// Rows in pairs (m even: even l; m + 1: odd l); records at
+// R[m (L/2) + k], l = 2k + (m & 1); one scalar load of two records per
+// row and block of two k, blocks below the diagonal or past Nl skipped.
+template <int L>
+__device__ __forceinline__ void harm_planar_sums(const float4* __restrict__ R, int nl1, float z1r,
+                                                 float z1i, float* ar, float* ai) {
+    constexpr int H = L / 2;
+    constexpr int KB = 2;            // k per block
+    const int ke = (nl1 + 1) >> 1;   // k past the last of row 0
+    float zr = 1.f, zi = 0.f;
+#pragma clang loop unroll(disable)
+    for (int m = 0; m < nl1; m += 2) {
+        const float4* Ra = R + m * H;
+        const float4* Rb = Ra + H;
+        const float zar = zr, zai = zi;
+        const float zbr = zr * z1r - zi * z1i, zbi = fmaf(zr, z1i, zi * z1r);
+        const int bf = (m >> 1) / KB;   // block of k = m / 2
+#pragma clang loop unroll(full)
+        for (int b = 0; b < H / KB; ++b) {
+            if (b < bf || KB * b >= ke) continue;
+            float4 ra[KB], rb[KB];
+#pragma clang loop unroll(full)
+            for (int j = 0; j < KB; ++j) {
+                ra[j] = Ra[KB * b + j];
+                rb[j] = Rb[KB * b + j];
+            }
+#pragma clang loop unroll(full)
+            for (int j = 0; j < KB; ++j) {
+                const int le = 2 * (KB * b + j);
+                ar[le] = fmaf(zai, ra[j].z, fmaf(zar, ra[j].x, ar[le]));
+                ai[le] = fmaf(zai, ra[j].w, fmaf(zar, ra[j].y, ai[le]));
+                ar[le + 1] = fmaf(zbi, rb[j].z, fmaf(zbr, rb[j].x, ar[le + 1]));
+                ai[le + 1] = fmaf(zbi, rb[j].w, fmaf(zbr, rb[j].y, ai[le + 1]));
+            }
+        }
+        zr = zbr * z1r - zbi * z1i;
+        zi = fmaf(zbr, z1i, zbi * z1r);
     }
 }
 
@@ -606,6 +667,16 @@ __global__ __launch_bounds__(kThreads, harm_waves(L)) void series_harm_kernel(
             // direction; the centre (knd below float epsilon) keeps only l = 0
             const bool centre = d <= S.a && S.n[0] * (I.kmag * d) < (double)kEpsFloat;
             r = centre ? d3{0.0, 0.0, 0.0} : norm(r);
+            if (SS.planar) {
+                // r.z = 0, so sin(theta) = 1 and e^{i phi} = r.x + i r.y
+                const RadialTrig trig = harm_trig(S, I.kmag, d);
+                float ar[L], ai[L];
+#pragma unroll
+                for (int l = 0; l < L; ++l) ar[l] = ai[l] = 0.f;
+                harm_planar_sums<L>(rec + SS.toff, nl1, (float)r.x, (float)r.y, ar, ai);
+                harm_radial<L>(S, I.kmag, d, trig, A, B, ar, ai, acc_r, acc_i);
+                continue;
+            }
             const float x = (float)r.z;
             const float st = sqrtf(fmaxf(0.f, fmaf(-x, x, 1.f)));
             float z1r = 1.f, z1i = 0.f;
@@ -905,6 +976,9 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
             const SeriesInst& I = batch.inst[S.inst];
             S.tl = bucket[S.inst];
             P->max_tl = std::max(P->max_tl, S.tl);
+            // pixels p = A + u X + v Y with X.z = Y.z = 0 and A.z = c.z give
+            // r.z = 0 exactly
+            S.planar = I.q.X.z == 0.0 && I.q.Y.z == 0.0 && I.q.A.z == S.s.p[2] ? 1 : 0;
             S.toff = nrec;
             nrec += (long long)S.tl * S.tl;
             // enough partial sums to occupy the GPU, at least 32 waves each
