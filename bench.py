@@ -95,14 +95,26 @@ def cpu_baseline(pkg, W, inst, waves, A, B, budget_s=12.0):
     }
 
 
-def harmonic_flops(Nl):
-    """FLOPs of the harmonic series kernel per (pixel, sphere), P = (Nl+1)(Nl+2)/2
-    pairs (m, l >= m): per pair the scaled Legendre step (3), the rotation by
-    e^{i m phi} (2) and the accumulation (8); per order the FP64 radial steps
-    of j_l and y_l (6), B_l h_l (6) and its product with the order sum (8);
-    per row the next diagonal value and phase (8)."""
+def harmonic_flops(Nl, planar=False):
+    """FLOPs of the harmonic series kernel per (pixel, sphere).  General form,
+    P = (Nl+1)(Nl+2)/2 pairs (m, l >= m): per pair the scaled Legendre step
+    (3), the rotation by e^{i m phi} (2) and the accumulation (8); per order
+    the FP64 radial steps of j_l and y_l (6), B_l h_l (6) and its product with
+    the order sum (8); per row the next diagonal value and phase (8).
+    Planar form (the slice contains the sphere centre and is normal to z, so
+    x = 0 and Pn_lm(0) sits in the records): only the pairs with l + m even,
+    8 each (cos and sin times the two complex records); per row the next
+    phase (6); per order as above (28)."""
     n = Nl + 1
+    if planar:
+        pairs = sum((n - m + 1) // 2 for m in range(n))
+        return 8 * pairs + 6 * n + 28 * n
     return 13 * n * (n + 1) // 2 + 28 * n
+
+
+def is_planar(inst, s):
+    """The harmonic kernel's planar case: X.z = Y.z = 0 and A.z = the centre's z."""
+    return inst["X"][2] == 0.0 and inst["Y"][2] == 0.0 and inst["A"][2] == s["p"][2]
 
 
 def direct_flops_per(Nl, nk):
@@ -261,7 +273,8 @@ def main():
     b64 = P.Buffers(inst, pkg.FP64_PARITY)
     terms = W.terms(inst)
     n2 = inst["N"] ** 2
-    series_flops = n2 * sum(harmonic_flops(s["Nl"]) for s in inst["spheres"])
+    series_flops = n2 * sum(harmonic_flops(s["Nl"], is_planar(inst, s)) for s in inst["spheres"])
+    planar = all(is_planar(inst, s) for s in inst["spheres"])
     direct_flops = n2 * sum(direct_flops_per(s["Nl"], inst["nk"]) for s in inst["spheres"])
     # the harmonic form is the choice the library makes for Monte-Carlo
     # illumination (every instance with >= 16 waves); set it explicitly
@@ -380,7 +393,9 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS if achieved else None,
             "traffic": traffic,
-            "kernel": "series_harm_kernel<32> (harmonic FP32 scattered-field series)",
+            "kernel": "series_harm_kernel<32> (harmonic FP32 scattered-field series, "
+                      + ("planar form: the slice contains the sphere centres)" if planar
+                         else "general form)"),
             "flops_per_launch": series_flops,
             "avg_launch_ms": avg_series_ms,
             "launches": ser_n,

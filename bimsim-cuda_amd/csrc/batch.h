@@ -31,9 +31,10 @@ int mie_batch_launch(bimsim_ctx* ctx, const SphereDev* d_sph, const double* d_km
                      int lmax, int quirks, void* work, double2* A, double2* B);
 
 // Pixel-invariant table of the incident-field series for the complex64
-// form: (i^l sign) * D_l for l = 0..m, then 1/l for l = 0..m (entry 0
-// unused); D_l from the same FP64 Legendre recurrences as the per-pixel
-// reference form, so the values are identical.  Appends 2 (m + 1) doubles.
+// form: for l = 0..m the four values (i^l sign) * D_l, (2l-1)/l, (l-1)/l
+// and 2l-1 (the ratios unused at l = 0); D_l from the same FP64 Legendre
+// recurrences as the per-pixel reference form, so the values are
+// identical.  Appends 4 (m + 1) doubles.
 void uf_fast_table(double ca, double cb, int m, std::vector<double>* out);
 
 // Incident fields of n instances into Uf (complex64 when f32, else

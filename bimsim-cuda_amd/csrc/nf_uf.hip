@@ -78,11 +78,12 @@ __device__ dc uf_series_value(const QuadArg& q, int uR, int vR, int pix, d3 f, d
     return cscl(cscl(cscl(sum, 2.0), (double)kPIf), amp);
 }
 
-// The series of uf_series_value for complex64 fields: (i^l sign) D_l and
-// 1/l from uf_fast_table (T), j_l and P_l by their upward recurrences
-// multiplying by 1/(kd) and 1/l with FMAs, the j_l threshold rule of
-// shift_j (spherical_bessel.h:109-122) unchanged.  Differs from the
-// reference order by rounding only.
+// The series of uf_series_value for complex64 fields, with the per-order
+// constants from uf_fast_table (T[4l ..]: (i^l sign) D_l, (2l-1)/l,
+// (l-1)/l, 2l-1): j_l and P_l by their upward recurrences multiplying by
+// 1/(kd) and by the tabled ratios with FMAs; the j_l threshold rule of
+// shift_j (spherical_bessel.h:109-122) unchanged (l > kd as an integer
+// compare).  Differs from the reference order by rounding only.
 __device__ dc uf_fast_value(const QuadArg& q, int uR, int vR, int pix, d3 f, d3 k, double kmag,
                             double amp, double ca, double cb, int m, const double* __restrict__ T) {
     int iu, iv;
@@ -94,32 +95,37 @@ __device__ dc uf_fast_value(const QuadArg& q, int uR, int vR, int pix, d3 f, d3 
     r = norm(r);
     const double kd = kmag * d;
     const double inv = 1.0 / kd;
+    // l > kd  <=>  l >= floor(kd) + 1 for kd >= 0
+    const int lk = (int)fmin(floor(kd), 1e9) + 1;
     double sn, cs;
     sincos(kd, &sn, &cs);
     double j0 = sn * inv;
     double j1 = fma(j0, inv, -(cs * inv));
     const double ct = dot(k, r);
-    const double* Ds = T;
-    const double* rl = T + (m + 1);
     double P0 = 1.0, P1 = ct;
-    double ar = j0 * Ds[0];
-    double ai = (j1 * ct) * Ds[1];
-    auto step = [&](int l) {
-        double jn = fma((double)(2 * l - 1) * j1, inv, -j0);
-        if ((double)l > kd && (jn < kJMin || jn > kJMax)) jn = 0.0;
+    double ar = j0 * T[0];
+    double ai = (j1 * ct) * T[4];
+    auto step = [&](int l, double acc) {
+        const double* t = T + 4 * l;
+        double jn = fma(t[3] * j1, inv, -j0);
+        // the threshold rule acts only where l > kd: skip its selects when
+        // no lane of the wave is there
+        const bool past = l >= lk;
+        if (__any(past))
+            if (past && (jn < kJMin || jn > kJMax)) jn = 0.0;
         j0 = j1;
         j1 = jn;
-        const double Pn = fma((double)(2 * l - 1) * ct, P1, -((double)(l - 1) * P0)) * rl[l];
+        const double Pn = fma(t[1] * ct, P1, -(t[2] * P0));
         P0 = P1;
         P1 = Pn;
-        return (jn * Pn) * Ds[l];
+        return fma(jn * Pn, t[0], acc);
     };
     int l = 2;
     for (; l + 1 <= m; l += 2) {
-        ar += step(l);
-        ai += step(l + 1);
+        ar = step(l, ar);
+        ai = step(l + 1, ai);
     }
-    if (l <= m) ar += step(l);
+    if (l <= m) ar = step(l, ar);
     return dc{((ar * 2.0) * (double)kPIf) * amp, ((ai * 2.0) * (double)kPIf) * amp};
 }
 
@@ -259,9 +265,8 @@ __global__ void add_outside_kernel(int npix, const unsigned char* __restrict__ m
 
 void uf_fast_table(double ca, double cb, int m, std::vector<double>* out) {
     const size_t base = out->size();
-    out->resize(base + 2 * (size_t)(m + 1), 0.0);
-    double* Ds = out->data() + base;
-    double* rl = Ds + (m + 1);
+    out->resize(base + 4 * (size_t)(m + 1), 0.0);
+    double* T = out->data() + base;
     // P_{n}(x) from P_{n-1}, P_{n-2} as ref::shift_legendre (legendre.h:20)
     auto leg = [](int n, double x, double& P0, double& P1) {
         const double Pn = ((double)(2 * n - 1) * x * P1 - (double)(n - 1) * P0) / (double)n;
@@ -277,8 +282,12 @@ void uf_fast_table(double ca, double cb, int m, std::vector<double>* out) {
             leg(l + 1, cb, Pb[0], Pb[1]);
         }
         const double D = ((Pa[1] - Pa[2]) - Pb[1]) + Pb[2];
-        Ds[l] = (l & 2) ? -D : D;   // i^l = 1, i, -1, -i
-        rl[l] = l > 0 ? 1.0 / (double)l : 0.0;
+        T[4 * l] = (l & 2) ? -D : D;   // i^l = 1, i, -1, -i
+        if (l > 0) {
+            T[4 * l + 1] = (2.0 * l - 1.0) / l;
+            T[4 * l + 2] = (l - 1.0) / l;
+        }
+        T[4 * l + 3] = 2.0 * l - 1.0;
     }
 }
 

@@ -74,6 +74,7 @@ struct SeriesPlan {
     long long coef_elems = 0, rhat_elems = 0;   // per chunk (maximum)
     struct Bucket {
         int L, item_begin, item_count, counter;
+        int planar;   // harmonic form: every sphere of its instances is planar
     };
     struct Chunk {
         int sph_begin, sph_count;

@@ -410,6 +410,7 @@ __global__ void harm_tsum_kernel(const SeriesInst* __restrict__ insts,
         const double* P = pl + I.pl_off + (tri_off(m, I.lh) + (l - m)) * I.nk;
         const double2* E = e + I.e_off + (long long)m * I.nk;
         const double2* F = phi + SS.woff;
+#pragma unroll 8
         for (int w = w0; w < w1; ++w) {
             const double p = P[w];
             const double2 ew = E[w];
@@ -540,7 +541,11 @@ __device__ __forceinline__ void harm_radial(const SphereDev& S, double kmag, dou
             dc jl = l == 0 ? j0 : j1;
             if (l >= 2) {
                 dc jn = csub(cmul(cscl(j1, (double)(2 * l - 1)), inv), j0);
-                if ((double)l > knd.r && (jn.r < kJMin || jn.r > kJMax)) jn = dc{0.0, 0.0};
+                // the threshold rule can only act where l > Re knd: skip its
+                // selects when no lane of the wave is there
+                const bool past = (double)l > knd.r;
+                if (__any(past))
+                    if (past && (jn.r < kJMin || jn.r > kJMax)) jn = dc{0.0, 0.0};
                 j0 = j1;
                 j1 = jn;
                 jl = jn;
@@ -562,14 +567,19 @@ __device__ __forceinline__ void harm_radial(const SphereDev& S, double kmag, dou
         double jl = l == 0 ? j0 : j1, yl = l == 0 ? y0 : y1;
         if (l >= 2) {
             double jn = fma((double)(2 * l - 1) * j1, inv, -j0);
-            if ((double)l > kd && (jn < kJMin || jn > kJMax)) jn = 0.0;
+            double yn = fma((double)(2 * l - 1) * y1, inv, -y0);
+            // the threshold rules act only where l > kd or y_l < kYMin: skip
+            // their selects when no lane of the wave is there
+            const bool past = (double)l > kd;
+            if (__any(past || yn < kYMin)) {
+                if (past && (jn < kJMin || jn > kJMax)) jn = 0.0;
+                if (yn < kYMin || (past && yn > 0.0)) {
+                    yn = 0.0;
+                    y1 = 0.0;
+                }
+            }
             j0 = j1;
             j1 = jn;
-            double yn = fma((double)(2 * l - 1) * y1, inv, -y0);
-            if (yn < kYMin || ((double)l > kd && yn > 0.0)) {
-                yn = 0.0;
-                y1 = 0.0;
-            }
             y0 = y1;
             y1 = yn;
             jl = jn;
@@ -637,7 +647,7 @@ constexpr int harm_waves(int L) { return L <= 32 ? 4 : (L <= 48 ? 3 : (L <= 80 ?
 // holds c_lm (l >= m + 2) and the diagonal factor sqrt((2m+1)/(2m)) at
 // l = m (where the recurrence does not read c); cst[2 (m kCS + l) + 1] is 1
 // at l = m, else 0, and injects the diagonal value (q is 0 below it).
-template <int L>
+template <int L, bool kPlanar>
 __global__ __launch_bounds__(kThreads, harm_waves(L)) void series_harm_kernel(
     const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph,
     const SeriesItem* __restrict__ items, int n_items, const double2* __restrict__ A,
@@ -667,7 +677,7 @@ __global__ __launch_bounds__(kThreads, harm_waves(L)) void series_harm_kernel(
             // direction; the centre (knd below float epsilon) keeps only l = 0
             const bool centre = d <= S.a && S.n[0] * (I.kmag * d) < (double)kEpsFloat;
             r = centre ? d3{0.0, 0.0, 0.0} : norm(r);
-            if (SS.planar) {
+            if (kPlanar || SS.planar) {
                 // r.z = 0, so sin(theta) = 1 and e^{i phi} = r.x + i r.y
                 const RadialTrig trig = harm_trig(S, I.kmag, d);
                 float ar[L], ai[L];
@@ -768,9 +778,16 @@ void launch_bucket(dim3 grid, hipStream_t st, const SeriesPlan& P, const SeriesP
 template <int L>
 void launch_harm(dim3 grid, hipStream_t st, const SeriesPlan& P, const SeriesPlan::Bucket& K,
                  const double2* A, const double2* B, float2* U) {
-    hipLaunchKernelGGL(series_harm_kernel<L>, grid, dim3(kThreads), 0, st, P.d_inst, P.d_sph,
-                       P.d_items + K.item_begin, K.item_count, A, B, P.d_rec, P.d_cst, U,
-                       P.d_counters + K.counter);
+    // buckets whose spheres are all planar take the kernel without the
+    // general branch (fewer registers live)
+    if (K.planar)
+        hipLaunchKernelGGL((series_harm_kernel<L, true>), grid, dim3(kThreads), 0, st, P.d_inst,
+                           P.d_sph, P.d_items + K.item_begin, K.item_count, A, B, P.d_rec,
+                           P.d_cst, U, P.d_counters + K.counter);
+    else
+        hipLaunchKernelGGL((series_harm_kernel<L, false>), grid, dim3(kThreads), 0, st, P.d_inst,
+                           P.d_sph, P.d_items + K.item_begin, K.item_count, A, B, P.d_rec,
+                           P.d_cst, U, P.d_counters + K.counter);
 }
 
 template <class T>
@@ -818,6 +835,12 @@ void series_plan_destroy(SeriesPlan* P) {
                     P->d_zpart, P->d_zfirst};
     for (void* q : harm) (void)hipFree(q);
     *P = SeriesPlan();
+}
+
+// The harmonic form's planar case: pixels p = A + u X + v Y with X.z = Y.z
+// = 0 and A.z = c.z have r.z = 0 exactly, so x = 0 at every pixel.
+static bool harm_planar(const QuadArg& q, const SphereDev& S) {
+    return q.X.z == 0.0 && q.Y.z == 0.0 && q.A.z == S.p[2];
 }
 
 int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, SeriesPlan* P,
@@ -932,9 +955,12 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
                          [&](int a, int b) { return cost[a] > cost[b]; });
         for (int L : kBuckets) {
             const int begin = (int)items.size();
+            int all_planar = 1;
             for (int i : order) {
                 if (bucket[i] != L) continue;
                 const SeriesInst& I = batch.inst[i];
+                for (int t = I.s_begin; t < I.s_begin + I.ns; ++t)
+                    if (!harm_planar(I.q, batch.sph[t].s)) all_planar = 0;
                 const int npix = I.uR * I.vR;
                 // the splits of a pixel block follow each other, so that they
                 // run side by side and share its coefficients in L2
@@ -943,7 +969,8 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
             }
             const int count = (int)items.size() - begin;
             if (count > 0)
-                ch.buckets.push_back(SeriesPlan::Bucket{L, begin, count, P->n_counters++});
+                ch.buckets.push_back(
+                    SeriesPlan::Bucket{L, begin, count, P->n_counters++, all_planar});
         }
         P->chunks.push_back(ch);
         i0 = i1;
@@ -976,9 +1003,7 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
             const SeriesInst& I = batch.inst[S.inst];
             S.tl = bucket[S.inst];
             P->max_tl = std::max(P->max_tl, S.tl);
-            // pixels p = A + u X + v Y with X.z = Y.z = 0 and A.z = c.z give
-            // r.z = 0 exactly
-            S.planar = I.q.X.z == 0.0 && I.q.Y.z == 0.0 && I.q.A.z == S.s.p[2] ? 1 : 0;
+            S.planar = harm_planar(I.q, S.s) ? 1 : 0;
             S.toff = nrec;
             nrec += (long long)S.tl * S.tl;
             // enough partial sums to occupy the GPU, at least 32 waves each
