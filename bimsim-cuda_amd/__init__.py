@@ -37,7 +37,7 @@ EXPORTS = (
     "bimsim_detector_resample", "bimsim_slice_ratio", "bimsim_batch_create",
     "bimsim_batch_launch", "bimsim_batch_destroy", "bimsim_batch_run", "bimsim_set_autotune",
     "bimsim_set_series_algorithm", "bimsim_detector_rotate_add", "bimsim_detector_ring_sums",
-    "bimsim_detector_from_line",
+    "bimsim_detector_from_line", "bimsim_field_backpropagate",
 )
 
 
@@ -158,6 +158,8 @@ def lib():
                                                c_double, c_int, c_void_p]
         L.bimsim_slice_ratio.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p,
                                          ctypes.c_longlong, c_void_p]
+        L.bimsim_field_backpropagate.argtypes = [c_void_p, c_int, c_int, c_void_p, c_int, c_int,
+                                                 c_double, c_double, c_double, c_double]
         L.bimsim_detector_rotate_add.argtypes = [c_void_p, c_int, c_double, c_int, c_int,
                                                  c_double, c_void_p, c_void_p]
         L.bimsim_detector_ring_sums.argtypes = [c_void_p, c_int, c_int, c_int, c_int, c_int,
@@ -336,6 +338,12 @@ class Context:
     def detector_resample(self, F, Du, Dv, ss, scale, accumulate, D, prec=FP64_PARITY):
         self._check(self.L.bimsim_detector_resample(self.h, prec, _ptr(F), Du, Dv, ss, scale,
                                                     1 if accumulate else 0, _ptr(D)))
+
+    def field_backpropagate(self, F, uR, vR, du, dv, lam, d, prec=FP64_PARITY, quirks=QUIRKS_REF):
+        """Angular spectrum F times -exp(i kz d) where propagating, else 0
+        (microscope.cu:423-489)."""
+        self._check(self.L.bimsim_field_backpropagate(self.h, prec, quirks, _ptr(F), uR, vR, du,
+                                                      dv, lam, d))
 
     def detector_rotate_add(self, deg, uR, vR, scale, src, D, prec=FP64_PARITY):
         """D += scale * src rotated by deg (microscope.cu:89-132)."""

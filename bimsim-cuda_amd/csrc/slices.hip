@@ -10,6 +10,7 @@
 //   absorbance etc.       microscopeStruct::getAbsorbance / getTransmittance
 //                         (microscope.cu:722-791)
 //   objective band-pass   bandpass kernel (microscope.cu:495-526)
+//   back-propagation      backpropagate kernel (microscope.cu:423-489, 592-605)
 // Every kernel is a grid-stride-free one-thread-per-element map; these are
 // HBM-bound and small next to the near-field series.  Built with
 // -ffp-contract=off so that the FP64 forms round like the reference's ops.
@@ -126,6 +127,42 @@ __global__ void bandpass_kernel(typename Cplx<R>::T* __restrict__ F, int uR, int
         z.y = 0;
         F[i] = z;
     }
+}
+
+// backpropagate (microscope.cu:423-489), on the angular spectrum: samples
+// with kx^2 + ky^2 < k^2 times -exp(i kz d), kz = sqrt(k^2 - kx^2 - ky^2),
+// the others 0.  The reference keeps Kx, kx, k and kz in float whatever
+// its precision, so they are float here too; the phase factor and the
+// product are in the field's precision.  Frequencies as in bandpass (the
+// negative ones one sample off with quirks).
+template <class R>
+__global__ void backprop_kernel(typename Cplx<R>::T* __restrict__ F, int uR, int vR, double du,
+                                double dv, double lambda, float d, int quirks) {
+    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= (size_t)uR * vR) return;
+    const int iv = (int)(i / uR), iu = (int)(i - (size_t)iv * uR);
+    const float twopi = 2 * kPIf;
+    // float Kx = 2*PI * uR * du: the last product in the field's precision
+    const float Kx = (float)((R)(twopi * (float)uR) * (R)du);
+    const float Ky = (float)((R)(twopi * (float)vR) * (R)dv);
+    const float dkx = Kx / (float)uR, dky = Ky / (float)vR;
+    const int nu = iu <= uR / 2 ? iu : -(quirks ? uR - 1 - iu : uR - iu);
+    const int nv = iv <= vR / 2 ? iv : -(quirks ? vR - 1 - iv : vR - iv);
+    const float kx = (float)((R)nu * (R)dkx), ky = (float)((R)nv * (R)dky);
+    const float k = (float)((R)twopi / (R)lambda);
+    typename Cplx<R>::T z = F[i];
+    if (kx * kx + ky * ky < k * k) {
+        const float kz = sqrtf(k * k - kx * kx - ky * ky);
+        const R ph = (R)(kz * d);
+        const R sr = -cos(ph), si = -sin(ph);
+        const R zr = z.x * sr - z.y * si, zi = z.x * si + z.y * sr;
+        z.x = zr;
+        z.y = zi;
+    } else {
+        z.x = 0;
+        z.y = 0;
+    }
+    F[i] = z;
 }
 
 bool prec_ok(int prec) { return prec >= BIMSIM_FP64_PARITY && prec <= BIMSIM_FP32_FAST; }
@@ -247,6 +284,23 @@ int bimsim_field_bandpass(bimsim_ctx* ctx, int prec, int quirks, void* F, int uR
     else
         hipLaunchKernelGGL(bandpass_kernel<double>, grid1(n), dim3(kThreads), 0, ctx->stream,
                            (double2*)F, uR, vR, du, dv, NAin / lambda, NAout / lambda, quirks);
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return BIMSIM_OK;
+}
+
+int bimsim_field_backpropagate(bimsim_ctx* ctx, int prec, int quirks, void* F, int uR, int vR,
+                               double du, double dv, double lambda, double d) {
+    if (!ctx) return BIMSIM_E_INVALID;
+    BIMSIM_CHECK_ARG(ctx, F && uR > 0 && vR > 0 && prec_ok(prec), "bad back-propagation arguments");
+    BIMSIM_CHECK_ARG(ctx, lambda > 0.0, "lambda must be positive");
+    BIMSIM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    const size_t n = (size_t)uR * vR;
+    if (prec == BIMSIM_FP32_FAST)
+        hipLaunchKernelGGL(backprop_kernel<float>, grid1(n), dim3(kThreads), 0, ctx->stream,
+                           (float2*)F, uR, vR, du, dv, lambda, (float)d, quirks);
+    else
+        hipLaunchKernelGGL(backprop_kernel<double>, grid1(n), dim3(kThreads), 0, ctx->stream,
+                           (double2*)F, uR, vR, du, dv, lambda, (float)d, quirks);
     BIMSIM_HIP_CHECK(ctx, hipGetLastError());
     return BIMSIM_OK;
 }

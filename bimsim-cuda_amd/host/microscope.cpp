@@ -86,6 +86,17 @@ void microscopeStruct::applyBandpass() {
     nf.Uf.fromAngularSpectrum();
 }
 
+// microscope.cu:592-605: U (not Uf) propagated by z_d in its angular
+// spectrum; the reference does not call it from Simulate.
+void microscopeStruct::applyBackpropagation() {
+    nf.U.toAngularSpectrum();
+    const double du = 1.0 / nf.pos.X.len(), dv = 1.0 / nf.pos.Y.len();
+    check(bimsim_field_backpropagate(context(), nf.U.prec, settings().quirks, nf.U.x_hat,
+                                     (int)nf.U.R[0], (int)nf.U.R[1], du, dv, nf.lambda, z_d),
+          "applyBackpropagation");
+    nf.U.fromAngularSpectrum();
+}
+
 void microscopeStruct::getFarField() {
     Ud = nf.U.crop(padding * (int)Ud.R[0], padding * (int)Ud.R[1], (int)Ud.R[0], (int)Ud.R[1]);
     Ufd = nf.Uf.crop(padding * (int)Ufd.R[0], padding * (int)Ufd.R[1], (int)Ufd.R[0], (int)Ufd.R[1]);

@@ -106,6 +106,7 @@ struct microscopeStruct {
     void setPos(bsPoint pMin, bsPoint pMax, bsVector normal);
 
     void applyBandpass();
+    void applyBackpropagation();
     void getFarField();
     void integrateDetector();
     void clearDetector();

@@ -242,6 +242,16 @@ int bimsim_field_fft(bimsim_ctx* ctx, int prec, void* F, int uR, int vR, int inv
 int bimsim_field_bandpass(bimsim_ctx* ctx, int prec, int quirks, void* F, int uR, int vR,
                           double du, double dv, double NAin, double NAout, double lambda);
 
+/* Propagation of an angular spectrum over a distance d (backpropagate
+ * kernel of microscopeStruct::applyBackpropagation, reference
+ * microscope.cu:423-489,592-605): samples with kx^2 + ky^2 < k^2 are
+ * multiplied by -exp(i kz d), kz = sqrt(k^2 - kx^2 - ky^2), the others
+ * set to 0; kx = 2 pi u du over the band-pass frequencies (quirks: the
+ * negative ones one sample off), k = 2 pi/lambda, in float as the
+ * reference. */
+int bimsim_field_backpropagate(bimsim_ctx* ctx, int prec, int quirks, void* F, int uR, int vR,
+                               double du, double dv, double lambda, double d);
+
 /* S[i] = op(F[i]) (fieldslice::Intensity / Mag / Real / Imag, reference
  * fieldslice.cu:6-27,110-230); op = bimsim_slice_op. */
 int bimsim_field_to_scalar(bimsim_ctx* ctx, int prec, int op, const void* F, long long n,

@@ -719,6 +719,35 @@ void interpolate_impl(const double* in, double* out, int uR, int vR, T deg, bool
         }
 }
 
+// ---------------------------------------------------------- back-propagation
+// backpropagate, microscope.cu:423-489: float Kx, dkx, kx, ky, k, kz (the
+// kernel declares them float); the phase factor and the product in T.
+template <class T>
+void backprop_impl(double* U, int uR, int vR, T du, T dv, T lambda, float d, bool quirks) {
+    const float twopi = 2 * kPIf;
+    const float Kx = (float)((T)(twopi * (float)uR) * du);
+    const float Ky = (float)((T)(twopi * (float)vR) * dv);
+    const float dkx = Kx / (float)uR, dky = Ky / (float)vR;
+    const float k = (float)((T)twopi / lambda);
+    for (int iv = 0; iv < vR; ++iv)
+        for (int iu = 0; iu < uR; ++iu) {
+            const int nu = iu <= uR / 2 ? iu : -(quirks ? uR - 1 - iu : uR - iu);
+            const int nv = iv <= vR / 2 ? iv : -(quirks ? vR - 1 - iv : vR - iv);
+            const float kx = (float)((T)nu * (T)dkx), ky = (float)((T)nv * (T)dky);
+            double* z = U + 2 * ((size_t)iv * uR + iu);
+            if (kx * kx + ky * ky < k * k) {
+                const float kz = std::sqrt(k * k - kx * kx - ky * ky);
+                const T ph = (T)(kz * d);
+                const T sr = -std::cos(ph), si = -std::sin(ph);
+                const T zr = (T)z[0], zi = (T)z[1];
+                z[0] = (double)(zr * sr - zi * si);
+                z[1] = (double)(zr * si + zi * sr);
+            } else {
+                z[0] = z[1] = 0.0;
+            }
+        }
+}
+
 // ---------------------------------------- extended-source detector images
 // Real images sampled like the reference's float textures at the same
 // coordinates: bilinear (exact weights), zero outside.
@@ -1121,6 +1150,15 @@ int orc_nf_interpolate(const double* Uin, double* Uout, int uR, int vR, double d
                        int centre_float, int fp32) {
     if (fp32) interpolate_impl<float>(Uin, Uout, uR, vR, (float)deg, centre_float != 0);
     else interpolate_impl<double>(Uin, Uout, uR, vR, deg, centre_float != 0);
+    return 0;
+}
+
+int orc_backpropagate(double* U, int uR, int vR, double du, double dv, double lambda, double d,
+                      int quirks, int fp32) {
+    if (fp32)
+        backprop_impl<float>(U, uR, vR, (float)du, (float)dv, (float)lambda, (float)d, quirks != 0);
+    else
+        backprop_impl<double>(U, uR, vR, du, dv, lambda, (float)d, quirks != 0);
     return 0;
 }
 

@@ -89,6 +89,11 @@ int orc_nf_interpolate(const double* Uin, double* Uout, int uR, int vR, double d
  * D += scale * src rotated by deg (InterpolateAndIntegrateUd, :89-132);
  * line[r] += the reference's tree sum over a ring of n_samples points
  * (simRing_Ud, :196-240); D = scale * line(|p - centre|) (:37-60). */
+/* backpropagate (microscope.cu:423-489) on an angular spectrum U (uR x vR
+ * complex, in place); Kx, kx, k and kz in float as the reference. */
+int orc_backpropagate(double* U, int uR, int vR, double du, double dv, double lambda, double d,
+                      int quirks, int fp32);
+
 int orc_det_rotate_add(const double* src, double* D, int uR, int vR, double deg, double scale,
                        int fp32);
 int orc_det_ring_sums(const double* src, double* line, int uR, int vR, int n_samples,

@@ -214,6 +214,14 @@ class Oracle:
                                   c_double(deg), c_int(centre_float), c_int(fp32))
         return out.view(np.complex128).reshape(vR, uR)
 
+    def backpropagate(self, F, du, dv, lam, d, quirks=1, fp32=0):
+        """Angular spectrum F (vR x uR complex) propagated by d (microscope.cu:423-489)."""
+        vR, uR = F.shape
+        out = _c2d(F).ravel().copy()
+        self.L.orc_backpropagate(_p(out), c_int(uR), c_int(vR), c_double(du), c_double(dv),
+                                 c_double(lam), c_double(d), c_int(quirks), c_int(fp32))
+        return out.view(np.complex128).reshape(vR, uR)
+
     # ---- extended-source detector images (real uR x vR arrays)
     def det_rotate_add(self, src, D, deg, scale=1.0, fp32=0):
         vR, uR = src.shape

@@ -96,3 +96,19 @@ def test_views_scale_resample_ratio(pkg, ctx, prec):
     assert normrel(host(T).reshape(Dn, Dn), blocks / 0.5) <= tol
     ctx.slice_ratio(pkg.RATIO_ABSORBANCE, D, Di, T, prec=prec)
     assert np.max(np.abs(host(T).reshape(Dn, Dn) + np.log10(blocks / 0.5))) <= 10 * tol
+
+
+@pytest.mark.parametrize("prec", [0, 2])
+@pytest.mark.parametrize("quirks", [0, 1])
+@pytest.mark.parametrize("N,lam,d", [(48, 2.45, 2.5), (30, 3.0, -1.25), (64, 0.8, 7.0)])
+def test_backpropagate(pkg, ctx, orc, prec, quirks, N, lam, d):
+    """microscopeStruct::applyBackpropagation's kernel (microscope.cu:423-489)
+    on a spectrum, against the oracle; the FFTs around it are the ones
+    tested above."""
+    L = 30.0
+    F0 = field(N, N + 3)
+    want = orc.backpropagate(F0, 1.0 / L, 1.0 / L, lam, d, quirks=quirks,
+                             fp32=1 if prec == 2 else 0)
+    F = torch.as_tensor(F0.ravel()).to(CT[prec]).cuda()
+    ctx.field_backpropagate(F, N, N, 1.0 / L, 1.0 / L, lam, d, prec=prec, quirks=quirks)
+    assert normrel(host(F).reshape(N, N), want) <= TOL[prec]
