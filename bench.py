@@ -17,7 +17,11 @@ bimsim-cuda_amd/workloads.py), as one graph-captured batch in FP32 against
 the plain FP64 form on two of its images.
 
 A third section ("lut_path") times the sphere-table path on one sphere of
-the 50-sphere 1024 x 1024 table workload (cfg3).
+the 50-sphere 1024 x 1024 table workload (cfg3); two more time that whole
+image through the harmonic series ("cfg3_image", with its FP64 batch pass
+and absorbance error) and the 1024-instance ragged batch ("cfg5_batch"),
+each as one graph-captured FP32 batch.  --no-spectral skips all of these
+reported-beside sections.
 
 usage: python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--no-spectral]
 Prints one JSON line.
@@ -178,6 +182,73 @@ def lut_path(pkg, P, W, ctx, torch, steps=3):
         "lookups_per_s": lookups / t_look,
         "image_ms_est_50_spheres": 50 * (t_tab + t_look) * 1e3,
     }
+
+
+def graph_batch(pkg, P, W, ctx, torch, insts, prec):
+    """A graph-captured batch of `insts` with its detector buffers."""
+    waves = [None if i.get("plane_wave") else W.mc_waves(i, pkg.mc_sample_na) for i in insts]
+    arr, spheres, Wv, n_img = P.batch_tables(insts, waves)
+    wd = torch.as_tensor(np.ascontiguousarray(Wv).ravel()).cuda()
+    rt = torch.float32 if prec == pkg.FP32_FAST else torch.float64
+    D = torch.zeros(n_img, dtype=rt, device="cuda")
+    Di = torch.zeros_like(D)
+    Ab = torch.zeros_like(D)
+    b = ctx.batch(arr, spheres, wd, D, Di, Ab, prec=prec, use_graph=prec == pkg.FP32_FAST)
+    return b, Di, Ab
+
+
+def time_batch(b, torch, steps):
+    b.launch()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.launch()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def large_workloads(pkg, P, W, ctx, torch, steps=5):
+    """The 50-sphere 1024 x 1024 image (cfg3) and the 1024-instance ragged
+    batch (cfg5), each as one graph-captured FP32 batch; cfg3 also as an FP64
+    batch pass for its absorbance error."""
+    out = {}
+    inst = W.config3()
+    b, Di, Ab = graph_batch(pkg, P, W, ctx, torch, [inst], pkg.FP32_FAST)
+    t = time_batch(b, torch, steps)
+    b6, Di6, Ab6 = graph_batch(pkg, P, W, ctx, torch, [inst], pkg.FP64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b6.launch()
+    torch.cuda.synchronize()
+    t6 = time.perf_counter() - t0
+    di = Di6.cpu().numpy()
+    bright = di > 1e-3 * di.max()
+    err = float(np.max(np.abs(Ab.cpu().numpy()[bright] - Ab6.cpu().numpy()[bright])))
+    b.close()
+    b6.close()
+    terms = W.terms(inst)
+    log(f"bench: cfg3 image {t * 1e3:.2f} ms FP32 ({terms / t:.3e} terms/s), "
+        f"FP64 batch pass {t6 * 1e3:.1f} ms, absorbance error {err:.1e}")
+    out["cfg3_image"] = {
+        "workload": "cfg3: 50 non-overlapping spheres a~U(1,10) um, lambda 4 um, 1024x1024 over "
+                    "200 um, 400 waves, harmonic series (no sphere tables), one graph-captured "
+                    "instance",
+        "ms_per_image": t * 1e3, "terms_per_s": terms / t, "images_per_s": 1.0 / t,
+        "fp64_batch_ms_per_image": t6 * 1e3, "absorbance_max_abs_err": err,
+    }
+    insts = W.config5()
+    b, Di, Ab = graph_batch(pkg, P, W, ctx, torch, insts, pkg.FP32_FAST)
+    t = time_batch(b, torch, max(1, steps // 2))
+    b.close()
+    terms = sum(W.terms(i) for i in insts)
+    log(f"bench: cfg5 batch of {len(insts)} images {t * 1e3:.1f} ms ({len(insts) / t:.0f} images/s)")
+    out["cfg5_batch"] = {
+        "workload": "cfg5: 1024 single-sphere instances, a log-U(0.5,25) um, lambda U(2.5,11) um, "
+                    "N in {256..1024} over 4a, half plane-wave, half 200-2000 Monte-Carlo waves; "
+                    "one graph-captured batch",
+        "ms_per_batch": t * 1e3, "images_per_s": len(insts) / t, "terms_per_s": terms / t,
+    }
+    return out
 
 
 def spectral(pkg, P, W, ctx, torch, stride=16, steps=3, fp64_images=2):
@@ -412,6 +483,7 @@ def main():
     if not args.no_spectral:
         out["spectral"] = spectral(pkg, P, W, ctx, torch)
         out["lut_path"] = lut_path(pkg, P, W, ctx, torch)
+        out.update(large_workloads(pkg, P, W, ctx, torch))
     if not args.no_cpu:
         log("bench: CPU baseline")
         A = b64.A.cpu().numpy()
