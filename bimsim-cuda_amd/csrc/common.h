@@ -69,6 +69,18 @@ __host__ __device__ inline dc cdiv_naive(dc a, dc b) {
 }
 __host__ __device__ inline dc cscl(dc a, double s) { return dc{a.r * s, a.i * s}; }
 __device__ inline dc cexp_i(double phi) { return dc{1.0 * cos(phi), 1.0 * sin(phi)}; }
+
+// Kahan-compensated complex float accumulation: acc += y with the running
+// compensation cmp (needs FMA contraction off, as every file here has).
+__device__ __forceinline__ void kahan(float& acc_r, float& acc_i, float& cmp_r, float& cmp_i,
+                                      float yr0, float yi0) {
+    const float yr = yr0 - cmp_r, yi = yi0 - cmp_i;
+    const float tr = acc_r + yr, ti = acc_i + yi;
+    cmp_r = (tr - acc_r) - yr;
+    cmp_i = (ti - acc_i) - yi;
+    acc_r = tr;
+    acc_i = ti;
+}
 __device__ inline dc csin_r(dc a) { return dc{sin(a.r) * cosh(a.i), cos(a.r) * sinh(a.i)}; }
 __device__ inline dc ccos_r(dc a) { return dc{cos(a.r) * cosh(a.i), -(sin(a.r) * sinh(a.i))}; }
 

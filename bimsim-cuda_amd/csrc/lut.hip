@@ -325,7 +325,8 @@ __global__ __launch_bounds__(kThreads) void uplut_f32_kernel(
     const float tscale = (float)(thetaR - 1) / kPIf;
     const float tmax = (float)(thetaR - 1);
     const d3 c = sub(ps, f);
-    float sr = 0.0f, si = 0.0f;
+    // waves summed in groups of 8 whose sums enter a Kahan accumulator
+    float sr = 0.0f, si = 0.0f, cr = 0.0f, ci = 0.0f;
     for (int w0 = 0; w0 < nk; w0 += kLutWaves) {
         const int nw = min(kLutWaves, nk - w0);
         __syncthreads();
@@ -336,24 +337,30 @@ __global__ __launch_bounds__(kThreads) void uplut_f32_kernel(
             wpi[i] = (float)ph.i;
         }
         __syncthreads();
+        for (int g = 0; g < nw; g += 8) {
+            float gr = 0.0f, gi = 0.0f;
 #pragma unroll 4
-        for (int w = 0; w < nw; ++w) {
-            const float4 k = wk[w];
-            float ct = fmaf(k.z, rz, fmaf(k.y, ry, fmaf(k.x, rx, cbias)));
-            ct = fminf(fmaxf(ct, -1.0f), 1.0f);
-            float t = acosf(ct) * tscale;
-            t = t <= tmax ? t : tmax;
-            const int ya = min((int)t, thetaR - 2);
-            const float wy = t - (float)ya;
-            const float* row = Lx + __umul24(ya, W2);
-            const f4_a8 A = *(const f4_a8*)row;
-            const f4_a8 B = *(const f4_a8*)(row + W2);
-            const float ar = fmaf(wx, A.z - A.x, A.x), aim = fmaf(wx, A.w - A.y, A.y);
-            const float br = fmaf(wx, B.z - B.x, B.x), bim = fmaf(wx, B.w - B.y, B.y);
-            const float vr = fmaf(wy, br - ar, ar), vi = fmaf(wy, bim - aim, aim);
-            const float pr = k.w, pi = wpi[w];
-            sr = fmaf(pr, vr, fmaf(-pi, vi, sr));
-            si = fmaf(pr, vi, fmaf(pi, vr, si));
+            for (int j = 0; j < 8; ++j) {
+                const int w = g + j;
+                if (w >= nw) break;
+                const float4 k = wk[w];
+                float ct = fmaf(k.z, rz, fmaf(k.y, ry, fmaf(k.x, rx, cbias)));
+                ct = fminf(fmaxf(ct, -1.0f), 1.0f);
+                float t = acosf(ct) * tscale;
+                t = t <= tmax ? t : tmax;
+                const int ya = min((int)t, thetaR - 2);
+                const float wy = t - (float)ya;
+                const float* row = Lx + __umul24(ya, W2);
+                const f4_a8 A = *(const f4_a8*)row;
+                const f4_a8 B = *(const f4_a8*)(row + W2);
+                const float ar = fmaf(wx, A.z - A.x, A.x), aim = fmaf(wx, A.w - A.y, A.y);
+                const float br = fmaf(wx, B.z - B.x, B.x), bim = fmaf(wx, B.w - B.y, B.y);
+                const float vr = fmaf(wy, br - ar, ar), vi = fmaf(wy, bim - aim, aim);
+                const float pr = k.w, pi = wpi[w];
+                gr = fmaf(pr, vr, fmaf(-pi, vi, gr));
+                gi = fmaf(pr, vi, fmaf(pi, vr, gi));
+            }
+            kahan(sr, si, cr, ci, gr, gi);
         }
     }
     if (live) {

@@ -150,16 +150,6 @@ __device__ __forceinline__ void series_blocks(const float* cr, const float* ci, 
     }
 }
 
-__device__ __forceinline__ void kahan(float& acc_r, float& acc_i, float& cmp_r, float& cmp_i,
-                                      float yr0, float yi0) {
-    const float yr = yr0 - cmp_r, yi = yi0 - cmp_i;
-    const float tr = acc_r + yr, ti = acc_i + yi;
-    cmp_r = (tr - acc_r) - yr;
-    cmp_i = (ti - acc_i) - yi;
-    acc_r = tr;
-    acc_i = ti;
-}
-
 // Phase 2: persistent work-queue kernel over the items of one bucket.
 template <int L>
 __global__ __launch_bounds__(kThreads) void series_f32_kernel(
