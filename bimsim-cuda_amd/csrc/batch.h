@@ -14,7 +14,15 @@ struct UfInst {
     int m, uR, vR, plane;
     long long off;   // element offset of the instance's field
     long long tab;   // complex64 fields: offset of its uf_fast_table, else -1
+    int flat;        // k.r = 0 at every pixel (uf_flat)
 };
+
+// The incident field's plane case: k along z and a slice normal to z
+// through the focus, so that k.(p - f) = 0 exactly at every pixel.
+inline bool uf_flat(const QuadArg& q, const d3& f, const d3& k) {
+    return k.x == 0.0 && k.y == 0.0 && (k.z == 1.0 || k.z == -1.0) && q.X.z == 0.0 &&
+           q.Y.z == 0.0 && q.A.z == f.z;
+}
 
 // Uf + Us sum of one instance, and the clearing of its detector images.
 struct SumInst {

@@ -191,7 +191,8 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
             uf_fast_table(ca, cb, I.m, &uftab);
         }
         uf[i] = UfInst{q, mk(I.focus), mk(I.kdir), b->kmag[i], I.amp, ca, cb, I.m, I.N, I.N,
-                       I.plane_wave ? 1 : 0, field, tab};
+                       I.plane_wave ? 1 : 0, field, tab,
+                       uf_flat(q, mk(I.focus), mk(I.kdir)) ? 1 : 0};
         sum[i] = SumInst{q, I.N, I.N, (int)sph_dev.size(), I.ns, field, I.image_off, I.Nd * I.Nd};
         for (int c = 0; c < 3; ++c) kdirs[3 * (size_t)i + c] = I.kdir[c];
         // spheres of this instance, coefficients packed

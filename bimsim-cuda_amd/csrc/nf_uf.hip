@@ -85,7 +85,8 @@ __device__ dc uf_series_value(const QuadArg& q, int uR, int vR, int pix, d3 f, d
 // shift_j (spherical_bessel.h:109-122) unchanged (l > kd as an integer
 // compare).  Differs from the reference order by rounding only.
 __device__ dc uf_fast_value(const QuadArg& q, int uR, int vR, int pix, d3 f, d3 k, double kmag,
-                            double amp, double ca, double cb, int m, const double* __restrict__ T) {
+                            double amp, double ca, double cb, int m, const double* __restrict__ T,
+                            bool flat) {
     int iu, iv;
     pixel_of(pix, uR, iu, iv);
     const d3 p = quad_at(q, (double)iu / (double)uR, (double)iv / (double)vR);
@@ -101,31 +102,79 @@ __device__ dc uf_fast_value(const QuadArg& q, int uR, int vR, int pix, d3 f, d3 
     sincos(kd, &sn, &cs);
     double j0 = sn * inv;
     double j1 = fma(j0, inv, -(cs * inv));
+    // one order: j_l from the recurrence (threshold rule where l > kd), then
+    // P_l and the term (t: the order's table entry)
+    // (chk: apply the rule; a block of orders skips it when no lane of the
+    // wave has l > kd there)
+    auto jstep = [&](int l, const double* t, bool chk) {
+        double jn = fma(t[3] * j1, inv, -j0);
+        if (chk && l >= lk && (jn < kJMin || jn > kJMax)) jn = 0.0;
+        j0 = j1;
+        j1 = jn;
+        return jn;
+    };
+    // the table is read four orders at a time, so that one wait on the
+    // scalar loads covers four orders
+    constexpr int kB = 4;
+    if (flat) {
+        // k.r = 0 at every pixel: P_l(0) = -((l-1)/l) P_{l-2}(0) and 0 for odd
+        // l, so the series is real; the values of the general form at ct = 0
+        double ar = j0 * T[0], P = 1.0;
+        auto fstep = [&](int l, const double* t, bool chk) {
+            const double jn = jstep(l, t, chk);
+            if (!(l & 1)) {
+                P = -(t[2] * P);
+                ar = fma(jn * P, t[0], ar);
+            }
+        };
+        int l = 2;
+        for (; l + kB - 1 <= m; l += kB) {
+            double tb[4 * kB];
+#pragma unroll
+            for (int x = 0; x < 4 * kB; ++x) tb[x] = T[4 * l + x];
+            if (__any(l + kB - 1 >= lk)) {
+#pragma unroll
+                for (int b = 0; b < kB; ++b) fstep(l + b, tb + 4 * b, true);
+            } else {
+#pragma unroll
+                for (int b = 0; b < kB; ++b) fstep(l + b, tb + 4 * b, false);
+            }
+        }
+        for (; l <= m; ++l) fstep(l, T + 4 * l, true);
+        return dc{((ar * 2.0) * (double)kPIf) * amp, 0.0};
+    }
     const double ct = dot(k, r);
     double P0 = 1.0, P1 = ct;
     double ar = j0 * T[0];
     double ai = (j1 * ct) * T[4];
-    auto step = [&](int l, double acc) {
-        const double* t = T + 4 * l;
-        double jn = fma(t[3] * j1, inv, -j0);
-        // the threshold rule acts only where l > kd: skip its selects when
-        // no lane of the wave is there
-        const bool past = l >= lk;
-        if (__any(past))
-            if (past && (jn < kJMin || jn > kJMax)) jn = 0.0;
-        j0 = j1;
-        j1 = jn;
+    auto step = [&](int l, const double* t, double acc, bool chk) {
+        const double jn = jstep(l, t, chk);
         const double Pn = fma(t[1] * ct, P1, -(t[2] * P0));
         P0 = P1;
         P1 = Pn;
         return fma(jn * Pn, t[0], acc);
     };
     int l = 2;
-    for (; l + 1 <= m; l += 2) {
-        ar = step(l, ar);
-        ai = step(l + 1, ai);
+    for (; l + kB - 1 <= m; l += kB) {   // l even: ar, ai, ar, ai
+        double tb[4 * kB];
+#pragma unroll
+        for (int x = 0; x < 4 * kB; ++x) tb[x] = T[4 * l + x];
+        if (__any(l + kB - 1 >= lk)) {
+            ar = step(l, tb, ar, true);
+            ai = step(l + 1, tb + 4, ai, true);
+            ar = step(l + 2, tb + 8, ar, true);
+            ai = step(l + 3, tb + 12, ai, true);
+        } else {
+            ar = step(l, tb, ar, false);
+            ai = step(l + 1, tb + 4, ai, false);
+            ar = step(l + 2, tb + 8, ar, false);
+            ai = step(l + 3, tb + 12, ai, false);
+        }
     }
-    if (l <= m) ar = step(l, ar);
+    for (; l <= m; ++l) {
+        if (l & 1) ai = step(l, T + 4 * l, ai, true);
+        else ar = step(l, T + 4 * l, ar, true);
+    }
     return dc{((ar * 2.0) * (double)kPIf) * amp, ((ai * 2.0) * (double)kPIf) * amp};
 }
 
@@ -153,11 +202,11 @@ __global__ __launch_bounds__(kThreads) void uf_series_kernel(QuadArg q, int uR, 
 __global__ __launch_bounds__(kThreads) void uf_fast_kernel(QuadArg q, int uR, int vR, d3 f, d3 k,
                                                            double kmag, double amp, double ca,
                                                            double cb, int m,
-                                                           const double* __restrict__ T,
+                                                           const double* __restrict__ T, int flat,
                                                            float2* __restrict__ Uf) {
     const int pix = blockIdx.x * kThreads + threadIdx.x;
     if (pix >= uR * vR) return;
-    store(Uf, pix, uf_fast_value(q, uR, vR, pix, f, k, kmag, amp, ca, cb, m, T));
+    store(Uf, pix, uf_fast_value(q, uR, vR, pix, f, k, kmag, amp, ca, cb, m, T, flat != 0));
 }
 
 template <class Out>
@@ -183,7 +232,7 @@ __global__ __launch_bounds__(kThreads) void uf_batch_kernel(const UfInst* __rest
                 v = uf_plane_value(I.q, I.uR, I.vR, pix, I.f, I.k, I.kmag, I.amp);
             else if constexpr (std::is_same<Out, float2>::value)   // tables for every instance
                 v = uf_fast_value(I.q, I.uR, I.vR, pix, I.f, I.k, I.kmag, I.amp, I.ca, I.cb, I.m,
-                                  tabs + I.tab);
+                                  tabs + I.tab, I.flat != 0);
             else
                 v = uf_series_value(I.q, I.uR, I.vR, pix, I.f, I.k, I.kmag, I.amp, I.ca, I.cb, I.m);
             store(Uf + I.off, pix, v);
@@ -372,7 +421,8 @@ extern "C" int bimsim_nf_uf(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int
             const double* T = nullptr;
             BIMSIM_TRY(dbl_table_get(ctx, tab, &T));
             hipLaunchKernelGGL(uf_fast_kernel, grid, dim3(kThreads), 0, ctx->stream, q, uR, vR, f, k,
-                               kmag, amp, cos_alpha, cos_beta, m, T, (float2*)Uf);
+                               kmag, amp, cos_alpha, cos_beta, m, T, uf_flat(q, f, k) ? 1 : 0,
+                               (float2*)Uf);
         } else
             hipLaunchKernelGGL(uf_series_kernel<double2>, grid, dim3(kThreads), 0, ctx->stream, q,
                                uR, vR, f, k, kmag, amp, cos_alpha, cos_beta, m, (double2*)Uf);

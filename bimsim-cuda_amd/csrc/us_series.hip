@@ -551,9 +551,15 @@ __device__ __forceinline__ void harm_radial(const SphereDev& S, double kmag, dou
     const double sn = T.t0, cs = T.t1;
     double j0 = sn * inv, j1 = fma(j0, inv, -(cs * inv));
     double y0 = -(cs * inv), y1 = fma(y0, inv, -(sn * inv));
+    // B_l read four orders at a time (indices clamped to Nl), so that one
+    // wait on the scalar loads covers four orders
+    double2 bq[4];
 #pragma clang loop unroll(full)
     for (int l = 0; l < L; ++l) {
         if (l >= nl1) continue;
+        if ((l & 3) == 0)
+#pragma clang loop unroll(full)
+            for (int q = 0; q < 4; ++q) bq[q] = Bs[min(l + q, nl1 - 1)];
         double jl = l == 0 ? j0 : j1, yl = l == 0 ? y0 : y1;
         if (l >= 2) {
             double jn = fma((double)(2 * l - 1) * j1, inv, -j0);
@@ -575,7 +581,7 @@ __device__ __forceinline__ void harm_radial(const SphereDev& S, double kmag, dou
             jl = jn;
             yl = yn;
         }
-        const double2 b = Bs[l];
+        const double2 b = bq[l & 3];
         mac(l, fma(b.x, jl, -(b.y * yl)), fma(b.x, yl, b.y * jl));
     }
 }
