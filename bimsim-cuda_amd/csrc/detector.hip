@@ -517,6 +517,9 @@ int detector_band(bimsim_ctx* ctx, const PostOp& bp, void* Uin, void* Ufin, int 
                                       (const C*)Ufin, vR, B, colsd, Nc, (C*)G, tw_u));
         int Bc = std::max(1, std::min(8, 2048 / vR));
         Bc = std::min(Bc, Nc);
+        // a narrow pass band leaves few columns: fewer per workgroup, so
+        // that the column pass still spreads over the chip
+        while (Bc > 1 && (Nc + Bc - 1) / Bc < 128) Bc /= 2;
         lds = sizeof(C) * 2 * (size_t)Bc * vR;
         BIMSIM_TRY(launch_cols_band<R>(ctx, vR, dim3((Nc + Bc - 1) / Bc, 2), lds, (C*)G, Nc, Bc,
                                        colsd, bp, tw_v));
