@@ -510,8 +510,15 @@ int detector_band(bimsim_ctx* ctx, const PostOp& bp, void* Uin, void* Ufin, int 
     BIMSIM_TRY(int_table_get(ctx, cols, &colsd));
     BIMSIM_TRY(scratch_get(ctx, kSlotFftA, sizeof(C) * 2 * (size_t)std::max(Nc, 1) * vR, &G));
     BIMSIM_TRY(scratch_get(ctx, kSlotFftB, sizeof(R) * 2 * (size_t)nrows * uR, &I));
+    // rows per workgroup: at most 1024 points, and at least 512 workgroups
+    // per field where the rows allow it
+    auto rows_per_group = [&](int nr) {
+        int b = std::max(1, std::min(8, 1024 / uR));
+        while (b > 1 && (nr + b - 1) / b < 512) b /= 2;
+        return b;
+    };
     if (Nc > 0) {
-        int B = std::max(1, std::min(8, 1024 / uR));
+        int B = rows_per_group(vR);
         size_t lds = sizeof(C) * 2 * (size_t)B * uR;
         BIMSIM_TRY(launch_rows_fwd<R>(ctx, uR, dim3((vR + B - 1) / B, 2), lds, (const C*)Uin,
                                       (const C*)Ufin, vR, B, colsd, Nc, (C*)G, tw_u));
@@ -524,7 +531,7 @@ int detector_band(bimsim_ctx* ctx, const PostOp& bp, void* Uin, void* Ufin, int 
         BIMSIM_TRY(launch_cols_band<R>(ctx, vR, dim3((Nc + Bc - 1) / Bc, 2), lds, (C*)G, Nc, Bc,
                                        colsd, bp, tw_v));
     }
-    int B = std::max(1, std::min(8, 1024 / uR));
+    int B = rows_per_group(nrows);
     size_t lds = sizeof(C) * 2 * (size_t)B * uR;
     if (Nc > 0) {
         BIMSIM_TRY(launch_rows_inv<R>(ctx, uR, dim3((nrows + B - 1) / B, 2), lds, (const C*)G, Nc,
