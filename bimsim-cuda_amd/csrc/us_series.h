@@ -44,7 +44,6 @@ struct SeriesSph {
     long long coff;     // offset of c'[l][pix] in the chunk's coefficient buffer
     long long roff;     // offset of r-hat[pix] in the chunk's r-hat buffer
     long long toff;     // harmonic form: offset of its harmonic records
-    int nchunk;         // harmonic form: wave chunks of its harmonic sums
     int tl;             // harmonic form: record stride (its instance's order bucket)
     int planar;         // harmonic form: the slice contains the centre and is normal
                         // to z, so every pixel has x = r-hat.z = 0
@@ -93,15 +92,11 @@ struct SeriesPlan {
     float2* d_coef = nullptr;
     float4* d_rhat = nullptr;
     // harmonic form
-    long long n_pl = 0, n_e = 0, n_rec = 0, n_tpart = 0, n_z = 0;
+    long long n_pl = 0, n_e = 0, n_rec = 0;
     int max_lh = 0, max_tl = 0;
-    int2* d_zmap = nullptr;       // (sphere, chunk) of every partial harmonic sum
-    long long* d_zpart = nullptr; // offset of each partial sum's (m, l) block
-    long long* d_zfirst = nullptr;// first partial sum of each sphere
     double* d_pl = nullptr;       // normalised Legendre values per (m, wave, l)
     double2* d_e = nullptr;       // exp(i m phi_k) per (m, wave)
     double2* d_phi = nullptr;     // wave weights (1/nk) subA exp(i k k.c) per sphere
-    double2* d_tpart = nullptr;   // partial harmonic sums per (sphere, chunk, m, l)
     float4* d_rec = nullptr;      // harmonic records per (sphere, m, l)
     float* d_cst = nullptr;       // scaled Legendre recurrence constants per (m, l)
     double* d_h64 = nullptr;      // FP64 Legendre constants and scales of the wave sums

@@ -377,97 +377,111 @@ __global__ void harm_legendre_kernel(const SeriesInst* __restrict__ insts,
     }
 }
 
-// Partial sums over one chunk of waves: thread l of block (m, z), z an entry
-// of zmap = (sphere, chunk), writes (T+, T-) before the 4 pi/(2l+1) factor
-// at part[(z * tl + m) * tl + l] (tl = the sphere's record stride).
-__global__ void harm_tsum_kernel(const SeriesInst* __restrict__ insts,
-                                 const SeriesSph* __restrict__ sph, const int2* __restrict__ zmap,
-                                 long long n_z, const long long* __restrict__ zpart,
-                                 const double* __restrict__ pl, const double2* __restrict__ e,
-                                 const double2* __restrict__ phi, double2* __restrict__ part) {
-    const int l = threadIdx.x;
-    const int m = blockIdx.x;
-    for (long long z = blockIdx.y; z < n_z; z += gridDim.y) {
-        const int2 sc = zmap[z];
-        const SeriesSph& SS = sph[sc.x];
-        const SeriesInst& I = insts[SS.inst];
-        const int nl1 = SS.s.Nl + 1;
-        if (m >= nl1 || l < m || l >= nl1) continue;
-        if (SS.planar && ((l + m) & 1)) continue;   // Pn_lm(0) = 0: not read
-        const int chunk_waves = (I.nk + SS.nchunk - 1) / SS.nchunk;
-        const int w0 = sc.y * chunk_waves, w1 = min(I.nk, w0 + chunk_waves);
-        dc tp = {0.0, 0.0}, tm = {0.0, 0.0};
-        const double* P = pl + I.pl_off + (tri_off(m, I.lh) + (l - m)) * I.nk;
-        const double2* E = e + I.e_off + (long long)m * I.nk;
-        const double2* F = phi + SS.woff;
-#pragma unroll 8
-        for (int w = w0; w < w1; ++w) {
-            const double p = P[w];
-            const double2 ew = E[w];
-            const double2 fw = F[w];
-            const dc a = cscl(dc{fw.x, fw.y}, p);
-            tp = cadd(tp, cmul(a, dc{ew.x, ew.y}));
-            tm = cadd(tm, cmul(a, dc{ew.x, -ew.y}));
-        }
-        const int tl = SS.tl;
-        double2* o = part + (zpart[z] + (long long)m * tl + l) * 2;
-        o[0] = make_double2(tp.r, tp.i);
-        o[1] = make_double2(tm.r, tm.i);
-    }
-}
+// Records per (sphere, m, l) from the wave tables.  Block (m, tile, s) forms
+// four records of row m of sphere s: orders l = 4 tile + j, or for a planar
+// sphere k = 4 tile + j with l = 2k + (m & 1).  Its threads run over the
+// instance's waves (coalesced reads of the Legendre, azimuth-phase and
+// wave-weight tables), summing T+- = sum_w phi_w Pn_lm e^{+-i m phi_w}; a
+// block reduction forms the records: 4 pi/(2l+1) K_lm (or Pn_lm(0) for
+// planar spheres) times U1 = T+ + T-, U2 = (Im T+ - Im T-, Re T- - Re T+)
+// (U1 = T+, U2 = 0 at m = 0), zero outside m <= l <= Nl.
+// Grid: (m < max bucket, tiles, spheres).
+constexpr int kRecThreads = 256;
+constexpr int kRecTile = 4;
 
-// Records per (sphere, m, l): the chunk sums times 4 pi/(2l+1) K_lm,
-// K_lm = prod_{j=m+1..l} a_jm / 2, as U1 and U2 at rec[toff + m * tl + l]
-// (zero outside m <= l <= Nl).  Planar spheres (x = 0 at every pixel) fold
-// Pn_lm(0) in instead of K_lm and keep only l + m even, row m holding
-// l = 2k + (m & 1) at rec[toff + m * (tl/2) + k].  Grid: (m, spheres),
-// threads l (or k).
-__global__ void harm_records_kernel(const SeriesSph* __restrict__ sph, int n_sph,
-                                    const long long* __restrict__ zfirst,
-                                    const long long* __restrict__ zpart,
-                                    const double2* __restrict__ part,
-                                    const double* __restrict__ h64, float4* __restrict__ rec) {
-    const int m = blockIdx.x;
-    for (int s = blockIdx.y; s < n_sph; s += gridDim.y) {
+__global__ __launch_bounds__(kRecThreads) void harm_records_kernel(
+    const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph, int n_sph,
+    const double* __restrict__ pl, const double2* __restrict__ e, const double2* __restrict__ phi,
+    const double* __restrict__ h64, float4* __restrict__ rec) {
+    __shared__ double red[kRecThreads / 64][4 * kRecTile];
+    const int m = blockIdx.x, tile = blockIdx.y;
+    for (int s = blockIdx.z; s < n_sph; s += gridDim.z) {
         const SeriesSph& SS = sph[s];
         const int tl = SS.tl;
-        // planar spheres: row m holds l = 2k + (m & 1), k < tl / 2
-        const int l = SS.planar ? 2 * (int)threadIdx.x + (m & 1) : (int)threadIdx.x;
-        if ((SS.planar ? (int)threadIdx.x >= tl / 2 : l >= tl) || m >= tl) continue;
-        float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+        const bool planar = SS.planar != 0;
+        const int width = planar ? tl / 2 : tl;   // records per row
+        if (m >= tl || kRecTile * tile >= width) continue;
+        const SeriesInst& I = insts[SS.inst];
         const int nl1 = SS.s.Nl + 1;
-        if (m < nl1 && l >= m && l < nl1) {
-            // a sphere's chunks are consecutive blocks of tl^2 (T+, T-) pairs
-            dc tp = {0.0, 0.0}, tm = {0.0, 0.0};
-            const double2* o = part + (zpart[zfirst[s]] + (long long)m * tl + l) * 2;
-            const long long cs = 2LL * tl * tl;
-#pragma unroll 8
-            for (int c = 0; c < SS.nchunk; ++c) {
-                const double2 a = o[c * cs], b = o[c * cs + 1];
-                tp = cadd(tp, dc{a.x, a.y});
-                tm = cadd(tm, dc{b.x, b.y});
-            }
-            double f;
-            if (SS.planar) {
-                // Pn_lm(0): Pn_mm(0) = Pn_00 prod_k sqrt((2k+1)/(2k)),
-                // Pn_lm(0) = -a_lm b_lm Pn_{l-2,m}(0), folded into the record
-                double pn = 0.28209479177387814;
-                for (int k = 1; k <= m; ++k) pn *= h64[kHDiag + k];
-                const double* ab = h64 + 2 * (m * kH);
-                for (int j = m + 2; j <= l; j += 2) pn = -(ab[2 * j] * ab[2 * j + 1]) * pn;
-                f = 4.0 * M_PI / (2.0 * l + 1.0) * pn;
-            } else {
-                f = 4.0 * M_PI / (2.0 * l + 1.0) * h64[kHK + m * kH + l];
-            }
-            if (m == 0)
-                r = make_float4((float)(f * tp.r), (float)(f * tp.i), 0.f, 0.f);
-            else
-                r = make_float4((float)(f * (tp.r + tm.r)), (float)(f * (tp.i + tm.i)),
-                                (float)(f * (tp.i - tm.i)), (float)(f * (tm.r - tp.r)));
+        int lj[kRecTile];
+        bool need[kRecTile];
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < kRecTile; ++j) {
+            const int k = kRecTile * tile + j;
+            lj[j] = planar ? 2 * k + (m & 1) : k;
+            need[j] = k < width && m < nl1 && lj[j] >= m && lj[j] < nl1;
+            any = any || need[j];
         }
-        const long long at = SS.planar ? (long long)m * (tl / 2) + threadIdx.x
-                                       : (long long)m * tl + l;
-        rec[SS.toff + at] = r;
+        // (T+ re, T+ im, T- re, T- im) of each record
+        double acc[4 * kRecTile];
+#pragma unroll
+        for (int q = 0; q < 4 * kRecTile; ++q) acc[q] = 0.0;
+        if (any) {
+            const double* P[kRecTile];
+#pragma unroll
+            for (int j = 0; j < kRecTile; ++j)
+                P[j] = pl + I.pl_off + (tri_off(m, I.lh) + (need[j] ? lj[j] - m : 0)) * I.nk;
+            const double2* E = e + I.e_off + (long long)m * I.nk;
+            const double2* F = phi + SS.woff;
+            for (int w = threadIdx.x; w < I.nk; w += kRecThreads) {
+                const double2 ew = E[w], fw = F[w];
+#pragma unroll
+                for (int j = 0; j < kRecTile; ++j) {
+                    if (!need[j]) continue;
+                    const double p = P[j][w];
+                    const double ar = fw.x * p, ai = fw.y * p;
+                    acc[4 * j] += ar * ew.x - ai * ew.y;
+                    acc[4 * j + 1] += ar * ew.y + ai * ew.x;
+                    acc[4 * j + 2] += ar * ew.x + ai * ew.y;
+                    acc[4 * j + 3] += ai * ew.x - ar * ew.y;
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4 * kRecTile; ++q) {
+            double v = acc[q];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+            acc[q] = v;
+        }
+        __syncthreads();   // red is reused for the next sphere
+        if ((threadIdx.x & 63) == 0)
+#pragma unroll
+            for (int q = 0; q < 4 * kRecTile; ++q) red[threadIdx.x >> 6][q] = acc[q];
+        __syncthreads();
+        if (threadIdx.x < kRecTile) {
+            const int j = threadIdx.x;
+            const int k = kRecTile * tile + j;
+            if (k < width) {
+                float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (need[j]) {
+                    dc tp = {0.0, 0.0}, tm = {0.0, 0.0};
+                    for (int q = 0; q < kRecThreads / 64; ++q) {
+                        tp = cadd(tp, dc{red[q][4 * j], red[q][4 * j + 1]});
+                        tm = cadd(tm, dc{red[q][4 * j + 2], red[q][4 * j + 3]});
+                    }
+                    const int l = lj[j];
+                    double f;
+                    if (planar) {
+                        // Pn_lm(0): Pn_mm(0) = Pn_00 prod_q sqrt((2q+1)/(2q)),
+                        // Pn_lm(0) = -a_lm b_lm Pn_{l-2,m}(0)
+                        double pn = 0.28209479177387814;
+                        for (int q = 1; q <= m; ++q) pn *= h64[kHDiag + q];
+                        const double* ab = h64 + 2 * (m * kH);
+                        for (int q = m + 2; q <= l; q += 2) pn = -(ab[2 * q] * ab[2 * q + 1]) * pn;
+                        f = 4.0 * M_PI / (2.0 * l + 1.0) * pn;
+                    } else {
+                        f = 4.0 * M_PI / (2.0 * l + 1.0) * h64[kHK + m * kH + l];
+                    }
+                    if (m == 0)
+                        r = make_float4((float)(f * tp.r), (float)(f * tp.i), 0.f, 0.f);
+                    else
+                        r = make_float4((float)(f * (tp.r + tm.r)), (float)(f * (tp.i + tm.i)),
+                                        (float)(f * (tp.i - tm.i)), (float)(f * (tm.r - tp.r)));
+                }
+                rec[SS.toff + (long long)m * width + k] = r;
+            }
+        }
     }
 }
 
@@ -826,9 +840,7 @@ void series_plan_destroy(SeriesPlan* P) {
     (void)hipFree(P->d_part);
     (void)hipFree(P->d_coef);
     (void)hipFree(P->d_rhat);
-    void* harm[] = {P->d_pl,   P->d_e,    P->d_phi,   P->d_tpart, P->d_rec, P->d_cst,
-                    P->d_h64, P->d_zmap,
-                    P->d_zpart, P->d_zfirst};
+    void* harm[] = {P->d_pl, P->d_e, P->d_phi, P->d_rec, P->d_cst, P->d_h64};
     for (void* q : harm) (void)hipFree(q);
     *P = SeriesPlan();
 }
@@ -971,12 +983,10 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
         P->chunks.push_back(ch);
         i0 = i1;
     }
-    std::vector<int2> zmap;
-    std::vector<long long> zpart, zfirst;
     std::vector<float> cst;
     std::vector<double> h64;
     if (P->harmonic) {
-        long long npl = 0, ne = 0, nrec = 0, npairs = 0;
+        long long npl = 0, ne = 0, nrec = 0;
         for (int i = 0; i < ni; ++i) {
             SeriesInst& I = batch.inst[i];
             int lh = 1;
@@ -989,11 +999,6 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
             ne += (long long)lh * I.nk;
             P->max_lh = std::max(P->max_lh, lh);
         }
-        long long sum_tri = 0;
-        for (int t = 0; t < P->n_sph; ++t) {
-            const long long b = bucket[batch.sph[t].inst];
-            sum_tri += b * b / 2;
-        }
         for (int t = 0; t < P->n_sph; ++t) {
             SeriesSph& S = batch.sph[t];
             const SeriesInst& I = batch.inst[S.inst];
@@ -1002,21 +1007,10 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
             S.planar = harm_planar(I.q, S.s) ? 1 : 0;
             S.toff = nrec;
             nrec += (long long)S.tl * S.tl;
-            // enough partial sums to occupy the GPU, at least 32 waves each
-            const long long want = ((1LL << 20) + sum_tri - 1) / std::max<long long>(1, sum_tri);
-            S.nchunk = (int)std::max<long long>(1, std::min<long long>(want, (I.nk + 31) / 32));
-            zfirst.push_back((long long)zmap.size());
-            for (int c = 0; c < S.nchunk; ++c) {
-                zmap.push_back(make_int2(t, c));
-                zpart.push_back(npairs);
-                npairs += (long long)S.tl * S.tl;
-            }
         }
         P->n_pl = npl;
         P->n_e = ne;
         P->n_rec = nrec;
-        P->n_tpart = 2 * npairs;
-        P->n_z = (long long)zmap.size();
         // c_lm = 4 b_lm / a_{l-1,m} (l >= m + 2) and the diagonal factors,
         // a_lm = sqrt((4l^2-1)/(l^2-m^2)), b_lm = sqrt(((l-1)^2-m^2)/(4(l-1)^2-1))
         auto a_ = [](int l, int m) {
@@ -1055,15 +1049,11 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
     BIMSIM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     int rc;
     if (P->harmonic &&
-        ((rc = upload(ctx, &P->d_zmap, zmap)) != BIMSIM_OK ||
-         (rc = upload(ctx, &P->d_zpart, zpart)) != BIMSIM_OK ||
-         (rc = upload(ctx, &P->d_zfirst, zfirst)) != BIMSIM_OK ||
-         (rc = upload(ctx, &P->d_cst, cst)) != BIMSIM_OK ||
+        ((rc = upload(ctx, &P->d_cst, cst)) != BIMSIM_OK ||
          (rc = upload(ctx, &P->d_h64, h64)) != BIMSIM_OK ||
          (rc = alloc(ctx, &P->d_pl, P->n_pl)) != BIMSIM_OK ||
          (rc = alloc(ctx, &P->d_e, P->n_e)) != BIMSIM_OK ||
          (rc = alloc(ctx, &P->d_phi, std::max<long long>(1, P->n_wave_rec))) != BIMSIM_OK ||
-         (rc = alloc(ctx, &P->d_tpart, P->n_tpart)) != BIMSIM_OK ||
          (rc = alloc(ctx, &P->d_rec, P->n_rec)) != BIMSIM_OK)) {
         series_plan_destroy(P);
         return rc;
@@ -1094,18 +1084,16 @@ int series_plan_launch_waves(bimsim_ctx* ctx, const SeriesPlan& P, const double*
     const hipStream_t st = ctx->stream;
     const dim3 wgrid((P.nk_max + 255) / 256, std::min(P.n_sph, 65535));
     if (P.harmonic) {
-        // per-wave Legendre values, azimuth phases and wave weights, then the
-        // per-sphere sums T
+        // per-wave Legendre values, azimuth phases and wave weights (and the
+        // counters cleared), then the records from them
         const dim3 lgrid((P.nk_max + 63) / 64, std::min(P.n_inst, 65535), P.max_lh);
         hipLaunchKernelGGL(harm_legendre_kernel, lgrid, dim3(64), 0, st, P.d_inst, P.d_sph,
                            P.n_inst, waves, P.d_h64, P.d_pl, P.d_e, P.d_phi, P.d_counters,
                            P.n_counters);
-        const dim3 tgrid(P.max_lh, (unsigned)std::min<long long>(P.n_z, 65535));
-        hipLaunchKernelGGL(harm_tsum_kernel, tgrid, dim3(128), 0, st, P.d_inst, P.d_sph, P.d_zmap,
-                           P.n_z, P.d_zpart, P.d_pl, P.d_e, P.d_phi, P.d_tpart);
-        const dim3 rgrid(P.max_tl, std::min(P.n_sph, 65535));
-        hipLaunchKernelGGL(harm_records_kernel, rgrid, dim3(128), 0, st, P.d_sph, P.n_sph,
-                           P.d_zfirst, P.d_zpart, P.d_tpart, P.d_h64, P.d_rec);
+        const dim3 rgrid(P.max_tl, (P.max_tl + kRecTile - 1) / kRecTile,
+                         std::min(P.n_sph, 65535));
+        hipLaunchKernelGGL(harm_records_kernel, rgrid, dim3(kRecThreads), 0, st, P.d_inst,
+                           P.d_sph, P.n_sph, P.d_pl, P.d_e, P.d_phi, P.d_h64, P.d_rec);
     } else {
         hipLaunchKernelGGL(series_waves_kernel, wgrid, dim3(256), 0, st, P.d_inst, P.d_sph, P.n_sph,
                            waves, P.d_wt, P.d_counters, P.n_counters);
