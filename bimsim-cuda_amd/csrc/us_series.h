@@ -1,5 +1,6 @@
 // Scattered-field series in the FP32 form for one or many instances
-// (slice, spheres, waves).  Two phases per chunk of instances:
+// (slice, spheres, waves), by one of two algorithms.
+// Direct form, two phases per chunk of instances:
 //   1. a coefficient kernel writes, per (sphere, pixel), the wave-independent
 //      products c'_l = s_l B_l h_l(kd) (or s_l A_l j_l(knd) inside) computed
 //      in FP64 with the reference recurrences, and r-hat;
@@ -9,6 +10,11 @@
 //      queue is empty.
 // Instances are grouped into chunks whose coefficient storage fits a memory
 // budget; the same buffers serve every chunk.
+// Harmonic form (addition theorem): per-wave Legendre values, azimuth
+// phases and wave weights, then per (sphere, m, l) records of the wave sums
+// (the wave tables, from the waves only), then one persistent launch per
+// order bucket evaluates per pixel the order sums of the records and the
+// FP64 radial factors.
 #pragma once
 
 #include <vector>
