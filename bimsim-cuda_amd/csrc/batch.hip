@@ -2,9 +2,12 @@
 // sequence (reference microscope.cu:849-857, nearfield.cpp:194-218) for many
 // instances at once, optionally captured into one HIP graph.
 //
-//   Mie coefficients   one launch, two lanes per (instance, sphere); for
+//   Mie coefficients   one launch, three waves per block of spheres; for
 //                      FP32 on a side stream beside the next two rows
 //   incident fields    one launch over (pixel blocks, instances)
+//   FP32 pre-pass      with one harmonic plan and few spheres instead: the
+//                      Legendre tables, then coefficients, records and
+//                      incident fields in one launch (series_plan_launch_prepass)
 //   scattered fields   FP32: coefficient kernel + one persistent work-queue
 //                      launch per order bucket (us_series.hip); FP64 forms:
 //                      the per-instance entry point
@@ -14,6 +17,7 @@
 
 #include "batch.h"
 #include "internal.h"
+#include "mie_device.h"
 #include "us_series.h"
 
 using namespace bimsim;
@@ -49,6 +53,11 @@ struct bimsim_batch {
     // beside the incident fields and wave tables (joined before the sums)
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // FP32 passes of one harmonic plan whose coefficient blocks fit
+    // kPrepassLds: coefficients, records and incident fields in one launch
+    // on the one stream instead
+    bool prepass = false;
+    PrepassArgs pre{};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -92,7 +101,13 @@ int enqueue(bimsim_batch* b) {
     bimsim_ctx* ctx = b->ctx;
     const size_t fb = b->f32 ? sizeof(float2) : sizeof(double2);
     const size_t rb = b->f32 ? sizeof(float) : sizeof(double);
-    if (b->f32) {
+    if (b->f32 && b->prepass) {
+        const bool mc = b->have_mc;
+        BIMSIM_TRY(series_plan_launch_prepass(ctx, mc ? b->plan_mc : b->plan_pw,
+                                              mc ? b->waves : b->d_kdirs, b->pre));
+        BIMSIM_TRY(series_plan_launch_sums(ctx, mc ? b->plan_mc : b->plan_pw, b->d_A, b->d_B,
+                                           (float2*)b->d_U));
+    } else if (b->f32) {
         // fork: coefficients on the side stream, fields and wave tables here
         BIMSIM_HIP_CHECK(ctx, hipEventRecord(b->ev_fork, ctx->stream));
         BIMSIM_HIP_CHECK(ctx, hipStreamWaitEvent(b->side, b->ev_fork, 0));
@@ -273,6 +288,13 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
             BIMSIM_TRY(series_plan_create_tuned(ctx, pw, budget, b->d_kdirs, b->d_A, b->d_B, &b->plan_pw));
             b->have_pw = true;
         }
+        const int wstride = b->lmax + 2;
+        b->pre = PrepassArgs{b->d_sph, b->d_kmag, b->n_sph, mie::mie_spheres_per_block(wstride),
+                             wstride, b->quirks, b->d_A, b->d_B, b->d_uf, b->n,
+                             (b->max_npix + 255) / 256, b->d_uftab, (float2*)b->d_Uf};
+        const SeriesPlan& P = b->have_mc ? b->plan_mc : b->plan_pw;
+        b->prepass = b->have_mc != b->have_pw && P.harmonic && b->n_sph > 0 &&
+                     prepass_lds(b->pre) <= kPrepassLds;
     }
     return BIMSIM_OK;
 }

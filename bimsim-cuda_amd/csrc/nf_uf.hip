@@ -10,184 +10,14 @@
 #include "internal.h"
 #include "batch.h"
 #include "refmath.h"
+#include "uf_device.h"
 
 namespace bimsim {
 namespace {
 
 constexpr int kThreads = 256;
 
-__device__ inline void pixel_of(int pix, int uR, int& iu, int& iv) {
-    iv = pix / uR;
-    iu = pix - iv * uR;
-}
-
-template <class Out>
-__device__ inline void store(Out* U, int i, dc v);
-template <>
-__device__ inline void store<double2>(double2* U, int i, dc v) { U[i] = make_double2(v.r, v.i); }
-template <>
-__device__ inline void store<float2>(float2* U, int i, dc v) {
-    U[i] = make_float2((float)v.r, (float)v.i);
-}
-
-// gpuScalarUf, nfScalarUf.cu:46-175: the field at pixel pix
-__device__ dc uf_series_value(const QuadArg& q, int uR, int vR, int pix, d3 f, d3 k, double kmag,
-                              double amp, double ca, double cb, int m) {
-    int iu, iv;
-    pixel_of(pix, uR, iu, iv);
-    const d3 p = quad_at(q, (double)iu / (double)uR, (double)iv / (double)vR);
-    d3 r = sub(p, f);
-    const double d = len(r);
-    if (d < (double)kEpsFloat) return dc{amp * 2.0 * (double)kPIf * (ca - cb), 0.0};
-    r = norm(r);
-    const dc im = {0.0, 1.0};
-    dc il = {1.0, 0.0};
-    double j[2];
-    const double kd = kmag * d;
-    ref::init_j(kd, j);
-    const double ct = dot(k, r);
-    double P0 = 1.0, P1 = ct;
-    double Pa[3] = {1.0, ca, 1.0}, Pb[3] = {1.0, cb, 1.0};
-    dc sum = {0.0, 0.0};
-    for (int l = 0; l <= m; ++l) {
-        double jl, Pl;
-        if (l == 0) {
-            jl = j[0];
-            Pl = P0;
-        } else if (l == 1) {
-            jl = j[1];
-            Pl = P1;
-            Pa[2] = Pa[0];
-            ref::shift_legendre(l + 1, ca, Pa[0], Pa[1]);
-            Pb[2] = Pb[0];
-            ref::shift_legendre(l + 1, cb, Pb[0], Pb[1]);
-        } else {
-            ref::shift_j(l, kd, j);
-            ref::shift_legendre(l, ct, P0, P1);
-            jl = j[1];
-            Pl = P1;
-            Pa[2] = Pa[0];
-            ref::shift_legendre(l + 1, ca, Pa[0], Pa[1]);
-            Pb[2] = Pb[0];
-            ref::shift_legendre(l + 1, cb, Pb[0], Pb[1]);
-        }
-        const double D = ((Pa[1] - Pa[2]) - Pb[1]) + Pb[2];
-        sum = cadd(sum, cscl(cscl(cscl(il, jl), Pl), D));
-        il = cmul(il, im);
-    }
-    return cscl(cscl(cscl(sum, 2.0), (double)kPIf), amp);
-}
-
-// The series of uf_series_value for complex64 fields, with the per-order
-// constants from uf_fast_table (T[4l ..]: (i^l sign) D_l, (2l-1)/l,
-// (l-1)/l, 2l-1): j_l and P_l by their upward recurrences multiplying by
-// 1/(kd) and by the tabled ratios with FMAs; the j_l threshold rule of
-// shift_j (spherical_bessel.h:109-122) unchanged (l > kd as an integer
-// compare).  Differs from the reference order by rounding only.
-__device__ dc uf_fast_value(const QuadArg& q, int uR, int vR, int pix, d3 f, d3 k, double kmag,
-                            double amp, double ca, double cb, int m, const double* __restrict__ T,
-                            bool flat) {
-    int iu, iv;
-    pixel_of(pix, uR, iu, iv);
-    const d3 p = quad_at(q, (double)iu / (double)uR, (double)iv / (double)vR);
-    d3 r = sub(p, f);
-    const double d = len(r);
-    if (d < (double)kEpsFloat) return dc{amp * 2.0 * (double)kPIf * (ca - cb), 0.0};
-    r = norm(r);
-    const double kd = kmag * d;
-    const double inv = 1.0 / kd;
-    // l > kd  <=>  l >= floor(kd) + 1 for kd >= 0
-    const int lk = (int)fmin(floor(kd), 1e9) + 1;
-    double sn, cs;
-    sincos(kd, &sn, &cs);
-    double j0 = sn * inv;
-    double j1 = fma(j0, inv, -(cs * inv));
-    // one order: j_l from the recurrence (threshold rule where l > kd), then
-    // P_l and the term (t: the order's table entry)
-    // (chk: apply the rule; a block of orders skips it when no lane of the
-    // wave has l > kd there)
-    auto jstep = [&](int l, const double* t, bool chk) {
-        double jn = fma(t[3] * j1, inv, -j0);
-        if (chk && l >= lk && (jn < kJMin || jn > kJMax)) jn = 0.0;
-        j0 = j1;
-        j1 = jn;
-        return jn;
-    };
-    // the table is read four orders at a time, so that one wait on the
-    // scalar loads covers four orders
-    constexpr int kB = 4;
-    if (flat) {
-        // k.r = 0 at every pixel: P_l(0) = -((l-1)/l) P_{l-2}(0) and 0 for odd
-        // l, so the series is real; the values of the general form at ct = 0
-        double ar = j0 * T[0], P = 1.0;
-        auto fstep = [&](int l, const double* t, bool chk) {
-            const double jn = jstep(l, t, chk);
-            if (!(l & 1)) {
-                P = -(t[2] * P);
-                ar = fma(jn * P, t[0], ar);
-            }
-        };
-        int l = 2;
-        for (; l + kB - 1 <= m; l += kB) {
-            double tb[4 * kB];
-#pragma unroll
-            for (int x = 0; x < 4 * kB; ++x) tb[x] = T[4 * l + x];
-            if (__any(l + kB - 1 >= lk)) {
-#pragma unroll
-                for (int b = 0; b < kB; ++b) fstep(l + b, tb + 4 * b, true);
-            } else {
-#pragma unroll
-                for (int b = 0; b < kB; ++b) fstep(l + b, tb + 4 * b, false);
-            }
-        }
-        for (; l <= m; ++l) fstep(l, T + 4 * l, true);
-        return dc{((ar * 2.0) * (double)kPIf) * amp, 0.0};
-    }
-    const double ct = dot(k, r);
-    double P0 = 1.0, P1 = ct;
-    double ar = j0 * T[0];
-    double ai = (j1 * ct) * T[4];
-    auto step = [&](int l, const double* t, double acc, bool chk) {
-        const double jn = jstep(l, t, chk);
-        const double Pn = fma(t[1] * ct, P1, -(t[2] * P0));
-        P0 = P1;
-        P1 = Pn;
-        return fma(jn * Pn, t[0], acc);
-    };
-    int l = 2;
-    for (; l + kB - 1 <= m; l += kB) {   // l even: ar, ai, ar, ai
-        double tb[4 * kB];
-#pragma unroll
-        for (int x = 0; x < 4 * kB; ++x) tb[x] = T[4 * l + x];
-        if (__any(l + kB - 1 >= lk)) {
-            ar = step(l, tb, ar, true);
-            ai = step(l + 1, tb + 4, ai, true);
-            ar = step(l + 2, tb + 8, ar, true);
-            ai = step(l + 3, tb + 12, ai, true);
-        } else {
-            ar = step(l, tb, ar, false);
-            ai = step(l + 1, tb + 4, ai, false);
-            ar = step(l + 2, tb + 8, ar, false);
-            ai = step(l + 3, tb + 12, ai, false);
-        }
-    }
-    for (; l <= m; ++l) {
-        if (l & 1) ai = step(l, T + 4 * l, ai, true);
-        else ar = step(l, T + 4 * l, ar, true);
-    }
-    return dc{((ar * 2.0) * (double)kPIf) * amp, ((ai * 2.0) * (double)kPIf) * amp};
-}
-
-// gpuScalarUfp, nfScalarUf.cu:9-43: the plane wave at pixel pix
-__device__ inline dc uf_plane_value(const QuadArg& q, int uR, int vR, int pix, d3 f, d3 k,
-                                    double kmag, double amp) {
-    int iu, iv;
-    pixel_of(pix, uR, iu, iv);
-    const d3 p = quad_at(q, (double)iu / (double)uR, (double)iv / (double)vR);
-    const d3 r = sub(p, f);
-    const double kr = kmag * dot(k, r);
-    return cscl(cexp_i(kr), amp);
-}
+using namespace ufd;
 
 template <class Out>
 __global__ __launch_bounds__(kThreads) void uf_series_kernel(QuadArg q, int uR, int vR, d3 f,
@@ -226,17 +56,8 @@ __global__ __launch_bounds__(kThreads) void uf_batch_kernel(const UfInst* __rest
     for (int i = blockIdx.y; i < n; i += gridDim.y) {
         const UfInst& I = insts[i];
         const int npix = I.uR * I.vR;
-        for (int pix = blockIdx.x * kThreads + threadIdx.x; pix < npix; pix += gridDim.x * kThreads) {
-            dc v;
-            if (I.plane)
-                v = uf_plane_value(I.q, I.uR, I.vR, pix, I.f, I.k, I.kmag, I.amp);
-            else if constexpr (std::is_same<Out, float2>::value)   // tables for every instance
-                v = uf_fast_value(I.q, I.uR, I.vR, pix, I.f, I.k, I.kmag, I.amp, I.ca, I.cb, I.m,
-                                  tabs + I.tab, I.flat != 0);
-            else
-                v = uf_series_value(I.q, I.uR, I.vR, pix, I.f, I.k, I.kmag, I.amp, I.ca, I.cb, I.m);
-            store(Uf + I.off, pix, v);
-        }
+        for (int pix = blockIdx.x * kThreads + threadIdx.x; pix < npix; pix += gridDim.x * kThreads)
+            uf_batch_pixel(I, pix, tabs, Uf);
     }
 }
 

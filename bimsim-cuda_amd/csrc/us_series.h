@@ -66,6 +66,30 @@ struct SeriesBatch {
     std::vector<SeriesSph> sph;
 };
 
+struct UfInst;
+
+// The other work of an FP32 batch's pre-pass, run in one launch with a
+// harmonic plan's records (series_plan_launch_prepass): the Mie
+// coefficients of n_mie_sph spheres (sphere i at mie_kmag[i]) in blocks of
+// spb, Bessel arrays of wstride in LDS, and the complex64 incident fields
+// of n_uf instances, uf_blocks blocks of 256 pixels each.
+struct PrepassArgs {
+    const SphereDev* mie_sph;
+    const double* mie_kmag;
+    int n_mie_sph, spb, wstride, quirks;
+    double2* A;
+    double2* B;
+    const UfInst* uf;
+    int n_uf, uf_blocks;
+    const double* uf_tabs;
+    float2* Uf;
+};
+
+// LDS of the pre-pass launch's coefficient blocks, which every block of it
+// reserves; the launch is taken only up to kPrepassLds.
+constexpr size_t kPrepassLds = 16384;
+size_t prepass_lds(const PrepassArgs& a);
+
 // Algorithms of the FP32 series.
 enum SeriesAlgo { kSeriesAuto = 0, kSeriesDirect = 1, kSeriesHarmonic = 2 };
 
@@ -132,6 +156,12 @@ void series_plan_destroy(SeriesPlan* plan);
 int series_plan_launch(bimsim_ctx* ctx, const SeriesPlan& plan, const double* waves,
                        const double2* A, const double2* B, float2* U);
 int series_plan_launch_waves(bimsim_ctx* ctx, const SeriesPlan& plan, const double* waves);
+// Legendre tables, then one launch of the records of P beside the
+// coefficients and incident fields of a (instead of
+// series_plan_launch_waves, the Mie launch and the incident-field launch).
+// P must be a harmonic plan and prepass_lds(a) <= kPrepassLds.
+int series_plan_launch_prepass(bimsim_ctx* ctx, const SeriesPlan& P, const double* waves,
+                               const PrepassArgs& a);
 int series_plan_launch_sums(bimsim_ctx* ctx, const SeriesPlan& plan, const double2* A,
                             const double2* B, float2* U);
 

@@ -15,11 +15,14 @@
 // Built with -ffp-contract=off so that the FP64 recurrences round like the
 // reference's; every FP32 FMA of the series is written out with fmaf.
 #include <algorithm>
+#include <climits>
 #include <cstdio>
 #include <cstring>
 #include <string>
 
+#include "mie_device.h"
 #include "refmath.h"
+#include "uf_device.h"
 #include "us_series.h"
 
 namespace bimsim {
@@ -389,13 +392,13 @@ __global__ void harm_legendre_kernel(const SeriesInst* __restrict__ insts,
 constexpr int kRecThreads = 256;
 constexpr int kRecTile = 4;
 
-__global__ __launch_bounds__(kRecThreads) void harm_records_kernel(
-    const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph, int n_sph,
-    const double* __restrict__ pl, const double2* __restrict__ e, const double2* __restrict__ phi,
+__device__ __forceinline__ void harm_records_block(
+    int m, int tile, int s_first, int s_step, const SeriesInst* __restrict__ insts,
+    const SeriesSph* __restrict__ sph, int n_sph, const double* __restrict__ pl,
+    const double2* __restrict__ e, const double2* __restrict__ phi,
     const double* __restrict__ h64, float4* __restrict__ rec) {
     __shared__ double red[kRecThreads / 64][4 * kRecTile];
-    const int m = blockIdx.x, tile = blockIdx.y;
-    for (int s = blockIdx.z; s < n_sph; s += gridDim.z) {
+    for (int s = s_first; s < n_sph; s += s_step) {
         const SeriesSph& SS = sph[s];
         const int tl = SS.tl;
         const bool planar = SS.planar != 0;
@@ -483,6 +486,58 @@ __global__ __launch_bounds__(kRecThreads) void harm_records_kernel(
             }
         }
     }
+}
+
+__global__ __launch_bounds__(kRecThreads) void harm_records_kernel(
+    const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph, int n_sph,
+    const double* __restrict__ pl, const double2* __restrict__ e, const double2* __restrict__ phi,
+    const double* __restrict__ h64, float4* __restrict__ rec) {
+    harm_records_block(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.z, insts, sph, n_sph, pl, e,
+                       phi, h64, rec);
+}
+
+// The FP32 batch's pre-pass as one launch: Mie coefficients (blocks
+// [0, n_mie), mie_block over spb spheres each, Bessel arrays in dynamic
+// LDS), the harmonic records (the next n_rec blocks, one per (m, tile,
+// sphere) as harm_records_kernel) and the incident fields (the rest, 256
+// pixels of one instance each).  None reads what another writes, and the
+// records read only the Legendre tables of the launch before, so the long
+// dependent chains of the coefficients run on a few compute units while
+// the records and fields fill the others.  The role is uniform per block.
+__global__ __launch_bounds__(kRecThreads) void prepass_kernel(
+    PrepassArgs a, int n_mie, int n_rec, int rec_m, int rec_tiles,
+    const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph, int n_sph,
+    const double* __restrict__ pl, const double2* __restrict__ e, const double2* __restrict__ phi,
+    const double* __restrict__ h64, float4* __restrict__ rec) {
+    static_assert(kRecThreads >= mie::kMieThreads, "the coefficient waves fit the block");
+    extern __shared__ mie::zc mie_lds[];
+    __shared__ SphereDev msph[mie::kSphPerLaunch];
+    __shared__ double mk[mie::kSphPerLaunch];
+    int b = (int)blockIdx.x;
+    const int t = (int)threadIdx.x;
+    if (b < n_mie) {
+        const int s0 = b * a.spb;
+        const int cnt = min(a.spb, a.n_mie_sph - s0);
+        if (t < cnt) {
+            msph[t] = a.mie_sph[s0 + t];
+            mk[t] = a.mie_kmag[s0 + t];
+        }
+        __syncthreads();
+        mie::mie_block(msph, mk, cnt, a.quirks, mie_lds, a.wstride, a.A, a.B);
+        return;
+    }
+    b -= n_mie;
+    if (b < n_rec) {
+        const int per_s = rec_m * rec_tiles;
+        const int s = b / per_s, r = b - s * per_s;
+        harm_records_block(r % rec_m, r / rec_m, s, n_sph, insts, sph, n_sph, pl, e, phi, h64,
+                           rec);
+        return;
+    }
+    b -= n_rec;
+    const int i = b / a.uf_blocks, pix = (b - i * a.uf_blocks) * kRecThreads + t;
+    const UfInst& I = a.uf[i];
+    if (pix < I.uR * I.vR) ufd::uf_batch_pixel(I, pix, a.uf_tabs, a.Uf);
 }
 
 // Trigonometric start values of the radial recurrences of one pixel and
@@ -1098,6 +1153,32 @@ int series_plan_launch_waves(bimsim_ctx* ctx, const SeriesPlan& P, const double*
         hipLaunchKernelGGL(series_waves_kernel, wgrid, dim3(256), 0, st, P.d_inst, P.d_sph, P.n_sph,
                            waves, P.d_wt, P.d_counters, P.n_counters);
     }
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return BIMSIM_OK;
+}
+
+size_t prepass_lds(const PrepassArgs& a) {
+    return sizeof(mie::zc) * 8 * (size_t)a.wstride * (size_t)std::min(a.spb, a.n_mie_sph);
+}
+
+int series_plan_launch_prepass(bimsim_ctx* ctx, const SeriesPlan& P, const double* waves,
+                               const PrepassArgs& a) {
+    if (!P.harmonic || P.n_inst == 0 || P.n_sph == 0 || a.n_mie_sph <= 0 || a.n_uf <= 0 ||
+        a.spb < 1 || a.spb > mie::kSphPerLaunch || a.uf_blocks < 1 || prepass_lds(a) > kPrepassLds)
+        return set_error(ctx, BIMSIM_E_INVALID, "pre-pass launch outside its limits");
+    const hipStream_t st = ctx->stream;
+    const dim3 lgrid((P.nk_max + 63) / 64, std::min(P.n_inst, 65535), P.max_lh);
+    hipLaunchKernelGGL(harm_legendre_kernel, lgrid, dim3(64), 0, st, P.d_inst, P.d_sph, P.n_inst,
+                       waves, P.d_h64, P.d_pl, P.d_e, P.d_phi, P.d_counters, P.n_counters);
+    const int n_mie = (a.n_mie_sph + a.spb - 1) / a.spb;
+    const int tiles = (P.max_tl + kRecTile - 1) / kRecTile;
+    const long long n_rec = (long long)P.max_tl * tiles * P.n_sph;
+    const long long total = n_mie + n_rec + (long long)a.uf_blocks * a.n_uf;
+    if (total > INT_MAX)
+        return set_error(ctx, BIMSIM_E_INVALID, "pre-pass launch outside its limits");
+    hipLaunchKernelGGL(prepass_kernel, dim3((unsigned)total), dim3(kRecThreads), prepass_lds(a), st,
+                       a, n_mie, (int)n_rec, P.max_tl, tiles, P.d_inst, P.d_sph, P.n_sph, P.d_pl,
+                       P.d_e, P.d_phi, P.d_h64, P.d_rec);
     BIMSIM_HIP_CHECK(ctx, hipGetLastError());
     return BIMSIM_OK;
 }
