@@ -20,7 +20,7 @@ constexpr size_t kMieLds = 61440;   // LDS budget of a block's Bessel arrays (pl
 
 constexpr int kLanes = 3;            // parts per sphere: one wave each
 constexpr int kWave = 64;
-constexpr int kMieThreads = kLanes * kWave;
+constexpr int kMieThreads = (kLanes + 1) * kWave;   // and a wave for the start orders
 static_assert(kSphPerLaunch <= kWave, "one lane per sphere in each part's wave");
 
 // Spheres per block so that 2 sets x 4 arrays x wstride complex values
@@ -30,15 +30,11 @@ inline int mie_spheres_per_block(int wstride) {
     return (int)std::max<size_t>(1, std::min<size_t>(kSphPerLaunch, kMieLds / per));
 }
 
-// One part of the Bessel sets of sphere S into its work w (j, y, j', y' of
-// ka, then of kna, wstride apart): role 0 J of ka, 1 Y of ka, 2 J of kna.
-__device__ inline void mie_bessel(const SphereDev& S, double kmag, int role, int quirks, zc* w,
-                           int wstride) {
+// The argument of part `role` of sphere S: ka for J and Y of ka (roles 0,
+// 1), kna for J of kna (role 2).
+__device__ inline zc mie_arg(const SphereDev& S, double kmag, int role) {
     const zc nc = {S.n[0], S.n[1]};
-    const zc z = role < 2 ? zc{kmag * S.a, 0.0} : zj::zsc(S.a, zj::zsc(kmag, nc));
-    zc* o = role < 2 ? w : w + 4 * wstride;
-    zj::sph_jy_cplx(S.Nl, z, quirks != 0, o, o + wstride, o + 2 * wstride, o + 3 * wstride,
-                    role == 1, role != 1);
+    return role < 2 ? zc{kmag * S.a, 0.0} : zj::zsc(S.a, zj::zsc(kmag, nc));
 }
 
 // A_l and B_l of sphere S from its two Bessel sets (w: ka set, then kna).
@@ -63,16 +59,47 @@ __device__ inline void mie_order(const SphereDev& S, int l, const zc* w, int wst
 }
 
 // Coefficients of the block's cnt spheres (sh, wavenumbers kh, in LDS);
-// lds holds 8 wstride complex values per sphere.  Wave r < 3 computes part
-// r of sphere (lane); every thread of the block (at least kMieThreads)
-// then takes (sphere, order) pairs.
+// lds holds 8 wstride complex values per sphere (j, y, j', y' of ka, then
+// of kna).  Wave r < 3 computes part r of sphere (lane): J of ka, Y of ka,
+// J of kna, as sph_jy_cplx (cbessjyva_sph) with the array clearing left out
+// (every order up to Nl is written on every path).  The start orders of the
+// Miller recurrences depend only on |z| and Nl, so wave 3 computes them for
+// every sphere and argument while waves 0-2 evaluate the start values; the
+// parts then finish with them.  Every thread of the block (at least
+// kMieThreads) then takes (sphere, order) pairs of the coefficient formula.
 __device__ __forceinline__ void mie_block(const SphereDev* sh, const double* kh, int cnt,
                                           int quirks, zc* lds, int wstride, double2* A,
                                           double2* B) {
+    __shared__ zj::CjyvaOrders ord[kSphPerLaunch][2];   // (sphere, ka / kna)
     const int t = (int)threadIdx.x;
-    const int s = t % kWave;
-    if (s < cnt && t < kMieThreads)
-        mie_bessel(sh[s], kh[s], t / kWave, quirks, lds + (size_t)8 * s * wstride, wstride);
+    const int s = t % kWave, role = t / kWave;
+    const bool part = s < cnt && role < kLanes;
+    zc z = {0.0, 0.0};
+    zc* o = lds;
+    bool want_y = false, want_j = false;
+    zj::CjyvaStart st = {};
+    if (part) {
+        z = mie_arg(sh[s], kh[s], role);
+        o = lds + (size_t)8 * s * wstride + (role < 2 ? 0 : 4 * wstride);
+        want_y = role == 1;
+        want_j = role != 1 || z.i != 0.0;
+        st = zj::cjyva_start(sh[s].Nl + 0.5, z, o, o + wstride, o + 2 * wstride, o + 3 * wstride,
+                             quirks != 0, want_y, want_j);
+    } else if (role == kLanes) {
+        for (int q = s; q < 2 * cnt; q += kWave) {
+            const int i = q >> 1, arg = q & 1;
+            const double v = sh[i].Nl + 0.5;
+            ord[i][arg] = zj::cjyva_orders(zj::zabs(mie_arg(sh[i], kh[i], 2 * arg)), (int)v);
+        }
+    }
+    __syncthreads();
+    if (part) {
+        if (!st.done)
+            zj::cjyva_rest(st, z, &ord[s][role < 2 ? 0 : 1], o, o + wstride, o + 2 * wstride,
+                           o + 3 * wstride, want_y, want_j);
+        zj::sph_jy_scale(sh[s].Nl, z, quirks != 0, o, o + wstride, o + 2 * wstride,
+                         o + 3 * wstride, want_y, want_j);
+    }
     __syncthreads();
     int lm = 0;
     for (int i = 0; i < cnt; ++i) lm = max(lm, sh[i].Nl);
@@ -82,7 +109,6 @@ __device__ __forceinline__ void mie_block(const SphereDev* sh, const double* kh,
         if (l <= sh[i].Nl) mie_order(sh[i], l, lds + (size_t)8 * i * wstride, wstride, A, B);
     }
 }
-
 
 }  // namespace mie
 }  // namespace bimsim

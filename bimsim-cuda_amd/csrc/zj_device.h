@@ -182,17 +182,29 @@ __device__ inline int msta2(double x, int n, int mp) {
 
 __device__ inline double sq(double v) { return v * v; }
 
-// Complex-argument J_{v0+k}, Y_{v0+k} and derivatives, k = 0..floor(v)
-// (CJYVA).  Arrays hold floor(v)+2 entries.  typo reproduces the reference's
-// y_{v0} asymptotic defect (cbessjy.cpp:541).
-//
-// The two halves can run apart (on different lanes): want_y = false stops
-// after J and J' (their values do not depend on the Y part) and leaves cyv,
-// cyvp untouched; want_j = false, for real z only (the complex-z Y
-// correction reads J), skips the Miller recurrence and leaves cjv, cjvp
-// untouched.  Each call writes only the arrays of the halves it computes.
-__device__ inline void cjyva(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyvp, bool typo,
-                             bool want_y = true, bool want_j = true) {
+// Start orders of CJYVA's Miller recurrence for |z| = a0 and top order n
+// (MSTA1 for 10^-200, MSTA2 for 15 digits at order n).  They depend on
+// nothing else, so a caller may compute them beside cjyva_start.
+struct CjyvaOrders {
+    int m1, m2;
+};
+__device__ inline CjyvaOrders cjyva_orders(double a0, int n) {
+    return CjyvaOrders{msta1(a0, 200), msta2(a0, n, 15)};
+}
+
+// CJYVA up to its recurrences: J, Y of orders v0 and v0 + 1 (done: |z| is
+// below 1e-100 and every array is written).
+struct CjyvaStart {
+    zc j0, j1, y0, y1;
+    double a0, v0;
+    int n;
+    bool done;
+};
+
+// CJYVA's first part (see cjyva): the power series or asymptotic
+// expansions for orders v0 and v0 + 1; writes the arrays only when done.
+__device__ inline CjyvaStart cjyva_start(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyvp,
+                                         bool typo, bool want_y, bool want_j) {
     const double eps = 1e-15, euler = 0.5772156649015329;
     const zc one = {1.0, 0.0}, zero = {0.0, 0.0};
     const double a0 = zabs(z);
@@ -214,7 +226,7 @@ __device__ inline void cjyva(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyv
         } else {
             cjvp[0] = zc{1e308, 0.0};
         }
-        return;
+        return CjyvaStart{zero, zero, zero, zero, a0, v0, n, true};
     }
     if (z1.r < 0.0) z1 = zneg(z);
     zc j0, j1, y0 = zero, y1 = zero;
@@ -317,6 +329,18 @@ __device__ inline void cjyva(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyv
             j1 = zmul(j1, f1);
         }
     }
+    return CjyvaStart{j0, j1, y0, y1, a0, v0, n, false};
+}
+
+// The rest of CJYVA from cjyva_start's values: the recurrences for orders
+// v0 + 2 .. v0 + n and the derivatives.  ord: the start orders of
+// cjyva_orders(st.a0, st.n), or null to compute them here where needed.
+__device__ inline void cjyva_rest(const CjyvaStart& st, zc z, const CjyvaOrders* ord, zc* cjv,
+                                  zc* cyv, zc* cjvp, zc* cyvp, bool want_y, bool want_j) {
+    const zc one = {1.0, 0.0}, zero = {0.0, 0.0};
+    const zc j0 = st.j0, j1 = st.j1, y0 = st.y0, y1 = st.y1;
+    const double a0 = st.a0, v0 = st.v0;
+    int n = st.n;
     if (want_j) {
         cjv[0] = j0;
         cjv[1] = j1;
@@ -331,7 +355,7 @@ __device__ inline void cjyva(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyv
             f1 = f;
         }
     } else if (n >= 2) {
-        int m = msta1(a0, 200);
+        int m = ord ? ord->m1 : msta1(a0, 200);
         if (m < n) {
             for (int k = m + 1; k <= n; ++k) {
                 if (want_j) cjv[k] = cjvp[k] = zero;
@@ -339,7 +363,7 @@ __device__ inline void cjyva(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyv
             }
             n = m;
         } else if (want_j) {
-            m = msta2(a0, n, 15);
+            m = ord ? ord->m2 : msta2(a0, n, 15);
         }
         if (want_j) {
             zc f2 = zero, f1 = {1.0e-100, 0.0}, f = zero;
@@ -433,6 +457,21 @@ __device__ inline void cjyva(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyv
     }
     cyvp[0] = zsub(zmul(zsc(v0, cyv[0]), zr), cyv[1]);
     for (int k = 1; k <= n; ++k) cyvp[k] = zsub(cyv[k - 1], zmul(zsc(k + v0, cyv[k]), zr));
+}
+
+// Complex-argument J_{v0+k}, Y_{v0+k} and derivatives, k = 0..floor(v)
+// (CJYVA).  Arrays hold floor(v)+2 entries.  typo reproduces the reference's
+// y_{v0} asymptotic defect (cbessjy.cpp:541).
+//
+// The two halves can run apart (on different lanes): want_y = false stops
+// after J and J' (their values do not depend on the Y part) and leaves cyv,
+// cyvp untouched; want_j = false, for real z only (the complex-z Y
+// correction reads J), skips the Miller recurrence and leaves cjv, cjvp
+// untouched.  Each call writes only the arrays of the halves it computes.
+__device__ inline void cjyva(double v, zc z, zc* cjv, zc* cyv, zc* cjvp, zc* cyvp, bool typo,
+                             bool want_y = true, bool want_j = true) {
+    const CjyvaStart st = cjyva_start(v, z, cjv, cyv, cjvp, cyvp, typo, want_y, want_j);
+    if (!st.done) cjyva_rest(st, z, nullptr, cjv, cyv, cjvp, cyvp, want_y, want_j);
 }
 
 // Real-argument J_{v0+k}, Y_{v0+k} and derivatives, k = 0..floor(v) (JYV,
@@ -595,19 +634,11 @@ __device__ inline void sph_jy_real(int nl, double x, bool quirks, double* j, dou
     }
 }
 
-// Spherical j_l, y_l, j'_l, y'_l for l = 0..nl (cbessjyva_sph,
-// cbessjy.cpp:718-737): scale by sqrt(pi/(2z)); the reference uses
-// pi = 3.14159 there.
-// want_j / want_y select the halves as in cjyva (want_j is forced for
-// complex z when Y is wanted); only the selected arrays are written.
-__device__ inline void sph_jy_cplx(int nl, zc z, bool quirks, zc* j, zc* y, zc* jp, zc* yp,
-                                   bool want_y = true, bool want_j = true) {
-    if (want_y && z.i != 0.0) want_j = true;
-    for (int k = 0; k <= nl + 1; ++k) {
-        if (want_j) j[k] = jp[k] = zc{0.0, 0.0};
-        if (want_y) y[k] = yp[k] = zc{0.0, 0.0};
-    }
-    cjyva(nl + 0.5, z, j, y, jp, yp, quirks, want_y, want_j);
+// The cylindrical-to-spherical step of sph_jy_cplx on orders 0..nl:
+// j_l = sqrt(pi / (2z)) J_{l+1/2}, j'_l = -j_l / (2z) + sqrt(pi / (2z)) J'
+// (pi = 3.14159 with quirks, as the reference).
+__device__ inline void sph_jy_scale(int nl, zc z, bool quirks, zc* j, zc* y, zc* jp, zc* yp,
+                                    bool want_y, bool want_j) {
     const double pi = quirks ? 3.14159 : M_PI;
     const zc z2 = zsc(2.0, z);
     // the reference evaluates these two per order; they do not depend on it
@@ -625,6 +656,22 @@ __device__ inline void sph_jy_cplx(int nl, zc z, bool quirks, zc* j, zc* y, zc* 
             yp[k] = zadd(zmul(inv, yk), zmul(ypk, s));
         }
     }
+}
+
+// Spherical j_l, y_l, j'_l, y'_l for l = 0..nl (cbessjyva_sph,
+// cbessjy.cpp:718-737): scale by sqrt(pi/(2z)); the reference uses
+// pi = 3.14159 there.
+// want_j / want_y select the halves as in cjyva (want_j is forced for
+// complex z when Y is wanted); only the selected arrays are written.
+__device__ inline void sph_jy_cplx(int nl, zc z, bool quirks, zc* j, zc* y, zc* jp, zc* yp,
+                                   bool want_y = true, bool want_j = true) {
+    if (want_y && z.i != 0.0) want_j = true;
+    for (int k = 0; k <= nl + 1; ++k) {
+        if (want_j) j[k] = jp[k] = zc{0.0, 0.0};
+        if (want_y) y[k] = yp[k] = zc{0.0, 0.0};
+    }
+    cjyva(nl + 0.5, z, j, y, jp, yp, quirks, want_y, want_j);
+    sph_jy_scale(nl, z, quirks, j, y, jp, yp, want_y, want_j);
 }
 
 }  // namespace zj
