@@ -374,20 +374,22 @@ __device__ inline void cjyva_rest(const CjyvaStart& st, zc z, const CjyvaOrders*
                 f1 = f;
             }
             const zc cs = zabs(j0) > zabs(j1) ? zdiv(j0, f) : zdiv(j1, f2);
+            #pragma unroll 4
             for (int k = 0; k <= n; ++k) cjv[k] = zmul(cjv[k], cs);
         }
     }
     if (want_j) {
         cjvp[0] = zsub(zmul(zsc(v0, cjv[0]), zr), cjv[1]);
+        #pragma unroll 4
         for (int k = 1; k <= n; ++k) cjvp[k] = zadd(zmul(zsc(-(k + v0), cjv[k]), zr), cjv[k - 1]);
     }
     if (!want_y) return;
     cyv[0] = y0;
     cyv[1] = y1;
-    const double ya0 = zabs(y0);
     int lb = 0;
-    {
+    if (z.i != 0.0) {
         // |y_{k-2}| and |y_{k-1}| carried along (one magnitude per order)
+        const double ya0 = zabs(y0);
         zc g0 = y0, g1 = y1;
         double a_g0 = ya0, a_g1 = zabs(y1);
         for (int k = 2; k <= n; ++k) {
@@ -399,6 +401,15 @@ __device__ inline void cjyva_rest(const CjyvaStart& st, zc z, const CjyvaOrders*
             g1 = g;
             a_g0 = a_g1;
             a_g1 = yak;
+        }
+    } else {
+        // the same recurrence; lb is read only by the complex-z correction
+        zc g0 = y0, g1 = y1;
+        for (int k = 2; k <= n; ++k) {
+            const zc g = zsub(zmul(zsc(2.0 * (v0 + k - 1.0), g1), zr), g0);
+            cyv[k] = g;
+            g0 = g1;
+            g1 = g;
         }
     }
     int lb0 = 0;
@@ -456,6 +467,7 @@ __device__ inline void cjyva_rest(const CjyvaStart& st, zc z, const CjyvaOrders*
         }
     }
     cyvp[0] = zsub(zmul(zsc(v0, cyv[0]), zr), cyv[1]);
+    #pragma unroll 4
     for (int k = 1; k <= n; ++k) cyvp[k] = zsub(cyv[k - 1], zmul(zsc(k + v0, cyv[k]), zr));
 }
 
@@ -644,6 +656,7 @@ __device__ inline void sph_jy_scale(int nl, zc z, bool quirks, zc* j, zc* y, zc*
     // the reference evaluates these two per order; they do not depend on it
     const zc s = zsqrt(rdiv(pi, z2));
     const zc inv = rdiv(-1.0, z2);
+#pragma unroll 4
     for (int k = 0; k <= nl; ++k) {
         if (want_j) {
             const zc jk = zmul(j[k], s), jpk = jp[k];
