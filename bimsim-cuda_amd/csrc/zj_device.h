@@ -247,25 +247,34 @@ __device__ inline CjyvaStart cjyva_start(double v, zc z, zc* cjv, zc* cyv, zc* c
     } else {
         const int kz = a0 >= 50.0 ? 8 : (a0 >= 35.0 ? 10 : 11);
         const zc z2r = rdiv(1.0, z2);   // the terms divide by k(2k -+ 1) z^2
+        // the four series (P and Q of orders v0 and v0 + 1) are independent:
+        // one loop steps all four, each with the operations it has alone
+        double mu[2];
+        zc pp[2], tp[2], qq[2], tq[2];
+#pragma unroll
         for (int l = 0; l < 2; ++l) {
-            const double mu = 4.0 * (l + v0) * (l + v0);
-            zc p = one, tp = one;
-            for (int k = 1; k <= kz; ++k) {
-                zc num = zsc(mu - sq(4.0 * k - 1.0),
-                             zsc(mu - sq(4.0 * k - 3.0), zsc(-0.78125e-2, tp)));
-                tp = zmul(zsc(1.0 / (k * (2.0 * k - 1.0)), num), z2r);
-                p = zadd(p, tp);
+            mu[l] = 4.0 * (l + v0) * (l + v0);
+            pp[l] = tp[l] = qq[l] = tq[l] = one;
+        }
+        for (int k = 1; k <= kz; ++k) {
+#pragma unroll
+            for (int l = 0; l < 2; ++l) {
+                const zc nump = zsc(mu[l] - sq(4.0 * k - 1.0),
+                                    zsc(mu[l] - sq(4.0 * k - 3.0), zsc(-0.78125e-2, tp[l])));
+                tp[l] = zmul(zsc(1.0 / (k * (2.0 * k - 1.0)), nump), z2r);
+                pp[l] = zadd(pp[l], tp[l]);
+                const zc numq = zsc(mu[l] - sq(4.0 * k + 1.0),
+                                    zsc(mu[l] - sq(4.0 * k - 1.0), zsc(-0.78125e-2, tq[l])));
+                tq[l] = zmul(zsc(1.0 / (k * (2.0 * k + 1.0)), numq), z2r);
+                qq[l] = zadd(qq[l], tq[l]);
             }
-            zc q = one, tq = one;
-            for (int k = 1; k <= kz; ++k) {
-                zc num = zsc(mu - sq(4.0 * k + 1.0),
-                             zsc(mu - sq(4.0 * k - 1.0), zsc(-0.78125e-2, tq)));
-                tq = zmul(zsc(1.0 / (k * (2.0 * k + 1.0)), num), z2r);
-                q = zadd(q, tq);
-            }
-            q = zmul(q, rdiv(0.125 * (mu - 1.0), z1));
+        }
+        const zc amp = zsqrt(rdiv(M_2_PI, z1));   // the same for both orders
+#pragma unroll
+        for (int l = 0; l < 2; ++l) {
+            const zc p = pp[l];
+            const zc q = zmul(qq[l], rdiv(0.125 * (mu[l] - 1.0), z1));
             const zc chi = zsub(z1, zc{(0.5 * (l + v0) + 0.25) * M_PI, 0.0});
-            const zc amp = zsqrt(rdiv(M_2_PI, z1));
             const zc c = zcos(chi), s = zsin(chi);
             const zc jj = zmul(amp, zsub(zmul(p, c), zmul(q, s)));
             if (l == 0) {
@@ -366,12 +375,18 @@ __device__ inline void cjyva_rest(const CjyvaStart& st, zc z, const CjyvaOrders*
             m = ord ? ord->m2 : msta2(a0, n, 15);
         }
         if (want_j) {
+            // each step's factor is formed one step ahead, beside the
+            // previous step's product; orders above n go to the spare entry
+            // n + 1 (m >= n here), so the step has no branch
             zc f2 = zero, f1 = {1.0e-100, 0.0}, f = zero;
+            double c = 2.0 * (v0 + m + 1.0);
             for (int k = m; k >= 0; --k) {
-                f = zsub(zmul(zsc(2.0 * (v0 + k + 1.0), f1), zr), f2);
-                if (k <= n) cjv[k] = f;
+                const double cn = 2.0 * (v0 + (k - 1) + 1.0);
+                f = zsub(zmul(zsc(c, f1), zr), f2);
+                cjv[min(k, n + 1)] = f;
                 f2 = f1;
                 f1 = f;
+                c = cn;
             }
             const zc cs = zabs(j0) > zabs(j1) ? zdiv(j0, f) : zdiv(j1, f2);
             #pragma unroll 4
@@ -403,13 +418,17 @@ __device__ inline void cjyva_rest(const CjyvaStart& st, zc z, const CjyvaOrders*
             a_g1 = yak;
         }
     } else {
-        // the same recurrence; lb is read only by the complex-z correction
+        // the same recurrence (each factor formed one step ahead); lb is
+        // read only by the complex-z correction
         zc g0 = y0, g1 = y1;
+        double c = 2.0 * (v0 + 2 - 1.0);
         for (int k = 2; k <= n; ++k) {
-            const zc g = zsub(zmul(zsc(2.0 * (v0 + k - 1.0), g1), zr), g0);
+            const double cn = 2.0 * (v0 + (k + 1) - 1.0);
+            const zc g = zsub(zmul(zsc(c, g1), zr), g0);
             cyv[k] = g;
             g0 = g1;
             g1 = g;
+            c = cn;
         }
     }
     int lb0 = 0;
