@@ -86,6 +86,28 @@ def test_batch_of_one_equals_the_pipeline(pkg, ctx, W, prec):
 
 
 @pytest.mark.parametrize("prec", [0, 2])
+def test_plane_wave_batch_equals_the_pipeline(pkg, ctx, W, prec):
+    """A batch of plane-wave instances only (the FP32 form runs its one
+    plane-wave series plan with the coefficients and incident fields in the
+    same launch as the records)."""
+    insts = [W.make_instance([dict(p=(2.0, -1.0, 0.0), a=3.0, n=complex(1.5, 0.02)),
+                              dict(p=(-6.0, 4.0, 0.0), a=2.2, n=complex(1.35, 0.07))],
+                             2.8, 48, 30.0, (0.0, 0.5), (0.0, 0.5), 1, 7, plane_wave=True),
+             W.make_instance([dict(p=(0.5, 0.5, 0.0), a=4.0, n=complex(1.6, 0.0))],
+                             3.1, 32, 24.0, (0.0, 0.4), (0.0, 0.4), 1, 8, plane_wave=True)]
+    waves = waves_of(pkg, W, insts)
+    b, D, Di, Ab, _ = run_batch(pkg, ctx, insts, waves, prec)
+    parts = zip(split(insts, host(D)), split(insts, host(Di)), split(insts, host(Ab)))
+    for inst, w, (d, di, ab) in zip(insts, waves, parts):
+        D1, Di1, Ab1 = single(pkg, ctx, inst, w, prec)
+        if prec == 0:
+            assert np.array_equal(d, D1) and np.array_equal(di, Di1)
+        else:
+            assert normrel(d, D1) <= 2e-6 and normrel(di, Di1) <= 2e-6
+    b.close()
+
+
+@pytest.mark.parametrize("prec", [0, 2])
 def test_ragged_batch_equals_per_instance_runs(pkg, ctx, W, prec):
     insts = ragged(W)
     waves = waves_of(pkg, W, insts)
