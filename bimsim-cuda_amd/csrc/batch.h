@@ -56,6 +56,25 @@ int uf_batch_launch(bimsim_ctx* ctx, const UfInst* d_inst, int n, int max_npix, 
 int sum_batch_launch(bimsim_ctx* ctx, const SumInst* d_inst, int n, int max_npix,
                      const double4* d_sph, int f32, const void* Uf, void* U, void* D, void* Di);
 
+// A batch instance's Uf + Us sum for the detector's first row pass to
+// apply as it loads the fields (instead of sum_batch_launch): its slice and
+// its ns spheres as (x, y, z, a) at sph.
+struct SumFuse {
+    QuadArg q;
+    int uR, vR, ns;
+    const double4* sph;
+};
+
+// bimsim_detector's work; with fz (complex64 fields on the band-limited
+// path, detector_fusable), U holds the scattered field alone and the sum
+// with Uf is formed as the rows load.  The images are accumulated into as
+// always, so they must be cleared before.
+bool detector_fusable(const bimsim_ctx* ctx, int uR, int vR);
+int detector_run(bimsim_ctx* ctx, int prec, int quirks, void* U, void* Uf, int uR, int vR,
+                 double du, double dv, double NAin, double NAout, double lambda, int padding,
+                 int Nd, int ss, void* D, void* Di, void* Abs, int write_filtered,
+                 const SumFuse* fz);
+
 // Zeroes `bytes` (a multiple of 4) at p with a kernel (graph-safe).
 int zero_launch(bimsim_ctx* ctx, void* p, size_t bytes);
 

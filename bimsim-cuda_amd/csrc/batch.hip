@@ -7,7 +7,9 @@
 //   incident fields    one launch over (pixel blocks, instances)
 //   FP32 pre-pass      with one harmonic plan and few spheres instead: the
 //                      Legendre tables, then coefficients, records and
-//                      incident fields in one launch (series_plan_launch_prepass)
+//                      incident fields in one launch (series_plan_launch_prepass);
+//                      with band-limited detectors the sum is then formed by
+//                      their first row pass and the images cleared here
 //   scattered fields   FP32: coefficient kernel + one persistent work-queue
 //                      launch per order bucket (us_series.hip); FP64 forms:
 //                      the per-instance entry point
@@ -58,6 +60,11 @@ struct bimsim_batch {
     // on the one stream instead
     bool prepass = false;
     PrepassArgs pre{};
+    // and with every detector on the band-limited path, the Uf + Us sum
+    // formed by the detectors' first row pass (the images cleared in the
+    // pre-pass launch)
+    bool fuse_sum = false;
+    std::vector<SumInst> sum_host;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -140,16 +147,22 @@ int enqueue(bimsim_batch* b) {
                                     (char*)b->d_U + fb * b->field_off[i], 0));
         }
     }
-    BIMSIM_TRY(sum_batch_launch(ctx, b->d_sum, b->n, b->max_npix, b->d_sph4, b->f32, b->d_Uf, b->d_U,
-                                b->D, b->Di));
+    if (!b->fuse_sum)
+        BIMSIM_TRY(sum_batch_launch(ctx, b->d_sum, b->n, b->max_npix, b->d_sph4, b->f32, b->d_Uf,
+                                    b->d_U, b->D, b->Di));
     for (int i = 0; i < b->n; ++i) {
         const bimsim_instance& I = b->inst[i];
         void* Ab = b->Abs ? (char*)b->Abs + rb * I.image_off : nullptr;
-        BIMSIM_TRY(bimsim_detector(ctx, b->prec, b->quirks, (char*)b->d_U + fb * b->field_off[i],
-                                   (char*)b->d_Uf + fb * b->field_off[i], I.N, I.N, b->du[i], b->dv[i],
-                                   I.objective[0], I.objective[1], I.lambda, I.padding, I.Nd, I.ss,
-                                   (char*)b->D + rb * I.image_off, (char*)b->Di + rb * I.image_off,
-                                   Ab, 0));
+        SumFuse fz{};
+        if (b->fuse_sum) {
+            const SumInst& S = b->sum_host[i];
+            fz = SumFuse{S.q, S.uR, S.vR, S.ns, b->d_sph4 + S.s_begin};
+        }
+        BIMSIM_TRY(detector_run(ctx, b->prec, b->quirks, (char*)b->d_U + fb * b->field_off[i],
+                                (char*)b->d_Uf + fb * b->field_off[i], I.N, I.N, b->du[i], b->dv[i],
+                                I.objective[0], I.objective[1], I.lambda, I.padding, I.Nd, I.ss,
+                                (char*)b->D + rb * I.image_off, (char*)b->Di + rb * I.image_off, Ab,
+                                0, b->fuse_sum ? &fz : nullptr));
     }
     return BIMSIM_OK;
 }
@@ -295,6 +308,14 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
         const SeriesPlan& P = b->have_mc ? b->plan_mc : b->plan_pw;
         b->prepass = b->have_mc != b->have_pw && P.harmonic && b->n_sph > 0 &&
                      prepass_lds(b->pre) <= kPrepassLds;
+        b->fuse_sum = b->prepass;
+        for (int i = 0; i < n; ++i) b->fuse_sum = b->fuse_sum && detector_fusable(ctx, inst[i].N, inst[i].N);
+        if (b->fuse_sum) {
+            b->sum_host = sum;
+            b->pre.sum = b->d_sum;
+            b->pre.D = (float*)b->D;
+            b->pre.Di = (float*)b->Di;
+        }
     }
     return BIMSIM_OK;
 }

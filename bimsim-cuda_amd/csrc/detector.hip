@@ -11,6 +11,8 @@
 // whole translation unit); FP32 ones in float with FP64-accurate twiddles.
 #include <type_traits>
 
+#include "batch.h"
+#include "geometry.h"
 #include "internal.h"
 
 namespace bimsim {
@@ -302,14 +304,32 @@ __device__ __forceinline__ typename C2<R>::T* fixed_fft(typename C2<R>::T* a,
     }
 }
 
+// A batch instance's Uf + Us at pixel pix from U = Us (a) and Uf (b), by
+// the rule of the batch's sum kernel: Uf without spheres, else U + Uf
+// outside every sphere (d < a is inside, nfSumUf.cu:31-48).
+template <class C>
+__device__ __forceinline__ C fused_sum(const SumFuse& fz, int pix, C a, C b) {
+    if (fz.ns == 0) return b;
+    const int iv = pix / fz.uR, iu = pix - iv * fz.uR;
+    const d3 p = quad_at(fz.q, (double)iu / (double)fz.uR, (double)iv / (double)fz.vR);
+    for (int s = 0; s < fz.ns; ++s) {
+        const double4 S = fz.sph[s];
+        if (len(sub(p, d3{S.x, S.y, S.z})) < S.w) return a;
+    }
+    a.x = a.x + b.x;
+    a.y = a.y + b.y;
+    return a;
+}
+
 // Forward FFT of B rows (length N) of U (blockIdx.y = 0) or Uf (= 1); the
 // pass-band columns cols[0..Nc) go to the compact spectrum
-// G[(field*Nc + c)*nrows + row].
-template <class R, int N>
+// G[(field*Nc + c)*nrows + row].  kSum: U is the scattered field alone and
+// the rows of U load as fused_sum(fz, ...).
+template <class R, int N, bool kSum>
 __global__ __launch_bounds__(kThreads) void rows_fwd_kernel(
     const typename C2<R>::T* __restrict__ U, const typename C2<R>::T* __restrict__ Uf, int nrows,
     int B, const int* __restrict__ cols, int Nc, typename C2<R>::T* __restrict__ G,
-    const typename C2<R>::T* __restrict__ tw) {
+    const typename C2<R>::T* __restrict__ tw, SumFuse fz) {
     using C = typename C2<R>::T;
     extern __shared__ unsigned char smem_raw[];
     C* a = reinterpret_cast<C*>(smem_raw);
@@ -318,7 +338,14 @@ __global__ __launch_bounds__(kThreads) void rows_fwd_kernel(
     const C* F = f ? Uf : U;
     const int row0 = blockIdx.x * B;
     const int nb = min(B, nrows - row0);
-    for (int t = threadIdx.x; t < nb * N; t += kThreads) a[t] = F[(size_t)row0 * N + t];
+    if (kSum && f == 0) {
+        for (int t = threadIdx.x; t < nb * N; t += kThreads) {
+            const size_t i = (size_t)row0 * N + t;
+            a[t] = fused_sum(fz, (int)i, U[i], Uf[i]);
+        }
+    } else {
+        for (int t = threadIdx.x; t < nb * N; t += kThreads) a[t] = F[(size_t)row0 * N + t];
+    }
     __syncthreads();
     const C* res = fixed_fft<R, N>(a, b, nb, tw, false);
     for (int t = threadIdx.x; t < nb * Nc; t += kThreads) {
@@ -437,15 +464,27 @@ bool fixed_size(int n) {
     }
 }
 
+// The band-limited LDS FFT path: both sizes tabled and a row pair in LDS.
+bool band_path(const bimsim_ctx* ctx, size_t csize, int uR, int vR) {
+    return fixed_size(uR) && fixed_size(vR) && csize * 2 * (size_t)std::max(uR, vR) <= ctx->lds_max;
+}
+
 template <class R>
 int launch_rows_fwd(bimsim_ctx* ctx, int N, dim3 grid, size_t lds, const typename C2<R>::T* U,
                     const typename C2<R>::T* Uf, int nrows, int B, const int* cols, int Nc,
-                    typename C2<R>::T* G, const typename C2<R>::T* tw) {
+                    typename C2<R>::T* G, const typename C2<R>::T* tw, const SumFuse* fz) {
+    // the summing form exists for complex64 fields (the FP32 batches) only
+    constexpr bool f32 = std::is_same<R, float>::value;
+    if (fz && !f32) return set_error(ctx, BIMSIM_E_UNSUPPORTED, "fused sum needs complex64 fields");
     switch (N) {
-#define BIMSIM_X(NN)                                                                            \
-    case NN:                                                                                    \
-        hipLaunchKernelGGL((rows_fwd_kernel<R, NN>), grid, dim3(kThreads), lds, ctx->stream, U, \
-                           Uf, nrows, B, cols, Nc, G, tw);                                      \
+#define BIMSIM_X(NN)                                                                          \
+    case NN:                                                                                  \
+        if (f32 && fz)                                                                        \
+            hipLaunchKernelGGL((rows_fwd_kernel<R, NN, f32>), grid, dim3(kThreads), lds,      \
+                               ctx->stream, U, Uf, nrows, B, cols, Nc, G, tw, *fz);           \
+        else                                                                                  \
+            hipLaunchKernelGGL((rows_fwd_kernel<R, NN, false>), grid, dim3(kThreads), lds,    \
+                               ctx->stream, U, Uf, nrows, B, cols, Nc, G, tw, SumFuse{});     \
         break;
         BIMSIM_FFT_SIZES(BIMSIM_X)
 #undef BIMSIM_X
@@ -491,7 +530,8 @@ int launch_rows_inv(bimsim_ctx* ctx, int N, dim3 grid, size_t lds, const typenam
 template <class R>
 int detector_band(bimsim_ctx* ctx, const PostOp& bp, void* Uin, void* Ufin, int uR, int vR,
                   int padding, int Nd, int ss, void* D, void* Di, void* Abs, int write_filtered,
-                  const typename C2<R>::T* tw_u, const typename C2<R>::T* tw_v) {
+                  const typename C2<R>::T* tw_u, const typename C2<R>::T* tw_v,
+                  const SumFuse* fz) {
     using C = typename C2<R>::T;
     // pass-band columns: |f_u| <= NAout/lambda with the band-pass's own index map
     std::vector<int> cols;
@@ -521,7 +561,7 @@ int detector_band(bimsim_ctx* ctx, const PostOp& bp, void* Uin, void* Ufin, int 
         int B = rows_per_group(vR);
         size_t lds = sizeof(C) * 2 * (size_t)B * uR;
         BIMSIM_TRY(launch_rows_fwd<R>(ctx, uR, dim3((vR + B - 1) / B, 2), lds, (const C*)Uin,
-                                      (const C*)Ufin, vR, B, colsd, Nc, (C*)G, tw_u));
+                                      (const C*)Ufin, vR, B, colsd, Nc, (C*)G, tw_u, fz));
         int Bc = std::max(1, std::min(8, 2048 / vR));
         Bc = std::min(Bc, Nc);
         // a narrow pass band leaves few columns: fewer per workgroup, so
@@ -588,13 +628,12 @@ int run_fft_pass(bimsim_ctx* ctx, typename C2<R>::T* data, int nlines, int n, lo
 template <class R>
 int detector_impl(bimsim_ctx* ctx, void* Uin, void* Ufin, int uR, int vR, double du, double dv,
                   double NAin, double NAout, double lambda, int padding, int Nd, int ss, void* D,
-                  void* Di, void* Abs, int write_filtered, int quirks) {
+                  void* Di, void* Abs, int write_filtered, int quirks, const SumFuse* fz) {
     using C = typename C2<R>::T;
     const size_t npix = (size_t)uR * vR;
     C* U = (C*)Uin;
     C* Uf = (C*)Ufin;
-    if (fixed_size(uR) && fixed_size(vR) &&
-        sizeof(C) * 2 * (size_t)std::max(uR, vR) <= ctx->lds_max) {
+    if (band_path(ctx, sizeof(C), uR, vR)) {
         const double *twd_u, *twd_v;
         const float *twf_u, *twf_v;
         BIMSIM_TRY(twiddles_get(ctx, uR, &twd_u, &twf_u));
@@ -612,8 +651,9 @@ int detector_impl(bimsim_ctx* ctx, void* Uin, void* Ufin, int uR, int vR, double
         const bool dbl = std::is_same<R, double>::value;
         return detector_band<R>(ctx, bp, Uin, Ufin, uR, vR, padding, Nd, ss, D, Di, Abs,
                                 write_filtered, dbl ? (const C*)twd_u : (const C*)twf_u,
-                                dbl ? (const C*)twd_v : (const C*)twf_v);
+                                dbl ? (const C*)twd_v : (const C*)twf_v, fz);
     }
+    if (fz) return set_error(ctx, BIMSIM_E_UNSUPPORTED, "fused sum needs the band-limited detector");
     if (!write_filtered) {
         void *a = nullptr, *b = nullptr;
         BIMSIM_TRY(scratch_get(ctx, kSlotFftA, sizeof(C) * npix, &a));
@@ -691,10 +731,16 @@ int field_fft_impl(bimsim_ctx* ctx, void* Fin, int uR, int vR, int inverse) {
 
 using namespace bimsim;
 
-extern "C" int bimsim_detector(bimsim_ctx* ctx, int prec, int quirks, void* U, void* Uf, int uR, int vR,
-                               double du, double dv, double NAin, double NAout, double lambda,
-                               int padding, int Nd, int ss, void* D, void* Di, void* Abs,
-                               int write_filtered) {
+namespace bimsim {
+
+bool detector_fusable(const bimsim_ctx* ctx, int uR, int vR) {
+    return band_path(ctx, sizeof(float2), uR, vR);
+}
+
+int detector_run(bimsim_ctx* ctx, int prec, int quirks, void* U, void* Uf, int uR, int vR,
+                 double du, double dv, double NAin, double NAout, double lambda, int padding,
+                 int Nd, int ss, void* D, void* Di, void* Abs, int write_filtered,
+                 const SumFuse* fz) {
     if (!ctx) return BIMSIM_E_INVALID;
     BIMSIM_CHECK_ARG(ctx, U && Uf && D && Di, "null pointer argument");
     BIMSIM_CHECK_ARG(ctx, uR > 0 && vR > 0 && Nd > 0 && ss > 0 && padding >= 0, "bad sizes");
@@ -707,13 +753,23 @@ extern "C" int bimsim_detector(bimsim_ctx* ctx, int prec, int quirks, void* U, v
     int rc;
     if (prec == BIMSIM_FP32_FAST)
         rc = detector_impl<float>(ctx, U, Uf, uR, vR, du, dv, NAin, NAout, lambda, padding, Nd, ss,
-                                  D, Di, Abs, write_filtered, quirks);
+                                  D, Di, Abs, write_filtered, quirks, fz);
     else
         rc = detector_impl<double>(ctx, U, Uf, uR, vR, du, dv, NAin, NAout, lambda, padding, Nd,
-                                   ss, D, Di, Abs, write_filtered, quirks);
+                                   ss, D, Di, Abs, write_filtered, quirks, fz);
     if (rc != BIMSIM_OK) return rc;
     timer_end(ctx, tmr);
     return BIMSIM_OK;
+}
+
+}  // namespace bimsim
+
+extern "C" int bimsim_detector(bimsim_ctx* ctx, int prec, int quirks, void* U, void* Uf, int uR, int vR,
+                               double du, double dv, double NAin, double NAout, double lambda,
+                               int padding, int Nd, int ss, void* D, void* Di, void* Abs,
+                               int write_filtered) {
+    return detector_run(ctx, prec, quirks, U, Uf, uR, vR, du, dv, NAin, NAout, lambda, padding, Nd,
+                        ss, D, Di, Abs, write_filtered, nullptr);
 }
 
 extern "C" int bimsim_field_fft(bimsim_ctx* ctx, int prec, void* F, int uR, int vR, int inverse) {

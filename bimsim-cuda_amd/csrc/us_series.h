@@ -67,12 +67,15 @@ struct SeriesBatch {
 };
 
 struct UfInst;
+struct SumInst;
 
 // The other work of an FP32 batch's pre-pass, run in one launch with a
 // harmonic plan's records (series_plan_launch_prepass): the Mie
 // coefficients of n_mie_sph spheres (sphere i at mie_kmag[i]) in blocks of
 // spb, Bessel arrays of wstride in LDS, and the complex64 incident fields
-// of n_uf instances, uf_blocks blocks of 256 pixels each.
+// of n_uf instances, uf_blocks blocks of 256 pixels each; with sum, also
+// the clearing of each instance's detector images in D and Di (sum[i] for
+// instance i), which the batch's sum kernel does otherwise.
 struct PrepassArgs {
     const SphereDev* mie_sph;
     const double* mie_kmag;
@@ -83,6 +86,9 @@ struct PrepassArgs {
     int n_uf, uf_blocks;
     const double* uf_tabs;
     float2* Uf;
+    const SumInst* sum;
+    float* D;
+    float* Di;
 };
 
 // LDS of the pre-pass launch's coefficient blocks, which every block of it

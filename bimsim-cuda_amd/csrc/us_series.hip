@@ -500,7 +500,8 @@ __global__ __launch_bounds__(kRecThreads) void harm_records_kernel(
 // [0, n_mie), mie_block over spb spheres each, Bessel arrays in dynamic
 // LDS), the harmonic records (the next n_rec blocks, one per (m, tile,
 // sphere) as harm_records_kernel) and the incident fields (the rest, 256
-// pixels of one instance each).  None reads what another writes, and the
+// pixels of one instance each, which also clear its detector images when
+// a.sum is given).  None reads what another writes, and the
 // records read only the Legendre tables of the launch before, so the long
 // dependent chains of the coefficients run on a few compute units while
 // the records and fields fill the others.  The role is uniform per block.
@@ -538,6 +539,13 @@ __global__ __launch_bounds__(kRecThreads) void prepass_kernel(
     const int i = b / a.uf_blocks, pix = (b - i * a.uf_blocks) * kRecThreads + t;
     const UfInst& I = a.uf[i];
     if (pix < I.uR * I.vR) ufd::uf_batch_pixel(I, pix, a.uf_tabs, a.Uf);
+    if (a.sum) {
+        const SumInst& S = a.sum[i];   // nd2 <= uR vR
+        if (pix < S.nd2) {
+            a.D[S.image_off + pix] = 0.f;
+            a.Di[S.image_off + pix] = 0.f;
+        }
+    }
 }
 
 // Trigonometric start values of the radial recurrences of one pixel and
