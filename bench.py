@@ -354,8 +354,9 @@ def main():
     log(f"bench: workload cfg2 N={inst['N']} nk={inst['nk']} "
         f"Nl={[s['Nl'] for s in inst['spheres']]} m={inst['m']} terms/image={terms:.3e}")
 
-    # headline: the image as one graph-captured batch of one instance (Mie
-    # coefficients on a side stream beside the incident field and wave tables)
+    # headline: the image as one graph-captured batch of one instance (the
+    # Legendre tables, one launch with the Mie coefficients, records and
+    # incident field, the harmonic sums, the detector)
     arr, spheres, Wv, n_img = P.batch_tables([inst], [waves_h])
     wd = torch.as_tensor(Wv.ravel()).cuda()
     D = torch.empty(n_img, dtype=torch.float32, device="cuda")
