@@ -420,6 +420,43 @@ __global__ __launch_bounds__(kThreads) void rows_inv_kernel(
     }
 }
 
+// rows_inv_kernel on the U and Uf rows of a block together (2 nb rows in
+// one transform), then, for a detector without supersampling (ss = 1) or
+// filtered-field output, detect_from_intensity on those rows: D += |U|^2,
+// Di += |Uf|^2 (|.| squared, as the intensity rows), Abs = -log10(D / Di)
+// over the crop columns [off, off + Nd); the rows are those of the crop.
+template <class R, int N>
+__global__ __launch_bounds__(kThreads) void rows_inv_detect_kernel(
+    const typename C2<R>::T* __restrict__ G, int Nc, int vR, const int* __restrict__ cols,
+    int row_begin, int nrows, int B, int off, int Nd, R* __restrict__ D, R* __restrict__ Di,
+    R* __restrict__ Abs, const typename C2<R>::T* __restrict__ tw) {
+    using C = typename C2<R>::T;
+    extern __shared__ unsigned char smem_raw[];
+    C* a = reinterpret_cast<C*>(smem_raw);
+    C* b = a + (size_t)2 * B * N;
+    const int r0 = row_begin + blockIdx.x * B;
+    const int nb = min(B, row_begin + nrows - r0);
+    for (int t = threadIdx.x; t < 2 * nb * N; t += kThreads) a[t] = cmk<C>((R)0, (R)0);
+    __syncthreads();
+    for (int t = threadIdx.x; t < 2 * nb * Nc; t += kThreads) {
+        const int f = t / (nb * Nc), u = t - f * (nb * Nc);
+        const int g = u / Nc, c = u - g * Nc;
+        a[(f * nb + g) * N + cols[c]] = G[((size_t)f * Nc + c) * vR + r0 + g];
+    }
+    __syncthreads();
+    const C* res = fixed_fft<R, N>(a, b, 2 * nb, tw, true);
+    for (int t = threadIdx.x; t < nb * Nd; t += kThreads) {
+        const int g = t / Nd, iu = t - g * Nd;
+        const C v = res[g * N + off + iu], w = res[(nb + g) * N + off + iu];
+        const R m = sqrt(v.x * v.x + v.y * v.y), mf = sqrt(w.x * w.x + w.y * w.y);
+        const size_t i = (size_t)(r0 - row_begin + g) * Nd + iu;
+        const R d = D[i] + m * m, di = Di[i] + mf * mf;
+        D[i] = d;
+        Di[i] = di;
+        if (Abs) Abs[i] = -log10(d / di);
+    }
+}
+
 template <class R>
 __global__ void fill_zero(R* p, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -526,6 +563,25 @@ int launch_rows_inv(bimsim_ctx* ctx, int N, dim3 grid, size_t lds, const typenam
     return BIMSIM_OK;
 }
 
+template <class R>
+int launch_rows_inv_detect(bimsim_ctx* ctx, int N, dim3 grid, size_t lds,
+                           const typename C2<R>::T* G, int Nc, int vR, const int* cols,
+                           int row_begin, int nrows, int B, int off, int Nd, R* D, R* Di, R* Abs,
+                           const typename C2<R>::T* tw) {
+    switch (N) {
+#define BIMSIM_X(NN)                                                                       \
+    case NN:                                                                               \
+        hipLaunchKernelGGL((rows_inv_detect_kernel<R, NN>), grid, dim3(kThreads), lds,     \
+                           ctx->stream, G, Nc, vR, cols, row_begin, nrows, B, off, Nd, D, \
+                           Di, Abs, tw);                                                   \
+        break;
+        BIMSIM_FFT_SIZES(BIMSIM_X)
+#undef BIMSIM_X
+    }
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return BIMSIM_OK;
+}
+
 // Band-limited detector for FFT lengths with a fixed plan.
 template <class R>
 int detector_band(bimsim_ctx* ctx, const PostOp& bp, void* Uin, void* Ufin, int uR, int vR,
@@ -573,6 +629,12 @@ int detector_band(bimsim_ctx* ctx, const PostOp& bp, void* Uin, void* Ufin, int 
     }
     int B = rows_per_group(nrows);
     size_t lds = sizeof(C) * 2 * (size_t)B * uR;
+    if (Nc > 0 && ss == 1 && !write_filtered && 2 * lds <= ctx->lds_max) {
+        // both fields' rows per workgroup, detection on them in the same pass
+        return launch_rows_inv_detect<R>(ctx, uR, dim3((nrows + B - 1) / B), 2 * lds, (const C*)G,
+                                         Nc, vR, colsd, row_begin, nrows, B, off, Nd, (R*)D,
+                                         (R*)Di, (R*)Abs, tw_u);
+    }
     if (Nc > 0) {
         BIMSIM_TRY(launch_rows_inv<R>(ctx, uR, dim3((nrows + B - 1) / B, 2), lds, (const C*)G, Nc,
                                       vR, colsd, row_begin, nrows, B, (R*)I,
