@@ -4,8 +4,9 @@ MI355X, batched FP32 form vs the plain FP64 form.
 A step is one absorbance image of the workload (incident field, FP64 Mie
 coefficients, scattered field of every sphere under every Monte-Carlo wave,
 Uf + Us, detector FFT / band-pass / intensity / -log10) with inputs resident
-in device memory, run as a graph-captured batch of one instance
-(bimsim_batch_launch).  The workload is the 3-sphere, 1000-wave, 512 x 512
+in device memory, run as a batch of one instance (bimsim_batch_launch), its
+kernels enqueued on one stream per step; the same batch replayed as a
+captured HIP graph is timed beside it.  The workload is the 3-sphere, 1000-wave, 512 x 512
 Monte-Carlo focusing configuration (cfg2 in bimsim-cuda_amd/workloads.py).
 The same image through the stepwise entry points (eager launches) and in
 the plain FP64 form are reported beside it.
@@ -23,7 +24,7 @@ and absorbance error) and the 1024-instance ragged batch ("cfg5_batch"),
 each as one graph-captured FP32 batch.  --no-spectral skips all of these
 reported-beside sections.
 
-usage: python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--no-spectral]
+usage: python bench.py [--gpus 1] [--steps 200] [--warmup 20] [--no-cpu] [--no-spectral]
 Prints one JSON line.
 """
 import argparse
@@ -207,7 +208,7 @@ def time_batch(b, torch, steps):
     return (time.perf_counter() - t0) / steps
 
 
-def large_workloads(pkg, P, W, ctx, torch, steps=5):
+def large_workloads(pkg, P, W, ctx, torch, steps=20):
     """The 50-sphere 1024 x 1024 image (cfg3) and the 1024-instance ragged
     batch (cfg5), each as one graph-captured FP32 batch; cfg3 also as an FP64
     batch pass for its absorbance error."""
@@ -251,7 +252,7 @@ def large_workloads(pkg, P, W, ctx, torch, steps=5):
     return out
 
 
-def spectral(pkg, P, W, ctx, torch, stride=16, steps=3, fp64_images=2):
+def spectral(pkg, P, W, ctx, torch, stride=16, steps=20, fp64_images=2):
     """Batched FP32 spectral imaging (one captured batch) against the plain
     FP64 form, images/s and absorbance error on the same instances."""
     insts = W.config4(stride=stride)
@@ -307,8 +308,8 @@ def spectral(pkg, P, W, ctx, torch, stride=16, steps=3, fp64_images=2):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-spectral", action="store_true")
@@ -354,15 +355,19 @@ def main():
     log(f"bench: workload cfg2 N={inst['N']} nk={inst['nk']} "
         f"Nl={[s['Nl'] for s in inst['spheres']]} m={inst['m']} terms/image={terms:.3e}")
 
-    # headline: the image as one graph-captured batch of one instance (the
-    # Legendre tables, one launch with the Mie coefficients, records and
-    # incident field, the harmonic sums, the detector)
+    # headline: the image as a batch of one instance (the Legendre tables,
+    # one launch with the Mie coefficients, records and incident field, the
+    # harmonic sums, the detector), its kernels enqueued on one stream per
+    # step: the host enqueues an image in ~0.02 ms, well inside its GPU
+    # time, and back-to-back steps then run without gaps; a captured graph
+    # of the same batch replays with a few microseconds between images and
+    # is timed beside it
     arr, spheres, Wv, n_img = P.batch_tables([inst], [waves_h])
     wd = torch.as_tensor(Wv.ravel()).cuda()
     D = torch.empty(n_img, dtype=torch.float32, device="cuda")
     Di = torch.empty_like(D)
     Ab = torch.empty_like(D)
-    batch = ctx.batch(arr, spheres, wd, D, Di, Ab, prec=pkg.FP32_FAST, use_graph=True)
+    batch = ctx.batch(arr, spheres, wd, D, Di, Ab, prec=pkg.FP32_FAST, use_graph=False)
     for _ in range(args.warmup):
         batch.launch()
     torch.cuda.synchronize()
@@ -372,7 +377,13 @@ def main():
         batch.launch()
     torch.cuda.synchronize()
     t32 = (time.perf_counter() - t0) / args.steps
-    log(f"bench: FP32 graph batch {t32 * 1e3:.3f} ms/image")
+    log(f"bench: FP32 batch {t32 * 1e3:.3f} ms/image")
+    Dg, Dig, Abg = torch.empty_like(D), torch.empty_like(D), torch.empty_like(D)
+    gbatch = ctx.batch(arr, spheres, wd, Dg, Dig, Abg, prec=pkg.FP32_FAST, use_graph=True)
+    t_graph = time_batch(gbatch, torch, args.steps)
+    graph_same = bool(torch.equal(Ab, Abg))
+    gbatch.close()
+    log(f"bench: FP32 graph batch {t_graph * 1e3:.3f} ms/image (same image: {graph_same})")
 
     # the same image through the stepwise entry points (eager launches)
     for _ in range(args.warmup):
@@ -445,12 +456,14 @@ def main():
             "workload": "cfg2: 3 spheres a~U(2,8) um, n~U(1.3,1.6)+iU(0,0.1), 1000 Monte-Carlo "
                         "waves over condenser NA 0.2-0.62, lambda 3 um, 512x512 slice over "
                         "60 um, objective NA 0.62, Legendre incident field; one image per step "
-                        "as a graph-captured batch of one instance",
+                        "as a batch of one instance enqueued on one stream",
             "N": inst["N"], "nk": inst["nk"], "spheres": len(inst["spheres"]),
             "Nl": [s["Nl"] for s in inst["spheres"]], "m": inst["m"],
             "terms_per_image": terms,
         },
         "images_per_s": 1.0 / t32,
+        "graph_batch": {"ms_per_step": t_graph * 1e3, "images_per_s": 1.0 / t_graph,
+                        "same_image": graph_same},
         "stepwise_fp32": {"ms_per_step": t_step * 1e3, "terms_per_s": terms / t_step,
                           "absorbance_max_abs_err": abs_err_step},
         "fp64_plain": {"terms_per_s": terms / t64, "images_per_s": 1.0 / t64,
