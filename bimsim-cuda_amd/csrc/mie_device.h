@@ -63,8 +63,8 @@ __device__ inline void mie_order(const SphereDev& S, int l, const zc* w, int wst
 // of kna).  Wave r < 3 computes part r of sphere (lane): J of ka, Y of ka,
 // J of kna, as sph_jy_cplx (cbessjyva_sph) with the array clearing left out
 // (every order up to Nl is written on every path).  The start orders of the
-// Miller recurrences depend only on |z| and Nl, so wave 3 computes them for
-// every sphere and argument while waves 0-2 evaluate the start values; the
+// Miller recurrences depend only on |z| and Nl, so wave 3 computes them, one
+// secant search per lane, while waves 0-2 evaluate the start values; the
 // parts then finish with them.  Every thread of the block (at least
 // kMieThreads) then takes (sphere, order) pairs of the coefficient formula.
 __device__ __forceinline__ void mie_block(const SphereDev* sh, const double* kh, int cnt,
@@ -86,10 +86,15 @@ __device__ __forceinline__ void mie_block(const SphereDev* sh, const double* kh,
         st = zj::cjyva_start(sh[s].Nl + 0.5, z, o, o + wstride, o + 2 * wstride, o + 3 * wstride,
                              quirks != 0, want_y, want_j);
     } else if (role == kLanes) {
-        for (int q = s; q < 2 * cnt; q += kWave) {
-            const int i = q >> 1, arg = q & 1;
+        // one search per lane: (sphere, argument, m1 or m2)
+        for (int q = s; q < 4 * cnt; q += kWave) {
+            const int i = q >> 2, arg = (q >> 1) & 1, which = q & 1;
             const double v = sh[i].Nl + 0.5;
-            ord[i][arg] = zj::cjyva_orders(zj::zabs(mie_arg(sh[i], kh[i], 2 * arg)), (int)v);
+            const int m = zj::cjyva_order(zj::zabs(mie_arg(sh[i], kh[i], 2 * arg)), (int)v, which);
+            if (which)
+                ord[i][arg].m2 = m;
+            else
+                ord[i][arg].m1 = m;
         }
     }
     __syncthreads();

@@ -191,6 +191,29 @@ struct CjyvaOrders {
 __device__ inline CjyvaOrders cjyva_orders(double a0, int n) {
     return CjyvaOrders{msta1(a0, 200), msta2(a0, n, 15)};
 }
+// One of the two (which = 0: m1, 1: m2) with the same secant code for both,
+// so that lanes computing different ones stay together: msta1 and msta2
+// differ only in the start and target of the search.
+__device__ inline int cjyva_order(double a0, int n, int which) {
+    int n0;
+    double target;
+    if (which == 0) {
+        n0 = (int)(1.1 * a0) + 1;
+        target = 200.0;
+    } else {
+        const double half = 0.5 * 15;
+        const double ejn = envelope(n, a0);
+        if (ejn <= half) {
+            target = 15;
+            n0 = (int)(1.1 * a0);
+            if (n0 < 1) n0 = 1;
+        } else {
+            target = half + ejn;
+            n0 = n;
+        }
+    }
+    return secant(n0, a0, target) + (which ? 10 : 0);
+}
 
 // CJYVA up to its recurrences: J, Y of orders v0 and v0 + 1 (done: |z| is
 // below 1e-100 and every array is written).
