@@ -25,9 +25,9 @@ int scratch_get(bimsim_ctx* ctx, int slot, size_t bytes, void** out) {
     auto& s = ctx->scratch[slot];
     if (s.second < bytes) {
         if (s.first) {
-            // the previous user of the slot may still be in flight
-            BIMSIM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-            BIMSIM_HIP_CHECK(ctx, hipFree(s.first));
+            // kernels in flight and captured batch graphs may still use the
+            // old buffer: it is freed with the context
+            ctx->scratch_retired.push_back(s.first);
             s.first = nullptr;
             s.second = 0;
         }
@@ -173,6 +173,7 @@ int bimsim_ctx_destroy(bimsim_ctx* ctx) {
     series_cache_release(ctx);
     for (auto& kv : ctx->scratch)
         if (kv.second.first) (void)hipFree(kv.second.first);
+    for (void* p : ctx->scratch_retired) (void)hipFree(p);
     for (auto& kv : ctx->int_tables) (void)hipFree(kv.second);
     for (auto& kv : ctx->dbl_tables) (void)hipFree(kv.second);
     for (auto& kv : ctx->twiddles) {

@@ -26,8 +26,11 @@ struct bimsim_ctx {
     };
     std::vector<Span> spans;
     std::vector<hipEvent_t> event_pool;
-    // grow-only device scratch, split into named slots
+    // grow-only device scratch, split into named slots; a slot's outgrown
+    // buffers are kept until the context ends, since a captured batch graph
+    // may still hold them
     std::map<int, std::pair<void*, size_t>> scratch;
+    std::vector<void*> scratch_retired;
     // FFT twiddle tables per length: e^{-2 pi i t/N}, t = 0..N-1
     std::map<int, std::pair<double*, float*>> twiddles;
     // device copies of small host tables, keyed by content (e.g. the
@@ -61,7 +64,8 @@ enum ScratchSlot {
 
 int set_error(bimsim_ctx* ctx, int code, const char* fmt, ...);
 
-// Device scratch of at least `bytes` in `slot` (grown with hipMalloc).
+// Device scratch of at least `bytes` in `slot` (grown with hipMalloc; an
+// outgrown buffer stays allocated until the context ends).
 int scratch_get(bimsim_ctx* ctx, int slot, size_t bytes, void** out);
 
 // Device copy of an int table (cached by content; uploaded once).

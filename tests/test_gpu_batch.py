@@ -147,6 +147,28 @@ def test_graph_replays_and_reads_current_inputs(pkg, ctx, W):
     b2.close()
 
 
+def test_graph_replays_after_the_scratch_grows(pkg, W):
+    """A captured batch keeps its image after a larger call on the same
+    context grows the context's detector scratch (outgrown buffers stay
+    allocated until the context ends)."""
+    c = pkg.Context(0)
+    try:
+        insts = [W.config(2, N=32, nk=16)]
+        waves = waves_of(pkg, W, insts)
+        b, D, Di, Ab, _ = run_batch(pkg, c, insts, waves, pkg.FP32_FAST, use_graph=True)
+        first = host(D).copy()
+        big = W.config(2, N=256, nk=16)
+        single(pkg, c, big, waves_of(pkg, W, [big])[0], pkg.FP32_FAST)
+        for _ in range(2):
+            D.zero_()
+            Di.zero_()
+            b.launch()
+            assert np.array_equal(host(D), first)
+        b.close()
+    finally:
+        c.close()
+
+
 def oracle_image(orc, inst, waves, fp32=0):
     import bimsim_amd.pipeline as P
     q = P.quad_of(inst)
