@@ -40,6 +40,7 @@ struct SeriesInst {
     long long part_off; // element offset of its split partial sums
     long long pl_off;   // harmonic form: its normalised Legendre table
     long long e_off;    // harmonic form: its azimuth phase table
+    long long perm_off; // harmonic form: its pixel order in the plan's permutation, or -1
 };
 
 struct SeriesSph {
@@ -135,6 +136,7 @@ struct SeriesPlan {
     double2* d_phi = nullptr;     // wave weights (1/nk) subA exp(i k k.c) per sphere
     float4* d_rec = nullptr;      // harmonic records per (sphere, m, l)
     float* d_cst = nullptr;       // scaled Legendre recurrence constants per (m, l)
+    int* d_perm = nullptr;        // pixel orders of the instances with perm_off >= 0
     double* d_h64 = nullptr;      // FP64 Legendre constants and scales of the wave sums
 };
 

@@ -390,6 +390,10 @@ __global__ void harm_legendre_kernel(const SeriesInst* __restrict__ insts,
 // (U1 = T+, U2 = 0 at m = 0), zero outside m <= l <= Nl.
 // Grid: (m < max bucket, tiles, spheres).
 constexpr int kRecThreads = 256;
+// Limits of the harmonic form's per-instance pixel orders: pixels in the
+// plan and pixel-sphere tests on the host to build them.
+constexpr long long kPermMaxPixels = 16LL << 20;
+constexpr long long kPermMaxTests = 128LL << 20;
 constexpr int kRecTile = 4;
 
 __device__ __forceinline__ void harm_records_block(
@@ -725,7 +729,7 @@ __global__ __launch_bounds__(kThreads, harm_waves(L)) void series_harm_kernel(
     const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph,
     const SeriesItem* __restrict__ items, int n_items, const double2* __restrict__ A,
     const double2* __restrict__ B, const float4* __restrict__ rec, const float* __restrict__ cst,
-    float2* __restrict__ U, int* __restrict__ counter) {
+    const int* __restrict__ perm, float2* __restrict__ U, int* __restrict__ counter) {
     __shared__ int s_item;
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
@@ -736,8 +740,11 @@ __global__ __launch_bounds__(kThreads, harm_waves(L)) void series_harm_kernel(
         const SeriesItem item = items[it];
         const SeriesInst& I = insts[item.inst];
         const int npix = I.uR * I.vR;
-        const int pix = item.pix0 + (int)threadIdx.x;
-        if (pix >= npix) continue;
+        const int j = item.pix0 + (int)threadIdx.x;
+        if (j >= npix) continue;
+        // pixels in the instance's order of uniform waves (outside every
+        // sphere, then inside each), so that a wave takes one radial path
+        const int pix = I.perm_off >= 0 ? perm[I.perm_off + j] : j;
         const int iv = pix / I.uR, iu = pix - iv * I.uR;
         const d3 p = quad_at(I.q, (double)iu / (double)I.uR, (double)iv / (double)I.vR);
         float acc_r = 0.f, acc_i = 0.f;
@@ -856,11 +863,11 @@ void launch_harm(dim3 grid, hipStream_t st, const SeriesPlan& P, const SeriesPla
     if (K.planar)
         hipLaunchKernelGGL((series_harm_kernel<L, true>), grid, dim3(kThreads), 0, st, P.d_inst,
                            P.d_sph, P.d_items + K.item_begin, K.item_count, A, B, P.d_rec,
-                           P.d_cst, U, P.d_counters + K.counter);
+                           P.d_cst, P.d_perm, U, P.d_counters + K.counter);
     else
         hipLaunchKernelGGL((series_harm_kernel<L, false>), grid, dim3(kThreads), 0, st, P.d_inst,
                            P.d_sph, P.d_items + K.item_begin, K.item_count, A, B, P.d_rec,
-                           P.d_cst, U, P.d_counters + K.counter);
+                           P.d_cst, P.d_perm, U, P.d_counters + K.counter);
 }
 
 template <class T>
@@ -897,6 +904,7 @@ void series_plan_destroy(SeriesPlan* P) {
     (void)hipFree(P->d_inst);
     (void)hipFree(P->d_sph);
     (void)hipFree(P->d_items);
+    (void)hipFree(P->d_perm);
     (void)hipFree(P->d_split_insts);
     (void)hipFree(P->d_counters);
     (void)hipFree(P->d_wt);
@@ -1108,9 +1116,56 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
             }
         }
     }
+    // harmonic form: the pixels of each instance with pixels inside a sphere
+    // in the order outside every sphere, then inside each sphere (stable
+    // within a class), so that the pixel kernel's waves mostly take one of
+    // the two radial paths (a wave across a sphere's edge takes both); the
+    // order changes no value, each pixel being computed on its own.  Left
+    // out for very large plans (host time and table size).
+    std::vector<int> perm;
+    for (SeriesInst& I : batch.inst) I.perm_off = -1;
+    if (P->harmonic) {
+        long long npx = 0, tests = 0;
+        for (const SeriesInst& I : batch.inst) {
+            npx += (long long)I.uR * I.vR;
+            tests += (long long)I.uR * I.vR * I.ns;
+        }
+        if (npx <= kPermMaxPixels && tests <= kPermMaxTests) {
+            std::vector<int> cls, start;
+            for (SeriesInst& I : batch.inst) {
+                const int npix = I.uR * I.vR;
+                cls.assign(npix, 0);
+                bool any = false;
+                for (int pix = 0; pix < npix; ++pix) {
+                    const int iv = pix / I.uR, iu = pix - iv * I.uR;
+                    const d3 p = quad_at(I.q, (double)iu / (double)I.uR, (double)iv / (double)I.vR);
+                    for (int t = 0; t < I.ns; ++t) {
+                        const SphereDev& S = batch.sph[I.s_begin + t].s;
+                        if (len(sub(p, mk(S.p))) <= S.a) {
+                            cls[pix] = t + 1;
+                            any = true;
+                            break;
+                        }
+                    }
+                }
+                if (!any) continue;
+                I.perm_off = (long long)perm.size();
+                start.assign(I.ns + 2, 0);
+                for (int c : cls) ++start[c + 1];
+                for (int c = 1; c <= I.ns + 1; ++c) start[c] += start[c - 1];
+                perm.resize(perm.size() + npix);
+                int* o = perm.data() + I.perm_off;
+                for (int pix = 0; pix < npix; ++pix) o[start[cls[pix]]++] = pix;
+            }
+        }
+    }
     BIMSIM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     BIMSIM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     int rc;
+    if (!perm.empty() && (rc = upload(ctx, &P->d_perm, perm)) != BIMSIM_OK) {
+        series_plan_destroy(P);
+        return rc;
+    }
     if (P->harmonic &&
         ((rc = upload(ctx, &P->d_cst, cst)) != BIMSIM_OK ||
          (rc = upload(ctx, &P->d_h64, h64)) != BIMSIM_OK ||
