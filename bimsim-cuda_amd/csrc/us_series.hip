@@ -742,8 +742,9 @@ __global__ __launch_bounds__(kThreads, harm_waves(L)) void series_harm_kernel(
         const int npix = I.uR * I.vR;
         const int j = item.pix0 + (int)threadIdx.x;
         if (j >= npix) continue;
-        // pixels in the instance's order of uniform waves (outside every
-        // sphere, then inside each), so that a wave takes one radial path
+        // pixels in the instance's order of uniform waves (inside each
+        // sphere, then outside every sphere), so that a wave takes one
+        // radial path
         const int pix = I.perm_off >= 0 ? perm[I.perm_off + j] : j;
         const int iv = pix / I.uR, iu = pix - iv * I.uR;
         const d3 p = quad_at(I.q, (double)iu / (double)I.uR, (double)iv / (double)I.vR);
@@ -1117,7 +1118,7 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
         }
     }
     // harmonic form: the pixels of each instance with pixels inside a sphere
-    // in the order outside every sphere, then inside each sphere (stable
+    // in the order inside each sphere, then outside every sphere (stable
     // within a class), so that the pixel kernel's waves mostly take one of
     // the two radial paths (a wave across a sphere's edge takes both); the
     // order changes no value, each pixel being computed on its own.  Left
@@ -1149,6 +1150,9 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
                     }
                 }
                 if (!any) continue;
+                // the inside classes (slower waves) first, so that the
+                // work queue starts them first; the outside class last
+                for (int& c : cls) c = c == 0 ? I.ns : c - 1;
                 I.perm_off = (long long)perm.size();
                 start.assign(I.ns + 2, 0);
                 for (int c : cls) ++start[c + 1];
