@@ -18,7 +18,7 @@ c_int, c_double, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
 OK, E_INVALID, E_HIP, E_NOMEM, E_UNSUPPORTED, E_NODEVICE = 0, -1, -2, -3, -4, -5
 FP64_PARITY, FP64, FP32_FAST = 0, 1, 2
 QUIRKS_OFF, QUIRKS_REF = 0, 1
-UF_SERIES, UF_PLANE, UF_LUT = 0, 1, 2
+UF_SERIES, UF_PLANE, UF_LUT, UF_MC = 0, 1, 2, 3
 SLICE_INTENSITY, SLICE_MAGNITUDE, SLICE_REAL, SLICE_IMAG = 0, 1, 2, 3
 RATIO_ABSORBANCE, RATIO_TRANSMITTANCE = 0, 1
 SERIES_AUTO, SERIES_DIRECT, SERIES_HARMONIC = 0, 1, 2
@@ -37,7 +37,7 @@ EXPORTS = (
     "bimsim_detector_resample", "bimsim_slice_ratio", "bimsim_batch_create",
     "bimsim_batch_launch", "bimsim_batch_destroy", "bimsim_batch_run", "bimsim_set_autotune",
     "bimsim_set_series_algorithm", "bimsim_detector_rotate_add", "bimsim_detector_ring_sums",
-    "bimsim_detector_from_line", "bimsim_field_backpropagate",
+    "bimsim_detector_from_line", "bimsim_field_backpropagate", "bimsim_nf_uf_mc",
 )
 
 
@@ -72,7 +72,7 @@ class Instance(ctypes.Structure):
     _fields_ = [("pos", Quad), ("N", c_int), ("m", c_int), ("focus", c_double * 3),
                 ("kdir", c_double * 3), ("lambda_", c_double), ("amp", c_double),
                 ("condenser", c_double * 2), ("objective", c_double * 2),
-                ("plane_wave", c_int), ("sphere_begin", c_int), ("ns", c_int),
+                ("uf_mode", c_int), ("sphere_begin", c_int), ("ns", c_int),
                 ("wave_begin", c_int), ("nk", c_int), ("padding", c_int), ("Nd", c_int),
                 ("ss", c_int), ("image_off", ctypes.c_longlong)]
 
@@ -118,6 +118,9 @@ def lib():
         L.bimsim_nf_uf.argtypes = [c_void_p, ctypes.POINTER(Quad), c_int, c_int, c_double * 3,
                                    c_double * 3, c_double, c_double, c_double, c_double, c_int,
                                    c_int, c_int, c_int, c_void_p]
+        L.bimsim_nf_uf_mc.argtypes = [c_void_p, ctypes.POINTER(Quad), c_int, c_int,
+                                      c_double * 3, c_double, c_void_p, c_int, c_double, c_int,
+                                      c_void_p]
         L.bimsim_nf_us.argtypes = [c_void_p, ctypes.POINTER(Quad), c_int, c_int, c_double * 3,
                                    c_double, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int,
                                    c_double, c_int, c_int, c_void_p, c_int]
@@ -207,6 +210,57 @@ def _ptr(t):
     return c_void_p(t.data_ptr()) if t is not None else None
 
 
+def _dtypes(prec):
+    """(complex dtype, real dtype) of fields and images at precision `prec`."""
+    import torch
+    if prec == FP32_FAST:
+        return torch.complex64, torch.float32
+    if prec in (FP64_PARITY, FP64):
+        return torch.complex128, torch.float64
+    raise ValueError(f"unknown precision {prec}")
+
+
+def check_tensor(t, dtype, n, name, optional=False):
+    """Raise ValueError unless `t` is a contiguous device tensor of `dtype`
+    with at least `n` elements (None passes when `optional`)."""
+    import torch
+    if t is None:
+        if optional:
+            return
+        raise ValueError(f"{name} is required")
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name} must be a torch tensor")
+    if t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype} (got {t.dtype})")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if t.numel() < n:
+        raise ValueError(f"{name} holds {t.numel()} elements, the call needs {n}")
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be on the GPU (got {t.device})")
+
+
+def check_field(t, prec, n, name, optional=False):
+    check_tensor(t, _dtypes(prec)[0], n, name, optional)
+
+
+def check_real(t, prec, n, name, optional=False):
+    check_tensor(t, _dtypes(prec)[1], n, name, optional)
+
+
+def coeff_extent(spheres):
+    """Elements of the packed A_l / B_l arrays the spheres address."""
+    return max((s.get("coeff_off", 0) + s["Nl"] + 1 for s in spheres), default=0)
+
+
+def check_waves(waves, name="waves"):
+    """Monte-Carlo directions: a float64 device tensor of rows (x, y, z)."""
+    import torch
+    check_tensor(waves, torch.float64, 3, name)
+    if waves.numel() % 3:
+        raise ValueError(f"{name} must hold rows of 3 doubles (got {waves.numel()} values)")
+
+
 class Context:
     """A bimsim_ctx bound to a device and a stream (torch's current stream by default)."""
 
@@ -259,20 +313,41 @@ class Context:
         self._check(self.L.bimsim_get_timings(self.h, ms, n))
         return {s: (ms[i], n[i]) for i, s in enumerate(self.STAGES)}
 
-    # ---- entry points (device tensors)
+    # ---- entry points (device tensors; shapes, dtypes and sizes are
+    # checked here, so that a mismatched buffer raises ValueError instead of
+    # reaching the device)
     def mie_coeffs(self, spheres, kmag, A, B, quirks=QUIRKS_REF):
+        import torch
+        n = coeff_extent(spheres)
+        check_tensor(A, torch.complex128, n, "A")
+        check_tensor(B, torch.complex128, n, "B")
         S = sphere_array(spheres)
         self._check(self.L.bimsim_mie_coeffs(self.h, S, len(spheres), kmag, quirks, _ptr(A),
                                              _ptr(B)))
 
     def nf_uf(self, q, uR, vR, focus, kdir, kmag, amp, ca, cb, m, Uf, mode=UF_SERIES,
               prec=FP64_PARITY, quirks=QUIRKS_REF):
+        check_field(Uf, prec, uR * vR, "Uf")
         self._check(self.L.bimsim_nf_uf(self.h, ctypes.byref(q), uR, vR, (c_double * 3)(*focus),
                                         (c_double * 3)(*kdir), kmag, amp, ca, cb, m, mode, prec,
                                         quirks, _ptr(Uf)))
 
+    def nf_uf_mc(self, q, uR, vR, focus, kmag, waves, subA, Uf, prec=FP64_PARITY):
+        """Incident field as the Monte-Carlo sum of the plane waves `waves`."""
+        check_waves(waves)
+        check_field(Uf, prec, uR * vR, "Uf")
+        self._check(self.L.bimsim_nf_uf_mc(self.h, ctypes.byref(q), uR, vR,
+                                           (c_double * 3)(*focus), kmag, _ptr(waves),
+                                           waves.numel() // 3, subA, prec, _ptr(Uf)))
+
     def nf_us(self, q, uR, vR, focus, kmag, spheres, A, B, waves, subA, U, prec=FP64_PARITY,
               quirks=QUIRKS_REF, accumulate=False):
+        import torch
+        n = coeff_extent(spheres)
+        check_tensor(A, torch.complex128, n, "A")
+        check_tensor(B, torch.complex128, n, "B")
+        check_waves(waves)
+        check_field(U, prec, uR * vR, "U")
         S = sphere_array(spheres)
         nk = waves.numel() // 3
         self._check(self.L.bimsim_nf_us(self.h, ctypes.byref(q), uR, vR, (c_double * 3)(*focus),
@@ -280,27 +355,42 @@ class Context:
                                         subA, prec, quirks, _ptr(U), 1 if accumulate else 0))
 
     def nf_sum_uf(self, q, uR, vR, spheres, Uf, U, prec=FP64_PARITY):
+        check_field(Uf, prec, uR * vR, "Uf")
+        check_field(U, prec, uR * vR, "U")
         S = sphere_array(spheres)
         self._check(self.L.bimsim_nf_sum_uf(self.h, ctypes.byref(q), uR, vR, S, len(spheres),
                                             prec, _ptr(Uf), _ptr(U)))
 
     def uf_bessel_lut(self, m, kmag, dmin, dmax, dR, lut, prec=FP64_PARITY):
+        check_real(lut, prec, dR * (m + 1), "lut")
         self._check(self.L.bimsim_uf_bessel_lut(self.h, m, kmag, dmin, dmax, dR, prec, _ptr(lut)))
 
     def nf_uf_lut(self, q, uR, vR, focus, kdir, amp, ca, cb, m, lut, dR, dmin, dmax, Uf,
                   prec=FP64_PARITY):
+        check_real(lut, prec, dR * (m + 1), "lut")
+        check_field(Uf, prec, uR * vR, "Uf")
         self._check(self.L.bimsim_nf_uf_lut(self.h, ctypes.byref(q), uR, vR,
                                             (c_double * 3)(*focus), (c_double * 3)(*kdir), amp,
                                             ca, cb, m, _ptr(lut), dR, dmin, dmax, prec, _ptr(Uf)))
 
     def sphere_lut(self, sphere, A, B, kmag, dmin, dmax, dR, aR, thetaR, Usp, Uip,
                    prec=FP64_PARITY):
+        import torch
+        n = coeff_extent([sphere])
+        check_tensor(A, torch.complex128, n, "A")
+        check_tensor(B, torch.complex128, n, "B")
+        check_field(Usp, prec, thetaR * dR, "Usp")
+        check_field(Uip, prec, thetaR * aR, "Uip", optional=True)
         S = sphere_array([sphere])
         self._check(self.L.bimsim_sphere_lut(self.h, S, _ptr(A), _ptr(B), kmag, dmin, dmax, dR,
                                              aR, thetaR, prec, _ptr(Usp), _ptr(Uip)))
 
     def nf_us_lut(self, q, uR, vR, focus, kmag, sphere, Usp, dR, Uip, aR, thetaR, dmin, dmax,
                   waves, subA, U, prec=FP64_PARITY):
+        check_field(Usp, prec, thetaR * dR, "Usp")
+        check_field(Uip, prec, thetaR * aR, "Uip", optional=True)
+        check_waves(waves)
+        check_field(U, prec, uR * vR, "U")
         S = sphere_array([sphere])
         nk = waves.numel() // 3
         self._check(self.L.bimsim_nf_us_lut(self.h, ctypes.byref(q), uR, vR,
@@ -309,66 +399,111 @@ class Context:
                                             subA, prec, _ptr(U)))
 
     def nf_interpolate(self, deg, uR, vR, U_src, Uf_src, U, Uf, prec=FP64_PARITY):
+        n = uR * vR
+        if (U_src is None) != (U is None) or (Uf_src is None) != (Uf is None):
+            raise ValueError("each field to rotate needs both its source and its output")
+        for t, name in ((U_src, "U_src"), (Uf_src, "Uf_src"), (U, "U"), (Uf, "Uf")):
+            check_field(t, prec, n, name, optional=True)
         self._check(self.L.bimsim_nf_interpolate(self.h, prec, deg, uR, vR, _ptr(U_src),
                                                  _ptr(Uf_src), _ptr(U), _ptr(Uf)))
 
     def sphere_radial_lut(self, sphere, kmag, dmin, dmax, dR, aR, h, j):
+        import torch
+        nl1 = sphere["Nl"] + 1
+        check_tensor(h, torch.complex128, dR * nl1, "h", optional=True)
+        check_tensor(j, torch.complex128, aR * nl1, "j", optional=True)
         S = sphere_array([sphere])
         self._check(self.L.bimsim_sphere_radial_lut(self.h, S, kmag, dmin, dmax, dR, aR, _ptr(h),
                                                     _ptr(j)))
 
     def field_fft(self, F, uR, vR, inverse=False, prec=FP64_PARITY):
+        check_field(F, prec, uR * vR, "F")
         self._check(self.L.bimsim_field_fft(self.h, prec, _ptr(F), uR, vR, 1 if inverse else 0))
 
     def field_bandpass(self, F, uR, vR, du, dv, NAin, NAout, lam, prec=FP64_PARITY,
                        quirks=QUIRKS_REF):
+        check_field(F, prec, uR * vR, "F")
         self._check(self.L.bimsim_field_bandpass(self.h, prec, quirks, _ptr(F), uR, vR, du, dv,
                                                  NAin, NAout, lam))
 
     def field_to_scalar(self, F, op, S, prec=FP64_PARITY):
+        check_field(F, prec, 1, "F")
+        check_real(S, prec, F.numel(), "S")
         self._check(self.L.bimsim_field_to_scalar(self.h, prec, op, _ptr(F), F.numel(), _ptr(S)))
 
     def field_scale(self, F, v, prec=FP64_PARITY):
+        check_field(F, prec, 1, "F")
         self._check(self.L.bimsim_field_scale(self.h, prec, _ptr(F), F.numel(), v))
 
     def field_crop(self, src, uR, vR, u, v, su, sv, dst, prec=FP64_PARITY):
+        check_field(src, prec, uR * vR, "src")
+        check_field(dst, prec, su * sv, "dst")
         self._check(self.L.bimsim_field_crop(self.h, prec, _ptr(src), uR, vR, u, v, su, sv,
                                              _ptr(dst)))
 
     def detector_resample(self, F, Du, Dv, ss, scale, accumulate, D, prec=FP64_PARITY):
+        check_field(F, prec, Du * ss * Dv * ss, "F")
+        check_real(D, prec, Du * Dv, "D")
         self._check(self.L.bimsim_detector_resample(self.h, prec, _ptr(F), Du, Dv, ss, scale,
                                                     1 if accumulate else 0, _ptr(D)))
 
     def field_backpropagate(self, F, uR, vR, du, dv, lam, d, prec=FP64_PARITY, quirks=QUIRKS_REF):
         """Angular spectrum F times -exp(i kz d) where propagating, else 0
         (microscope.cu:423-489)."""
+        check_field(F, prec, uR * vR, "F")
         self._check(self.L.bimsim_field_backpropagate(self.h, prec, quirks, _ptr(F), uR, vR, du,
                                                       dv, lam, d))
 
     def detector_rotate_add(self, deg, uR, vR, scale, src, D, prec=FP64_PARITY):
         """D += scale * src rotated by deg (microscope.cu:89-132)."""
+        check_real(src, prec, uR * vR, "src")
+        check_real(D, prec, uR * vR, "D")
         self._check(self.L.bimsim_detector_rotate_add(self.h, prec, deg, uR, vR, scale,
                                                       _ptr(src), _ptr(D)))
 
     def detector_ring_sums(self, uR, vR, n_samples, n_rings, src, line, prec=FP64_PARITY):
         """line[r] += the ring sums of src (microscope.cu:138-240)."""
+        check_real(src, prec, uR * vR, "src")
+        check_real(line, prec, n_rings, "line")
         self._check(self.L.bimsim_detector_ring_sums(self.h, prec, uR, vR, n_samples, n_rings,
                                                      _ptr(src), _ptr(line)))
 
     def detector_from_line(self, uR, vR, scale, line, D, prec=FP64_PARITY):
         """D = scale * line(|p - centre|) (microscope.cu:37-86)."""
+        check_real(line, prec, 1, "line")
+        check_real(D, prec, uR * vR, "D")
         self._check(self.L.bimsim_detector_from_line(self.h, prec, uR, vR, scale, _ptr(line),
                                                      line.numel(), _ptr(D)))
 
     def slice_ratio(self, op, D, Di, out, prec=FP64_PARITY):
+        check_real(D, prec, 1, "D")
+        check_real(Di, prec, D.numel(), "Di")
+        check_real(out, prec, D.numel(), "out")
         self._check(self.L.bimsim_slice_ratio(self.h, prec, op, _ptr(D), _ptr(Di), D.numel(),
                                               _ptr(out)))
+
+    @staticmethod
+    def check_batch(instances, spheres, waves, D, Di, Abs, prec):
+        """The buffers a batch reads and writes, against its instance table."""
+        import torch
+        n_img = max((I.image_off + I.Nd * I.Nd for I in instances), default=0)
+        check_real(D, prec, n_img, "D")
+        check_real(Di, prec, n_img, "Di")
+        check_real(Abs, prec, n_img, "Abs", optional=True)
+        nw = max((I.wave_begin + I.nk for I in instances
+                  if I.uf_mode != UF_PLANE and I.ns > 0 or I.uf_mode == UF_MC), default=0)
+        if nw > 0:
+            check_tensor(waves, torch.float64, 3 * nw, "waves")
+        for I in instances:
+            if I.ns < 0 or I.sphere_begin < 0 or I.sphere_begin + I.ns > len(spheres):
+                raise ValueError("an instance's sphere range lies outside the sphere list")
 
     def batch(self, instances, spheres, waves, D, Di, Abs=None, prec=FP32_FAST,
               quirks=QUIRKS_REF, use_graph=False):
         """A bimsim_batch over `instances` (a ctypes Instance array); runs the
         pass once.  Keep the returned Batch (and the tensors) alive while
         launching it again."""
+        self.check_batch(instances, spheres, waves, D, Di, Abs, prec)
         S = sphere_array(spheres)
         h = c_void_p()
         self._check(self.L.bimsim_batch_create(self.h, instances, len(instances), S, len(spheres),
@@ -378,6 +513,7 @@ class Context:
 
     def batch_run(self, instances, spheres, waves, D, Di, Abs=None, prec=FP32_FAST,
                   quirks=QUIRKS_REF):
+        self.check_batch(instances, spheres, waves, D, Di, Abs, prec)
         S = sphere_array(spheres)
         self._check(self.L.bimsim_batch_run(self.h, instances, len(instances), S, len(spheres),
                                             _ptr(waves), prec, quirks, _ptr(D), _ptr(Di),
@@ -385,6 +521,11 @@ class Context:
 
     def detector(self, U, Uf, uR, vR, du, dv, NAin, NAout, lam, padding, Nd, ss, D, Di,
                  Abs=None, prec=FP64_PARITY, quirks=QUIRKS_REF, write_filtered=True):
+        check_field(U, prec, uR * vR, "U")
+        check_field(Uf, prec, uR * vR, "Uf")
+        for t, name in ((D, "D"), (Di, "Di")):
+            check_real(t, prec, Nd * Nd, name)
+        check_real(Abs, prec, Nd * Nd, "Abs", optional=True)
         self._check(self.L.bimsim_detector(self.h, prec, quirks, _ptr(U), _ptr(Uf), uR, vR, du,
                                            dv, NAin, NAout, lam, padding, Nd, ss, _ptr(D),
                                            _ptr(Di), _ptr(Abs), 1 if write_filtered else 0))

@@ -11,10 +11,14 @@ struct UfInst {
     QuadArg q;
     d3 f, k;
     double kmag, amp, ca, cb;
-    int m, uR, vR, plane;
+    int m, uR, vR;
+    int mode;        // bimsim_uf_mode: series, plane wave or Monte-Carlo sum
     long long off;   // element offset of the instance's field
     long long tab;   // complex64 fields: offset of its uf_fast_table, else -1
     int flat;        // k.r = 0 at every pixel (uf_flat)
+    int nk;          // Monte-Carlo sum: waves [wave_off, wave_off + nk)
+    long long wave_off;
+    double sA;       // Monte-Carlo sum: weight (1.0f/nk) subA of every wave
 };
 
 // The incident field's plane case: k along z and a slice normal to z
@@ -47,8 +51,18 @@ void uf_fast_table(double ca, double cb, int m, std::vector<double>* out);
 
 // Incident fields of n instances into Uf (complex64 when f32, else
 // complex128); tabs holds the instances' uf_fast_table rows (f32 only).
+// Monte-Carlo instances read their waves from `waves` (complex128 only).
 int uf_batch_launch(bimsim_ctx* ctx, const UfInst* d_inst, int n, int max_npix, int f32,
-                    const double* tabs, void* Uf);
+                    const double* tabs, const double* waves, void* Uf);
+
+// Incident fields of the Monte-Carlo instances list[0..n) of d_inst into the
+// complex64 field Uf, in tiles of 64 x 64 pixels (the per-pixel batch
+// kernels leave those instances alone in complex64 fields).
+int uf_mc_batch_launch(bimsim_ctx* ctx, const UfInst* d_inst, const int* list, int n,
+                       int max_tiles, const double* waves, float2* Uf);
+
+// Workgroups of uf_mc_batch_launch for a uR x vR field.
+int mc_tiles(int uR, int vR);
 
 // U = Uf or U += Uf outside the spheres, for n instances; d_sph holds
 // (x, y, z, a) per sphere.  Also zeroes each instance's images in D and Di

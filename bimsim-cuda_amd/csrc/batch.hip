@@ -48,6 +48,10 @@ struct bimsim_batch {
     double4* d_sph4 = nullptr;
     void *d_Uf = nullptr, *d_U = nullptr;
     double* d_kdirs = nullptr;      // plane-wave directions, one row per instance
+    // complex64 batches: the instances whose incident field is the
+    // Monte-Carlo sum, evaluated by the tiled kernel after the per-pixel one
+    int* d_mc_list = nullptr;
+    int n_mc_uf = 0, mc_tiles_max = 0;
     SeriesPlan plan_mc, plan_pw;    // FP32 series: Monte-Carlo / plane-wave instances
     bool have_mc = false, have_pw = false;
     hipStream_t capture_stream = nullptr;
@@ -98,7 +102,7 @@ void release(bimsim_batch* b) {
     series_plan_destroy(&b->plan_mc);
     series_plan_destroy(&b->plan_pw);
     void* ptrs[] = {b->d_sph, b->d_kmag, b->d_work, b->d_A, b->d_B, b->d_uf, b->d_uftab,
-                    b->d_sum, b->d_sph4, b->d_Uf, b->d_U, b->d_kdirs};
+                    b->d_sum, b->d_sph4, b->d_Uf, b->d_U, b->d_kdirs, b->d_mc_list};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -112,6 +116,8 @@ int enqueue(bimsim_batch* b) {
         const bool mc = b->have_mc;
         BIMSIM_TRY(series_plan_launch_prepass(ctx, mc ? b->plan_mc : b->plan_pw,
                                               mc ? b->waves : b->d_kdirs, b->pre));
+        BIMSIM_TRY(uf_mc_batch_launch(ctx, b->d_uf, b->d_mc_list, b->n_mc_uf, b->mc_tiles_max,
+                                      b->waves, (float2*)b->d_Uf));
         BIMSIM_TRY(series_plan_launch_sums(ctx, mc ? b->plan_mc : b->plan_pw, b->d_A, b->d_B,
                                            (float2*)b->d_U));
     } else if (b->f32) {
@@ -127,7 +133,10 @@ int enqueue(bimsim_batch* b) {
         BIMSIM_HIP_CHECK(ctx, hipEventRecord(b->ev_join, b->side));
         if (b->have_mc) BIMSIM_TRY(series_plan_launch_waves(ctx, b->plan_mc, b->waves));
         if (b->have_pw) BIMSIM_TRY(series_plan_launch_waves(ctx, b->plan_pw, b->d_kdirs));
-        BIMSIM_TRY(uf_batch_launch(ctx, b->d_uf, b->n, b->max_npix, b->f32, b->d_uftab, b->d_Uf));
+        BIMSIM_TRY(uf_batch_launch(ctx, b->d_uf, b->n, b->max_npix, b->f32, b->d_uftab, b->waves,
+                                   b->d_Uf));
+        BIMSIM_TRY(uf_mc_batch_launch(ctx, b->d_uf, b->d_mc_list, b->n_mc_uf, b->mc_tiles_max,
+                                      b->waves, (float2*)b->d_Uf));
         BIMSIM_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, b->ev_join, 0));
         if (b->have_mc)
             BIMSIM_TRY(series_plan_launch_sums(ctx, b->plan_mc, b->d_A, b->d_B, (float2*)b->d_U));
@@ -136,12 +145,14 @@ int enqueue(bimsim_batch* b) {
     } else {
         BIMSIM_TRY(mie_batch_launch(ctx, b->d_sph, b->d_kmag, b->n_sph, b->lmax, b->quirks,
                                     b->d_work, b->d_A, b->d_B));
-        BIMSIM_TRY(uf_batch_launch(ctx, b->d_uf, b->n, b->max_npix, b->f32, b->d_uftab, b->d_Uf));
+        BIMSIM_TRY(uf_batch_launch(ctx, b->d_uf, b->n, b->max_npix, b->f32, b->d_uftab, b->waves,
+                                   b->d_Uf));
         for (int i = 0; i < b->n; ++i) {
             const bimsim_instance& I = b->inst[i];
             if (I.ns == 0) continue;
-            const double* w = I.plane_wave ? b->d_kdirs + 3 * (size_t)i : b->waves + 3 * (size_t)I.wave_begin;
-            const int nk = I.plane_wave ? 1 : I.nk;
+            const bool plane = I.uf_mode == BIMSIM_UF_PLANE;
+            const double* w = plane ? b->d_kdirs + 3 * (size_t)i : b->waves + 3 * (size_t)I.wave_begin;
+            const int nk = plane ? 1 : I.nk;
             BIMSIM_TRY(bimsim_nf_us(ctx, &I.pos, I.N, I.N, I.focus, b->kmag[i], b->inst_spheres[i].data(),
                                     I.ns, b->d_A, b->d_B, w, nk, b->subA[i], b->prec, b->quirks,
                                     (char*)b->d_U + fb * b->field_off[i], 0));
@@ -182,6 +193,7 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
     std::vector<double> uftab;
     std::vector<SumInst> sum(n);
     std::vector<double> kdirs(3 * (size_t)n);
+    std::vector<int> mc_list;
     SeriesBatch mc, pw;
     b->inst_spheres.resize(n);
     b->kmag.resize(n);
@@ -197,7 +209,12 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
         BIMSIM_CHECK_ARG(ctx, I.m >= 1 && I.m <= 100000, "instance m out of range");
         BIMSIM_CHECK_ARG(ctx, I.ns >= 0 && I.sphere_begin >= 0 && I.sphere_begin + I.ns <= n_spheres,
                          "instance sphere range outside the sphere array");
-        BIMSIM_CHECK_ARG(ctx, I.plane_wave || (I.nk >= 1 && I.wave_begin >= 0), "instance needs waves");
+        BIMSIM_CHECK_ARG(ctx, I.uf_mode == BIMSIM_UF_SERIES || I.uf_mode == BIMSIM_UF_PLANE ||
+                                  I.uf_mode == BIMSIM_UF_MC,
+                         "instance incident field must be the series, a plane wave or the "
+                         "Monte-Carlo sum");
+        const bool plane = I.uf_mode == BIMSIM_UF_PLANE;
+        BIMSIM_CHECK_ARG(ctx, plane || (I.nk >= 1 && I.wave_begin >= 0), "instance needs waves");
         BIMSIM_CHECK_ARG(ctx, I.Nd > 0 && I.ss > 0 && I.padding >= 0 &&
                                   (long long)(2 * I.padding + 1) * I.Nd * I.ss <= I.N && I.image_off >= 0,
                          "instance detector does not fit its near field");
@@ -205,7 +222,7 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
         BIMSIM_CHECK_ARG(ctx, lx > 0.0 && ly > 0.0, "instance slice has a zero edge");
         b->kmag[i] = 2.0 * (double)kPIf / I.lambda;
         const double ca = std::cos(std::asin(I.condenser[0])), cb = std::cos(std::asin(I.condenser[1]));
-        b->subA[i] = I.plane_wave ? I.amp : 2.0 * (double)kPIf * I.amp * ((1.0 - cb) - (1.0 - ca));
+        b->subA[i] = plane ? I.amp : 2.0 * (double)kPIf * I.amp * ((1.0 - cb) - (1.0 - ca));
         b->du[i] = 1.0 / lx;
         b->dv[i] = 1.0 / ly;
         b->image_elems = std::max(b->image_elems, (size_t)(I.image_off + (long long)I.Nd * I.Nd));
@@ -214,29 +231,34 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
         b->field_off[i] = field;
         const QuadArg q = to_quad(I.pos);
         long long tab = -1;
-        if (b->f32 && !I.plane_wave) {
+        if (b->f32 && I.uf_mode == BIMSIM_UF_SERIES) {
             tab = (long long)uftab.size();
             uf_fast_table(ca, cb, I.m, &uftab);
         }
         uf[i] = UfInst{q, mk(I.focus), mk(I.kdir), b->kmag[i], I.amp, ca, cb, I.m, I.N, I.N,
-                       I.plane_wave ? 1 : 0, field, tab,
-                       uf_flat(q, mk(I.focus), mk(I.kdir)) ? 1 : 0};
+                       I.uf_mode, field, tab, uf_flat(q, mk(I.focus), mk(I.kdir)) ? 1 : 0,
+                       plane ? 1 : I.nk, plane ? 0 : (long long)I.wave_begin,
+                       plane ? I.amp : (double)(1.0f / (float)I.nk) * b->subA[i]};
+        if (b->f32 && I.uf_mode == BIMSIM_UF_MC) {
+            mc_list.push_back(i);
+            b->mc_tiles_max = std::max(b->mc_tiles_max, mc_tiles(I.N, I.N));
+        }
         sum[i] = SumInst{q, I.N, I.N, (int)sph_dev.size(), I.ns, field, I.image_off, I.Nd * I.Nd};
         for (int c = 0; c < 3; ++c) kdirs[3 * (size_t)i + c] = I.kdir[c];
         // spheres of this instance, coefficients packed
-        SeriesBatch& sb = I.plane_wave ? pw : mc;
+        SeriesBatch& sb = plane ? pw : mc;
         SeriesInst si{};
         si.q = q;
         si.f = mk(I.focus);
         si.kmag = b->kmag[i];
-        const int nk = I.plane_wave ? 1 : I.nk;
+        const int nk = plane ? 1 : I.nk;
         si.sA = (double)(1.0f / (float)nk) * b->subA[i];
         si.uR = si.vR = I.N;
         si.s_begin = (int)sb.sph.size();
         si.ns = I.ns;
         si.nk = nk;
         si.accumulate = 0;
-        si.wave_off = I.plane_wave ? i : I.wave_begin;
+        si.wave_off = plane ? i : I.wave_begin;
         si.u_off = field;
         for (int s = 0; s < I.ns; ++s) {
             bimsim_sphere S = spheres[I.sphere_begin + s];
@@ -279,6 +301,8 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
     BIMSIM_TRY(dev_upload(ctx, &b->d_sum, sum));
     BIMSIM_TRY(dev_upload(ctx, &b->d_sph4, sph4));
     BIMSIM_TRY(dev_upload(ctx, &b->d_kdirs, kdirs));
+    BIMSIM_TRY(dev_upload(ctx, &b->d_mc_list, mc_list));
+    b->n_mc_uf = (int)mc_list.size();
     BIMSIM_TRY(dev_alloc(ctx, &b->d_work, sizeof(double2) * 8 * (size_t)(b->lmax + 2) * std::max(1, b->n_sph)));
     BIMSIM_TRY(dev_alloc(ctx, (void**)&b->d_A, sizeof(double2) * (size_t)std::max<long long>(1, coef)));
     BIMSIM_TRY(dev_alloc(ctx, (void**)&b->d_B, sizeof(double2) * (size_t)std::max<long long>(1, coef)));
@@ -304,7 +328,7 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
         const int wstride = b->lmax + 2;
         b->pre = PrepassArgs{b->d_sph, b->d_kmag, b->n_sph, mie::mie_spheres_per_block(wstride),
                              wstride, b->quirks, b->d_A, b->d_B, b->d_uf, b->n,
-                             (b->max_npix + 255) / 256, b->d_uftab, (float2*)b->d_Uf};
+                             (b->max_npix + 255) / 256, b->d_uftab, b->waves, (float2*)b->d_Uf};
         const SeriesPlan& P = b->have_mc ? b->plan_mc : b->plan_pw;
         b->prepass = b->have_mc != b->have_pw && P.harmonic && b->n_sph > 0 &&
                      prepass_lds(b->pre) <= kPrepassLds;
@@ -335,7 +359,9 @@ int bimsim_batch_create(bimsim_ctx* ctx, const bimsim_instance* inst, int n,
     if (!quirks)
         return set_error(ctx, BIMSIM_E_UNSUPPORTED, "batches run with the reference recurrences only");
     bool need_waves = false;
-    for (int i = 0; i < n; ++i) need_waves |= !inst[i].plane_wave && inst[i].ns > 0;
+    for (int i = 0; i < n; ++i)
+        need_waves |= inst[i].uf_mode == BIMSIM_UF_MC ||
+                      (inst[i].uf_mode != BIMSIM_UF_PLANE && inst[i].ns > 0);
     BIMSIM_CHECK_ARG(ctx, waves || !need_waves, "waves must be given");
     bimsim_batch* b = new bimsim_batch();
     b->ctx = ctx;

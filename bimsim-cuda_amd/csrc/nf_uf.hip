@@ -52,13 +52,123 @@ __global__ __launch_bounds__(kThreads) void uf_plane_kernel(QuadArg q, int uR, i
 template <class Out>
 __global__ __launch_bounds__(kThreads) void uf_batch_kernel(const UfInst* __restrict__ insts, int n,
                                                             const double* __restrict__ tabs,
+                                                            const double* __restrict__ waves,
                                                             Out* __restrict__ Uf) {
     for (int i = blockIdx.y; i < n; i += gridDim.y) {
         const UfInst& I = insts[i];
         const int npix = I.uR * I.vR;
         for (int pix = blockIdx.x * kThreads + threadIdx.x; pix < npix; pix += gridDim.x * kThreads)
-            uf_batch_pixel(I, pix, tabs, Uf);
+            uf_batch_pixel(I, pix, tabs, waves, Uf);
     }
+}
+
+// Monte-Carlo incident field, FP64 forms: one pixel per thread, the waves
+// summed in order (uf_mc_value).
+template <class Out>
+__global__ __launch_bounds__(kThreads) void uf_mc_kernel(QuadArg q, int uR, int vR, d3 f,
+                                                         double kmag,
+                                                         const double* __restrict__ W, int nk,
+                                                         double sA, Out* __restrict__ Uf) {
+    const int pix = blockIdx.x * kThreads + threadIdx.x;
+    if (pix >= uR * vR) return;
+    store(Uf, pix, uf_mc_value(q, uR, vR, pix, f, kmag, W, nk, sA));
+}
+
+// Monte-Carlo incident field, complex64 form.  With p = A + X iu/uR +
+// Y iv/vR each wave's phase splits into a row and a column factor,
+//   sA e^{i k K.(p - f)} = [sA e^{i k (K.(A - f) + K.Y iv/vR)}] e^{i k K.X iu/uR},
+// so a tile of kMcTile x kMcTile pixels is a complex rank-nk product: per
+// chunk of kMcChunk waves the workgroup evaluates the tile's row and column
+// factors (phases in FP64, reduced to [-pi, pi], then sincosf) into LDS and
+// every thread accumulates its 4 x 4 pixels, 4 FMAs per pixel and wave.
+constexpr int kMcTile = 64;
+constexpr int kMcChunk = 32;
+constexpr double kTwoPi = 6.283185307179586476925;
+
+__device__ __forceinline__ float2 cis_reduced(double x) {
+    const double r = fma(-kTwoPi, rint(x * (1.0 / kTwoPi)), x);
+    float sn, cs;
+    sincosf((float)r, &sn, &cs);
+    return make_float2(cs, sn);
+}
+
+// Tile `tile` of instance I's Monte-Carlo field.
+__device__ __forceinline__ void uf_mc_tile(const UfInst& I, int tile,
+                                           const double* __restrict__ waves,
+                                           float2* __restrict__ Uf) {
+    static_assert(kThreads == 256 && kMcTile == 64, "16 x 16 threads of 4 x 4 pixels");
+    __shared__ __align__(16) float2 sa[kMcChunk][kMcTile];   // column factors
+    __shared__ __align__(16) float2 sb[kMcChunk][kMcTile];   // row factors times the weight
+    __shared__ double sw[kMcChunk][3];          // k K.X/uR, k K.Y/vR, k K.(A - f)
+    const int tu = (I.uR + kMcTile - 1) / kMcTile, tv = (I.vR + kMcTile - 1) / kMcTile;
+    if (tile >= tu * tv) return;   // uniform per workgroup
+    const int u0 = (tile % tu) * kMcTile, v0 = (tile / tu) * kMcTile;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const double* W = waves + 3 * I.wave_off;
+    const d3 af = sub(I.q.A, I.f);
+    float ar[4][4], ai[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) ar[i][j] = ai[i][j] = 0.f;
+    for (int w0 = 0; w0 < I.nk; w0 += kMcChunk) {
+        const int nw = min(kMcChunk, I.nk - w0);
+        __syncthreads();   // the previous chunk's factors are consumed
+        if (threadIdx.x < nw) {
+            const int w = w0 + threadIdx.x;
+            const d3 K = {W[3 * w], W[3 * w + 1], W[3 * w + 2]};
+            sw[threadIdx.x][0] = I.kmag * dot(K, I.q.X) / (double)I.uR;
+            sw[threadIdx.x][1] = I.kmag * dot(K, I.q.Y) / (double)I.vR;
+            sw[threadIdx.x][2] = I.kmag * dot(K, af);
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < nw * kMcTile; e += kThreads) {
+            const int wl = e / kMcTile, c = e - wl * kMcTile;
+            sa[wl][c] = cis_reduced(sw[wl][0] * (double)(u0 + c));
+            const float2 b = cis_reduced(fma(sw[wl][1], (double)(v0 + c), sw[wl][2]));
+            sb[wl][c] = make_float2((float)I.sA * b.x, (float)I.sA * b.y);
+        }
+        __syncthreads();
+        for (int wl = 0; wl < nw; ++wl) {
+            const float4 a01 = *reinterpret_cast<const float4*>(&sa[wl][4 * tx]);
+            const float4 a23 = *reinterpret_cast<const float4*>(&sa[wl][4 * tx + 2]);
+            const float4 b01 = *reinterpret_cast<const float4*>(&sb[wl][4 * ty]);
+            const float4 b23 = *reinterpret_cast<const float4*>(&sb[wl][4 * ty + 2]);
+            const float2 a[4] = {{a01.x, a01.y}, {a01.z, a01.w}, {a23.x, a23.y}, {a23.z, a23.w}};
+            const float2 b[4] = {{b01.x, b01.y}, {b01.z, b01.w}, {b23.x, b23.y}, {b23.z, b23.w}};
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    ar[i][j] = fmaf(b[i].x, a[j].x, fmaf(-b[i].y, a[j].y, ar[i][j]));
+                    ai[i][j] = fmaf(b[i].x, a[j].y, fmaf(b[i].y, a[j].x, ai[i][j]));
+                }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int iv = v0 + 4 * ty + i;
+        if (iv >= I.vR) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int iu = u0 + 4 * tx + j;
+            if (iu < I.uR) Uf[I.off + (long long)iv * I.uR + iu] = make_float2(ar[i][j], ai[i][j]);
+        }
+    }
+}
+
+// Grid: (tiles, Monte-Carlo instances of the batch).
+__global__ __launch_bounds__(kThreads) void uf_mc_tile_kernel(const UfInst* __restrict__ insts,
+                                                              const int* __restrict__ list,
+                                                              const double* __restrict__ waves,
+                                                              float2* __restrict__ Uf) {
+    uf_mc_tile(insts[list[blockIdx.y]], (int)blockIdx.x, waves, Uf);
+}
+
+__global__ __launch_bounds__(kThreads) void uf_mc_one_kernel(UfInst I,
+                                                             const double* __restrict__ waves,
+                                                             float2* __restrict__ Uf) {
+    uf_mc_tile(I, (int)blockIdx.x, waves, Uf);
 }
 
 // U = Uf (no spheres) or U += Uf outside every sphere (d < a is inside,
@@ -162,16 +272,30 @@ void uf_fast_table(double ca, double cb, int m, std::vector<double>* out) {
 }
 
 int uf_batch_launch(bimsim_ctx* ctx, const UfInst* d_inst, int n, int max_npix, int f32,
-                    const double* tabs, void* Uf) {
+                    const double* tabs, const double* waves, void* Uf) {
     const dim3 grid((max_npix + kThreads - 1) / kThreads, std::min(n, 65535));
     if (f32)
         hipLaunchKernelGGL(uf_batch_kernel<float2>, grid, dim3(kThreads), 0, ctx->stream, d_inst, n,
-                           tabs, (float2*)Uf);
+                           tabs, waves, (float2*)Uf);
     else
         hipLaunchKernelGGL(uf_batch_kernel<double2>, grid, dim3(kThreads), 0, ctx->stream, d_inst,
-                           n, tabs, (double2*)Uf);
+                           n, tabs, waves, (double2*)Uf);
     BIMSIM_HIP_CHECK(ctx, hipGetLastError());
     return BIMSIM_OK;
+}
+
+int uf_mc_batch_launch(bimsim_ctx* ctx, const UfInst* d_inst, const int* list, int n,
+                       int max_tiles, const double* waves, float2* Uf) {
+    if (n <= 0) return BIMSIM_OK;
+    if (n > 65535) return set_error(ctx, BIMSIM_E_UNSUPPORTED, "too many Monte-Carlo instances");
+    hipLaunchKernelGGL(uf_mc_tile_kernel, dim3(max_tiles, n), dim3(kThreads), 0, ctx->stream,
+                       d_inst, list, waves, Uf);
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return BIMSIM_OK;
+}
+
+int mc_tiles(int uR, int vR) {
+    return ((uR + kMcTile - 1) / kMcTile) * ((vR + kMcTile - 1) / kMcTile);
 }
 
 int sum_batch_launch(bimsim_ctx* ctx, const SumInst* d_inst, int n, int max_npix,
@@ -249,6 +373,48 @@ extern "C" int bimsim_nf_uf(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int
                                uR, vR, f, k, kmag, amp, cos_alpha, cos_beta, m, (double2*)Uf);
     }
     BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    timer_end(ctx, tmr);
+    return BIMSIM_OK;
+}
+
+extern "C" int bimsim_nf_uf_mc(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int vR,
+                               const double focus[3], double kmag, const double* waves, int nk,
+                               double subA, int prec, void* Uf) {
+    if (!ctx) return BIMSIM_E_INVALID;
+    BIMSIM_CHECK_ARG(ctx, pos && focus && waves && Uf, "null pointer argument");
+    BIMSIM_CHECK_ARG(ctx, uR > 0 && vR > 0 && (long long)uR * vR <= (1LL << 28),
+                     "slice resolution must be positive and at most 2^28 pixels");
+    BIMSIM_CHECK_ARG(ctx, nk >= 1, "the Monte-Carlo sum needs at least one wave");
+    BIMSIM_CHECK_ARG(ctx, prec >= BIMSIM_FP64_PARITY && prec <= BIMSIM_FP32_FAST, "unknown precision");
+    BIMSIM_CHECK_ARG(ctx, kmag > 0.0 && std::isfinite(kmag), "kmag must be positive");
+    BIMSIM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    const QuadArg q = to_quad(*pos);
+    const d3 f = mk(focus);
+    // the weight of every wave, (1.0f/nk) subA as the scattered field
+    const double sA = (double)(1.0f / (float)nk) * subA;
+    const int tmr = timer_begin(ctx, kStageUf);
+    if (prec == BIMSIM_FP32_FAST) {
+        UfInst I{};
+        I.q = q;
+        I.f = f;
+        I.k = d3{0.0, 0.0, 1.0};
+        I.kmag = kmag;
+        I.uR = uR;
+        I.vR = vR;
+        I.mode = BIMSIM_UF_MC;
+        I.off = 0;
+        I.nk = nk;
+        I.wave_off = 0;
+        I.sA = sA;
+        hipLaunchKernelGGL(uf_mc_one_kernel, dim3(mc_tiles(uR, vR)), dim3(kThreads), 0, ctx->stream,
+                           I, waves, (float2*)Uf);
+        BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    } else {
+        const dim3 grid((uR * vR + kThreads - 1) / kThreads);
+        hipLaunchKernelGGL(uf_mc_kernel<double2>, grid, dim3(kThreads), 0, ctx->stream, q, uR, vR,
+                           f, kmag, waves, nk, sA, (double2*)Uf);
+        BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    }
     timer_end(ctx, tmr);
     return BIMSIM_OK;
 }

@@ -188,19 +188,42 @@ __device__ inline dc uf_plane_value(const QuadArg& q, int uR, int vR, int pix, d
     return cscl(cexp_i(kr), amp);
 }
 
+// The Monte-Carlo incident field at pixel pix: the plane waves of
+// gpuScalarUfp (nfScalarUf.cu:9-43) along the nk directions W, each with
+// the weight sA = (1.0f/nk) subA that the scattered field gives the same
+// wave (nfScalarUs.cu:461,572), summed in wave order in FP64.
+__device__ inline dc uf_mc_value(const QuadArg& q, int uR, int vR, int pix, d3 f, double kmag,
+                                 const double* __restrict__ W, int nk, double sA) {
+    int iu, iv;
+    pixel_of(pix, uR, iu, iv);
+    const d3 p = quad_at(q, (double)iu / (double)uR, (double)iv / (double)vR);
+    const d3 r = sub(p, f);
+    dc sum = {0.0, 0.0};
+    for (int w = 0; w < nk; ++w) {
+        const d3 K = {W[3 * w], W[3 * w + 1], W[3 * w + 2]};
+        sum = cadd(sum, cscl(cexp_i(kmag * dot(K, r)), sA));
+    }
+    return sum;
+}
+
 // Pixel pix of batch instance I into its field (complex64 fields use the
-// instance's uf_fast_table row at tabs + I.tab).
+// instance's uf_fast_table row at tabs + I.tab; their Monte-Carlo instances
+// are left to uf_mc_batch_launch).
 template <class Out>
 __device__ inline void uf_batch_pixel(const UfInst& I, int pix, const double* __restrict__ tabs,
-                                      Out* __restrict__ Uf) {
+                                      const double* __restrict__ waves, Out* __restrict__ Uf) {
     dc v;
-    if (I.plane)
+    if (I.mode == BIMSIM_UF_PLANE) {
         v = uf_plane_value(I.q, I.uR, I.vR, pix, I.f, I.k, I.kmag, I.amp);
-    else if constexpr (std::is_same<Out, float2>::value)   // tables for every instance
+    } else if (I.mode == BIMSIM_UF_MC) {
+        if constexpr (std::is_same<Out, float2>::value) return;
+        v = uf_mc_value(I.q, I.uR, I.vR, pix, I.f, I.kmag, waves + 3 * I.wave_off, I.nk, I.sA);
+    } else if constexpr (std::is_same<Out, float2>::value) {   // tables for every instance
         v = uf_fast_value(I.q, I.uR, I.vR, pix, I.f, I.k, I.kmag, I.amp, I.ca, I.cb, I.m,
                           tabs + I.tab, I.flat != 0);
-    else
+    } else {
         v = uf_series_value(I.q, I.uR, I.vR, pix, I.f, I.k, I.kmag, I.amp, I.ca, I.cb, I.m);
+    }
     store(Uf + I.off, pix, v);
 }
 

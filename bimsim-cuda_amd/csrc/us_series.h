@@ -86,6 +86,7 @@ struct PrepassArgs {
     const UfInst* uf;
     int n_uf, uf_blocks;
     const double* uf_tabs;
+    const double* uf_waves;   // waves of the Monte-Carlo incident fields
     float2* Uf;
     const SumInst* sum;
     float* D;

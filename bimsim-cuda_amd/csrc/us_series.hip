@@ -542,7 +542,7 @@ __global__ __launch_bounds__(kRecThreads) void prepass_kernel(
     b -= n_rec;
     const int i = b / a.uf_blocks, pix = (b - i * a.uf_blocks) * kRecThreads + t;
     const UfInst& I = a.uf[i];
-    if (pix < I.uR * I.vR) ufd::uf_batch_pixel(I, pix, a.uf_tabs, a.Uf);
+    if (pix < I.uR * I.vR) ufd::uf_batch_pixel(I, pix, a.uf_tabs, a.uf_waves, a.Uf);
     if (a.sum) {
         const SumInst& S = a.sum[i];   // nd2 <= uR vR
         if (pix < S.nd2) {

@@ -5,7 +5,7 @@ libbimsim_hip.so on the context stream."""
 import numpy as np
 import torch
 
-from . import FP32_FAST, QUIRKS_REF, UF_PLANE, UF_SERIES, Instance, Quad
+from . import FP32_FAST, QUIRKS_REF, UF_MC, UF_PLANE, UF_SERIES, Instance, Quad
 
 
 def quad_of(inst):
@@ -39,9 +39,15 @@ def simulate(ctx, inst, waves, prec, buf=None, quirks=QUIRKS_REF, write_filtered
         buf = Buffers(inst, prec)
     q = quad_of(inst)
     N = inst["N"]
-    mode = UF_PLANE if inst.get("plane_wave") else UF_SERIES
-    ctx.nf_uf(q, N, N, inst["focus"], inst["kdir"], inst["kmag"], inst["amp"], inst["cos_alpha"],
-              inst["cos_beta"], inst["m"], buf.Uf, mode=mode, prec=prec, quirks=quirks)
+    mode = uf_mode_of(inst)
+    if mode == UF_MC:
+        # the incident field as the Monte-Carlo sum of the same waves as Us
+        ctx.nf_uf_mc(q, N, N, inst["focus"], inst["kmag"], waves, inst["subA"], buf.Uf,
+                     prec=prec)
+    else:
+        ctx.nf_uf(q, N, N, inst["focus"], inst["kdir"], inst["kmag"], inst["amp"],
+                  inst["cos_alpha"], inst["cos_beta"], inst["m"], buf.Uf, mode=mode, prec=prec,
+                  quirks=quirks)
     ctx.mie_coeffs(inst["spheres"], inst["kmag"], buf.A, buf.B, quirks=quirks)
     subA = inst["subA"]
     if inst.get("plane_wave"):
@@ -57,6 +63,14 @@ def simulate(ctx, inst, waves, prec, buf=None, quirks=QUIRKS_REF, write_filtered
                  inst["lam"], inst["padding"], inst["Nd"], inst["ss"], buf.D, buf.Di, buf.Abs,
                  prec=prec, quirks=quirks, write_filtered=write_filtered)
     return buf
+
+
+def uf_mode_of(inst):
+    """Incident-field form of an instance: a plane wave, the Monte-Carlo sum
+    of its waves (inst["uf_mc"]) or the Legendre series."""
+    if inst.get("plane_wave"):
+        return UF_PLANE
+    return UF_MC if inst.get("uf_mc") else UF_SERIES
 
 
 def batch_tables(insts, waves):
@@ -80,12 +94,12 @@ def batch_tables(insts, waves):
         I.amp = inst["amp"]
         I.condenser[:] = list(inst["cond"])
         I.objective[:] = list(inst["obj"])
-        I.plane_wave = 1 if inst.get("plane_wave") else 0
+        I.uf_mode = uf_mode_of(inst)
         I.sphere_begin = len(spheres)
         I.ns = len(inst["spheres"])
         spheres.extend(inst["spheres"])
         I.wave_begin = wb
-        if I.plane_wave:
+        if I.uf_mode == UF_PLANE:
             I.nk = 1
         else:
             w = np.asarray(waves[i], dtype=np.float64).reshape(-1, 3)

@@ -23,8 +23,9 @@ origin, k = +z, detector without padding and ss = 1):
          L=100, condenser and objective 0.2-0.5, 400 waves
   cfg5   ragged: single-sphere instances, a log-uniform [0.5, 25],
          lambda~U(2.5,11), n~U(1.2,1.7)+iU(0,0.2), N from {256,384,512,768,
-         1024}, L = 4a; half of them a plane wave, the others 200-2000
-         Monte-Carlo waves; condenser and objective 0-0.5
+         1024}, L = 4a, 200-2000 Monte-Carlo waves each; the incident field
+         by the Legendre series or (Bernoulli 1/2) as the Monte-Carlo sum of
+         the same waves; condenser and objective 0-0.5
 """
 import math
 import struct
@@ -69,7 +70,7 @@ def incident_order(kmag, dmax):
 
 
 def make_instance(spheres, lam, N, L, cond, obj, nk, wave_seed, amp=1.0, m=None,
-                  plane_wave=False):
+                  plane_wave=False, uf_mc=False):
     lam = f32(lam)
     kmag = (2.0 * PI_F) / lam
     spheres = [dict(s) for s in spheres]
@@ -93,7 +94,7 @@ def make_instance(spheres, lam, N, L, cond, obj, nk, wave_seed, amp=1.0, m=None,
         Y=(0.0, float(L), 0.0), focus=(0.0, 0.0, 0.0), kdir=(0.0, 0.0, 1.0), amp=amp,
         cond=cond, obj=obj, cos_alpha=ca,
         cos_beta=cb, m=m, spheres=spheres, ncoef=off, nk=nk, wave_seed=wave_seed,
-        plane_wave=plane_wave, padding=0, Nd=N, ss=1, du=1.0 / L, dv=1.0 / L,
+        plane_wave=plane_wave, uf_mc=uf_mc, padding=0, Nd=N, ss=1, du=1.0 / L, dv=1.0 / L,
     )
     inst["subA"] = (2.0 * PI_F) * amp * ((1.0 - cb) - (1.0 - ca))
     return inst
@@ -187,18 +188,20 @@ def config5(count=1024, sizes=(256, 384, 512, 768, 1024), nk_range=(200, 2000)):
         lam = g.uniform(2.5, 11.0)
         n = complex(g.uniform(1.2, 1.7), g.uniform(0.0, 0.2))
         N = sizes[min(len(sizes) - 1, int(g.uniform(0, len(sizes))))]
-        plane = g.uniform() < 0.5
-        nk = 1 if plane else nk_range[0] + int(g.uniform(0, nk_range[1] - nk_range[0] + 1))
+        mc = g.uniform() < 0.5
+        nk = nk_range[0] + int(g.uniform(0, nk_range[1] - nk_range[0] + 1))
         inst = make_instance([dict(p=(0.0, 0.0, 0.0), a=a, n=n)], lam, N, 4.0 * a, (0.0, 0.5),
-                             (0.0, 0.5), nk, SEED_BASE + 0x1000 + i, plane_wave=plane)
+                             (0.0, 0.5), nk, SEED_BASE + 0x1000 + i, uf_mc=mc)
         out.append(inst)
     return out
 
 
 def terms(inst):
-    """Mie series terms of one image: N^2 nk sum_s (Nl+1) + N^2 (m+1)."""
+    """Mie series terms of one image: N^2 nk sum_s (Nl+1) + N^2 (m+1), the
+    incident-field part N^2 nk when it is the Monte-Carlo sum."""
     n2 = inst["N"] * inst["N"]
-    return n2 * inst["nk"] * sum(s["Nl"] + 1 for s in inst["spheres"]) + n2 * (inst["m"] + 1)
+    uf = inst["nk"] if inst.get("uf_mc") else inst["m"] + 1
+    return n2 * inst["nk"] * sum(s["Nl"] + 1 for s in inst["spheres"]) + n2 * uf
 
 
 def us_flops(inst):

@@ -64,11 +64,14 @@ typedef enum {
 } bimsim_quirks;
 
 /* Incident-field evaluation (nearfieldStruct::scalarUf / scalarUfLut /
- * gpuScalarUfp, reference nfScalarUf.cu:9-217, nfScalarUfLut.cu:11-188). */
+ * gpuScalarUfp, reference nfScalarUf.cu:9-217, nfScalarUfLut.cu:11-188).
+ * BIMSIM_UF_MC is the Monte-Carlo sum of the plane waves that also drive the
+ * scattered field (bimsim_nf_uf_mc); the reference sums only Us that way. */
 typedef enum {
     BIMSIM_UF_SERIES = 0,
     BIMSIM_UF_PLANE = 1,
-    BIMSIM_UF_LUT = 2
+    BIMSIM_UF_LUT = 2,
+    BIMSIM_UF_MC = 3
 } bimsim_uf_mode;
 
 /* Scattered-field evaluation (scalarUs / scalarUpLut, reference
@@ -190,6 +193,21 @@ int bimsim_nf_uf(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int vR,
                  const double focus[3], const double kdir[3], double kmag,
                  double amp, double cos_alpha, double cos_beta, int m,
                  int mode, int prec, int quirks, void* Uf);
+
+/* Incident field as the Monte-Carlo sum of nk plane waves: Uf(p) =
+ * sum_w (1.0f/nk) subA exp(i kmag K_w.(p - f)), each term the plane wave of
+ * gpuScalarUfp (reference nfScalarUf.cu:9-43) with the weights the
+ * scattered field gives the same waves (nfScalarUs.cu:461,572).  waves:
+ * device, nk*3 doubles (the list nearfieldStruct::calcWaves draws,
+ * nearfield.cpp:96-108).  FP64 forms sum the waves per pixel in order;
+ * BIMSIM_FP32_FAST factors each phase into its row and column parts
+ * (exp(i k K.(A-f)) exp(i k K.X iu/uR) exp(i k K.Y iv/vR)) and sums the
+ * products over LDS-staged tiles.  Uf is overwritten.  The reference has
+ * no such mode (its Uf is the Legendre series or one plane wave); it is
+ * the focused field that converges to the series as nk grows. */
+int bimsim_nf_uf_mc(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int vR,
+                    const double focus[3], double kmag, const double* waves, int nk,
+                    double subA, int prec, void* Uf);
 
 /* Scattered/internal field of ns spheres summed over nk plane waves
  * (nearfieldStruct::scalarUs, reference nfScalarUs.cu:494-658: precompute +
@@ -382,7 +400,10 @@ typedef struct {
     double amp;             /* incident amplitude A                             */
     double condenser[2];    /* NAin, NAout                                      */
     double objective[2];    /* NAin, NAout                                      */
-    int    plane_wave;      /* 1: plane wave along kdir (Uf and a single wave)  */
+    int    uf_mode;         /* incident field (bimsim_uf_mode): BIMSIM_UF_SERIES
+                               (Legendre series), BIMSIM_UF_PLANE (plane wave
+                               along kdir for Uf and a single wave for Us) or
+                               BIMSIM_UF_MC (Monte-Carlo sum of the waves)   */
     int    sphere_begin;    /* spheres [sphere_begin, sphere_begin + ns)        */
     int    ns;
     int    wave_begin;      /* Monte-Carlo waves [wave_begin, wave_begin + nk)  */

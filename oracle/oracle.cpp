@@ -1049,6 +1049,37 @@ int orc_nf_uf(const bimsim_quad* q, int uR, int vR, const double f[3], const dou
     return 0;
 }
 
+// Monte-Carlo incident field: per pixel the sum over the waves of the plane
+// wave of gpuScalarUfp (nfScalarUf.cu:9-43: p = ABCD(u, v), r = p - f,
+// exp(i kmag k.r) A) with A = (1.0f/nk) subA, the weight the scattered
+// field gives each wave (nfScalarUs.cu:461 scale, :572 subA), in wave order.
+int orc_nf_uf_mc(const bimsim_quad* q, int uR, int vR, const double f[3], double kmag,
+                 const double* waves, int nk, double subA, int fp32, double* Uf, int nthreads) {
+    if (uR <= 0 || vR <= 0 || nk <= 0) return -1;
+    auto run = [&](auto zero) {
+        using T = decltype(zero);
+        quad<T> qq(*q);
+        const v3<T> fv = mk3<T>(f);
+        const T w = (T)(1.0f / nk) * (T)subA;
+        std::vector<v3<T>> K(nk);
+        for (int i = 0; i < nk; ++i) K[i] = mk3<T>(waves + 3 * i);
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
+        for (int iv = 0; iv < vR; ++iv) {
+            for (int iu = 0; iu < uR; ++iu) {
+                T u = (T)iu / (T)uR, v = (T)iv / (T)vR;
+                const v3<T> r = qq.at(u, v) - fv;
+                rc<T> sum;
+                for (int i = 0; i < nk; ++i) sum = sum + scl(cexp_(rc<T>(0, (T)kmag * K[i].dot(r))), w);
+                const int px = iv * uR + iu;
+                Uf[2 * px] = (double)sum.r;
+                Uf[2 * px + 1] = (double)sum.i;
+            }
+        }
+    };
+    if (fp32) run(0.0f); else run(0.0);
+    return 0;
+}
+
 int orc_nf_us(const bimsim_quad* q, int uR, int vR, const double f[3], double kmag,
               const bimsim_sphere* s, int ns, const double* A, const double* B,
               const double* waves, int nk, double subA, int fp32, int quirks, double* U,

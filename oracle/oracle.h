@@ -58,6 +58,8 @@ int orc_nf_uf(const bimsim_quad* q, int uR, int vR, const double f[3], const dou
               double kmag, double amp, double cos_alpha, double cos_beta, int m, int mode,
               int fp32, int quirks, const double* lut, int dR, double dmin, double dmax,
               double* Uf, int nthreads);
+int orc_nf_uf_mc(const bimsim_quad* q, int uR, int vR, const double f[3], double kmag,
+                 const double* waves, int nk, double subA, int fp32, double* Uf, int nthreads);
 int orc_nf_us(const bimsim_quad* q, int uR, int vR, const double f[3], double kmag,
               const bimsim_sphere* s, int ns, const double* A, const double* B,
               const double* waves, int nk, double subA, int fp32, int quirks, double* U,

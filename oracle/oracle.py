@@ -143,6 +143,17 @@ class Oracle:
         assert rc == 0
         return Uf.view(np.complex128).reshape(vR, uR)
 
+    def nf_uf_mc(self, q, uR, vR, focus, kmag, waves, subA, fp32=0):
+        """Monte-Carlo incident field: sum of the plane waves `waves`."""
+        Uf = np.zeros(2 * uR * vR)
+        f = (c_double * 3)(*focus)
+        w = np.ascontiguousarray(waves, dtype=np.float64).ravel()
+        rc = self.L.orc_nf_uf_mc(ctypes.byref(q), c_int(uR), c_int(vR), f, c_double(kmag),
+                                 _p(w), c_int(len(w) // 3), c_double(subA), c_int(fp32), _p(Uf),
+                                 c_int(self.nthreads))
+        assert rc == 0
+        return Uf.view(np.complex128).reshape(vR, uR)
+
     def nf_us(self, q, uR, vR, focus, kmag, spheres, A, B, waves, subA, fp32=0, quirks=1, U=None):
         acc = U is not None
         Ud = _c2d(U).ravel().copy() if acc else np.zeros(2 * uR * vR)

@@ -100,3 +100,38 @@ def test_sphere_lut_dims_match_oracle(pkg, orc):
                  n=complex(1.4, 0.01), Nl=10, coeff_off=0)
         R = int(rng.integers(16, 300))
         assert pkg.sphere_lut_dims(q, s, R) == orc.sphere_lut_dims(qo, s, R)
+
+
+def test_buffer_checks_reject_mismatched_tensors(pkg):
+    """The binding checks every buffer against the call before it reaches the
+    library: dtype for the precision, contiguity, size, device."""
+    torch = pytest.importorskip("torch")
+    c64 = torch.zeros(16, dtype=torch.complex64)
+    with pytest.raises(ValueError, match="complex128"):
+        pkg.check_field(c64, pkg.FP64_PARITY, 16, "U")
+    with pytest.raises(ValueError, match="contiguous"):
+        pkg.check_field(torch.zeros(32, dtype=torch.complex64)[::2], pkg.FP32_FAST, 16, "U")
+    with pytest.raises(ValueError, match="needs 17"):
+        pkg.check_field(c64, pkg.FP32_FAST, 17, "U")
+    with pytest.raises(ValueError, match="GPU"):
+        pkg.check_field(c64, pkg.FP32_FAST, 16, "U")
+    with pytest.raises(ValueError, match="float32"):
+        pkg.check_real(torch.zeros(4, dtype=torch.float64), pkg.FP32_FAST, 4, "D")
+    pkg.check_field(None, pkg.FP32_FAST, 16, "Uip", optional=True)
+    with pytest.raises(ValueError, match="required"):
+        pkg.check_field(None, pkg.FP32_FAST, 16, "U")
+    assert pkg.coeff_extent([dict(Nl=5, coeff_off=3), dict(Nl=2, coeff_off=0)]) == 9
+
+
+def test_batch_checks_use_the_instance_table(pkg):
+    """Images and waves must cover what the instance table addresses."""
+    torch = pytest.importorskip("torch")
+    arr = (pkg.Instance * 2)()
+    arr[0].Nd, arr[0].image_off, arr[0].uf_mode, arr[0].ns = 8, 0, pkg.UF_SERIES, 0
+    arr[1].Nd, arr[1].image_off, arr[1].uf_mode, arr[1].ns = 4, 64, pkg.UF_MC, 0
+    arr[1].wave_begin, arr[1].nk = 10, 5
+    D = torch.zeros(79, dtype=torch.float32)
+    with pytest.raises(ValueError, match="D holds 79 elements, the call needs 80"):
+        pkg.Context.check_batch(arr, [], None, D, D, None, pkg.FP32_FAST)
+    with pytest.raises(ValueError, match="float32"):
+        pkg.Context.check_batch(arr, [], None, D.double(), D, None, pkg.FP32_FAST)

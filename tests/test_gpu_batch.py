@@ -71,7 +71,9 @@ def ragged(W):
     c = W.make_instance([dict(p=(2.0, -1.0, 0.0), a=3.0, n=complex(1.5, 0.02))], 2.8, 48, 30.0,
                         (0.0, 0.5), (0.0, 0.5), 1, 7, plane_wave=True)
     d = W.make_instance([], 3.0, 32, 20.0, (0.0, 0.4), (0.0, 0.4), 8, 9)
-    return [a, b, c, d]
+    e = W.make_instance([dict(p=(-1.0, 2.0, 0.0), a=2.5, n=complex(1.45, 0.04))], 2.6, 40, 24.0,
+                        (0.1, 0.5), (0.0, 0.5), 37, 10, uf_mc=True)
+    return [a, b, c, d, e]
 
 
 @pytest.mark.parametrize("prec", [0, 2])
@@ -82,6 +84,24 @@ def test_batch_of_one_equals_the_pipeline(pkg, ctx, W, prec):
     D1, Di1, Ab1 = single(pkg, ctx, inst, waves[0], prec)
     assert np.array_equal(host(D), D1) and np.array_equal(host(Di), Di1)
     assert np.array_equal(host(Ab), Ab1)
+    b.close()
+
+
+@pytest.mark.parametrize("prec", [0, 2])
+def test_mc_focused_batch_of_one_equals_the_pipeline(pkg, ctx, W, prec):
+    """An instance whose incident field is the Monte-Carlo sum of its waves:
+    the batch and the stepwise entry points run the same per-pixel (FP64)
+    or tiled (FP32) code, so the images agree bit for bit."""
+    inst = W.make_instance([dict(p=(1.5, -2.0, 0.0), a=3.0, n=complex(1.4, 0.03))], 2.9, 96,
+                           30.0, (0.1, 0.5), (0.0, 0.5), 70, 21, uf_mc=True)
+    waves = waves_of(pkg, W, [inst])
+    b, D, Di, Ab, _ = run_batch(pkg, ctx, [inst], waves, prec)
+    D1, Di1, Ab1 = single(pkg, ctx, inst, waves[0], prec)
+    assert np.array_equal(host(Di), Di1)
+    if prec == 0:
+        assert np.array_equal(host(D), D1) and np.array_equal(host(Ab), Ab1)
+    else:
+        assert normrel(host(D), D1) <= 2e-6
     b.close()
 
 
@@ -173,9 +193,12 @@ def oracle_image(orc, inst, waves, fp32=0):
     import bimsim_amd.pipeline as P
     q = P.quad_of(inst)
     N = inst["N"]
-    Uf = orc.nf_uf(q, N, N, inst["focus"], inst["kdir"], inst["kmag"], inst["amp"],
-                   inst["cos_alpha"], inst["cos_beta"], inst["m"],
-                   mode=1 if inst.get("plane_wave") else 0)
+    if inst.get("uf_mc"):
+        Uf = orc.nf_uf_mc(q, N, N, inst["focus"], inst["kmag"], waves, inst["subA"])
+    else:
+        Uf = orc.nf_uf(q, N, N, inst["focus"], inst["kdir"], inst["kmag"], inst["amp"],
+                       inst["cos_alpha"], inst["cos_beta"], inst["m"],
+                       mode=1 if inst.get("plane_wave") else 0)
     if inst["spheres"]:
         s0 = inst["spheres"]
         A = np.zeros(inst["ncoef"], dtype=complex)

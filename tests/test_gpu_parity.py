@@ -85,6 +85,27 @@ def test_incident_field_matches_oracle(pkg, ctx, orc, W, mode):
     assert normrel(host(U32), ref) <= 1e-6
 
 
+@pytest.mark.parametrize("N,nk,z", [(48, 40, 0.0), (70, 333, 0.8), (130, 65, -0.4)])
+def test_mc_incident_field_matches_oracle(pkg, ctx, orc, W, N, nk, z):
+    """The Monte-Carlo incident field: FP64 per pixel in wave order, FP32 by
+    LDS tiles of row and column factors (ragged edges at N = 70, 130; wave
+    chunks with a remainder at nk = 333, 65)."""
+    import bimsim_amd.pipeline as P
+    inst = W.make_instance([], 2.7, N, 26.0, (0.15, 0.55), (0.0, 0.5), nk, 31)
+    inst["A"] = (inst["A"][0], inst["A"][1], z)
+    q = P.quad_of(inst)
+    waves = W.mc_waves(inst, pkg.mc_sample_na)
+    focus = (0.4, -0.3, 0.1)
+    ref = orc.nf_uf_mc(q, N, N, focus, inst["kmag"], waves, inst["subA"]).ravel()
+    wd = dev(waves.ravel(), torch.float64)
+    U64 = torch.empty(N * N, dtype=torch.complex128, device="cuda")
+    ctx.nf_uf_mc(q, N, N, focus, inst["kmag"], wd, inst["subA"], U64, prec=pkg.FP64_PARITY)
+    assert normrel(host(U64), ref) <= 1e-10
+    U32 = torch.empty(N * N, dtype=torch.complex64, device="cuda")
+    ctx.nf_uf_mc(q, N, N, focus, inst["kmag"], wd, inst["subA"], U32, prec=pkg.FP32_FAST)
+    assert normrel(host(U32), ref) <= 2e-6
+
+
 # ------------------------------------------------------------ scattered field
 def _us_case(pkg, W, cid, N, nk):
     inst = W.config(cid, N=N, nk=nk)

@@ -181,3 +181,113 @@ def test_sum_without_spheres_is_the_incident_field(orc):
     far = dict(p=(0.0, 0.0, 50.0), a=1.0, n=1.5 + 0j, Nl=1, coeff_off=0)
     assert np.array_equal(orc.nf_sum_uf(q, 6, 6, [big], Uf, U), U)
     assert np.allclose(orc.nf_sum_uf(q, 6, 6, [far], Uf, U), U + Uf, atol=0)
+
+
+@pytest.mark.parametrize("n,z", [(1.45 + 0.03j, 0.0), (1.3 + 0.0j, 0.7)])
+def test_scattered_series_is_the_brute_force_sum(orc, n, z):
+    """With the reference recurrences off, the oracle's scattered/internal
+    series equals sum_w (1/nk) subA e^{i k K_w.c} sum_l c_l P_l(K_w.r-hat)
+    with c_l = B_l h_l(kd) outside (A_l j_l(n kd) inside), every term from
+    scipy's Bessel functions and Legendre polynomials."""
+    import oracle
+    lam, a = 2.5, 3.0
+    kmag = 2.0 * PI_F / lam
+    N, L = 10, 14.0
+    q = oracle.Quad.xy(L, z)
+    p = (0.6, -1.1, 0.0)
+    nl = orc.sphere_order(a, lam)
+    A, B = orc.mie_coeffs(nl, kmag, a, n, quirks=0)
+    assert np.max(np.abs(B)) > 0.1
+    s = dict(p=p, a=a, n=n, Nl=nl, coeff_off=0)
+    rng = np.random.default_rng(11)
+    waves = rng.normal(size=(6, 3))
+    waves[:, 2] = np.abs(waves[:, 2]) + 1.5
+    waves /= np.linalg.norm(waves, axis=1)[:, None]
+    subA, focus = 0.7, (0.2, 0.1, 0.0)
+    U = orc.nf_us(q, N, N, focus, kmag, [s], A, B, waves, subA, quirks=0)
+    w1 = float(np.float32(1.0) / np.float32(len(waves)))
+    c = np.array(p) - np.array(focus)
+    l = np.arange(nl + 1)
+    want = np.zeros((N, N), dtype=complex)
+    for iv in range(N):
+        for iu in range(N):
+            r = np.array([-L / 2 + L * iu / N, -L / 2 + L * iv / N, z]) - np.array(p)
+            d = np.linalg.norm(r)
+            rh = r / d
+            if d <= a:
+                cl = A * special.spherical_jn(l, n * kmag * d)
+            else:
+                cl = B * sph_h(l, kmag * d)
+            for k in waves:
+                P = special.eval_legendre(l, float(k @ rh))
+                want[iv, iu] += w1 * subA * np.exp(1j * kmag * float(k @ c)) * np.sum(cl * P)
+    inside = np.array([[np.linalg.norm(np.array([-L / 2 + L * iu / N, -L / 2 + L * iv / N, z]) -
+                                       np.array(p)) <= a for iu in range(N)] for iv in range(N)])
+    assert inside.sum() > 5 and (~inside).sum() > 30
+    scale = np.max(np.abs(want))
+    assert np.max(np.abs(U[~inside] - want[~inside])) / scale < 1e-10
+    assert np.max(np.abs(U[inside] - want[inside])) / scale < 1e-10
+
+
+def test_mc_incident_field_is_the_internal_field_of_an_index_one_sphere(orc):
+    """The Monte-Carlo incident field sums the same weighted plane waves that
+    an index-1 sphere's internal series reproduces (by the Mie and Legendre
+    path of the scattered field), so the two agree inside such a sphere."""
+    import oracle
+    lam, a = 3.0, 6.0
+    kmag = 2.0 * PI_F / lam
+    N, L = 12, 20.0
+    q = oracle.Quad.xy(L, 0.5)
+    p = (1.0, -0.5, 0.0)
+    nl = orc.sphere_order(a, lam) + 25
+    A, B = orc.mie_coeffs(nl, kmag, a, 1.0 + 0.0j, quirks=0)
+    s = dict(p=p, a=a, n=1.0 + 0.0j, Nl=nl, coeff_off=0)
+    rng = np.random.default_rng(4)
+    waves = rng.normal(size=(9, 3))
+    waves[:, 2] = np.abs(waves[:, 2]) + 1.0
+    waves /= np.linalg.norm(waves, axis=1)[:, None]
+    subA, focus = 0.8, (0.3, 0.2, -0.1)
+    Ui = orc.nf_us(q, N, N, focus, kmag, [s], A, B, waves, subA, quirks=0)
+    Uf = orc.nf_uf_mc(q, N, N, focus, kmag, waves, subA)
+    iv, iu = np.mgrid[0:N, 0:N]
+    P = np.stack([-L / 2 + L * iu / N, -L / 2 + L * iv / N, np.full((N, N), 0.5)], axis=-1)
+    inside = np.linalg.norm(P - np.array(p), axis=-1) <= a
+    assert inside.sum() > 20
+    assert np.max(np.abs(Uf[inside] - Ui[inside])) < 1e-10
+    # and everywhere it is the weighted plane-wave sum itself
+    w1 = float(np.float32(1.0) / np.float32(len(waves)))
+    want = w1 * subA * np.exp(1j * kmag * np.einsum("wc,ijc->wij", waves,
+                                                    P - np.array(focus))).sum(axis=0)
+    assert np.max(np.abs(Uf - want)) < 1e-12
+
+
+def test_mc_incident_field_of_one_wave_is_the_plane_wave(orc):
+    import oracle
+    q = oracle.Quad.xy(9.0, 0.3)
+    k = (0.6, 0.0, 0.8)
+    kmag = 2.0 * PI_F / 2.0
+    Up = orc.nf_uf(q, 7, 7, (0.1, 0.0, 0.0), k, kmag, 1.7, 1.0, 0.5, 10, mode=1)
+    Um = orc.nf_uf_mc(q, 7, 7, (0.1, 0.0, 0.0), kmag, np.array([k]), 1.7)
+    assert np.max(np.abs(Up - Um)) < 1e-14
+
+
+def test_mc_incident_field_converges_to_the_series(orc, pkg):
+    """With many waves drawn uniformly over the condenser annulus the sum
+    tends to the focused field (the Legendre series, itself pinned to the
+    annulus integral), within its Monte-Carlo error."""
+    import oracle
+    lam, z = 2.5, 1.2
+    kmag = 2.0 * PI_F / lam
+    NAin, NAout = 0.2, 0.6
+    ca, cb = math.cos(math.asin(NAin)), math.cos(math.asin(NAout))
+    N, L = 6, 8.0
+    q = oracle.Quad.xy(L, z)
+    nk = 40000
+    waves = orc.mc_sample_na(nk, (0.0, 0.0, 1.0), NAin, NAout, 1, 77)
+    subA = 2.0 * PI_F * 1.0 * ((1.0 - cb) - (1.0 - ca))
+    Umc = orc.nf_uf_mc(q, N, N, (0.0, 0.0, 0.0), kmag, waves, subA)
+    m = 60
+    Us = orc.nf_uf(q, N, N, (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), kmag, 1.0, ca, cb, m, quirks=0)
+    err = np.max(np.abs(Umc - Us)) / np.max(np.abs(Us))
+    assert np.max(np.abs(Us.imag)) > 0.1 * np.max(np.abs(Us))   # a phase-sign slip would show
+    assert err < 3.0 / math.sqrt(nk), err
