@@ -8,6 +8,7 @@ loading fails loudly when libbimsim_hip.so has not been built.
 """
 import ctypes
 import os
+import weakref
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libbimsim_hip.so")
@@ -38,6 +39,7 @@ EXPORTS = (
     "bimsim_batch_launch", "bimsim_batch_destroy", "bimsim_batch_run", "bimsim_set_autotune",
     "bimsim_set_series_algorithm", "bimsim_detector_rotate_add", "bimsim_detector_ring_sums",
     "bimsim_detector_from_line", "bimsim_field_backpropagate", "bimsim_nf_uf_mc",
+    "bimsim_detector_zero_filled",
 )
 
 
@@ -74,7 +76,7 @@ class Instance(ctypes.Structure):
                 ("condenser", c_double * 2), ("objective", c_double * 2),
                 ("uf_mode", c_int), ("sphere_begin", c_int), ("ns", c_int),
                 ("wave_begin", c_int), ("nk", c_int), ("padding", c_int), ("Nd", c_int),
-                ("ss", c_int), ("image_off", ctypes.c_longlong)]
+                ("ss", c_int), ("image_off", ctypes.c_longlong), ("zero_fill", c_int)]
 
 
 def sphere_array(spheres):
@@ -129,6 +131,10 @@ def lib():
         L.bimsim_detector.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_int,
                                       c_double, c_double, c_double, c_double, c_double, c_int,
                                       c_int, c_int, c_void_p, c_void_p, c_void_p, c_int]
+        L.bimsim_detector_zero_filled.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p,
+                                                  c_int, c_int, c_int, c_double, c_double,
+                                                  c_double, c_double, c_double, c_void_p,
+                                                  c_void_p, c_void_p]
         L.bimsim_uf_bessel_lut.argtypes = [c_void_p, c_int, c_double, c_double, c_double, c_int,
                                            c_int, c_void_p]
         L.bimsim_nf_uf_lut.argtypes = [c_void_p, ctypes.POINTER(Quad), c_int, c_int,
@@ -275,9 +281,13 @@ class Context:
         if rc != OK:
             raise BimsimError(rc, "bimsim_ctx_create failed (no gfx950 device?)")
         self.h = h
+        # batches created on this context: destroyed before it is
+        self._batches = weakref.WeakSet()
 
     def close(self):
         if getattr(self, "h", None):
+            for b in list(getattr(self, "_batches", ())):
+                b.close()
             self.L.bimsim_ctx_destroy(self.h)
             self.h = None
 
@@ -509,7 +519,9 @@ class Context:
         self._check(self.L.bimsim_batch_create(self.h, instances, len(instances), S, len(spheres),
                                                _ptr(waves), prec, quirks, _ptr(D), _ptr(Di),
                                                _ptr(Abs), 1 if use_graph else 0, ctypes.byref(h)))
-        return Batch(self, h, (instances, S, waves, D, Di, Abs))
+        b = Batch(self, h, (instances, S, waves, D, Di, Abs))
+        self._batches.add(b)
+        return b
 
     def batch_run(self, instances, spheres, waves, D, Di, Abs=None, prec=FP32_FAST,
                   quirks=QUIRKS_REF):
@@ -518,6 +530,18 @@ class Context:
         self._check(self.L.bimsim_batch_run(self.h, instances, len(instances), S, len(spheres),
                                             _ptr(waves), prec, quirks, _ptr(D), _ptr(Di),
                                             _ptr(Abs)))
+
+    def detector_zero_filled(self, U, Uf, uR, vR, fill, du, dv, NAin, NAout, lam, D, Di,
+                             Abs=None, prec=FP64_PARITY, quirks=QUIRKS_REF):
+        """The detector on the field zero-filled to fill x its size for the FFT."""
+        check_field(U, prec, uR * vR, "U")
+        check_field(Uf, prec, uR * vR, "Uf")
+        for t, name in ((D, "D"), (Di, "Di")):
+            check_real(t, prec, uR * uR, name)
+        check_real(Abs, prec, uR * uR, "Abs", optional=True)
+        self._check(self.L.bimsim_detector_zero_filled(self.h, prec, quirks, _ptr(U), _ptr(Uf), uR,
+                                                       vR, fill, du, dv, NAin, NAout, lam, _ptr(D),
+                                                       _ptr(Di), _ptr(Abs)))
 
     def detector(self, U, Uf, uR, vR, du, dv, NAin, NAout, lam, padding, Nd, ss, D, Di,
                  Abs=None, prec=FP64_PARITY, quirks=QUIRKS_REF, write_filtered=True):
@@ -543,9 +567,10 @@ class Batch:
         self.ctx._check(self.ctx.L.bimsim_batch_launch(self.h))
 
     def close(self):
-        if getattr(self, "h", None):
+        # a batch lives on its context: it is gone once the context is
+        if getattr(self, "h", None) and getattr(self.ctx, "h", None):
             self.ctx.L.bimsim_batch_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

@@ -89,6 +89,12 @@ int detector_run(bimsim_ctx* ctx, int prec, int quirks, void* U, void* Uf, int u
                  int Nd, int ss, void* D, void* Di, void* Abs, int write_filtered,
                  const SumFuse* fz);
 
+// bimsim_detector_zero_filled's work (the images accumulate, like
+// detector_run's).
+int detector_zero_filled_run(bimsim_ctx* ctx, int prec, int quirks, const void* U, const void* Uf,
+                             int uR, int vR, int fill, double du, double dv, double NAin,
+                             double NAout, double lambda, void* D, void* Di, void* Abs);
+
 // Zeroes `bytes` (a multiple of 4) at p with a kernel (graph-safe).
 int zero_launch(bimsim_ctx* ctx, void* p, size_t bytes);
 

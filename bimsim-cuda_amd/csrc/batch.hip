@@ -164,6 +164,14 @@ int enqueue(bimsim_batch* b) {
     for (int i = 0; i < b->n; ++i) {
         const bimsim_instance& I = b->inst[i];
         void* Ab = b->Abs ? (char*)b->Abs + rb * I.image_off : nullptr;
+        if (I.zero_fill >= 2) {
+            BIMSIM_TRY(detector_zero_filled_run(
+                ctx, b->prec, b->quirks, (char*)b->d_U + fb * b->field_off[i],
+                (char*)b->d_Uf + fb * b->field_off[i], I.N, I.N, I.zero_fill, b->du[i], b->dv[i],
+                I.objective[0], I.objective[1], I.lambda, (char*)b->D + rb * I.image_off,
+                (char*)b->Di + rb * I.image_off, Ab));
+            continue;
+        }
         SumFuse fz{};
         if (b->fuse_sum) {
             const SumInst& S = b->sum_host[i];
@@ -215,6 +223,10 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
                          "Monte-Carlo sum");
         const bool plane = I.uf_mode == BIMSIM_UF_PLANE;
         BIMSIM_CHECK_ARG(ctx, plane || (I.nk >= 1 && I.wave_begin >= 0), "instance needs waves");
+        BIMSIM_CHECK_ARG(ctx, I.zero_fill <= 1 || (I.zero_fill <= 8 && I.Nd == I.N &&
+                                                    I.padding == 0 && I.ss == 1),
+                         "a zero-filled detector takes Nd = N, padding 0, ss 1 and a factor "
+                         "of at most 8");
         BIMSIM_CHECK_ARG(ctx, I.Nd > 0 && I.ss > 0 && I.padding >= 0 &&
                                   (long long)(2 * I.padding + 1) * I.Nd * I.ss <= I.N && I.image_off >= 0,
                          "instance detector does not fit its near field");
@@ -333,7 +345,9 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
         b->prepass = b->have_mc != b->have_pw && P.harmonic && b->n_sph > 0 &&
                      prepass_lds(b->pre) <= kPrepassLds;
         b->fuse_sum = b->prepass;
-        for (int i = 0; i < n; ++i) b->fuse_sum = b->fuse_sum && detector_fusable(ctx, inst[i].N, inst[i].N);
+        for (int i = 0; i < n; ++i)
+            b->fuse_sum = b->fuse_sum && inst[i].zero_fill <= 1 &&
+                          detector_fusable(ctx, inst[i].N, inst[i].N);
         if (b->fuse_sum) {
             b->sum_host = sum;
             b->pre.sum = b->d_sum;

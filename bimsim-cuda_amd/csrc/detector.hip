@@ -760,6 +760,45 @@ int detector_impl(bimsim_ctx* ctx, void* Uin, void* Ufin, int uR, int vR, double
     return BIMSIM_OK;
 }
 
+// dst (M x Mv) = src (uR x vR) at the corner, zeros elsewhere.
+template <class C>
+__global__ void embed_kernel(const C* __restrict__ src, int uR, int vR, C* __restrict__ dst, int M,
+                             int Mv) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)M * Mv) return;
+    const int iv = (int)(i / M), iu = (int)(i - (long long)iv * M);
+    C v;
+    v.x = 0;
+    v.y = 0;
+    if (iu < uR && iv < vR) v = src[(long long)iv * uR + iu];
+    dst[i] = v;
+}
+
+// The detector on the near field zero-filled to (fill uR) x (fill vR): both
+// fields embedded at the corner of scratch arrays, the FFT and band-pass on
+// the finer frequency grid (du/fill, dv/fill), the crop of the field's own
+// uR x uR samples at the corner.
+template <class R>
+int detector_zero_filled_impl(bimsim_ctx* ctx, const void* U, const void* Uf, int uR, int vR,
+                              int fill, double du, double dv, double NAin, double NAout,
+                              double lambda, void* D, void* Di, void* Abs, int quirks) {
+    using C = typename C2<R>::T;
+    const int M = fill * uR, Mv = fill * vR;
+    const size_t n = (size_t)M * Mv;
+    void *a = nullptr, *b = nullptr;
+    BIMSIM_TRY(scratch_get(ctx, kSlotDet, sizeof(C) * 2 * n, &a));
+    b = (char*)a + sizeof(C) * n;
+    const dim3 grid((unsigned)((n + 255) / 256));
+    hipLaunchKernelGGL(embed_kernel<C>, grid, dim3(256), 0, ctx->stream, (const C*)U, uR, vR, (C*)a,
+                       M, Mv);
+    hipLaunchKernelGGL(embed_kernel<C>, grid, dim3(256), 0, ctx->stream, (const C*)Uf, uR, vR,
+                       (C*)b, M, Mv);
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    // the embedded copies are ours: the transform may run in place on them
+    return detector_impl<R>(ctx, a, b, M, Mv, du / fill, dv / fill, NAin, NAout, lambda, 0, uR, 1,
+                            D, Di, Abs, 1, quirks, nullptr);
+}
+
 // fieldslice::toAngularSpectrum / fromAngularSpectrum (fieldslice.cpp:
 // 25-94): in-place forward (e^{-i}) or inverse (e^{+i}) 2D FFT, the inverse
 // followed by the 1/(R0 R1) of ScaleField.
@@ -832,6 +871,40 @@ extern "C" int bimsim_detector(bimsim_ctx* ctx, int prec, int quirks, void* U, v
                                int write_filtered) {
     return detector_run(ctx, prec, quirks, U, Uf, uR, vR, du, dv, NAin, NAout, lambda, padding, Nd,
                         ss, D, Di, Abs, write_filtered, nullptr);
+}
+
+namespace bimsim {
+
+int detector_zero_filled_run(bimsim_ctx* ctx, int prec, int quirks, const void* U, const void* Uf,
+                             int uR, int vR, int fill, double du, double dv, double NAin,
+                             double NAout, double lambda, void* D, void* Di, void* Abs) {
+    if (!ctx) return BIMSIM_E_INVALID;
+    BIMSIM_CHECK_ARG(ctx, U && Uf && D && Di, "null pointer argument");
+    BIMSIM_CHECK_ARG(ctx, uR > 0 && uR == vR, "the zero-filled detector needs a square field");
+    BIMSIM_CHECK_ARG(ctx, fill >= 1 && fill <= 8 && (long long)fill * uR * fill * vR <= (1LL << 28),
+                     "zero-fill factor must be in [1, 8] and the filled field at most 2^28 samples");
+    BIMSIM_CHECK_ARG(ctx, lambda > 0.0 && du > 0.0 && dv > 0.0, "lambda, du and dv must be positive");
+    BIMSIM_CHECK_ARG(ctx, prec >= BIMSIM_FP64_PARITY && prec <= BIMSIM_FP32_FAST, "unknown precision");
+    BIMSIM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    const int tmr = timer_begin(ctx, kStageDet);
+    const int rc = prec == BIMSIM_FP32_FAST
+                       ? detector_zero_filled_impl<float>(ctx, U, Uf, uR, vR, fill, du, dv, NAin,
+                                                          NAout, lambda, D, Di, Abs, quirks)
+                       : detector_zero_filled_impl<double>(ctx, U, Uf, uR, vR, fill, du, dv, NAin,
+                                                           NAout, lambda, D, Di, Abs, quirks);
+    if (rc != BIMSIM_OK) return rc;
+    timer_end(ctx, tmr);
+    return BIMSIM_OK;
+}
+
+}  // namespace bimsim
+
+extern "C" int bimsim_detector_zero_filled(bimsim_ctx* ctx, int prec, int quirks, const void* U,
+                                           const void* Uf, int uR, int vR, int fill, double du,
+                                           double dv, double NAin, double NAout, double lambda,
+                                           void* D, void* Di, void* Abs) {
+    return detector_zero_filled_run(ctx, prec, quirks, U, Uf, uR, vR, fill, du, dv, NAin, NAout,
+                                    lambda, D, Di, Abs);
 }
 
 extern "C" int bimsim_field_fft(bimsim_ctx* ctx, int prec, void* F, int uR, int vR, int inverse) {

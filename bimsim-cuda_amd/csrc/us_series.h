@@ -51,13 +51,24 @@ struct SeriesSph {
     long long coff;     // offset of c'[l][pix] in the chunk's coefficient buffer
     long long roff;     // offset of r-hat[pix] in the chunk's r-hat buffer
     long long toff;     // harmonic form: offset of its harmonic records
-    int tl;             // harmonic form: record stride (its instance's order bucket)
+    int tl;             // harmonic form: record stride (its order bucket)
     int planar;         // harmonic form: the slice contains the centre and is normal
                         // to z, so every pixel has x = r-hat.z = 0
 };
 
 struct SeriesItem {
-    int inst, pix0, split, pad;
+    int inst, pix0, split;
+    int grp;            // harmonic form: its sphere group
+};
+
+// Harmonic form: the spheres of one instance in one order bucket, a
+// consecutive range of the sphere table (the plan sorts each instance's
+// spheres by bucket).  The first group of an instance in launch order writes
+// (or, with the instance's accumulate, adds to) its field, later groups add.
+struct SeriesGroup {
+    int s_begin, ns;
+    int accumulate;
+    int L;
 };
 
 // Host description of a batch: the caller fills every SeriesInst field up
@@ -138,6 +149,7 @@ struct SeriesPlan {
     float4* d_rec = nullptr;      // harmonic records per (sphere, m, l)
     float* d_cst = nullptr;       // scaled Legendre recurrence constants per (m, l)
     int* d_perm = nullptr;        // pixel orders of the instances with perm_off >= 0
+    SeriesGroup* d_grp = nullptr; // sphere groups of the harmonic form
     double* d_h64 = nullptr;      // FP64 Legendre constants and scales of the wave sums
 };
 

@@ -380,138 +380,163 @@ __global__ void harm_legendre_kernel(const SeriesInst* __restrict__ insts,
     }
 }
 
-// Records per (sphere, m, l) from the wave tables.  Block (m, tile, s) forms
-// four records of row m of sphere s: orders l = 4 tile + j, or for a planar
-// sphere k = 4 tile + j with l = 2k + (m & 1).  Its threads run over the
-// instance's waves (coalesced reads of the Legendre, azimuth-phase and
-// wave-weight tables), summing T+- = sum_w phi_w Pn_lm e^{+-i m phi_w}; a
-// block reduction forms the records: 4 pi/(2l+1) K_lm (or Pn_lm(0) for
-// planar spheres) times U1 = T+ + T-, U2 = (Im T+ - Im T-, Re T- - Re T+)
-// (U1 = T+, U2 = 0 at m = 0), zero outside m <= l <= Nl.
-// Grid: (m < max bucket, tiles, spheres).
+// Records per (sphere, m, l) from the wave tables.  Block (m, instance)
+// forms row m of the records of every sphere of the instance: with
+// G_w = Pn_lm(cos th_w) e^{i m phi_w} and the sphere's wave weights phi_w,
+//   S = sum_w phi_w G_w,  S' = sum_w phi_w conj(G_w),
+// i.e. per (order, sphere) four real sums a = sum Re phi Re G,
+// b = sum Im phi Im G, c = sum Re phi Im G, d = sum Im phi Re G, with
+// S = (a - b, c + d) and S' = (a + b, d - c); the records hold
+//   U1 = S + S' = (2a, 2d),  U2 = (Im S - Im S', Re S' - Re S) = (2c, 2b)
+// (at m = 0, G is real: U1 = S = (a, d), U2 = 0).  A record is 4 pi/(2l+1)
+// K_lm (or Pn_lm(0) for a planar sphere) times (U1, U2).  The waves run in
+// chunks of kRecW staged in LDS (the Legendre values of the row's orders,
+// the azimuth phases, the weights of the instance's spheres); each thread
+// owns up to kRecPer (order, four spheres) combinations and sums them in
+// FP64, so the wave tables are read once per row for all spheres.  Planar-
+// only instances form only the orders l = m, m + 2, ...; the orders a
+// sphere does not need (past its Nl, or l + m odd when planar) are not
+// written (the plan clears the record tables once).
+// Grid: (m < the largest Legendre order count, instances).
 constexpr int kRecThreads = 256;
 // Limits of the harmonic form's per-instance pixel orders: pixels in the
 // plan and pixel-sphere tests on the host to build them.
 constexpr long long kPermMaxPixels = 16LL << 20;
 constexpr long long kPermMaxTests = 128LL << 20;
-constexpr int kRecTile = 4;
+constexpr int kRecW = 16;         // waves per LDS chunk
+constexpr int kRecL = 128;        // orders per tile
+constexpr int kRecS = 64;         // spheres per tile (16 groups of four)
+constexpr int kRecPer = 2;        // combinations per thread and pass
 
-__device__ __forceinline__ void harm_records_block(
-    int m, int tile, int s_first, int s_step, const SeriesInst* __restrict__ insts,
-    const SeriesSph* __restrict__ sph, int n_sph, const double* __restrict__ pl,
-    const double2* __restrict__ e, const double2* __restrict__ phi,
+__device__ __forceinline__ void harm_records_row(
+    int m, int i, const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph,
+    const double* __restrict__ pl, const double2* __restrict__ e, const double2* __restrict__ phi,
     const double* __restrict__ h64, float4* __restrict__ rec) {
-    __shared__ double red[kRecThreads / 64][4 * kRecTile];
-    for (int s = s_first; s < n_sph; s += s_step) {
-        const SeriesSph& SS = sph[s];
-        const int tl = SS.tl;
-        const bool planar = SS.planar != 0;
-        const int width = planar ? tl / 2 : tl;   // records per row
-        if (m >= tl || kRecTile * tile >= width) continue;
-        const SeriesInst& I = insts[SS.inst];
-        const int nl1 = SS.s.Nl + 1;
-        int lj[kRecTile];
-        bool need[kRecTile];
-        bool any = false;
+    __shared__ double sP[kRecW][kRecL];      // Pn_lm(cos th_w) of the tile's orders
+    __shared__ double2 sF[kRecW][kRecS];     // wave weights of the tile's spheres
+    __shared__ double2 sE[kRecW];            // e^{i m phi_w}
+    const SeriesInst& I = insts[i];
+    if (m >= I.lh) return;
+    const int t = (int)threadIdx.x;
+    bool all_planar = true;
+    for (int s = 0; s < I.ns; ++s) all_planar = all_planar && sph[I.s_begin + s].planar != 0;
+    const int lstep = all_planar ? 2 : 1;
+    const int nlv = (I.lh - m + lstep - 1) / lstep;   // orders l = m + j lstep
+    const double* Prow = pl + I.pl_off + tri_off(m, I.lh) * I.nk;
+    const double2* E = e + I.e_off + (long long)m * I.nk;
+    for (int s0 = 0; s0 < I.ns; s0 += kRecS) {
+        const int nst = min(kRecS, I.ns - s0);
+        const int nq = (nst + 3) >> 2;
+        for (int j0 = 0; j0 < nlv; j0 += kRecL) {
+            const int nlt = min(kRecL, nlv - j0);
+            const int combos = nq * nlt;
+            for (int c0 = 0; c0 < combos; c0 += kRecPer * kRecThreads) {
+                // combination c: order index c % nlt, sphere group c / nlt
+                double acc[kRecPer][4][4];
 #pragma unroll
-        for (int j = 0; j < kRecTile; ++j) {
-            const int k = kRecTile * tile + j;
-            lj[j] = planar ? 2 * k + (m & 1) : k;
-            need[j] = k < width && m < nl1 && lj[j] >= m && lj[j] < nl1;
-            any = any || need[j];
-        }
-        // (T+ re, T+ im, T- re, T- im) of each record
-        double acc[4 * kRecTile];
+                for (int u = 0; u < kRecPer; ++u)
 #pragma unroll
-        for (int q = 0; q < 4 * kRecTile; ++q) acc[q] = 0.0;
-        if (any) {
-            const double* P[kRecTile];
+                    for (int v = 0; v < 4; ++v)
 #pragma unroll
-            for (int j = 0; j < kRecTile; ++j)
-                P[j] = pl + I.pl_off + (tri_off(m, I.lh) + (need[j] ? lj[j] - m : 0)) * I.nk;
-            const double2* E = e + I.e_off + (long long)m * I.nk;
-            const double2* F = phi + SS.woff;
-            for (int w = threadIdx.x; w < I.nk; w += kRecThreads) {
-                const double2 ew = E[w], fw = F[w];
-#pragma unroll
-                for (int j = 0; j < kRecTile; ++j) {
-                    if (!need[j]) continue;
-                    const double p = P[j][w];
-                    const double ar = fw.x * p, ai = fw.y * p;
-                    acc[4 * j] += ar * ew.x - ai * ew.y;
-                    acc[4 * j + 1] += ar * ew.y + ai * ew.x;
-                    acc[4 * j + 2] += ar * ew.x + ai * ew.y;
-                    acc[4 * j + 3] += ai * ew.x - ar * ew.y;
-                }
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < 4 * kRecTile; ++q) {
-            double v = acc[q];
-            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-            acc[q] = v;
-        }
-        __syncthreads();   // red is reused for the next sphere
-        if ((threadIdx.x & 63) == 0)
-#pragma unroll
-            for (int q = 0; q < 4 * kRecTile; ++q) red[threadIdx.x >> 6][q] = acc[q];
-        __syncthreads();
-        if (threadIdx.x < kRecTile) {
-            const int j = threadIdx.x;
-            const int k = kRecTile * tile + j;
-            if (k < width) {
-                float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (need[j]) {
-                    dc tp = {0.0, 0.0}, tm = {0.0, 0.0};
-                    for (int q = 0; q < kRecThreads / 64; ++q) {
-                        tp = cadd(tp, dc{red[q][4 * j], red[q][4 * j + 1]});
-                        tm = cadd(tm, dc{red[q][4 * j + 2], red[q][4 * j + 3]});
+                        for (int q = 0; q < 4; ++q) acc[u][v][q] = 0.0;
+                for (int w0 = 0; w0 < I.nk; w0 += kRecW) {
+                    const int nw = min(kRecW, I.nk - w0);
+                    __syncthreads();   // the previous chunk is consumed
+                    for (int x = t; x < nw * nlt; x += kRecThreads) {
+                        const int w = x / nlt, jj = x - w * nlt;
+                        const int l = m + (j0 + jj) * lstep;
+                        sP[w][jj] = Prow[(long long)(l - m) * I.nk + w0 + w];
                     }
-                    const int l = lj[j];
-                    double f;
-                    if (planar) {
-                        // Pn_lm(0): Pn_mm(0) = Pn_00 prod_q sqrt((2q+1)/(2q)),
-                        // Pn_lm(0) = -a_lm b_lm Pn_{l-2,m}(0)
-                        double pn = 0.28209479177387814;
-                        for (int q = 1; q <= m; ++q) pn *= h64[kHDiag + q];
-                        const double* ab = h64 + 2 * (m * kH);
-                        for (int q = m + 2; q <= l; q += 2) pn = -(ab[2 * q] * ab[2 * q + 1]) * pn;
-                        f = 4.0 * M_PI / (2.0 * l + 1.0) * pn;
-                    } else {
-                        f = 4.0 * M_PI / (2.0 * l + 1.0) * h64[kHK + m * kH + l];
+                    for (int x = t; x < nw * nst; x += kRecThreads) {
+                        const int w = x / nst, ss = x - w * nst;
+                        sF[w][ss] = phi[sph[I.s_begin + s0 + ss].woff + w0 + w];
                     }
-                    if (m == 0)
-                        r = make_float4((float)(f * tp.r), (float)(f * tp.i), 0.f, 0.f);
-                    else
-                        r = make_float4((float)(f * (tp.r + tm.r)), (float)(f * (tp.i + tm.i)),
-                                        (float)(f * (tp.i - tm.i)), (float)(f * (tm.r - tp.r)));
+                    if (t < nw) sE[t] = E[w0 + t];
+                    __syncthreads();
+#pragma unroll
+                    for (int u = 0; u < kRecPer; ++u) {
+                        const int c = c0 + u * kRecThreads + t;
+                        if (c >= combos) continue;
+                        const int jj = c % nlt, g = c / nlt;
+                        for (int w = 0; w < nw; ++w) {
+                            const double p = sP[w][jj];
+                            const double2 ew = sE[w];
+                            const double gr = p * ew.x, gi = p * ew.y;
+#pragma unroll
+                            for (int v = 0; v < 4; ++v) {
+                                const double2 f = sF[w][min(4 * g + v, nst - 1)];
+                                acc[u][v][0] = fma(f.x, gr, acc[u][v][0]);
+                                acc[u][v][1] = fma(f.y, gi, acc[u][v][1]);
+                                acc[u][v][2] = fma(f.x, gi, acc[u][v][2]);
+                                acc[u][v][3] = fma(f.y, gr, acc[u][v][3]);
+                            }
+                        }
+                    }
                 }
-                rec[SS.toff + (long long)m * width + k] = r;
+                // the records of this pass's combinations
+#pragma unroll
+                for (int u = 0; u < kRecPer; ++u) {
+                    const int c = c0 + u * kRecThreads + t;
+                    if (c >= combos) continue;
+                    const int jj = c % nlt, g = c / nlt;
+                    const int l = m + (j0 + jj) * lstep;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int ss = 4 * g + v;
+                        if (ss >= nst) continue;
+                        const SeriesSph& SS = sph[I.s_begin + s0 + ss];
+                        const bool planar = SS.planar != 0;
+                        if (l > SS.s.Nl || l >= SS.tl || (planar && ((l + m) & 1))) continue;
+                        double f;
+                        if (planar) {
+                            // Pn_mm(0) = Pn_00 prod_q sqrt((2q+1)/(2q)),
+                            // Pn_lm(0) = -a_lm b_lm Pn_{l-2,m}(0)
+                            double pn = 0.28209479177387814;
+                            for (int q = 1; q <= m; ++q) pn *= h64[kHDiag + q];
+                            const double* ab = h64 + 2 * (m * kH);
+                            for (int q = m + 2; q <= l; q += 2) pn = -(ab[2 * q] * ab[2 * q + 1]) * pn;
+                            f = 4.0 * M_PI / (2.0 * l + 1.0) * pn;
+                        } else {
+                            f = 4.0 * M_PI / (2.0 * l + 1.0) * h64[kHK + m * kH + l];
+                        }
+                        const double a = acc[u][v][0], b = acc[u][v][1];
+                        const double cc = acc[u][v][2], d = acc[u][v][3];
+                        float4 r;
+                        if (m == 0)
+                            r = make_float4((float)(f * a), (float)(f * d), 0.f, 0.f);
+                        else
+                            r = make_float4((float)(f * 2.0 * a), (float)(f * 2.0 * d),
+                                            (float)(f * 2.0 * cc), (float)(f * 2.0 * b));
+                        const long long o = planar ? (long long)m * (SS.tl / 2) + (l - (m & 1)) / 2
+                                                   : (long long)m * SS.tl + l;
+                        rec[SS.toff + o] = r;
+                    }
+                }
             }
         }
     }
 }
 
 __global__ __launch_bounds__(kRecThreads) void harm_records_kernel(
-    const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph, int n_sph,
+    const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph, int n_inst,
     const double* __restrict__ pl, const double2* __restrict__ e, const double2* __restrict__ phi,
     const double* __restrict__ h64, float4* __restrict__ rec) {
-    harm_records_block(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.z, insts, sph, n_sph, pl, e,
-                       phi, h64, rec);
+    for (int i = blockIdx.y; i < n_inst; i += gridDim.y)
+        harm_records_row(blockIdx.x, i, insts, sph, pl, e, phi, h64, rec);
 }
 
 // The FP32 batch's pre-pass as one launch: Mie coefficients (blocks
 // [0, n_mie), mie_block over spb spheres each, Bessel arrays in dynamic
-// LDS), the harmonic records (the next n_rec blocks, one per (m, tile,
-// sphere) as harm_records_kernel) and the incident fields (the rest, 256
+// LDS), the harmonic records (the next n_rec blocks, one per (m, instance)
+// as harm_records_kernel) and the incident fields (the rest, 256
 // pixels of one instance each, which also clear its detector images when
 // a.sum is given).  None reads what another writes, and the
 // records read only the Legendre tables of the launch before, so the long
 // dependent chains of the coefficients run on a few compute units while
 // the records and fields fill the others.  The role is uniform per block.
 __global__ __launch_bounds__(kRecThreads) void prepass_kernel(
-    PrepassArgs a, int n_mie, int n_rec, int rec_m, int rec_tiles,
-    const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph, int n_sph,
+    PrepassArgs a, int n_mie, int n_rec, int rec_m,
+    const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph,
     const double* __restrict__ pl, const double2* __restrict__ e, const double2* __restrict__ phi,
     const double* __restrict__ h64, float4* __restrict__ rec) {
     static_assert(kRecThreads >= mie::kMieThreads, "the coefficient waves fit the block");
@@ -533,10 +558,7 @@ __global__ __launch_bounds__(kRecThreads) void prepass_kernel(
     }
     b -= n_mie;
     if (b < n_rec) {
-        const int per_s = rec_m * rec_tiles;
-        const int s = b / per_s, r = b - s * per_s;
-        harm_records_block(r % rec_m, r / rec_m, s, n_sph, insts, sph, n_sph, pl, e, phi, h64,
-                           rec);
+        harm_records_row(b % rec_m, b / rec_m, insts, sph, pl, e, phi, h64, rec);
         return;
     }
     b -= n_rec;
@@ -727,9 +749,10 @@ constexpr int harm_waves(int L) { return L <= 32 ? 4 : (L <= 48 ? 3 : (L <= 80 ?
 template <int L, bool kPlanar>
 __global__ __launch_bounds__(kThreads, harm_waves(L)) void series_harm_kernel(
     const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph,
-    const SeriesItem* __restrict__ items, int n_items, const double2* __restrict__ A,
-    const double2* __restrict__ B, const float4* __restrict__ rec, const float* __restrict__ cst,
-    const int* __restrict__ perm, float2* __restrict__ U, int* __restrict__ counter) {
+    const SeriesGroup* __restrict__ grps, const SeriesItem* __restrict__ items, int n_items,
+    const double2* __restrict__ A, const double2* __restrict__ B, const float4* __restrict__ rec,
+    const float* __restrict__ cst, const int* __restrict__ perm, float2* __restrict__ U,
+    int* __restrict__ counter) {
     __shared__ int s_item;
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
@@ -748,8 +771,9 @@ __global__ __launch_bounds__(kThreads, harm_waves(L)) void series_harm_kernel(
         const int pix = I.perm_off >= 0 ? perm[I.perm_off + j] : j;
         const int iv = pix / I.uR, iu = pix - iv * I.uR;
         const d3 p = quad_at(I.q, (double)iu / (double)I.uR, (double)iv / (double)I.vR);
+        const SeriesGroup G = grps[item.grp];
         float acc_r = 0.f, acc_i = 0.f;
-        for (int s = I.s_begin; s < I.s_begin + I.ns; ++s) {
+        for (int s = G.s_begin; s < G.s_begin + G.ns; ++s) {
             const SeriesSph& SS = sph[s];
             const SphereDev S = SS.s;
             const int nl1 = S.Nl + 1;
@@ -843,7 +867,7 @@ __global__ __launch_bounds__(kThreads, harm_waves(L)) void series_harm_kernel(
         }
         float2* o = U + I.u_off + pix;
         float2 v = make_float2(0.f, 0.f);
-        if (I.accumulate) v = *o;
+        if (G.accumulate) v = *o;
         *o = make_float2(v.x + acc_r, v.y + acc_i);
     }
 }
@@ -863,12 +887,12 @@ void launch_harm(dim3 grid, hipStream_t st, const SeriesPlan& P, const SeriesPla
     // general branch (fewer registers live)
     if (K.planar)
         hipLaunchKernelGGL((series_harm_kernel<L, true>), grid, dim3(kThreads), 0, st, P.d_inst,
-                           P.d_sph, P.d_items + K.item_begin, K.item_count, A, B, P.d_rec,
-                           P.d_cst, P.d_perm, U, P.d_counters + K.counter);
+                           P.d_sph, P.d_grp, P.d_items + K.item_begin, K.item_count, A, B,
+                           P.d_rec, P.d_cst, P.d_perm, U, P.d_counters + K.counter);
     else
         hipLaunchKernelGGL((series_harm_kernel<L, false>), grid, dim3(kThreads), 0, st, P.d_inst,
-                           P.d_sph, P.d_items + K.item_begin, K.item_count, A, B, P.d_rec,
-                           P.d_cst, P.d_perm, U, P.d_counters + K.counter);
+                           P.d_sph, P.d_grp, P.d_items + K.item_begin, K.item_count, A, B,
+                           P.d_rec, P.d_cst, P.d_perm, U, P.d_counters + K.counter);
 }
 
 template <class T>
@@ -906,6 +930,7 @@ void series_plan_destroy(SeriesPlan* P) {
     (void)hipFree(P->d_sph);
     (void)hipFree(P->d_items);
     (void)hipFree(P->d_perm);
+    (void)hipFree(P->d_grp);
     (void)hipFree(P->d_split_insts);
     (void)hipFree(P->d_counters);
     (void)hipFree(P->d_wt);
@@ -964,7 +989,33 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
     for (const SeriesInst& I : batch.inst) min_nk = std::min(min_nk, I.nk);
     P->harmonic = ctx->series_algo == kSeriesHarmonic ||
                   (ctx->series_algo == kSeriesAuto && min_nk >= 16);
-    if (P->harmonic) split_mult = 1;
+    if (P->harmonic) {
+        split_mult = 1;
+        // each instance's spheres sorted by their own order bucket, so that
+        // a bucket's spheres form one consecutive group per instance
+        for (const SeriesInst& I : batch.inst)
+            std::stable_sort(batch.sph.begin() + I.s_begin, batch.sph.begin() + I.s_begin + I.ns,
+                             [](const SeriesSph& a, const SeriesSph& b) {
+                                 return series_capacity(a.s.Nl + 1) < series_capacity(b.s.Nl + 1);
+                             });
+    }
+    // harmonic form: the sphere groups of each instance, in bucket order
+    std::vector<SeriesGroup> groups;
+    std::vector<std::vector<int>> inst_groups(ni);
+    if (P->harmonic) {
+        for (int i = 0; i < ni; ++i) {
+            const SeriesInst& I = batch.inst[i];
+            for (int s = I.s_begin; s < I.s_begin + I.ns;) {
+                const int L = series_capacity(batch.sph[s].s.Nl + 1);
+                int e = s + 1;
+                while (e < I.s_begin + I.ns && series_capacity(batch.sph[e].s.Nl + 1) == L) ++e;
+                inst_groups[i].push_back((int)groups.size());
+                groups.push_back(SeriesGroup{s, e - s,
+                                             inst_groups[i].size() == 1 ? I.accumulate : 1, L});
+                s = e;
+            }
+        }
+    }
     // split the waves until the queue holds several items per CU (each
     // split keeps at least 32 waves)
     const long long target = 4LL * ctx->cu_count;
@@ -1037,15 +1088,27 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
             const int begin = (int)items.size();
             int all_planar = 1;
             for (int i : order) {
-                if (bucket[i] != L) continue;
                 const SeriesInst& I = batch.inst[i];
+                const int npix = I.uR * I.vR;
+                if (P->harmonic) {
+                    // the instance's group of this bucket, if any
+                    for (int g : inst_groups[i]) {
+                        const SeriesGroup& G = groups[g];
+                        if (G.L != L) continue;
+                        for (int t = G.s_begin; t < G.s_begin + G.ns; ++t)
+                            if (!harm_planar(I.q, batch.sph[t].s)) all_planar = 0;
+                        for (int p0 = 0; p0 < npix; p0 += kThreads)
+                            items.push_back(SeriesItem{i, p0, 0, g});
+                    }
+                    continue;
+                }
+                if (bucket[i] != L) continue;
                 for (int t = I.s_begin; t < I.s_begin + I.ns; ++t)
                     if (!harm_planar(I.q, batch.sph[t].s)) all_planar = 0;
-                const int npix = I.uR * I.vR;
                 // the splits of a pixel block follow each other, so that they
                 // run side by side and share its coefficients in L2
                 for (int p0 = 0; p0 < npix; p0 += kThreads)
-                    for (int t = 0; t < I.nsplit; ++t) items.push_back(SeriesItem{i, p0, t, 0});
+                    for (int t = 0; t < I.nsplit; ++t) items.push_back(SeriesItem{i, p0, t, -1});
             }
             const int count = (int)items.size() - begin;
             if (count > 0)
@@ -1074,7 +1137,7 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
         for (int t = 0; t < P->n_sph; ++t) {
             SeriesSph& S = batch.sph[t];
             const SeriesInst& I = batch.inst[S.inst];
-            S.tl = bucket[S.inst];
+            S.tl = series_capacity(S.s.Nl + 1);
             P->max_tl = std::max(P->max_tl, S.tl);
             S.planar = harm_planar(I.q, S.s) ? 1 : 0;
             S.toff = nrec;
@@ -1166,7 +1229,8 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
     BIMSIM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     BIMSIM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     int rc;
-    if (!perm.empty() && (rc = upload(ctx, &P->d_perm, perm)) != BIMSIM_OK) {
+    if ((!perm.empty() && (rc = upload(ctx, &P->d_perm, perm)) != BIMSIM_OK) ||
+        (rc = upload(ctx, &P->d_grp, groups)) != BIMSIM_OK) {
         series_plan_destroy(P);
         return rc;
     }
@@ -1180,6 +1244,10 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
         series_plan_destroy(P);
         return rc;
     }
+    // the records kernel writes only the records the spheres use: the rest
+    // of each table stays zero, as the pixel kernel's blocks may read past Nl
+    if (P->harmonic && P->n_rec > 0)
+        BIMSIM_HIP_CHECK(ctx, hipMemset(P->d_rec, 0, sizeof(float4) * (size_t)P->n_rec));
     if ((rc = upload(ctx, &P->d_inst, batch.inst)) != BIMSIM_OK ||
         (rc = upload(ctx, &P->d_sph, batch.sph)) != BIMSIM_OK ||
         (rc = upload(ctx, &P->d_items, items)) != BIMSIM_OK ||
@@ -1212,10 +1280,9 @@ int series_plan_launch_waves(bimsim_ctx* ctx, const SeriesPlan& P, const double*
         hipLaunchKernelGGL(harm_legendre_kernel, lgrid, dim3(64), 0, st, P.d_inst, P.d_sph,
                            P.n_inst, waves, P.d_h64, P.d_pl, P.d_e, P.d_phi, P.d_counters,
                            P.n_counters);
-        const dim3 rgrid(P.max_tl, (P.max_tl + kRecTile - 1) / kRecTile,
-                         std::min(P.n_sph, 65535));
+        const dim3 rgrid(P.max_lh, std::min(P.n_inst, 65535));
         hipLaunchKernelGGL(harm_records_kernel, rgrid, dim3(kRecThreads), 0, st, P.d_inst,
-                           P.d_sph, P.n_sph, P.d_pl, P.d_e, P.d_phi, P.d_h64, P.d_rec);
+                           P.d_sph, P.n_inst, P.d_pl, P.d_e, P.d_phi, P.d_h64, P.d_rec);
     } else {
         hipLaunchKernelGGL(series_waves_kernel, wgrid, dim3(256), 0, st, P.d_inst, P.d_sph, P.n_sph,
                            waves, P.d_wt, P.d_counters, P.n_counters);
@@ -1238,13 +1305,12 @@ int series_plan_launch_prepass(bimsim_ctx* ctx, const SeriesPlan& P, const doubl
     hipLaunchKernelGGL(harm_legendre_kernel, lgrid, dim3(64), 0, st, P.d_inst, P.d_sph, P.n_inst,
                        waves, P.d_h64, P.d_pl, P.d_e, P.d_phi, P.d_counters, P.n_counters);
     const int n_mie = (a.n_mie_sph + a.spb - 1) / a.spb;
-    const int tiles = (P.max_tl + kRecTile - 1) / kRecTile;
-    const long long n_rec = (long long)P.max_tl * tiles * P.n_sph;
+    const long long n_rec = (long long)P.max_lh * P.n_inst;
     const long long total = n_mie + n_rec + (long long)a.uf_blocks * a.n_uf;
     if (total > INT_MAX)
         return set_error(ctx, BIMSIM_E_INVALID, "pre-pass launch outside its limits");
     hipLaunchKernelGGL(prepass_kernel, dim3((unsigned)total), dim3(kRecThreads), prepass_lds(a), st,
-                       a, n_mie, (int)n_rec, P.max_tl, tiles, P.d_inst, P.d_sph, P.n_sph, P.d_pl,
+                       a, n_mie, (int)n_rec, P.max_lh, P.d_inst, P.d_sph, P.d_pl,
                        P.d_e, P.d_phi, P.d_h64, P.d_rec);
     BIMSIM_HIP_CHECK(ctx, hipGetLastError());
     return BIMSIM_OK;

@@ -59,6 +59,11 @@ def simulate(ctx, inst, waves, prec, buf=None, quirks=QUIRKS_REF, write_filtered
     ctx.nf_sum_uf(q, N, N, inst["spheres"], buf.Uf, buf.U, prec=prec)
     buf.D.zero_()
     buf.Di.zero_()
+    if inst.get("zero_fill", 0) >= 2:
+        ctx.detector_zero_filled(buf.U, buf.Uf, N, N, inst["zero_fill"], inst["du"], inst["dv"],
+                                 inst["obj"][0], inst["obj"][1], inst["lam"], buf.D, buf.Di,
+                                 buf.Abs, prec=prec, quirks=quirks)
+        return buf
     ctx.detector(buf.U, buf.Uf, N, N, inst["du"], inst["dv"], inst["obj"][0], inst["obj"][1],
                  inst["lam"], inst["padding"], inst["Nd"], inst["ss"], buf.D, buf.Di, buf.Abs,
                  prec=prec, quirks=quirks, write_filtered=write_filtered)
@@ -110,6 +115,7 @@ def batch_tables(insts, waves):
         I.Nd = inst["Nd"]
         I.ss = inst["ss"]
         I.image_off = img
+        I.zero_fill = inst.get("zero_fill", 0)
         img += inst["Nd"] * inst["Nd"]
     W = np.concatenate(rows) if rows else np.zeros((1, 3))
     return arr, spheres, W, img

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define BIMSIM_ABI_VERSION 1
+#define BIMSIM_ABI_VERSION 2
 
 typedef struct bimsim_ctx bimsim_ctx;
 
@@ -243,6 +243,20 @@ int bimsim_detector(bimsim_ctx* ctx, int prec, int quirks, void* U, void* Uf, in
                     double lambda, int padding, int Nd, int ss, void* D,
                     void* Di, void* Abs, int write_filtered);
 
+/* The detector on the near field zero-filled to (fill uR) x (fill vR) for
+ * the FFT: U and Uf sit at the corner [0, uR) x [0, vR) of zero arrays, the
+ * objective band-pass acts on the finer frequency grid (du/fill, dv/fill),
+ * and the crop is the field's own uR x uR samples there (Nd = uR = vR,
+ * ss = 1); D, Di accumulate and Abs = -log10(D/Di) as bimsim_detector.  It
+ * keeps the band-pass from wrapping the field around its edges.  No
+ * counterpart in the reference, whose --padding instead simulates a larger
+ * near field (microscope.cu:641-655,709-720); with fill = 1 it is
+ * bimsim_detector with padding 0.  U and Uf are left unchanged. */
+int bimsim_detector_zero_filled(bimsim_ctx* ctx, int prec, int quirks, const void* U,
+                                const void* Uf, int uR, int vR, int fill, double du, double dv,
+                                double NAin, double NAout, double lambda, void* D, void* Di,
+                                void* Abs);
+
 /* ------------------------------------------- slice building blocks
  * The steps of microscopeStruct::SimulateImaging one at a time, for host
  * code that calls them separately.  Fields are complex128 (FP64
@@ -412,6 +426,9 @@ typedef struct {
     int    Nd;
     int    ss;
     long long image_off;    /* element offset of the instance's Nd x Nd images  */
+    int    zero_fill;       /* >= 2: the detector of bimsim_detector_zero_filled
+                               with this factor (Nd = N, padding 0, ss 1);
+                               0 or 1: the reference detector               */
 } bimsim_instance;
 
 typedef struct bimsim_batch bimsim_batch;

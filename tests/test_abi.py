@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol(pkg):
     for n in names:
         assert hasattr(L, n), n
     assert set(pkg.EXPORTS) <= set(names)
-    assert L.bimsim_abi_version() == 1
+    assert L.bimsim_abi_version() == 2
 
 
 def test_sphere_order_matches_oracle(pkg, orc):

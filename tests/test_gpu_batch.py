@@ -106,6 +106,26 @@ def test_mc_focused_batch_of_one_equals_the_pipeline(pkg, ctx, W, prec):
 
 
 @pytest.mark.parametrize("prec", [0, 2])
+def test_zero_filled_detector_batch_equals_the_pipeline(pkg, ctx, W, prec):
+    """Instances with the 2x zero-filled detector beside reference-detector
+    ones in one batch."""
+    a = W.config(1, N=64, nk=20)
+    a["zero_fill"] = 2
+    b_ = W.config(2, N=32, nk=24)
+    insts = [a, b_]
+    waves = waves_of(pkg, W, insts)
+    b, D, Di, Ab, _ = run_batch(pkg, ctx, insts, waves, prec)
+    parts = zip(split(insts, host(D)), split(insts, host(Di)), split(insts, host(Ab)))
+    for inst, w, (d, di, ab) in zip(insts, waves, parts):
+        D1, Di1, Ab1 = single(pkg, ctx, inst, w, prec)
+        if prec == 0:
+            assert np.array_equal(d, D1) and np.array_equal(di, Di1)
+        else:
+            assert normrel(d, D1) <= 2e-6 and normrel(di, Di1) <= 2e-6
+    b.close()
+
+
+@pytest.mark.parametrize("prec", [0, 2])
 def test_plane_wave_batch_equals_the_pipeline(pkg, ctx, W, prec):
     """A batch of plane-wave instances only (the FP32 form runs its one
     plane-wave series plan with the coefficients and incident fields in the

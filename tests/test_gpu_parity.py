@@ -106,6 +106,34 @@ def test_mc_incident_field_matches_oracle(pkg, ctx, orc, W, N, nk, z):
     assert normrel(host(U32), ref) <= 2e-6
 
 
+@pytest.mark.parametrize("fill,N", [(2, 64), (2, 96), (3, 32)])
+def test_zero_filled_detector_matches_oracle(pkg, ctx, orc, fill, N):
+    """The detector on the field zero-filled to fill x N: the oracle's
+    detector on the same embedded arrays, frequency step 1/(fill L)."""
+    rng = np.random.default_rng(fill * 100 + N)
+    U = rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N))
+    Uf = 1.0 + 0.1 * (rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N)))
+    L, lam, NAin, NAout = 30.0, 2.0, 0.1, 0.5
+    M = fill * N
+    Ue = np.zeros((M, M), dtype=complex)
+    Ufe = np.zeros((M, M), dtype=complex)
+    Ue[:N, :N] = U
+    Ufe[:N, :N] = Uf
+    out = orc.detector(Ue, Ufe, 1.0 / (fill * L), 1.0 / (fill * L), NAin, NAout, lam, 0, N, 1)
+    for prec, tol in ((pkg.FP64_PARITY, 1e-10), (pkg.FP32_FAST, 2e-5)):
+        cd = torch.complex64 if prec == pkg.FP32_FAST else torch.complex128
+        rd = torch.float32 if prec == pkg.FP32_FAST else torch.float64
+        Ud, Ufd = dev(U.ravel(), cd), dev(Uf.ravel(), cd)
+        D = torch.zeros(N * N, dtype=rd, device="cuda")
+        Di, Ab = torch.zeros_like(D), torch.zeros_like(D)
+        ctx.detector_zero_filled(Ud, Ufd, N, N, fill, 1.0 / L, 1.0 / L, NAin, NAout, lam, D, Di,
+                                 Ab, prec=prec)
+        assert normrel(host(D), out[2].ravel()) <= tol
+        assert normrel(host(Di), out[3].ravel()) <= tol
+        # the inputs are left unchanged
+        assert np.array_equal(host(Ud), U.ravel().astype(np.complex64 if prec == 2 else complex))
+
+
 # ------------------------------------------------------------ scattered field
 def _us_case(pkg, W, cid, N, nk):
     inst = W.config(cid, N=N, nk=nk)
