@@ -2,6 +2,8 @@
 // slice, spheres, wavelength and waves in one pass).
 #pragma once
 
+#include <vector>
+
 #include "internal.h"
 
 namespace bimsim {
@@ -52,8 +54,12 @@ void uf_fast_table(double ca, double cb, int m, std::vector<double>* out);
 // Incident fields of n instances into Uf (complex64 when f32, else
 // complex128); tabs holds the instances' uf_fast_table rows (f32 only).
 // Monte-Carlo instances read their waves from `waves` (complex128 only).
+// With sum (one SumInst per instance), the launch also clears the
+// instances' images in D and Di (float when f32, else double).
+struct SumInst;
 int uf_batch_launch(bimsim_ctx* ctx, const UfInst* d_inst, int n, int max_npix, int f32,
-                    const double* tabs, const double* waves, void* Uf);
+                    const double* tabs, const double* waves, void* Uf,
+                    const SumInst* sum = nullptr, void* D = nullptr, void* Di = nullptr);
 
 // Incident fields of the Monte-Carlo instances list[0..n) of d_inst into the
 // complex64 field Uf, in tiles of 64 x 64 pixels (the per-pixel batch
@@ -88,6 +94,31 @@ int detector_run(bimsim_ctx* ctx, int prec, int quirks, void* U, void* Uf, int u
                  double du, double dv, double NAin, double NAout, double lambda, int padding,
                  int Nd, int ss, void* D, void* Di, void* Abs, int write_filtered,
                  const SumFuse* fz);
+
+// The band-limited detector over many instances of one FFT size N (uR = vR
+// = N) in three launches per pass.  Instances qualify when det_batchable:
+// ss = 1, no zero fill, the N-point row pair fits LDS.
+struct DetParams {
+    long long uoff;        // element offset of U and Uf
+    long long image_off;   // its D, Di, Abs images
+    int padding, Nd;
+    double du, dv, NAin, NAout, lambda;
+    SumFuse fz;            // Uf + Us formed as the rows load (fused batches)
+};
+struct DetBatch {
+    int N = 0, n = 0, B = 0, B2 = 0, Bc = 0, max_nc = 0, max_nd = 0;
+    bool f32 = false, fused = false;
+    void* d_inst = nullptr;
+    int* d_cols = nullptr;
+    void* d_G = nullptr;
+    const void* tw = nullptr;
+};
+bool det_batchable(const bimsim_ctx* ctx, int prec, int N, int ss, int zero_fill);
+int det_batch_create(bimsim_ctx* ctx, int prec, int quirks, int N, const std::vector<DetParams>& p,
+                     bool fused, DetBatch* out);
+int det_batch_launch(bimsim_ctx* ctx, const DetBatch& b, const void* U, const void* Uf, void* D,
+                     void* Di, void* Abs);
+void det_batch_destroy(DetBatch* b);
 
 // bimsim_detector_zero_filled's work (the images accumulate, like
 // detector_run's).

@@ -15,6 +15,7 @@
 //                      the per-instance entry point
 //   Uf + Us            one launch over (pixel blocks, instances)
 //   detector           per instance (its own FFT size)
+#include <map>
 #include <vector>
 
 #include "batch.h"
@@ -69,6 +70,10 @@ struct bimsim_batch {
     // pre-pass launch)
     bool fuse_sum = false;
     std::vector<SumInst> sum_host;
+    // detectors: one batched pass per FFT size, and the instances that
+    // need their own (supersampling, zero fill, sizes without a fixed plan)
+    std::vector<DetBatch> det;
+    std::vector<int> det_single;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -101,6 +106,7 @@ void release(bimsim_batch* b) {
     if (b->ev_join) (void)hipEventDestroy(b->ev_join);
     series_plan_destroy(&b->plan_mc);
     series_plan_destroy(&b->plan_pw);
+    for (DetBatch& d : b->det) det_batch_destroy(&d);
     void* ptrs[] = {b->d_sph, b->d_kmag, b->d_work, b->d_A, b->d_B, b->d_uf, b->d_uftab,
                     b->d_sum, b->d_sph4, b->d_Uf, b->d_U, b->d_kdirs, b->d_mc_list};
     for (void* p : ptrs)
@@ -133,8 +139,9 @@ int enqueue(bimsim_batch* b) {
         BIMSIM_HIP_CHECK(ctx, hipEventRecord(b->ev_join, b->side));
         if (b->have_mc) BIMSIM_TRY(series_plan_launch_waves(ctx, b->plan_mc, b->waves));
         if (b->have_pw) BIMSIM_TRY(series_plan_launch_waves(ctx, b->plan_pw, b->d_kdirs));
+        // (with the sum formed by the detectors, this launch clears the images)
         BIMSIM_TRY(uf_batch_launch(ctx, b->d_uf, b->n, b->max_npix, b->f32, b->d_uftab, b->waves,
-                                   b->d_Uf));
+                                   b->d_Uf, b->fuse_sum ? b->d_sum : nullptr, b->D, b->Di));
         BIMSIM_TRY(uf_mc_batch_launch(ctx, b->d_uf, b->d_mc_list, b->n_mc_uf, b->mc_tiles_max,
                                       b->waves, (float2*)b->d_Uf));
         BIMSIM_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, b->ev_join, 0));
@@ -161,7 +168,9 @@ int enqueue(bimsim_batch* b) {
     if (!b->fuse_sum)
         BIMSIM_TRY(sum_batch_launch(ctx, b->d_sum, b->n, b->max_npix, b->d_sph4, b->f32, b->d_Uf,
                                     b->d_U, b->D, b->Di));
-    for (int i = 0; i < b->n; ++i) {
+    for (const DetBatch& d : b->det)
+        BIMSIM_TRY(det_batch_launch(ctx, d, b->d_U, b->d_Uf, b->D, b->Di, b->Abs));
+    for (int i : b->det_single) {
         const bimsim_instance& I = b->inst[i];
         void* Ab = b->Abs ? (char*)b->Abs + rb * I.image_off : nullptr;
         if (I.zero_fill >= 2) {
@@ -344,7 +353,9 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
         const SeriesPlan& P = b->have_mc ? b->plan_mc : b->plan_pw;
         b->prepass = b->have_mc != b->have_pw && P.harmonic && b->n_sph > 0 &&
                      prepass_lds(b->pre) <= kPrepassLds;
-        b->fuse_sum = b->prepass;
+        // the Uf + Us sum in the detectors' first row pass: the images are
+        // then cleared by the pre-pass launch or the incident-field launch
+        b->fuse_sum = true;
         for (int i = 0; i < n; ++i)
             b->fuse_sum = b->fuse_sum && inst[i].zero_fill <= 1 &&
                           detector_fusable(ctx, inst[i].N, inst[i].N);
@@ -354,6 +365,35 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
             b->pre.D = (float*)b->D;
             b->pre.Di = (float*)b->Di;
         }
+    }
+    // detectors by FFT size
+    std::map<int, std::vector<DetParams>> by_size;
+    for (int i = 0; i < n; ++i) {
+        const bimsim_instance& I = inst[i];
+        if (!det_batchable(ctx, b->prec, I.N, I.ss, I.zero_fill)) {
+            b->det_single.push_back(i);
+            continue;
+        }
+        DetParams q{};
+        q.uoff = b->field_off[i];
+        q.image_off = I.image_off;
+        q.padding = I.padding;
+        q.Nd = I.Nd;
+        q.du = b->du[i];
+        q.dv = b->dv[i];
+        q.NAin = I.objective[0];
+        q.NAout = I.objective[1];
+        q.lambda = I.lambda;
+        if (b->fuse_sum) {
+            const SumInst& S = sum[i];
+            q.fz = SumFuse{S.q, S.uR, S.vR, S.ns, b->d_sph4 + S.s_begin};
+        }
+        by_size[I.N].push_back(q);
+    }
+    for (auto& kv : by_size) {
+        b->det.emplace_back();
+        BIMSIM_TRY(det_batch_create(ctx, b->prec, b->quirks, kv.first, kv.second, b->fuse_sum,
+                                    &b->det.back()));
     }
     return BIMSIM_OK;
 }

@@ -78,6 +78,17 @@ __device__ inline bool passband(const PostOp& op, int iu, int iv) {
     return !(fmag < op.fmin || fmag > op.fmax);
 }
 
+// One instance of a batched detector pass (band-limited path, ss = 1, the
+// images accumulate): its fields at uoff, pass-band columns at cols_off in
+// the batch's column table, compact spectrum (2 Nc N values) at g_off,
+// images at image_off; crop rows and columns [off, off + Nd).
+struct DetInst {
+    long long uoff, g_off, image_off;
+    int cols_off, Nc, off, Nd;
+    PostOp bp;
+    SumFuse fz;
+};
+
 template <class R>
 __device__ inline void butterfly(typename C2<R>::T* x, int r, R sgn) {
     using C = typename C2<R>::T;
@@ -326,17 +337,16 @@ __device__ __forceinline__ C fused_sum(const SumFuse& fz, int pix, C a, C b) {
 // G[(field*Nc + c)*nrows + row].  kSum: U is the scattered field alone and
 // the rows of U load as fused_sum(fz, ...).
 template <class R, int N, bool kSum>
-__global__ __launch_bounds__(kThreads) void rows_fwd_kernel(
+__device__ __forceinline__ void rows_fwd_body(
     const typename C2<R>::T* __restrict__ U, const typename C2<R>::T* __restrict__ Uf, int nrows,
     int B, const int* __restrict__ cols, int Nc, typename C2<R>::T* __restrict__ G,
-    const typename C2<R>::T* __restrict__ tw, SumFuse fz) {
+    const typename C2<R>::T* __restrict__ tw, const SumFuse& fz, int blk, int f) {
     using C = typename C2<R>::T;
     extern __shared__ unsigned char smem_raw[];
     C* a = reinterpret_cast<C*>(smem_raw);
     C* b = a + (size_t)B * N;
-    const int f = blockIdx.y;
     const C* F = f ? Uf : U;
-    const int row0 = blockIdx.x * B;
+    const int row0 = blk * B;
     const int nb = min(B, nrows - row0);
     if (kSum && f == 0) {
         for (int t = threadIdx.x; t < nb * N; t += kThreads) {
@@ -354,20 +364,26 @@ __global__ __launch_bounds__(kThreads) void rows_fwd_kernel(
     }
 }
 
+template <class R, int N, bool kSum>
+__global__ __launch_bounds__(kThreads) void rows_fwd_kernel(
+    const typename C2<R>::T* __restrict__ U, const typename C2<R>::T* __restrict__ Uf, int nrows,
+    int B, const int* __restrict__ cols, int Nc, typename C2<R>::T* __restrict__ G,
+    const typename C2<R>::T* __restrict__ tw, SumFuse fz) {
+    rows_fwd_body<R, N, kSum>(U, Uf, nrows, B, cols, Nc, G, tw, fz, blockIdx.x, blockIdx.y);
+}
+
 // Pass-band columns of the compact spectrum, each: forward FFT along v,
 // objective band-pass, inverse FFT, scale 1/(uR vR).  In place in G.
 template <class R, int M>
-__global__ __launch_bounds__(kThreads) void cols_band_kernel(typename C2<R>::T* __restrict__ G,
-                                                             int Nc, int B,
-                                                             const int* __restrict__ cols,
-                                                             PostOp bp,
-                                                             const typename C2<R>::T* __restrict__ tw) {
+__device__ __forceinline__ void cols_band_body(typename C2<R>::T* __restrict__ G, int Nc, int B,
+                                               const int* __restrict__ cols, const PostOp& bp,
+                                               const typename C2<R>::T* __restrict__ tw, int blk,
+                                               int f) {
     using C = typename C2<R>::T;
     extern __shared__ unsigned char smem_raw[];
     C* a = reinterpret_cast<C*>(smem_raw);
     C* b = a + (size_t)B * M;
-    const int f = blockIdx.y;
-    const int c0 = blockIdx.x * B;
+    const int c0 = blk * B;
     const int nb = min(B, Nc - c0);
     C* line = G + ((size_t)f * Nc + c0) * M;
     for (int t = threadIdx.x; t < nb * M; t += kThreads) a[t] = line[t];
@@ -385,6 +401,15 @@ __global__ __launch_bounds__(kThreads) void cols_band_kernel(typename C2<R>::T* 
         const C v = res[t];
         line[t] = cmk<C>(v.x * sc, v.y * sc);
     }
+}
+
+template <class R, int M>
+__global__ __launch_bounds__(kThreads) void cols_band_kernel(typename C2<R>::T* __restrict__ G,
+                                                             int Nc, int B,
+                                                             const int* __restrict__ cols,
+                                                             PostOp bp,
+                                                             const typename C2<R>::T* __restrict__ tw) {
+    cols_band_body<R, M>(G, Nc, B, cols, bp, tw, blockIdx.x, blockIdx.y);
 }
 
 // Inverse FFT of rows [row_begin, row_begin + nrows) of U/Uf from the compact
@@ -426,15 +451,15 @@ __global__ __launch_bounds__(kThreads) void rows_inv_kernel(
 // Di += |Uf|^2 (|.| squared, as the intensity rows), Abs = -log10(D / Di)
 // over the crop columns [off, off + Nd); the rows are those of the crop.
 template <class R, int N>
-__global__ __launch_bounds__(kThreads) void rows_inv_detect_kernel(
+__device__ __forceinline__ void rows_inv_detect_body(
     const typename C2<R>::T* __restrict__ G, int Nc, int vR, const int* __restrict__ cols,
     int row_begin, int nrows, int B, int off, int Nd, R* __restrict__ D, R* __restrict__ Di,
-    R* __restrict__ Abs, const typename C2<R>::T* __restrict__ tw) {
+    R* __restrict__ Abs, const typename C2<R>::T* __restrict__ tw, int blk) {
     using C = typename C2<R>::T;
     extern __shared__ unsigned char smem_raw[];
     C* a = reinterpret_cast<C*>(smem_raw);
     C* b = a + (size_t)2 * B * N;
-    const int r0 = row_begin + blockIdx.x * B;
+    const int r0 = row_begin + blk * B;
     const int nb = min(B, row_begin + nrows - r0);
     for (int t = threadIdx.x; t < 2 * nb * N; t += kThreads) a[t] = cmk<C>((R)0, (R)0);
     __syncthreads();
@@ -454,6 +479,62 @@ __global__ __launch_bounds__(kThreads) void rows_inv_detect_kernel(
         D[i] = d;
         Di[i] = di;
         if (Abs) Abs[i] = -log10(d / di);
+    }
+}
+
+template <class R, int N>
+__global__ __launch_bounds__(kThreads) void rows_inv_detect_kernel(
+    const typename C2<R>::T* __restrict__ G, int Nc, int vR, const int* __restrict__ cols,
+    int row_begin, int nrows, int B, int off, int Nd, R* __restrict__ D, R* __restrict__ Di,
+    R* __restrict__ Abs, const typename C2<R>::T* __restrict__ tw) {
+    rows_inv_detect_body<R, N>(G, Nc, vR, cols, row_begin, nrows, B, off, Nd, D, Di, Abs, tw,
+                               blockIdx.x);
+}
+
+// ------------------------------------------ batched band-limited detector
+// The three passes above for many instances of one FFT size N at once: the
+// block's instance from DetInst[blockIdx.z], its fields, columns, compact
+// spectrum and images at their offsets.  Blocks past an instance's rows or
+// columns return (uniformly per block).
+template <class R, int N, bool kSum>
+__global__ __launch_bounds__(kThreads) void rows_fwd_batch_kernel(
+    const DetInst* __restrict__ insts, int n, const typename C2<R>::T* __restrict__ U,
+    const typename C2<R>::T* __restrict__ Uf, int B, const int* __restrict__ cols,
+    typename C2<R>::T* __restrict__ G, const typename C2<R>::T* __restrict__ tw) {
+    for (int z = blockIdx.z; z < n; z += gridDim.z) {
+        const DetInst& I = insts[z];
+        if (I.Nc == 0) continue;
+        rows_fwd_body<R, N, kSum>(U + I.uoff, Uf + I.uoff, N, B, cols + I.cols_off, I.Nc,
+                                  G + I.g_off, tw, I.fz, blockIdx.x, blockIdx.y);
+        __syncthreads();   // the block's LDS is reused by its next instance
+    }
+}
+
+template <class R, int M>
+__global__ __launch_bounds__(kThreads) void cols_band_batch_kernel(
+    const DetInst* __restrict__ insts, int n, int B, const int* __restrict__ cols,
+    typename C2<R>::T* __restrict__ G, const typename C2<R>::T* __restrict__ tw) {
+    for (int z = blockIdx.z; z < n; z += gridDim.z) {
+        const DetInst& I = insts[z];
+        if ((int)blockIdx.x * B >= I.Nc) continue;
+        cols_band_body<R, M>(G + I.g_off, I.Nc, B, cols + I.cols_off, I.bp, tw, blockIdx.x,
+                             blockIdx.y);
+        __syncthreads();
+    }
+}
+
+template <class R, int N>
+__global__ __launch_bounds__(kThreads) void rows_inv_detect_batch_kernel(
+    const DetInst* __restrict__ insts, int n, int B, const int* __restrict__ cols,
+    const typename C2<R>::T* __restrict__ G, R* __restrict__ D, R* __restrict__ Di,
+    R* __restrict__ Abs, const typename C2<R>::T* __restrict__ tw) {
+    for (int z = blockIdx.z; z < n; z += gridDim.z) {
+        const DetInst& I = insts[z];
+        if ((int)blockIdx.x * B >= I.Nd) continue;
+        rows_inv_detect_body<R, N>(G + I.g_off, I.Nc, N, cols + I.cols_off, I.off, I.Nd, B, I.off,
+                                   I.Nd, D + I.image_off, Di + I.image_off,
+                                   Abs ? Abs + I.image_off : nullptr, tw, blockIdx.x);
+        __syncthreads();
     }
 }
 
@@ -861,6 +942,148 @@ int detector_run(bimsim_ctx* ctx, int prec, int quirks, void* U, void* Uf, int u
     if (rc != BIMSIM_OK) return rc;
     timer_end(ctx, tmr);
     return BIMSIM_OK;
+}
+
+// ------------------------------------------ batched band-limited detector
+namespace {
+
+int rows_per_wg(int n) { return std::max(1, std::min(8, 1024 / n)); }
+
+template <class R>
+int det_batch_launch_impl(bimsim_ctx* ctx, const DetBatch& b, const void* U, const void* Uf,
+                          void* D, void* Di, void* Abs) {
+    using C = typename C2<R>::T;
+    const int N = b.N;
+    const DetInst* I = (const DetInst*)b.d_inst;
+    const C* tw = (const C*)b.tw;
+    const unsigned nz = (unsigned)std::min(b.n, 65535);
+    const size_t lds_r = sizeof(C) * 2 * (size_t)b.B * N;
+    const size_t lds_c = sizeof(C) * 2 * (size_t)b.Bc * N;
+    const hipStream_t st = ctx->stream;
+    constexpr bool f32 = std::is_same<R, float>::value;
+    switch (N) {
+#define BIMSIM_X(NN)                                                                               \
+    case NN:                                                                                       \
+        if (f32 && b.fused)                                                                        \
+            hipLaunchKernelGGL((rows_fwd_batch_kernel<R, NN, f32>), dim3((NN + b.B - 1) / b.B, 2, nz), \
+                               dim3(kThreads), lds_r, st, I, b.n, (const C*)U, (const C*)Uf, b.B,  \
+                               b.d_cols, (C*)b.d_G, tw);                                           \
+        else                                                                                       \
+            hipLaunchKernelGGL((rows_fwd_batch_kernel<R, NN, false>),                              \
+                               dim3((NN + b.B - 1) / b.B, 2, nz), dim3(kThreads), lds_r, st, I,    \
+                               b.n, (const C*)U, (const C*)Uf, b.B, b.d_cols, (C*)b.d_G, tw);     \
+        if (b.max_nc > 0)                                                                          \
+            hipLaunchKernelGGL((cols_band_batch_kernel<R, NN>),                                    \
+                               dim3((b.max_nc + b.Bc - 1) / b.Bc, 2, nz), dim3(kThreads), lds_c,   \
+                               st, I, b.n, b.Bc, b.d_cols, (C*)b.d_G, tw);                        \
+        hipLaunchKernelGGL((rows_inv_detect_batch_kernel<R, NN>),                                  \
+                           dim3((b.max_nd + b.B2 - 1) / b.B2, 1, nz), dim3(kThreads),              \
+                           2 * sizeof(C) * 2 * (size_t)b.B2 * NN, st, I, b.n, b.B2, b.d_cols,      \
+                           (const C*)b.d_G, (R*)D, (R*)Di, (R*)Abs, tw);                          \
+        break;
+        BIMSIM_FFT_SIZES(BIMSIM_X)
+#undef BIMSIM_X
+        default:
+            return set_error(ctx, BIMSIM_E_UNSUPPORTED, "no batched detector for N = %d", N);
+    }
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return BIMSIM_OK;
+}
+
+}  // namespace
+
+bool det_batchable(const bimsim_ctx* ctx, int prec, int N, int ss, int zero_fill) {
+    const size_t cs = prec == BIMSIM_FP32_FAST ? sizeof(float2) : sizeof(double2);
+    return ss == 1 && zero_fill <= 1 && band_path(ctx, cs, N, N) &&
+           2 * cs * 2 * (size_t)rows_per_wg(N) * N <= ctx->lds_max;
+}
+
+int det_batch_create(bimsim_ctx* ctx, int prec, int quirks, int N, const std::vector<DetParams>& p,
+                     bool fused, DetBatch* out) {
+    *out = DetBatch();
+    const bool f32 = prec == BIMSIM_FP32_FAST;
+    const size_t cs = f32 ? sizeof(float2) : sizeof(double2);
+    std::vector<DetInst> insts;
+    std::vector<int> cols;
+    long long g = 0;
+    int max_nc = 0, max_nd = 0;
+    for (const DetParams& q : p) {
+        DetInst I{};
+        I.uoff = q.uoff;
+        I.image_off = q.image_off;
+        I.Nd = q.Nd;
+        I.off = q.padding * q.Nd;
+        BIMSIM_CHECK_ARG(ctx, I.off + q.Nd <= N, "detector crop exceeds the near-field grid");
+        PostOp& bp = I.bp;
+        bp.kind = 1;
+        bp.uR = N;
+        bp.vR = N;
+        bp.quirks = quirks;
+        bp.du = q.du;
+        bp.dv = q.dv;
+        bp.fmin = q.NAin / q.lambda;
+        bp.fmax = q.NAout / q.lambda;
+        bp.scale = 1.0 / ((unsigned)N * (unsigned)N);
+        I.fz = q.fz;
+        // pass-band columns, as detector_band
+        I.cols_off = (int)cols.size();
+        for (int iu = 0; iu < N; ++iu) {
+            double u;
+            if (iu <= N / 2) u = (double)iu * bp.du;
+            else u = -(double)(quirks ? N - 1 - iu : N - iu) * bp.du;
+            if (!(std::fabs(u) > bp.fmax)) cols.push_back(iu);
+        }
+        I.Nc = (int)cols.size() - I.cols_off;
+        I.g_off = g;
+        g += 2LL * I.Nc * N;
+        max_nc = std::max(max_nc, I.Nc);
+        max_nd = std::max(max_nd, I.Nd);
+        insts.push_back(I);
+    }
+    out->N = N;
+    out->n = (int)insts.size();
+    out->f32 = f32;
+    out->fused = fused && f32;
+    out->max_nc = max_nc;
+    out->max_nd = max_nd;
+    out->B = rows_per_wg(N);
+    out->B2 = rows_per_wg(N);
+    out->Bc = std::max(1, std::min(8, 2048 / N));
+    const double* twd;
+    const float* twf;
+    BIMSIM_TRY(twiddles_get(ctx, N, &twd, &twf));
+    out->tw = f32 ? (const void*)twf : (const void*)twd;
+    if (cols.empty()) cols.push_back(0);
+    BIMSIM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    if (hipMalloc(&out->d_inst, sizeof(DetInst) * insts.size()) != hipSuccess ||
+        hipMalloc((void**)&out->d_cols, sizeof(int) * cols.size()) != hipSuccess ||
+        hipMalloc(&out->d_G, cs * (size_t)std::max<long long>(1, g)) != hipSuccess) {
+        det_batch_destroy(out);
+        return set_error(ctx, BIMSIM_E_NOMEM, "out of device memory for the batched detector");
+    }
+    BIMSIM_HIP_CHECK(ctx, hipMemcpy(out->d_inst, insts.data(), sizeof(DetInst) * insts.size(),
+                                    hipMemcpyHostToDevice));
+    BIMSIM_HIP_CHECK(ctx, hipMemcpy(out->d_cols, cols.data(), sizeof(int) * cols.size(),
+                                    hipMemcpyHostToDevice));
+    return BIMSIM_OK;
+}
+
+int det_batch_launch(bimsim_ctx* ctx, const DetBatch& b, const void* U, const void* Uf, void* D,
+                     void* Di, void* Abs) {
+    if (b.n == 0) return BIMSIM_OK;
+    const int tmr = timer_begin(ctx, kStageDet);
+    const int rc = b.f32 ? det_batch_launch_impl<float>(ctx, b, U, Uf, D, Di, Abs)
+                         : det_batch_launch_impl<double>(ctx, b, U, Uf, D, Di, Abs);
+    if (rc != BIMSIM_OK) return rc;
+    timer_end(ctx, tmr);
+    return BIMSIM_OK;
+}
+
+void det_batch_destroy(DetBatch* b) {
+    if (b->d_inst) (void)hipFree(b->d_inst);
+    if (b->d_cols) (void)hipFree(b->d_cols);
+    if (b->d_G) (void)hipFree(b->d_G);
+    *b = DetBatch();
 }
 
 }  // namespace bimsim

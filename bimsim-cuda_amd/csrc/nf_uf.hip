@@ -49,16 +49,25 @@ __global__ __launch_bounds__(kThreads) void uf_plane_kernel(QuadArg q, int uR, i
 }
 
 // Incident fields of a batch of instances.  Grid: (pixel blocks, instances).
-template <class Out>
+// With sum given, also clears each instance's detector images (sum[i],
+// nd2 <= uR vR), which the detectors then accumulate into.
+template <class Out, class R>
 __global__ __launch_bounds__(kThreads) void uf_batch_kernel(const UfInst* __restrict__ insts, int n,
                                                             const double* __restrict__ tabs,
                                                             const double* __restrict__ waves,
-                                                            Out* __restrict__ Uf) {
+                                                            Out* __restrict__ Uf,
+                                                            const SumInst* __restrict__ sum,
+                                                            R* __restrict__ D, R* __restrict__ Di) {
     for (int i = blockIdx.y; i < n; i += gridDim.y) {
         const UfInst& I = insts[i];
         const int npix = I.uR * I.vR;
-        for (int pix = blockIdx.x * kThreads + threadIdx.x; pix < npix; pix += gridDim.x * kThreads)
+        for (int pix = blockIdx.x * kThreads + threadIdx.x; pix < npix; pix += gridDim.x * kThreads) {
+            if (sum && pix < sum[i].nd2) {
+                D[sum[i].image_off + pix] = R(0);
+                Di[sum[i].image_off + pix] = R(0);
+            }
             uf_batch_pixel(I, pix, tabs, waves, Uf);
+        }
     }
 }
 
@@ -272,14 +281,15 @@ void uf_fast_table(double ca, double cb, int m, std::vector<double>* out) {
 }
 
 int uf_batch_launch(bimsim_ctx* ctx, const UfInst* d_inst, int n, int max_npix, int f32,
-                    const double* tabs, const double* waves, void* Uf) {
+                    const double* tabs, const double* waves, void* Uf, const SumInst* sum,
+                    void* D, void* Di) {
     const dim3 grid((max_npix + kThreads - 1) / kThreads, std::min(n, 65535));
     if (f32)
-        hipLaunchKernelGGL(uf_batch_kernel<float2>, grid, dim3(kThreads), 0, ctx->stream, d_inst, n,
-                           tabs, waves, (float2*)Uf);
+        hipLaunchKernelGGL((uf_batch_kernel<float2, float>), grid, dim3(kThreads), 0, ctx->stream,
+                           d_inst, n, tabs, waves, (float2*)Uf, sum, (float*)D, (float*)Di);
     else
-        hipLaunchKernelGGL(uf_batch_kernel<double2>, grid, dim3(kThreads), 0, ctx->stream, d_inst,
-                           n, tabs, waves, (double2*)Uf);
+        hipLaunchKernelGGL((uf_batch_kernel<double2, double>), grid, dim3(kThreads), 0, ctx->stream,
+                           d_inst, n, tabs, waves, (double2*)Uf, sum, (double*)D, (double*)Di);
     BIMSIM_HIP_CHECK(ctx, hipGetLastError());
     return BIMSIM_OK;
 }

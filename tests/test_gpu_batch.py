@@ -187,6 +187,34 @@ def test_graph_replays_and_reads_current_inputs(pkg, ctx, W):
     b2.close()
 
 
+@pytest.mark.parametrize("prec,shape", [(2, "sweep"), (2, "one"), (0, "one")])
+def test_graph_of_the_headline_path(pkg, ctx, W, prec, shape):
+    """The headline's path as a captured graph: a spectral sweep (batched
+    detector per FFT size, Uf + Us formed by its first row pass) and one
+    image (the pre-pass launch with coefficients, records and incident
+    field); FP64 batches captured too.  Every replay, with eager work and
+    synchronisations between replays, equals the eager pass bit for bit."""
+    if shape == "sweep":
+        insts = W.config4(n_nu=256, N=64, ns=5, nk=32, stride=37)
+    else:
+        insts = [W.config(2, N=64, nk=40)]
+    waves = waves_of(pkg, W, insts)
+    be, De, Die, Abe, _ = run_batch(pkg, ctx, insts, waves, prec)
+    be.launch()
+    want = host(Abe).copy(), host(De).copy()
+    bg, D, Di, Ab, _ = run_batch(pkg, ctx, insts, waves, prec, use_graph=True)
+    for _ in range(3):
+        D.fill_(7.0)
+        Di.fill_(7.0)
+        be.launch()          # eager work between the replays
+        torch.cuda.synchronize()
+        bg.launch()
+        torch.cuda.synchronize()
+        assert np.array_equal(host(Ab), want[0]) and np.array_equal(host(D), want[1])
+    be.close()
+    bg.close()
+
+
 def test_graph_replays_after_the_scratch_grows(pkg, W):
     """A captured batch keeps its image after a larger call on the same
     context grows the context's detector scratch (outgrown buffers stay
