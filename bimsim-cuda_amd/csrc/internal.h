@@ -45,6 +45,8 @@ struct bimsim_ctx {
     bool autotune = true;
     int series_algo = 0;   // FP32 series: 0 auto, 1 direct, 2 harmonic
     std::map<std::string, std::pair<int, int>> tune;
+    // harmonic series: register-budget variant per bucket, per batch shape
+    std::map<std::string, std::vector<int>> tune_harm;
 };
 
 namespace bimsim {

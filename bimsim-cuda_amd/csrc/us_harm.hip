@@ -1,0 +1,53 @@
+// The harmonic pixel kernel of one order bucket (HARM_L, set by the build:
+// this file is compiled once per bucket) in its register-budget variants.
+#include "us_harm.h"
+
+#ifndef HARM_L
+#error "us_harm.hip is compiled once per order bucket with -DHARM_L=<orders>"
+#endif
+
+namespace bimsim {
+namespace {
+
+template <int L, int W>
+void launch_variant(const HarmLaunch& a) {
+    // buckets whose spheres are all planar take the kernel without the
+    // general branch (fewer registers live)
+    if (a.planar)
+        hipLaunchKernelGGL((harm::series_harm_kernel<L, true, W>), a.grid,
+                           dim3(harm::kHarmThreads), 0, a.stream, a.inst, a.sph, a.grp, a.items,
+                           a.n_items, a.A, a.B, a.rec, a.cst, a.perm, a.U, a.counter);
+    else
+        hipLaunchKernelGGL((harm::series_harm_kernel<L, false, W>), a.grid,
+                           dim3(harm::kHarmThreads), 0, a.stream, a.inst, a.sph, a.grp, a.items,
+                           a.n_items, a.A, a.B, a.rec, a.cst, a.perm, a.U, a.counter);
+}
+
+}  // namespace
+
+template <int L>
+int harm_launch(const HarmLaunch& a, int variant) {
+    constexpr int w0 = harm_variant_waves(L, 0);
+    constexpr int w1 = harm_variant_waves(L, 1);
+    constexpr int w2 = harm_variant_waves(L, 2);
+    switch (variant) {
+        case 0:
+            launch_variant<L, w0>(a);
+            return 0;
+        case 1:
+            launch_variant<L, w1>(a);
+            return 0;
+        case 2:
+            if constexpr (w2 > 0) {
+                launch_variant<L, w2>(a);
+                return 0;
+            }
+            return -1;
+        default:
+            return -1;
+    }
+}
+
+template int harm_launch<HARM_L>(const HarmLaunch& a, int variant);
+
+}  // namespace bimsim

@@ -112,17 +112,61 @@ size_t prepass_lds(const PrepassArgs& a);
 // Algorithms of the FP32 series.
 enum SeriesAlgo { kSeriesAuto = 0, kSeriesDirect = 1, kSeriesHarmonic = 2 };
 
+// One launch of the harmonic pixel kernel over a bucket's work items.
+struct HarmLaunch {
+    dim3 grid;
+    hipStream_t stream;
+    const SeriesInst* inst;
+    const SeriesSph* sph;
+    const SeriesGroup* grp;
+    const SeriesItem* items;
+    int n_items;
+    const double2* A;
+    const double2* B;
+    const float4* rec;
+    const float* cst;
+    const int* perm;
+    float2* U;
+    int* counter;
+    int planar;   // every sphere of the bucket's groups is planar
+};
+
+// Register budgets of the harmonic kernel: variant v of the bucket of L
+// orders runs harm_variant_waves(L, v) waves per SIMD (0: no such variant).
+// The order sums alone take 2L VGPRs, so the larger budgets spill some
+// values to scratch; more waves in flight hide the latency of the FP64
+// radial chains and of the record loads better, and the plan's tuner
+// times the variants on the batch's own inputs.
+constexpr int kHarmVariants = 3;
+constexpr int kHarmDefaultVariant = 1;
+constexpr int harm_variant_waves(int L, int v) {
+    return L <= 16 ? 6 + v
+           : L <= 24 ? 5 + v
+           : L <= 40 ? 4 + v
+           : L <= 48 ? 3 + v
+           : L <= 64 ? 2 + v
+           : L <= 80 ? (v < 2 ? 2 + v : 0)
+                     : (v < 2 ? 1 + v : 0);
+}
+
+// Launches bucket L in variant v (defined per bucket in us_harm.hip);
+// returns 0, or -1 for a variant that does not exist.
+template <int L>
+int harm_launch(const HarmLaunch& a, int variant);
+
 struct SeriesPlan {
     int n_inst = 0, n_sph = 0;
     bool harmonic = false;    // addition-theorem form
     int split_mult = 1;       // waves split into up to this many items per block
     int groups_per_cu = 8;    // resident workgroups of the persistent launch
     int nk_max = 0, max_npix = 0;
+    long long field_elems = 0;   // elements of U the plan writes
     long long n_wave_rec = 0, n_part = 0;
     long long coef_elems = 0, rhat_elems = 0;   // per chunk (maximum)
     struct Bucket {
         int L, item_begin, item_count, counter;
-        int planar;   // harmonic form: every sphere of its instances is planar
+        int planar;    // harmonic form: every sphere of its instances is planar
+        int variant;   // harmonic form: register budget (harm_variant_waves)
     };
     struct Chunk {
         int sph_begin, sph_count;

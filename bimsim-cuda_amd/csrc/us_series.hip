@@ -574,325 +574,12 @@ __global__ __launch_bounds__(kRecThreads) void prepass_kernel(
     }
 }
 
-// Trigonometric start values of the radial recurrences of one pixel and
-// sphere, evaluated before the order sums so that their temporaries do not
-// coincide with the sums' registers: sin, cos of kd outside the sphere;
-// sin, cos of Re knd and sinh, cosh of Im knd inside.
-struct RadialTrig {
-    double t0, t1, t2, t3;
-};
-
-__device__ __forceinline__ RadialTrig harm_trig(const SphereDev& S, double kmag, double d) {
-    RadialTrig T;
-    if (d <= S.a) {
-        const dc knd = cscl(dc{S.n[0], S.n[1]}, kmag * d);
-        T.t0 = sin(knd.r);
-        T.t1 = cos(knd.r);
-        T.t2 = sinh(knd.i);
-        T.t3 = cosh(knd.i);
-    } else {
-        sincos(kmag * d, &T.t0, &T.t1);
-        T.t2 = T.t3 = 0.0;
-    }
-    return T;
-}
-
-// Radial factors of one pixel and sphere applied to its order sums:
-// acc += sum_{l <= Nl} c_l (ar_l + i ai_l), c_l = B_l (j_l + i y_l)(kd)
-// outside the sphere, A_l j_l(knd) inside (calc_Us / calc_Ui,
-// nfScalarUs.cu:50-95,201-245), in FP64 with the reference's upward
-// recurrences and threshold rules (spherical_bessel.h:85-141); the steps
-// multiply by 1/(kd) or 1/(knd) where the reference divides.
-template <int L>
-__device__ __forceinline__ void harm_radial(const SphereDev& S, double kmag, double d,
-                                            const RadialTrig& T, const double2* __restrict__ A,
-                                            const double2* __restrict__ B, const float* ar,
-                                            const float* ai, float& acc_r, float& acc_i) {
-    const int nl1 = S.Nl + 1;
-    auto mac = [&](int l, double cr, double ci) {
-        const float fr = (float)cr, fi = (float)ci;
-        acc_r = fmaf(fr, ar[l], fmaf(-fi, ai[l], acc_r));
-        acc_i = fmaf(fr, ai[l], fmaf(fi, ar[l], acc_i));
-    };
-    if (d <= S.a) {
-        const double2* As = A + S.coeff_off;
-        const dc knd = cscl(dc{S.n[0], S.n[1]}, kmag * d);
-        if (knd.r < (double)kEpsFloat) {
-            // calc_Ui returns A_0 for every wave: only the l = 0 term
-            const double2 a0 = As[0];
-            mac(0, a0.x, a0.y);
-            return;
-        }
-        // init_jc / shift_jc (csin, ccos as csin_r / ccos_r)
-        const dc inv = cdiv_naive(dc{1.0, 0.0}, knd);
-        const dc sn = {T.t0 * T.t3, T.t1 * T.t2}, cs = {T.t1 * T.t3, -(T.t0 * T.t2)};
-        dc j0 = cmul(sn, inv);
-        dc j1 = csub(cmul(j0, inv), cmul(cs, inv));
-#pragma clang loop unroll(full)
-        for (int l = 0; l < L; ++l) {
-            if (l >= nl1) continue;
-            dc jl = l == 0 ? j0 : j1;
-            if (l >= 2) {
-                dc jn = csub(cmul(cscl(j1, (double)(2 * l - 1)), inv), j0);
-                // the threshold rule can only act where l > Re knd: skip its
-                // selects when no lane of the wave is there
-                const bool past = (double)l > knd.r;
-                if (__any(past))
-                    if (past && (jn.r < kJMin || jn.r > kJMax)) jn = dc{0.0, 0.0};
-                j0 = j1;
-                j1 = jn;
-                jl = jn;
-            }
-            const dc c = cmul(ld2(As, l), jl);
-            mac(l, c.r, c.i);
-        }
-        return;
-    }
-    const double2* Bs = B + S.coeff_off;
-    const double kd = kmag * d;
-    const double inv = 1.0 / kd;
-    const double sn = T.t0, cs = T.t1;
-    double j0 = sn * inv, j1 = fma(j0, inv, -(cs * inv));
-    double y0 = -(cs * inv), y1 = fma(y0, inv, -(sn * inv));
-    // B_l read four orders at a time (indices clamped to Nl), so that one
-    // wait on the scalar loads covers four orders
-    double2 bq[4];
-#pragma clang loop unroll(full)
-    for (int l = 0; l < L; ++l) {
-        if (l >= nl1) continue;
-        if ((l & 3) == 0)
-#pragma clang loop unroll(full)
-            for (int q = 0; q < 4; ++q) bq[q] = Bs[min(l + q, nl1 - 1)];
-        double jl = l == 0 ? j0 : j1, yl = l == 0 ? y0 : y1;
-        if (l >= 2) {
-            double jn = fma((double)(2 * l - 1) * j1, inv, -j0);
-            double yn = fma((double)(2 * l - 1) * y1, inv, -y0);
-            // the threshold rules act only where l > kd or y_l < kYMin: skip
-            // their selects when no lane of the wave is there
-            const bool past = (double)l > kd;
-            if (__any(past || yn < kYMin)) {
-                if (past && (jn < kJMin || jn > kJMax)) jn = 0.0;
-                if (yn < kYMin || (past && yn > 0.0)) {
-                    yn = 0.0;
-                    y1 = 0.0;
-                }
-            }
-            j0 = j1;
-            j1 = jn;
-            y0 = y1;
-            y1 = yn;
-            jl = jn;
-            yl = yn;
-        }
-        const double2 b = bq[l & 3];
-        mac(l, fma(b.x, jl, -(b.y * yl)), fma(b.x, yl, b.y * jl));
-    }
-}
-
-// Order sums of a planar sphere (x = 0 at every pixel): the Legendre
-// values are in the records, so row m only rotates them by e^{i m phi}:
-//   ar_l + i ai_l += cos(m phi) U1_lm + sin(m phi) U2_lm,  l = m, m+2, ...
-// Rows in pairs (m even: even l; m + 1: odd l); records at
-// R[m (L/2) + k], l = 2k + (m & 1); one scalar load of two records per
-// row and block of two k, blocks below the diagonal or past Nl skipped.
-template <int L>
-__device__ __forceinline__ void harm_planar_sums(const float4* __restrict__ R, int nl1, float z1r,
-                                                 float z1i, float* ar, float* ai) {
-    constexpr int H = L / 2;
-    constexpr int KB = 2;            // k per block
-    const int ke = (nl1 + 1) >> 1;   // k past the last of row 0
-    float zr = 1.f, zi = 0.f;
-#pragma clang loop unroll(disable)
-    for (int m = 0; m < nl1; m += 2) {
-        const float4* Ra = R + m * H;
-        const float4* Rb = Ra + H;
-        const float zar = zr, zai = zi;
-        const float zbr = zr * z1r - zi * z1i, zbi = fmaf(zr, z1i, zi * z1r);
-        const int bf = (m >> 1) / KB;   // block of k = m / 2
-#pragma clang loop unroll(full)
-        for (int b = 0; b < H / KB; ++b) {
-            if (b < bf || KB * b >= ke) continue;
-            float4 ra[KB], rb[KB];
-#pragma clang loop unroll(full)
-            for (int j = 0; j < KB; ++j) {
-                ra[j] = Ra[KB * b + j];
-                rb[j] = Rb[KB * b + j];
-            }
-#pragma clang loop unroll(full)
-            for (int j = 0; j < KB; ++j) {
-                const int le = 2 * (KB * b + j);
-                ar[le] = fmaf(zai, ra[j].z, fmaf(zar, ra[j].x, ar[le]));
-                ai[le] = fmaf(zai, ra[j].w, fmaf(zar, ra[j].y, ai[le]));
-                ar[le + 1] = fmaf(zbi, rb[j].z, fmaf(zbr, rb[j].x, ar[le + 1]));
-                ai[le + 1] = fmaf(zbi, rb[j].w, fmaf(zbr, rb[j].y, ai[le + 1]));
-            }
-        }
-        zr = zbr * z1r - zbi * z1i;
-        zi = fmaf(zbr, z1i, zbi * z1r);
-    }
-}
-
-// Waves per SIMD the harmonic kernel is compiled for: the order sums take
-// 2L VGPRs, so small buckets can run four waves per SIMD.
-constexpr int harm_waves(int L) { return L <= 32 ? 4 : (L <= 48 ? 3 : (L <= 80 ? 2 : 1)); }
-
-// Per-pixel harmonic sum, persistent over (instance, pixel block) items.
-// Row m of the order sums runs the scaled recurrence
-//   q_mm = Pn_mm,  q_lm = 2x q_{l-1,m} - c_lm q_{l-2,m}
-// (Pn_lm = K_lm q_lm, K_lm = prod_{j=m+1..l} a_jm / 2, folded into the
-// records) in blocks of four orders: one scalar load of four records and
-// one of their constants per block, then straight-line FP32 code (blocks
-// wholly below the diagonal or past Nl are skipped).  cst[2 (m kCS + l)]
-// holds c_lm (l >= m + 2) and the diagonal factor sqrt((2m+1)/(2m)) at
-// l = m (where the recurrence does not read c); cst[2 (m kCS + l) + 1] is 1
-// at l = m, else 0, and injects the diagonal value (q is 0 below it).
-template <int L, bool kPlanar>
-__global__ __launch_bounds__(kThreads, harm_waves(L)) void series_harm_kernel(
-    const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph,
-    const SeriesGroup* __restrict__ grps, const SeriesItem* __restrict__ items, int n_items,
-    const double2* __restrict__ A, const double2* __restrict__ B, const float4* __restrict__ rec,
-    const float* __restrict__ cst, const int* __restrict__ perm, float2* __restrict__ U,
-    int* __restrict__ counter) {
-    __shared__ int s_item;
-    for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
-        __syncthreads();
-        const int it = __builtin_amdgcn_readfirstlane(s_item);
-        __syncthreads();
-        if (it >= n_items) break;
-        const SeriesItem item = items[it];
-        const SeriesInst& I = insts[item.inst];
-        const int npix = I.uR * I.vR;
-        const int j = item.pix0 + (int)threadIdx.x;
-        if (j >= npix) continue;
-        // pixels in the instance's order of uniform waves (inside each
-        // sphere, then outside every sphere), so that a wave takes one
-        // radial path
-        const int pix = I.perm_off >= 0 ? perm[I.perm_off + j] : j;
-        const int iv = pix / I.uR, iu = pix - iv * I.uR;
-        const d3 p = quad_at(I.q, (double)iu / (double)I.uR, (double)iv / (double)I.vR);
-        const SeriesGroup G = grps[item.grp];
-        float acc_r = 0.f, acc_i = 0.f;
-        for (int s = G.s_begin; s < G.s_begin + G.ns; ++s) {
-            const SeriesSph& SS = sph[s];
-            const SphereDev S = SS.s;
-            const int nl1 = S.Nl + 1;
-            d3 r = sub(p, mk(S.p));
-            const double d = len(r);
-            // direction; the centre (knd below float epsilon) keeps only l = 0
-            const bool centre = d <= S.a && S.n[0] * (I.kmag * d) < (double)kEpsFloat;
-            r = centre ? d3{0.0, 0.0, 0.0} : norm(r);
-            if (kPlanar || SS.planar) {
-                // r.z = 0, so sin(theta) = 1 and e^{i phi} = r.x + i r.y
-                const RadialTrig trig = harm_trig(S, I.kmag, d);
-                float ar[L], ai[L];
-#pragma unroll
-                for (int l = 0; l < L; ++l) ar[l] = ai[l] = 0.f;
-                harm_planar_sums<L>(rec + SS.toff, nl1, (float)r.x, (float)r.y, ar, ai);
-                harm_radial<L>(S, I.kmag, d, trig, A, B, ar, ai, acc_r, acc_i);
-                continue;
-            }
-            const float x = (float)r.z;
-            const float st = sqrtf(fmaxf(0.f, fmaf(-x, x, 1.f)));
-            float z1r = 1.f, z1i = 0.f;
-            if (st > 1e-30f) {
-                z1r = (float)r.x / st;
-                z1i = (float)r.y / st;
-            }
-            const float x2 = 2.f * x;
-            const RadialTrig trig = harm_trig(S, I.kmag, d);
-            float ar[L], ai[L];
-#pragma unroll
-            for (int l = 0; l < L; ++l) ar[l] = ai[l] = 0.f;
-            const float4* R = rec + SS.toff;   // row stride L: the instance's bucket
-            const int be = (nl1 + 3) >> 2;
-            float zr = 1.f, zi = 0.f;
-            float pmm = 0.28209479177387814f;   // 1/sqrt(4 pi)
-            // rows in pairs (m, m + 1), m even: both start in block m / 4, so
-            // one wait covers two rows' loads and the two recurrences
-            // interleave (a row m + 1 = Nl + 1 has zero records)
-#pragma clang loop unroll(disable)
-            for (int m = 0; m < nl1; m += 2) {
-                const float* Ca = cst + 2 * m * kCS;
-                const float* Cb = Ca + 2 * kCS;
-                const float4* Ra = R + m * L;
-                const float4* Rb = Ra + L;
-                const float pa = pmm, pb = Cb[2 * (m + 1)] * st * pmm;
-                const float zar = zr, zai = zi;
-                const float zbr = zr * z1r - zi * z1i, zbi = fmaf(zr, z1i, zi * z1r);
-                const int bf = m >> 2;
-                float qa0 = 0.f, qa1 = 0.f, qb0 = 0.f, qb1 = 0.f;
-#pragma clang loop unroll(full)
-                for (int b = 0; b < L / 4; ++b) {
-                    if (b < bf || b >= be) continue;
-                    const float4 ca0 = *reinterpret_cast<const float4*>(Ca + 8 * b);
-                    const float4 ca1 = *reinterpret_cast<const float4*>(Ca + 8 * b + 4);
-                    const float4 cb0 = *reinterpret_cast<const float4*>(Cb + 8 * b);
-                    const float4 cb1 = *reinterpret_cast<const float4*>(Cb + 8 * b + 4);
-                    const float cca[4] = {ca0.x, ca0.z, ca1.x, ca1.z};
-                    const float eea[4] = {ca0.y, ca0.w, ca1.y, ca1.w};
-                    const float ccb[4] = {cb0.x, cb0.z, cb1.x, cb1.z};
-                    const float eeb[4] = {cb0.y, cb0.w, cb1.y, cb1.w};
-                    float4 ra[4], rb[4];
-#pragma clang loop unroll(full)
-                    for (int j = 0; j < 4; ++j) {
-                        ra[j] = Ra[4 * b + j];
-                        rb[j] = Rb[4 * b + j];
-                    }
-#pragma clang loop unroll(full)
-                    for (int j = 0; j < 4; ++j) {
-                        // the diagonal value enters at l = m (ee = 1 there, else
-                        // 0); below it q stays 0
-                        const int l = 4 * b + j;
-                        const float qa = fmaf(x2, qa1, fmaf(-cca[j], qa0, eea[j] * pa));
-                        const float qb = fmaf(x2, qb1, fmaf(-ccb[j], qb0, eeb[j] * pb));
-                        qa0 = qa1;
-                        qa1 = qa;
-                        qb0 = qb1;
-                        qb1 = qb;
-                        const float ta1 = qa * zar, ta2 = qa * zai;
-                        const float tb1 = qb * zbr, tb2 = qb * zbi;
-                        ar[l] = fmaf(ta2, ra[j].z, fmaf(ta1, ra[j].x, ar[l]));
-                        ai[l] = fmaf(ta2, ra[j].w, fmaf(ta1, ra[j].y, ai[l]));
-                        ar[l] = fmaf(tb2, rb[j].z, fmaf(tb1, rb[j].x, ar[l]));
-                        ai[l] = fmaf(tb2, rb[j].w, fmaf(tb1, rb[j].y, ai[l]));
-                    }
-                }
-                // the next pair's diagonal value and phase
-                pmm = Cb[2 * (kCS + m + 2)] * st * pb;
-                zr = zbr * z1r - zbi * z1i;
-                zi = fmaf(zbr, z1i, zbi * z1r);
-            }
-            harm_radial<L>(S, I.kmag, d, trig, A, B, ar, ai, acc_r, acc_i);
-        }
-        float2* o = U + I.u_off + pix;
-        float2 v = make_float2(0.f, 0.f);
-        if (G.accumulate) v = *o;
-        *o = make_float2(v.x + acc_r, v.y + acc_i);
-    }
-}
-
 template <int L>
 void launch_bucket(dim3 grid, hipStream_t st, const SeriesPlan& P, const SeriesPlan::Bucket& K,
                    float2* U) {
     hipLaunchKernelGGL(series_f32_kernel<L>, grid, dim3(kThreads), 0, st, P.d_inst, P.d_sph,
                        P.d_items + K.item_begin, K.item_count, P.d_coef, P.d_rhat, P.d_wt, U,
                        P.d_part, P.d_counters + K.counter);
-}
-
-template <int L>
-void launch_harm(dim3 grid, hipStream_t st, const SeriesPlan& P, const SeriesPlan::Bucket& K,
-                 const double2* A, const double2* B, float2* U) {
-    // buckets whose spheres are all planar take the kernel without the
-    // general branch (fewer registers live)
-    if (K.planar)
-        hipLaunchKernelGGL((series_harm_kernel<L, true>), grid, dim3(kThreads), 0, st, P.d_inst,
-                           P.d_sph, P.d_grp, P.d_items + K.item_begin, K.item_count, A, B,
-                           P.d_rec, P.d_cst, P.d_perm, U, P.d_counters + K.counter);
-    else
-        hipLaunchKernelGGL((series_harm_kernel<L, false>), grid, dim3(kThreads), 0, st, P.d_inst,
-                           P.d_sph, P.d_grp, P.d_items + K.item_begin, K.item_count, A, B,
-                           P.d_rec, P.d_cst, P.d_perm, U, P.d_counters + K.counter);
 }
 
 template <class T>
@@ -981,6 +668,7 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
         cost[i] = (double)npix * I.ns * I.nk_pad * bucket[i];
         P->nk_max = std::max(P->nk_max, I.nk_pad);
         P->max_npix = std::max(P->max_npix, npix);
+        P->field_elems = std::max(P->field_elems, I.u_off + (long long)npix);
     }
     if (next_sph != P->n_sph) return set_error(ctx, BIMSIM_E_INVALID, "unused spheres in batch");
     // the harmonic form replaces the wave loop by (Nl+1)(Nl+2)/2 terms per
@@ -1112,8 +800,8 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
             }
             const int count = (int)items.size() - begin;
             if (count > 0)
-                ch.buckets.push_back(
-                    SeriesPlan::Bucket{L, begin, count, P->n_counters++, all_planar});
+                ch.buckets.push_back(SeriesPlan::Bucket{L, begin, count, P->n_counters++,
+                                                        all_planar, kHarmDefaultVariant});
         }
         P->chunks.push_back(ch);
         i0 = i1;
@@ -1316,6 +1004,46 @@ int series_plan_launch_prepass(bimsim_ctx* ctx, const SeriesPlan& P, const doubl
     return BIMSIM_OK;
 }
 
+namespace {
+
+// One bucket's persistent launch (the counter must be zero).
+int launch_one_bucket(bimsim_ctx* ctx, const SeriesPlan& P, const SeriesPlan::Bucket& K,
+                      const double2* A, const double2* B, float2* U) {
+    const hipStream_t st = ctx->stream;
+    // resident workgroups: the queue keeps them busy until it drains
+    const dim3 grid(std::min(K.item_count, P.groups_per_cu * ctx->cu_count));
+    const HarmLaunch hl{grid, st, P.d_inst, P.d_sph, P.d_grp, P.d_items + K.item_begin,
+                        K.item_count, A, B, P.d_rec, P.d_cst, P.d_perm, U,
+                        P.d_counters + K.counter, K.planar};
+    int hv = 0;
+    switch (K.L) {
+#define BIMSIM_CASE(LL) \
+    case LL:                                                \
+        if (P.harmonic) hv = harm_launch<LL>(hl, K.variant); \
+        else launch_bucket<LL>(grid, st, P, K, U);          \
+        break;
+        BIMSIM_CASE(16)
+        BIMSIM_CASE(24)
+        BIMSIM_CASE(32)
+        BIMSIM_CASE(40)
+        BIMSIM_CASE(48)
+        BIMSIM_CASE(64)
+        BIMSIM_CASE(80)
+        BIMSIM_CASE(96)
+        BIMSIM_CASE(112)
+#undef BIMSIM_CASE
+        default:
+            return set_error(ctx, BIMSIM_E_UNSUPPORTED, "no FP32 series kernel for %d orders", K.L);
+    }
+    if (hv != 0)
+        return set_error(ctx, BIMSIM_E_INVALID, "no variant %d of the %d-order kernel", K.variant,
+                         K.L);
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return BIMSIM_OK;
+}
+
+}  // namespace
+
 int series_plan_launch_sums(bimsim_ctx* ctx, const SeriesPlan& P, const double2* A,
                             const double2* B, float2* U) {
     if (P.n_inst == 0 || P.n_sph == 0) return BIMSIM_OK;
@@ -1328,30 +1056,7 @@ int series_plan_launch_sums(bimsim_ctx* ctx, const SeriesPlan& P, const double2*
             BIMSIM_HIP_CHECK(ctx, hipGetLastError());
         }
         const int tmr = timer_begin(ctx, kStageSeries);
-        for (const SeriesPlan::Bucket& K : ch.buckets) {
-            // resident workgroups: the queue keeps them busy until it drains
-            const dim3 grid(std::min(K.item_count, P.groups_per_cu * ctx->cu_count));
-            switch (K.L) {
-#define BIMSIM_CASE(LL) \
-    case LL:                                                \
-        if (P.harmonic) launch_harm<LL>(grid, st, P, K, A, B, U); \
-        else launch_bucket<LL>(grid, st, P, K, U);          \
-        break;
-                BIMSIM_CASE(16)
-                BIMSIM_CASE(24)
-                BIMSIM_CASE(32)
-                BIMSIM_CASE(40)
-                BIMSIM_CASE(48)
-                BIMSIM_CASE(64)
-                BIMSIM_CASE(80)
-                BIMSIM_CASE(96)
-                BIMSIM_CASE(112)
-#undef BIMSIM_CASE
-                default:
-                    return set_error(ctx, BIMSIM_E_UNSUPPORTED, "no FP32 series kernel for %d orders", K.L);
-            }
-            BIMSIM_HIP_CHECK(ctx, hipGetLastError());
-        }
+        for (const SeriesPlan::Bucket& K : ch.buckets) BIMSIM_TRY(launch_one_bucket(ctx, P, K, A, B, U));
         timer_end(ctx, tmr);
     }
     if (!P.split_insts.empty()) {
@@ -1381,6 +1086,77 @@ std::string shape_key(const bimsim_ctx* ctx, const SeriesBatch& batch) {
 
 }  // namespace
 
+namespace {
+
+// The register budget of each bucket of a harmonic plan, chosen by timing
+// its variants on the plan's own inputs (into a scratch field); the choice
+// is kept in the context per batch shape.
+int harm_tune(bimsim_ctx* ctx, const std::string& key, const double* waves, const double2* A,
+              const double2* B, SeriesPlan* P) {
+    std::vector<SeriesPlan::Bucket*> bks;
+    for (SeriesPlan::Chunk& ch : P->chunks)
+        for (SeriesPlan::Bucket& K : ch.buckets) bks.push_back(&K);
+    auto hit = ctx->tune_harm.find(key);
+    if (hit != ctx->tune_harm.end() && hit->second.size() == bks.size()) {
+        for (size_t i = 0; i < bks.size(); ++i) bks[i]->variant = hit->second[i];
+        return BIMSIM_OK;
+    }
+    const long long field = P->field_elems;
+    float2* tmp = nullptr;
+    if (hipMalloc((void**)&tmp, sizeof(float2) * (size_t)std::max<long long>(1, field)) != hipSuccess)
+        return set_error(ctx, BIMSIM_E_NOMEM, "out of device memory for tuning");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const bool timing = ctx->timing;
+    ctx->timing = false;
+    int rc = zero_launch(ctx, tmp, sizeof(float2) * (size_t)std::max<long long>(1, field));
+    if (rc == BIMSIM_OK) rc = series_plan_launch_waves(ctx, *P, waves);
+    std::vector<int> chosen;
+    for (SeriesPlan::Bucket* K : bks) {
+        if (rc != BIMSIM_OK) break;
+        float best = 1e30f;
+        int best_v = K->variant;
+        for (int v = 0; v < kHarmVariants && rc == BIMSIM_OK; ++v) {
+            if (harm_variant_waves(K->L, v) <= 0) continue;
+            K->variant = v;
+            float ms = 0.f;
+            for (int r = 0; r < 2 && rc == BIMSIM_OK; ++r) {   // a warm run, then the timed one
+                rc = zero_launch(ctx, P->d_counters + K->counter, sizeof(int));
+                if (rc != BIMSIM_OK) break;
+                (void)hipEventRecord(e0, ctx->stream);
+                rc = launch_one_bucket(ctx, *P, *K, A, B, tmp);
+                (void)hipEventRecord(e1, ctx->stream);
+                (void)hipEventSynchronize(e1);
+                (void)hipEventElapsedTime(&ms, e0, e1);
+            }
+            if (rc == BIMSIM_OK && ms < best) {
+                best = ms;
+                best_v = v;
+            }
+        }
+        K->variant = best_v;
+        chosen.push_back(best_v);
+    }
+    ctx->timing = timing;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(tmp);
+    BIMSIM_TRY(rc);
+    ctx->tune_harm[key] = chosen;
+    if (std::getenv("BIMSIM_TUNE_LOG")) {
+        std::string msg;
+        for (size_t i = 0; i < bks.size(); ++i)
+            msg += " L" + std::to_string(bks[i]->L) + ":" +
+                   std::to_string(harm_variant_waves(bks[i]->L, chosen[i]));
+        std::fprintf(stderr, "bimsim: harmonic waves per SIMD%s\n", msg.c_str());
+    }
+    return BIMSIM_OK;
+}
+
+}  // namespace
+
 int series_plan_create_tuned(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget,
                              const double* waves, const double2* A, const double2* B, SeriesPlan* P) {
     // the heuristic plan first: long queues keep it
@@ -1391,6 +1167,12 @@ int series_plan_create_tuned(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_bu
         for (const SeriesPlan::Bucket& K : ch.buckets) items += K.item_count;
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     (void)hipStreamIsCapturing(ctx->stream, &cap);
+    if (P->harmonic) {
+        // the harmonic form tunes the register budget of each bucket
+        batch = work;
+        if (!ctx->autotune || cap != hipStreamCaptureStatusNone) return BIMSIM_OK;
+        return harm_tune(ctx, shape_key(ctx, batch) + "|harm", waves, A, B, P);
+    }
     if (!ctx->autotune || items > 16LL * ctx->cu_count || cap != hipStreamCaptureStatusNone) {
         batch = work;
         return BIMSIM_OK;
