@@ -39,7 +39,7 @@ EXPORTS = (
     "bimsim_batch_launch", "bimsim_batch_destroy", "bimsim_batch_run", "bimsim_set_autotune",
     "bimsim_set_series_algorithm", "bimsim_detector_rotate_add", "bimsim_detector_ring_sums",
     "bimsim_detector_from_line", "bimsim_field_backpropagate", "bimsim_nf_uf_mc",
-    "bimsim_detector_zero_filled",
+    "bimsim_detector_zero_filled", "bimsim_detector_noalloc",
 )
 
 
@@ -135,6 +135,10 @@ def lib():
                                                   c_int, c_int, c_int, c_double, c_double,
                                                   c_double, c_double, c_double, c_void_p,
                                                   c_void_p, c_void_p]
+        L.bimsim_detector_noalloc.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p,
+                                              c_int, c_int, c_double, c_double, c_double,
+                                              c_double, c_double, c_int, c_int, c_int, c_void_p,
+                                              c_void_p, c_void_p]
         L.bimsim_uf_bessel_lut.argtypes = [c_void_p, c_int, c_double, c_double, c_double, c_int,
                                            c_int, c_void_p]
         L.bimsim_nf_uf_lut.argtypes = [c_void_p, ctypes.POINTER(Quad), c_int, c_int,
@@ -542,6 +546,19 @@ class Context:
         self._check(self.L.bimsim_detector_zero_filled(self.h, prec, quirks, _ptr(U), _ptr(Uf), uR,
                                                        vR, fill, du, dv, NAin, NAout, lam, _ptr(D),
                                                        _ptr(Di), _ptr(Abs)))
+
+    def detector_noalloc(self, U, Uf, uR, vR, du, dv, NAin, NAout, lam, padding, Nd, ss, D, Di,
+                         Abs=None, prec=FP64_PARITY, quirks=QUIRKS_REF):
+        """bimsim_detector (U and Uf left unchanged) on the library's fixed
+        work memory: no device allocation."""
+        check_field(U, prec, uR * vR, "U")
+        check_field(Uf, prec, uR * vR, "Uf")
+        for t, name in ((D, "D"), (Di, "Di")):
+            check_real(t, prec, Nd * Nd, name)
+        check_real(Abs, prec, Nd * Nd, "Abs", optional=True)
+        self._check(self.L.bimsim_detector_noalloc(self.h, prec, quirks, _ptr(U), _ptr(Uf), uR, vR,
+                                                   du, dv, NAin, NAout, lam, padding, Nd, ss,
+                                                   _ptr(D), _ptr(Di), _ptr(Abs)))
 
     def detector(self, U, Uf, uR, vR, du, dv, NAin, NAout, lam, padding, Nd, ss, D, Di,
                  Abs=None, prec=FP64_PARITY, quirks=QUIRKS_REF, write_filtered=True):

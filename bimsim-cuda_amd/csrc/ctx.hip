@@ -82,8 +82,7 @@ int twiddles_get(bimsim_ctx* ctx, int n, const double** d, const float** f) {
         std::vector<double> hd(2 * (size_t)n);
         std::vector<float> hf(2 * (size_t)n);
         for (int t = 0; t < n; ++t) {
-            // angle reduced to the first octant pattern through the exact
-            // integer fraction t/n; cos/sin evaluated in double
+            // cos/sin of the angle of the exact integer fraction t/n, in double
             const double ang = -2.0 * M_PI * (double)t / (double)n;
             hd[2 * t] = std::cos(ang);
             hd[2 * t + 1] = std::sin(ang);
@@ -155,6 +154,9 @@ int bimsim_ctx_create(int device, void* hip_stream, bimsim_ctx** out) {
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return BIMSIM_E_HIP;
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return BIMSIM_E_NODEVICE;
     if (hipSetDevice(device) != hipSuccess) return BIMSIM_E_HIP;
+    // the allocation-free detector's work memory on this device (a failure
+    // only disables that entry point, which then reports it)
+    (void)noalloc_scratch_init(device);
     bimsim_ctx* ctx = new bimsim_ctx();
     ctx->device = device;
     ctx->cu_count = prop.multiProcessorCount;

@@ -9,6 +9,7 @@
 // into D and Di (accumulating, like resample_intensity) and -log10(D/Di).
 // FP64 instantiations run in FP64 end to end (-ffp-contract=off for the
 // whole translation unit); FP32 ones in float with FP64-accurate twiddles.
+#include <mutex>
 #include <type_traits>
 
 #include "batch.h"
@@ -663,30 +664,41 @@ int launch_rows_inv_detect(bimsim_ctx* ctx, int N, dim3 grid, size_t lds,
     return BIMSIM_OK;
 }
 
-// Band-limited detector for FFT lengths with a fixed plan.
-template <class R>
-int detector_band(bimsim_ctx* ctx, const PostOp& bp, void* Uin, void* Ufin, int uR, int vR,
-                  int padding, int Nd, int ss, void* D, void* Di, void* Abs, int write_filtered,
-                  const typename C2<R>::T* tw_u, const typename C2<R>::T* tw_v,
-                  const SumFuse* fz) {
-    using C = typename C2<R>::T;
-    // pass-band columns: |f_u| <= NAout/lambda with the band-pass's own index map
-    std::vector<int> cols;
+// Pass-band columns (|f_u| <= NAout/lambda with the band-pass's own index
+// map): |u| grows with iu up to uR/2 and shrinks after it, so they are the
+// first n1 columns and the last n2.
+void band_columns(const PostOp& bp, int uR, int* n1, int* n2) {
+    *n1 = *n2 = 0;
     for (int iu = 0; iu < uR; ++iu) {
         double u;
         if (iu <= uR / 2) u = (double)iu * bp.du;
         else u = -(double)(bp.quirks ? uR - 1 - iu : uR - iu) * bp.du;
-        if (!(std::fabs(u) > bp.fmax)) cols.push_back(iu);
+        if (!(std::fabs(u) > bp.fmax)) ++(iu <= uR / 2 ? *n1 : *n2);
     }
-    const int Nc = (int)cols.size();
+}
+
+std::vector<int> band_column_list(const PostOp& bp, int uR) {
+    int n1, n2;
+    band_columns(bp, uR, &n1, &n2);
+    std::vector<int> cols;
+    for (int c = 0; c < n1; ++c) cols.push_back(c);
+    for (int c = 0; c < n2; ++c) cols.push_back(uR - n2 + c);
+    return cols;
+}
+
+// Band-limited detector for FFT lengths with a fixed plan, on the given
+// work buffers: the pass-band column list colsd (Nc entries), the compact
+// spectrum G (2 Nc vR complex) and, unless the inverse row pass detects
+// on its own rows, the intensity rows I (2 nrows uR real).
+template <class R>
+int detector_band(bimsim_ctx* ctx, const PostOp& bp, void* Uin, void* Ufin, int uR, int vR,
+                  int padding, int Nd, int ss, void* D, void* Di, void* Abs, int write_filtered,
+                  const typename C2<R>::T* tw_u, const typename C2<R>::T* tw_v,
+                  const SumFuse* fz, const int* colsd, int Nc, void* G, void* I) {
+    using C = typename C2<R>::T;
     const int S = Nd * ss, off = padding * S;
     const int row_begin = write_filtered ? 0 : off;
     const int nrows = write_filtered ? vR : S;
-    const int* colsd = nullptr;
-    void *G = nullptr, *I = nullptr;
-    BIMSIM_TRY(int_table_get(ctx, cols, &colsd));
-    BIMSIM_TRY(scratch_get(ctx, kSlotFftA, sizeof(C) * 2 * (size_t)std::max(Nc, 1) * vR, &G));
-    BIMSIM_TRY(scratch_get(ctx, kSlotFftB, sizeof(R) * 2 * (size_t)nrows * uR, &I));
     // rows per workgroup: at most 1024 points, and at least 512 workgroups
     // per field where the rows allow it
     auto rows_per_group = [&](int nr) {
@@ -792,9 +804,17 @@ int detector_impl(bimsim_ctx* ctx, void* Uin, void* Ufin, int uR, int vR, double
         bp.fmax = NAout / lambda;
         bp.scale = 1.0 / ((unsigned)uR * (unsigned)vR);   // ScaleField(1.0/(R0*R1))
         const bool dbl = std::is_same<R, double>::value;
+        const std::vector<int> cols = band_column_list(bp, uR);
+        const int Nc = (int)cols.size();
+        const int nrows = write_filtered ? vR : Nd * ss;
+        const int* colsd = nullptr;
+        void *G = nullptr, *I = nullptr;
+        BIMSIM_TRY(int_table_get(ctx, cols, &colsd));
+        BIMSIM_TRY(scratch_get(ctx, kSlotFftA, sizeof(C) * 2 * (size_t)std::max(Nc, 1) * vR, &G));
+        BIMSIM_TRY(scratch_get(ctx, kSlotFftB, sizeof(R) * 2 * (size_t)nrows * uR, &I));
         return detector_band<R>(ctx, bp, Uin, Ufin, uR, vR, padding, Nd, ss, D, Di, Abs,
                                 write_filtered, dbl ? (const C*)twd_u : (const C*)twf_u,
-                                dbl ? (const C*)twd_v : (const C*)twf_v, fz);
+                                dbl ? (const C*)twd_v : (const C*)twf_v, fz, colsd, Nc, G, I);
     }
     if (fz) return set_error(ctx, BIMSIM_E_UNSUPPORTED, "fused sum needs the band-limited detector");
     if (!write_filtered) {
@@ -1027,13 +1047,9 @@ int det_batch_create(bimsim_ctx* ctx, int prec, int quirks, int N, const std::ve
         I.fz = q.fz;
         // pass-band columns, as detector_band
         I.cols_off = (int)cols.size();
-        for (int iu = 0; iu < N; ++iu) {
-            double u;
-            if (iu <= N / 2) u = (double)iu * bp.du;
-            else u = -(double)(quirks ? N - 1 - iu : N - iu) * bp.du;
-            if (!(std::fabs(u) > bp.fmax)) cols.push_back(iu);
-        }
-        I.Nc = (int)cols.size() - I.cols_off;
+        const std::vector<int> c = band_column_list(bp, N);
+        cols.insert(cols.end(), c.begin(), c.end());
+        I.Nc = (int)c.size();
         I.g_off = g;
         g += 2LL * I.Nc * N;
         max_nc = std::max(max_nc, I.Nc);
@@ -1087,6 +1103,141 @@ void det_batch_destroy(DetBatch* b) {
 }
 
 }  // namespace bimsim
+
+// ---------------------------------------------- detector without allocation
+// bimsim_detector_noalloc keeps its work memory (twiddles, pass-band column
+// list, compact spectrum, intensity rows) in `scratch`, a file-scope device
+// array that every GPU holds from the library's load until the process
+// ends.  Its device address is taken once per device at context creation
+// (noalloc_scratch_init, with the device current) and kept per device; the
+// array's name is a host-side stand-in and never reaches a kernel.
+namespace {
+
+constexpr size_t kNoAllocBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kNoAllocDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kNoAllocDevices] = {};
+bool g_noalloc_tried[kNoAllocDevices] = {};
+
+// e^{-2 pi i t/n}, t < n, evaluated in double (as the context's tables)
+template <class C>
+__global__ void twiddle_fill_kernel(C* __restrict__ tw, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const double ang = -2.0 * M_PI * (double)t / (double)n;
+    tw[t].x = cos(ang);
+    tw[t].y = sin(ang);
+}
+
+// the first n1 and the last n2 of uR columns
+__global__ void band_cols_kernel(int* __restrict__ cols, int n1, int n2, int uR) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n1 + n2) cols[c] = c < n1 ? c : uR - n2 + (c - n1);
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+template <class R>
+int detector_noalloc_impl(bimsim_ctx* ctx, unsigned char* work, int quirks, const void* U,
+                          const void* Uf, int uR, int vR, double du, double dv, double NAin,
+                          double NAout, double lambda, int padding, int Nd, int ss, void* D,
+                          void* Di, void* Abs) {
+    using C = typename C2<R>::T;
+    PostOp bp{};
+    bp.kind = 1;
+    bp.uR = uR;
+    bp.vR = vR;
+    bp.quirks = quirks;
+    bp.du = du;
+    bp.dv = dv;
+    bp.fmin = NAin / lambda;
+    bp.fmax = NAout / lambda;
+    bp.scale = 1.0 / ((unsigned)uR * (unsigned)vR);   // ScaleField(1.0/(R0*R1))
+    int n1, n2;
+    band_columns(bp, uR, &n1, &n2);
+    const int Nc = n1 + n2;
+    const int nrows = Nd * ss;
+    // work layout: twiddles of uR and vR, columns, compact spectrum, rows
+    const size_t o_tu = 0, o_tv = align256(sizeof(C) * uR);
+    const size_t o_cols = o_tv + align256(sizeof(C) * vR);
+    const size_t o_G = o_cols + align256(sizeof(int) * std::max(Nc, 1));
+    const size_t o_I = o_G + align256(sizeof(C) * 2 * (size_t)std::max(Nc, 1) * vR);
+    const size_t need = o_I + sizeof(R) * 2 * (size_t)nrows * uR;
+    if (need > kNoAllocBytes)
+        return set_error(ctx, BIMSIM_E_INVALID,
+                         "the detector needs %zu bytes of work memory; the allocation-free "
+                         "entry point has %zu",
+                         need, kNoAllocBytes);
+    C* tu = (C*)(work + o_tu);
+    C* tv = (C*)(work + o_tv);
+    int* cols = (int*)(work + o_cols);
+    hipLaunchKernelGGL(twiddle_fill_kernel<C>, dim3((uR + 255) / 256), dim3(256), 0, ctx->stream,
+                       tu, uR);
+    hipLaunchKernelGGL(twiddle_fill_kernel<C>, dim3((vR + 255) / 256), dim3(256), 0, ctx->stream,
+                       tv, vR);
+    if (Nc > 0)
+        hipLaunchKernelGGL(band_cols_kernel, dim3((Nc + 255) / 256), dim3(256), 0, ctx->stream,
+                           cols, n1, n2, uR);
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return detector_band<R>(ctx, bp, const_cast<void*>(U), const_cast<void*>(Uf), uR, vR, padding,
+                            Nd, ss, D, Di, Abs, 0, tu, tv, nullptr, cols, Nc, work + o_G,
+                            work + o_I);
+}
+
+}  // namespace
+
+namespace bimsim {
+
+int noalloc_scratch_init(int device) {
+    if (device < 0 || device >= kNoAllocDevices) return BIMSIM_E_INVALID;
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (g_noalloc_tried[device]) return g_noalloc_ptr[device] ? BIMSIM_OK : BIMSIM_E_HIP;
+    g_noalloc_tried[device] = true;
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        return BIMSIM_E_HIP;
+    }
+    g_noalloc_ptr[device] = ptr;
+    return BIMSIM_OK;
+}
+
+}  // namespace bimsim
+
+extern "C" int bimsim_detector_noalloc(bimsim_ctx* ctx, int prec, int quirks, const void* U,
+                                       const void* Uf, int uR, int vR, double du, double dv,
+                                       double NAin, double NAout, double lambda, int padding,
+                                       int Nd, int ss, void* D, void* Di, void* Abs) {
+    if (!ctx) return BIMSIM_E_INVALID;
+    BIMSIM_CHECK_ARG(ctx, U && Uf && D && Di, "null pointer argument");
+    BIMSIM_CHECK_ARG(ctx, uR > 0 && vR > 0 && Nd > 0 && ss > 0 && padding >= 0, "bad sizes");
+    BIMSIM_CHECK_ARG(ctx, (long long)(2 * padding + 1) * Nd * ss <= std::min(uR, vR),
+                     "detector crop exceeds the near-field grid");
+    BIMSIM_CHECK_ARG(ctx, lambda > 0.0 && du > 0.0 && dv > 0.0, "lambda, du and dv must be positive");
+    BIMSIM_CHECK_ARG(ctx, prec >= BIMSIM_FP64_PARITY && prec <= BIMSIM_FP32_FAST, "unknown precision");
+    const bool f32 = prec == BIMSIM_FP32_FAST;
+    if (!band_path(ctx, f32 ? sizeof(float2) : sizeof(double2), uR, vR))
+        return set_error(ctx, BIMSIM_E_UNSUPPORTED,
+                         "the allocation-free detector takes FFT sizes of 64..4096 from "
+                         "{2^k, 3 2^k} whose rows fit LDS (got %d x %d)",
+                         uR, vR);
+    void* work = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_noalloc_mu);
+        if (ctx->device >= 0 && ctx->device < kNoAllocDevices) work = g_noalloc_ptr[ctx->device];
+    }
+    if (!work)
+        return set_error(ctx, BIMSIM_E_HIP, "the work memory of device %d was not set up",
+                         ctx->device);
+    BIMSIM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    return f32 ? detector_noalloc_impl<float>(ctx, (unsigned char*)work, quirks, U, Uf, uR, vR, du,
+                                              dv, NAin, NAout, lambda, padding, Nd, ss, D, Di, Abs)
+               : detector_noalloc_impl<double>(ctx, (unsigned char*)work, quirks, U, Uf, uR, vR,
+                                               du, dv, NAin, NAout, lambda, padding, Nd, ss, D,
+                                               Di, Abs);
+}
 
 extern "C" int bimsim_detector(bimsim_ctx* ctx, int prec, int quirks, void* U, void* Uf, int uR, int vR,
                                double du, double dv, double NAin, double NAout, double lambda,

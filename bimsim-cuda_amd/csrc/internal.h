@@ -79,6 +79,10 @@ int dbl_table_get(bimsim_ctx* ctx, const std::vector<double>& v, const double** 
 // Twiddle table for length n on the device (double and float copies).
 int twiddles_get(bimsim_ctx* ctx, int n, const double** d, const float** f);
 
+// Takes the device address of bimsim_detector_noalloc's work memory on
+// `device` (current) once per device; called by bimsim_ctx_create.
+int noalloc_scratch_init(int device);
+
 // Stages of the timers.
 enum TimerStage { kStageUf = 0, kStageUs = 1, kStageDet = 2, kStageSeries = 3, kNumStages = 4 };
 

@@ -257,6 +257,22 @@ int bimsim_detector_zero_filled(bimsim_ctx* ctx, int prec, int quirks, const voi
                                 double NAin, double NAout, double lambda, void* D, void* Di,
                                 void* Abs);
 
+/* bimsim_detector (write_filtered = 0: U and Uf are left unchanged) without
+ * any device allocation.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory, such as code under HIP graph capture,
+ * tight serving loops and harnesses that forbid hipMalloc.  Its work memory
+ * (twiddles, pass-band columns, compact spectrum, intensity rows) is a
+ * fixed 64 MiB device array that each GPU the library runs on holds until
+ * the process ends; calls needing more return BIMSIM_E_INVALID.  FFT sizes
+ * must have a fixed plan (64..4096 of 2^k and 3 2^k, rows fitting LDS),
+ * else BIMSIM_E_UNSUPPORTED.  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (other streams, other threads or contexts)
+ * race and must be serialised by the caller. */
+int bimsim_detector_noalloc(bimsim_ctx* ctx, int prec, int quirks, const void* U, const void* Uf,
+                            int uR, int vR, double du, double dv, double NAin, double NAout,
+                            double lambda, int padding, int Nd, int ss, void* D, void* Di,
+                            void* Abs);
+
 /* ------------------------------------------- slice building blocks
  * The steps of microscopeStruct::SimulateImaging one at a time, for host
  * code that calls them separately.  Fields are complex128 (FP64
