@@ -1062,9 +1062,16 @@ int det_batch_create(bimsim_ctx* ctx, int prec, int quirks, int N, const std::ve
     out->fused = fused && f32;
     out->max_nc = max_nc;
     out->max_nd = max_nd;
-    out->B = rows_per_wg(N);
-    out->B2 = rows_per_wg(N);
-    out->Bc = std::max(1, std::min(8, 2048 / N));
+    // rows (columns) per workgroup, fewer when the batch is small so that
+    // each pass still spreads over the chip
+    const int n = (int)insts.size();
+    int B = rows_per_wg(N), B2 = B, Bc = std::max(1, std::min(8, 2048 / N));
+    while (B > 1 && (long long)n * ((N + B - 1) / B) < 512) B /= 2;
+    while (B2 > 1 && (long long)n * ((max_nd + B2 - 1) / B2) < 512) B2 /= 2;
+    while (Bc > 1 && (long long)n * ((max_nc + Bc - 1) / Bc) < 128) Bc /= 2;
+    out->B = B;
+    out->B2 = B2;
+    out->Bc = Bc;
     const double* twd;
     const float* twf;
     BIMSIM_TRY(twiddles_get(ctx, N, &twd, &twf));

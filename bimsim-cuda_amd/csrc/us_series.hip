@@ -517,12 +517,126 @@ __device__ __forceinline__ void harm_records_row(
     }
 }
 
+// Records of few instances: block (m, tile, s) forms four records of row m
+// of sphere s (orders l = 4 tile + j, or for a planar sphere k = 4 tile + j
+// with l = 2k + (m & 1)), its threads running over the instance's waves
+// (coalesced reads of the Legendre, azimuth-phase and wave-weight tables)
+// and a block reduction forming S and S' as harm_records_row's; records
+// outside m <= l <= Nl are written as zeros.  More blocks than the row form
+// when a batch holds few instances, so that the waves spread over the chip.
+// Grid: (m < largest record stride, tiles, spheres).
+constexpr int kRecTile = 4;
+
+__device__ __forceinline__ void harm_records_tile(
+    int m, int tile, int s_first, int s_step, const SeriesInst* __restrict__ insts,
+    const SeriesSph* __restrict__ sph, int n_sph, const double* __restrict__ pl,
+    const double2* __restrict__ e, const double2* __restrict__ phi,
+    const double* __restrict__ h64, float4* __restrict__ rec) {
+    __shared__ double red[kRecThreads / 64][4 * kRecTile];
+    for (int s = s_first; s < n_sph; s += s_step) {
+        const SeriesSph& SS = sph[s];
+        const int tl = SS.tl;
+        const bool planar = SS.planar != 0;
+        const int width = planar ? tl / 2 : tl;   // records per row
+        if (m >= tl || kRecTile * tile >= width) continue;
+        const SeriesInst& I = insts[SS.inst];
+        const int nl1 = SS.s.Nl + 1;
+        int lj[kRecTile];
+        bool need[kRecTile];
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < kRecTile; ++j) {
+            const int k = kRecTile * tile + j;
+            lj[j] = planar ? 2 * k + (m & 1) : k;
+            need[j] = k < width && m < nl1 && lj[j] >= m && lj[j] < nl1;
+            any = any || need[j];
+        }
+        // (T+ re, T+ im, T- re, T- im) of each record
+        double acc[4 * kRecTile];
+#pragma unroll
+        for (int q = 0; q < 4 * kRecTile; ++q) acc[q] = 0.0;
+        if (any) {
+            const double* P[kRecTile];
+#pragma unroll
+            for (int j = 0; j < kRecTile; ++j)
+                P[j] = pl + I.pl_off + (tri_off(m, I.lh) + (need[j] ? lj[j] - m : 0)) * I.nk;
+            const double2* E = e + I.e_off + (long long)m * I.nk;
+            const double2* F = phi + SS.woff;
+            for (int w = threadIdx.x; w < I.nk; w += kRecThreads) {
+                const double2 ew = E[w], fw = F[w];
+#pragma unroll
+                for (int j = 0; j < kRecTile; ++j) {
+                    if (!need[j]) continue;
+                    const double p = P[j][w];
+                    const double ar = fw.x * p, ai = fw.y * p;
+                    acc[4 * j] += ar * ew.x - ai * ew.y;
+                    acc[4 * j + 1] += ar * ew.y + ai * ew.x;
+                    acc[4 * j + 2] += ar * ew.x + ai * ew.y;
+                    acc[4 * j + 3] += ai * ew.x - ar * ew.y;
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4 * kRecTile; ++q) {
+            double v = acc[q];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+            acc[q] = v;
+        }
+        __syncthreads();   // red is reused for the next sphere
+        if ((threadIdx.x & 63) == 0)
+#pragma unroll
+            for (int q = 0; q < 4 * kRecTile; ++q) red[threadIdx.x >> 6][q] = acc[q];
+        __syncthreads();
+        if (threadIdx.x < kRecTile) {
+            const int j = threadIdx.x;
+            const int k = kRecTile * tile + j;
+            if (k < width) {
+                float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (need[j]) {
+                    dc tp = {0.0, 0.0}, tm = {0.0, 0.0};
+                    for (int q = 0; q < kRecThreads / 64; ++q) {
+                        tp = cadd(tp, dc{red[q][4 * j], red[q][4 * j + 1]});
+                        tm = cadd(tm, dc{red[q][4 * j + 2], red[q][4 * j + 3]});
+                    }
+                    const int l = lj[j];
+                    double f;
+                    if (planar) {
+                        // Pn_lm(0): Pn_mm(0) = Pn_00 prod_q sqrt((2q+1)/(2q)),
+                        // Pn_lm(0) = -a_lm b_lm Pn_{l-2,m}(0)
+                        double pn = 0.28209479177387814;
+                        for (int q = 1; q <= m; ++q) pn *= h64[kHDiag + q];
+                        const double* ab = h64 + 2 * (m * kH);
+                        for (int q = m + 2; q <= l; q += 2) pn = -(ab[2 * q] * ab[2 * q + 1]) * pn;
+                        f = 4.0 * M_PI / (2.0 * l + 1.0) * pn;
+                    } else {
+                        f = 4.0 * M_PI / (2.0 * l + 1.0) * h64[kHK + m * kH + l];
+                    }
+                    if (m == 0)
+                        r = make_float4((float)(f * tp.r), (float)(f * tp.i), 0.f, 0.f);
+                    else
+                        r = make_float4((float)(f * (tp.r + tm.r)), (float)(f * (tp.i + tm.i)),
+                                        (float)(f * (tp.i - tm.i)), (float)(f * (tm.r - tp.r)));
+                }
+                rec[SS.toff + (long long)m * width + k] = r;
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(kRecThreads) void harm_records_kernel(
     const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph, int n_inst,
     const double* __restrict__ pl, const double2* __restrict__ e, const double2* __restrict__ phi,
     const double* __restrict__ h64, float4* __restrict__ rec) {
     for (int i = blockIdx.y; i < n_inst; i += gridDim.y)
         harm_records_row(blockIdx.x, i, insts, sph, pl, e, phi, h64, rec);
+}
+
+__global__ __launch_bounds__(kRecThreads) void harm_records_tile_kernel(
+    const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph, int n_sph,
+    const double* __restrict__ pl, const double2* __restrict__ e, const double2* __restrict__ phi,
+    const double* __restrict__ h64, float4* __restrict__ rec) {
+    harm_records_tile(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.z, insts, sph, n_sph, pl, e, phi,
+                      h64, rec);
 }
 
 // The FP32 batch's pre-pass as one launch: Mie coefficients (blocks
@@ -535,8 +649,8 @@ __global__ __launch_bounds__(kRecThreads) void harm_records_kernel(
 // dependent chains of the coefficients run on a few compute units while
 // the records and fields fill the others.  The role is uniform per block.
 __global__ __launch_bounds__(kRecThreads) void prepass_kernel(
-    PrepassArgs a, int n_mie, int n_rec, int rec_m,
-    const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph,
+    PrepassArgs a, int n_mie, int n_rec, int rec_m, int rec_tiles,
+    const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph, int n_sph,
     const double* __restrict__ pl, const double2* __restrict__ e, const double2* __restrict__ phi,
     const double* __restrict__ h64, float4* __restrict__ rec) {
     static_assert(kRecThreads >= mie::kMieThreads, "the coefficient waves fit the block");
@@ -558,7 +672,9 @@ __global__ __launch_bounds__(kRecThreads) void prepass_kernel(
     }
     b -= n_mie;
     if (b < n_rec) {
-        harm_records_row(b % rec_m, b / rec_m, insts, sph, pl, e, phi, h64, rec);
+        const int per_s = rec_m * rec_tiles;
+        const int s = b / per_s, r = b - s * per_s;
+        harm_records_tile(r % rec_m, r / rec_m, s, n_sph, insts, sph, n_sph, pl, e, phi, h64, rec);
         return;
     }
     b -= n_rec;
@@ -968,9 +1084,19 @@ int series_plan_launch_waves(bimsim_ctx* ctx, const SeriesPlan& P, const double*
         hipLaunchKernelGGL(harm_legendre_kernel, lgrid, dim3(64), 0, st, P.d_inst, P.d_sph,
                            P.n_inst, waves, P.d_h64, P.d_pl, P.d_e, P.d_phi, P.d_counters,
                            P.n_counters);
-        const dim3 rgrid(P.max_lh, std::min(P.n_inst, 65535));
-        hipLaunchKernelGGL(harm_records_kernel, rgrid, dim3(kRecThreads), 0, st, P.d_inst,
-                           P.d_sph, P.n_inst, P.d_pl, P.d_e, P.d_phi, P.d_h64, P.d_rec);
+        if ((long long)P.max_lh * P.n_inst >= 2LL * ctx->cu_count) {
+            // one block per (row, instance) over all its spheres
+            const dim3 rgrid(P.max_lh, std::min(P.n_inst, 65535));
+            hipLaunchKernelGGL(harm_records_kernel, rgrid, dim3(kRecThreads), 0, st, P.d_inst,
+                               P.d_sph, P.n_inst, P.d_pl, P.d_e, P.d_phi, P.d_h64, P.d_rec);
+        } else {
+            // few instances: blocks per (row, four records, sphere)
+            const dim3 rgrid(P.max_tl, (P.max_tl + kRecTile - 1) / kRecTile,
+                             std::min(P.n_sph, 65535));
+            hipLaunchKernelGGL(harm_records_tile_kernel, rgrid, dim3(kRecThreads), 0, st,
+                               P.d_inst, P.d_sph, P.n_sph, P.d_pl, P.d_e, P.d_phi, P.d_h64,
+                               P.d_rec);
+        }
     } else {
         hipLaunchKernelGGL(series_waves_kernel, wgrid, dim3(256), 0, st, P.d_inst, P.d_sph, P.n_sph,
                            waves, P.d_wt, P.d_counters, P.n_counters);
@@ -993,12 +1119,13 @@ int series_plan_launch_prepass(bimsim_ctx* ctx, const SeriesPlan& P, const doubl
     hipLaunchKernelGGL(harm_legendre_kernel, lgrid, dim3(64), 0, st, P.d_inst, P.d_sph, P.n_inst,
                        waves, P.d_h64, P.d_pl, P.d_e, P.d_phi, P.d_counters, P.n_counters);
     const int n_mie = (a.n_mie_sph + a.spb - 1) / a.spb;
-    const long long n_rec = (long long)P.max_lh * P.n_inst;
+    const int tiles = (P.max_tl + kRecTile - 1) / kRecTile;
+    const long long n_rec = (long long)P.max_tl * tiles * P.n_sph;
     const long long total = n_mie + n_rec + (long long)a.uf_blocks * a.n_uf;
     if (total > INT_MAX)
         return set_error(ctx, BIMSIM_E_INVALID, "pre-pass launch outside its limits");
     hipLaunchKernelGGL(prepass_kernel, dim3((unsigned)total), dim3(kRecThreads), prepass_lds(a), st,
-                       a, n_mie, (int)n_rec, P.max_lh, P.d_inst, P.d_sph, P.d_pl,
+                       a, n_mie, (int)n_rec, P.max_tl, tiles, P.d_inst, P.d_sph, P.n_sph, P.d_pl,
                        P.d_e, P.d_phi, P.d_h64, P.d_rec);
     BIMSIM_HIP_CHECK(ctx, hipGetLastError());
     return BIMSIM_OK;
