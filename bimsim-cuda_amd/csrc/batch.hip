@@ -21,6 +21,7 @@
 #include "batch.h"
 #include "internal.h"
 #include "mie_device.h"
+#include "nf_us.h"
 #include "us_series.h"
 
 using namespace bimsim;
@@ -53,6 +54,9 @@ struct bimsim_batch {
     // Monte-Carlo sum, evaluated by the tiled kernel after the per-pixel one
     int* d_mc_list = nullptr;
     int n_mc_uf = 0, mc_tiles_max = 0;
+    // complex64 batches: instances with orders beyond the FP32 series
+    // kernels, whose scattered field the FP64 form computes
+    std::vector<int> wide;
     SeriesPlan plan_mc, plan_pw;    // FP32 series: Monte-Carlo / plane-wave instances
     bool have_mc = false, have_pw = false;
     hipStream_t capture_stream = nullptr;
@@ -113,6 +117,44 @@ void release(bimsim_batch* b) {
         if (p) (void)hipFree(p);
 }
 
+// FP64 batches: coefficients, incident fields, then each instance's
+// scattered field through the per-instance entry point.
+int enqueue_fp64_fields(bimsim_batch* b) {
+    bimsim_ctx* ctx = b->ctx;
+    const size_t fb = sizeof(double2);
+    BIMSIM_TRY(mie_batch_launch(ctx, b->d_sph, b->d_kmag, b->n_sph, b->lmax, b->quirks, b->d_work,
+                                b->d_A, b->d_B));
+    BIMSIM_TRY(uf_batch_launch(ctx, b->d_uf, b->n, b->max_npix, b->f32, b->d_uftab, b->waves,
+                               b->d_Uf));
+    for (int i = 0; i < b->n; ++i) {
+        const bimsim_instance& I = b->inst[i];
+        if (I.ns == 0) continue;
+        const bool plane = I.uf_mode == BIMSIM_UF_PLANE;
+        const double* w = plane ? b->d_kdirs + 3 * (size_t)i : b->waves + 3 * (size_t)I.wave_begin;
+        const int nk = plane ? 1 : I.nk;
+        BIMSIM_TRY(bimsim_nf_us(ctx, &I.pos, I.N, I.N, I.focus, b->kmag[i],
+                                b->inst_spheres[i].data(), I.ns, b->d_A, b->d_B, w, nk, b->subA[i],
+                                b->prec, b->quirks, (char*)b->d_U + fb * b->field_off[i], 0));
+    }
+    return BIMSIM_OK;
+}
+
+// Complex64 batches: the scattered fields of the instances beyond the FP32
+// series orders (the FP64 form, rounded).
+int enqueue_wide(bimsim_batch* b) {
+    bimsim_ctx* ctx = b->ctx;
+    for (int i : b->wide) {
+        const bimsim_instance& I = b->inst[i];
+        const bool plane = I.uf_mode == BIMSIM_UF_PLANE;
+        const double* w = plane ? b->d_kdirs + 3 * (size_t)i : b->waves + 3 * (size_t)I.wave_begin;
+        BIMSIM_TRY(us_wide_fp32(ctx, &I.pos, I.N, I.N, I.focus, b->kmag[i],
+                                b->inst_spheres[i].data(), I.ns, b->d_A, b->d_B, w,
+                                plane ? 1 : I.nk, b->subA[i],
+                                (float2*)b->d_U + b->field_off[i], 0));
+    }
+    return BIMSIM_OK;
+}
+
 // One pass of the batch on ctx->stream.
 int enqueue(bimsim_batch* b) {
     bimsim_ctx* ctx = b->ctx;
@@ -150,21 +192,9 @@ int enqueue(bimsim_batch* b) {
         if (b->have_pw)
             BIMSIM_TRY(series_plan_launch_sums(ctx, b->plan_pw, b->d_A, b->d_B, (float2*)b->d_U));
     } else {
-        BIMSIM_TRY(mie_batch_launch(ctx, b->d_sph, b->d_kmag, b->n_sph, b->lmax, b->quirks,
-                                    b->d_work, b->d_A, b->d_B));
-        BIMSIM_TRY(uf_batch_launch(ctx, b->d_uf, b->n, b->max_npix, b->f32, b->d_uftab, b->waves,
-                                   b->d_Uf));
-        for (int i = 0; i < b->n; ++i) {
-            const bimsim_instance& I = b->inst[i];
-            if (I.ns == 0) continue;
-            const bool plane = I.uf_mode == BIMSIM_UF_PLANE;
-            const double* w = plane ? b->d_kdirs + 3 * (size_t)i : b->waves + 3 * (size_t)I.wave_begin;
-            const int nk = plane ? 1 : I.nk;
-            BIMSIM_TRY(bimsim_nf_us(ctx, &I.pos, I.N, I.N, I.focus, b->kmag[i], b->inst_spheres[i].data(),
-                                    I.ns, b->d_A, b->d_B, w, nk, b->subA[i], b->prec, b->quirks,
-                                    (char*)b->d_U + fb * b->field_off[i], 0));
-        }
+        BIMSIM_TRY(enqueue_fp64_fields(b));
     }
+    if (b->f32) BIMSIM_TRY(enqueue_wide(b));
     if (!b->fuse_sum)
         BIMSIM_TRY(sum_batch_launch(ctx, b->d_sum, b->n, b->max_npix, b->d_sph4, b->f32, b->d_Uf,
                                     b->d_U, b->D, b->Di));
@@ -305,14 +335,20 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
             ss.inst = (int)sb.inst.size();
             sb.sph.push_back(ss);
         }
-        if (I.ns > 0) sb.inst.push_back(si);
+        // complex64 batches: orders beyond the FP32 series go to the FP64 form
+        int nl1 = 1;
+        for (const bimsim_sphere& S : b->inst_spheres[i]) nl1 = std::max(nl1, S.Nl + 1);
+        if (b->f32 && I.ns > 0 && series_capacity(nl1) < 0) {
+            b->wide.push_back(i);
+            sb.sph.resize(sb.sph.size() - I.ns);
+        } else if (I.ns > 0) {
+            sb.inst.push_back(si);
+        }
         field += npix;
     }
     b->n_sph = (int)sph_dev.size();
     b->ncoef = coef;
     b->field_elems = field;
-    if (b->f32 && b->lmax + 1 > 112)
-        return set_error(ctx, BIMSIM_E_UNSUPPORTED, "FP32 series supports Nl <= 111 (got %d)", b->lmax);
     const size_t fb = b->f32 ? sizeof(float2) : sizeof(double2);
     BIMSIM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     BIMSIM_TRY(dev_upload(ctx, &b->d_sph, sph_dev));

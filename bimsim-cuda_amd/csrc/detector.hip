@@ -326,7 +326,13 @@ __device__ __forceinline__ C fused_sum(const SumFuse& fz, int pix, C a, C b) {
     const d3 p = quad_at(fz.q, (double)iu / (double)fz.uR, (double)iv / (double)fz.vR);
     for (int s = 0; s < fz.ns; ++s) {
         const double4 S = fz.sph[s];
-        if (len(sub(p, d3{S.x, S.y, S.z})) < S.w) return a;
+        // |p - c| < a from the squared distance, the square root taken only
+        // within a relative 1e-12 of the boundary, where the rounded distance
+        // could decide otherwise than its square
+        const d3 r = sub(p, d3{S.x, S.y, S.z});
+        const double d2 = dot(r, r), a2 = S.w * S.w;
+        if (d2 < a2 * (1.0 - 1e-12)) return a;
+        if (d2 <= a2 * (1.0 + 1e-12) && sqrt(d2) < S.w) return a;
     }
     a.x = a.x + b.x;
     a.y = a.y + b.y;

@@ -62,6 +62,7 @@ enum ScratchSlot {
     kSlotDet = 7,
     kSlotMisc = 8,
     kSlotUfLut = 9,   // incident-field Bessel table
+    kSlotWide = 10,   // complex128 scattered field of FP32 calls beyond the FP32 orders
 };
 
 int set_error(bimsim_ctx* ctx, int code, const char* fmt, ...);

@@ -21,4 +21,11 @@ struct WaveF {
     float kx2, ky2, kz2, pr, pi, pad0, pad1, pad2;
 };
 
+// The scattered field of an FP32 call whose orders exceed the FP32 series
+// kernels (Nl + 1 > 112): the hoisted FP64 form into context scratch, then
+// U (+)= that field rounded to complex64.
+int us_wide_fp32(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int vR, const double focus[3],
+                 double kmag, const bimsim_sphere* spheres, int ns, const void* A, const void* B,
+                 const double* waves, int nk, double subA, float2* U, int accumulate);
+
 }  // namespace bimsim

@@ -223,6 +223,16 @@ __global__ void clear_c32(float2* U, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) U[i] = make_float2(0.f, 0.f);
 }
+// U (+)= the complex128 field W rounded to complex64
+__global__ void narrow_add_kernel(const double2* __restrict__ W, float2* __restrict__ U, int n,
+                                  int accumulate) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double2 w = W[i];
+    float2 v = make_float2(0.f, 0.f);
+    if (accumulate) v = U[i];
+    U[i] = make_float2(v.x + (float)w.x, v.y + (float)w.y);
+}
 
 SphereDev to_dev(const bimsim_sphere& S) {
     SphereDev d;
@@ -236,6 +246,24 @@ SphereDev to_dev(const bimsim_sphere& S) {
 }
 
 }  // namespace
+}  // namespace bimsim
+
+namespace bimsim {
+
+int us_wide_fp32(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int vR, const double focus[3],
+                 double kmag, const bimsim_sphere* spheres, int ns, const void* A, const void* B,
+                 const double* waves, int nk, double subA, float2* U, int accumulate) {
+    const int npix = uR * vR;
+    void* wide = nullptr;
+    BIMSIM_TRY(scratch_get(ctx, kSlotWide, sizeof(double2) * (size_t)npix, &wide));
+    BIMSIM_TRY(bimsim_nf_us(ctx, pos, uR, vR, focus, kmag, spheres, ns, A, B, waves, nk, subA,
+                            BIMSIM_FP64, BIMSIM_QUIRKS_REF, wide, 0));
+    hipLaunchKernelGGL(narrow_add_kernel, dim3((npix + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                       ctx->stream, (const double2*)wide, U, npix, accumulate ? 1 : 0);
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return BIMSIM_OK;
+}
+
 }  // namespace bimsim
 
 using namespace bimsim;
@@ -309,6 +337,11 @@ extern "C" int bimsim_nf_us(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int
                                (s > 0 || accumulate) ? 1 : 0);
             BIMSIM_HIP_CHECK(ctx, hipGetLastError());
         }
+    } else if (series_capacity(lmax + 1) < 0) {
+        // orders beyond the FP32 series kernels: the FP64 form, rounded
+        timer_end(ctx, tmr);
+        return us_wide_fp32(ctx, pos, uR, vR, focus, kmag, spheres, ns, A, B, waves, nk, subA,
+                            (float2*)U, accumulate);
     } else {
         // FP32 series: a batch of one instance
         SeriesBatch batch;
