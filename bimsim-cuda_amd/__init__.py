@@ -35,7 +35,8 @@ EXPORTS = (
     "bimsim_sphere_lut_dims", "bimsim_sphere_lut", "bimsim_nf_us_lut", "bimsim_nf_interpolate",
     "bimsim_sphere_radial_lut", "bimsim_field_fft", "bimsim_field_bandpass",
     "bimsim_field_to_scalar", "bimsim_field_scale", "bimsim_field_crop",
-    "bimsim_detector_resample", "bimsim_slice_ratio", "bimsim_batch_create",
+    "bimsim_detector_resample", "bimsim_field3_to_scalar", "bimsim_detector_resample3",
+    "bimsim_slice_ratio", "bimsim_batch_create",
     "bimsim_batch_launch", "bimsim_batch_destroy", "bimsim_batch_run", "bimsim_set_autotune",
     "bimsim_set_series_algorithm", "bimsim_detector_rotate_add", "bimsim_detector_ring_sums",
     "bimsim_detector_from_line", "bimsim_field_backpropagate", "bimsim_nf_uf_mc",

@@ -34,26 +34,34 @@ __device__ inline R cabs2(typename Cplx<R>::T z) {
     return z.x * z.x + z.y * z.y;
 }
 
+// field_intensity (fieldslice.cu:6-27): |x|^2, or |x|^2 + |y|^2 + |z|^2 for a
+// vector field (Y and Z both given); |.| = sqrt(r*r + i*i) as rts::complex::abs
 template <class R>
-__global__ void field_to_scalar_kernel(const typename Cplx<R>::T* __restrict__ F, size_t n, int op,
+__device__ inline R intensity3(const typename Cplx<R>::T* X, const typename Cplx<R>::T* Y,
+                               const typename Cplx<R>::T* Z, size_t i) {
+    const R xm = sqrt(cabs2<R>(X[i]));
+    if (Y && Z) {
+        const R ym = sqrt(cabs2<R>(Y[i]));
+        const R zm = sqrt(cabs2<R>(Z[i]));
+        return xm * xm + ym * ym + zm * zm;
+    }
+    return xm * xm;
+}
+
+template <class R>
+__global__ void field_to_scalar_kernel(const typename Cplx<R>::T* __restrict__ X,
+                                       const typename Cplx<R>::T* __restrict__ Y,
+                                       const typename Cplx<R>::T* __restrict__ Z, size_t n, int op,
                                        R* __restrict__ S) {
     const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
-    const typename Cplx<R>::T z = F[i];
     R v;
     switch (op) {
-        case BIMSIM_SLICE_INTENSITY: {
-            const R m = sqrt(cabs2<R>(z));
-            v = m * m;
-            break;
-        }
-        case BIMSIM_SLICE_MAGNITUDE: {
-            const R m = sqrt(cabs2<R>(z));
-            v = sqrt(m * m);
-            break;
-        }
-        case BIMSIM_SLICE_REAL: v = z.x; break;
-        default: v = z.y; break;
+        case BIMSIM_SLICE_INTENSITY: v = intensity3<R>(X, Y, Z, i); break;
+        // Mag: field_intensity then field_sqrt (fieldslice.cu:156-171)
+        case BIMSIM_SLICE_MAGNITUDE: v = sqrt(intensity3<R>(X, Y, Z, i)); break;
+        case BIMSIM_SLICE_REAL: v = X[i].x; break;
+        default: v = X[i].y; break;
     }
     S[i] = v;
 }
@@ -80,8 +88,12 @@ __global__ void crop_kernel(const C* __restrict__ src, int uR, int u, int v, int
 // resample_intensity (accumulate = 1: D += scale*I/(ss*ss)) and
 // resample_intensity_without_integrating (accumulate = 0: D = I/(ss*ss)).
 // The field row stride is Du*ss (the detector-resolution field).
+// resample_intensity / resample_intensity_without_integrating
+// (fieldslice.cu:30-108): each of Y, Z adds its |.|^2 when given
 template <class R>
-__global__ void resample_kernel(const typename Cplx<R>::T* __restrict__ F, int Du, int Dv, int ss,
+__global__ void resample_kernel(const typename Cplx<R>::T* __restrict__ X,
+                                const typename Cplx<R>::T* __restrict__ Y,
+                                const typename Cplx<R>::T* __restrict__ Z, int Du, int Dv, int ss,
                                 R scale, int accumulate, R* __restrict__ D) {
     const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
     if (i >= (size_t)Du * Dv) return;
@@ -90,8 +102,10 @@ __global__ void resample_kernel(const typename Cplx<R>::T* __restrict__ F, int D
     for (int ix = 0; ix < ss; ++ix)
         for (int iy = 0; iy < ss; ++iy) {
             const size_t fi = (size_t)(iv * ss + iy) * ss * Du + (size_t)iu * ss + ix;
-            const R m = sqrt(cabs2<R>(F[fi]));
-            I += m * m;
+            const R xm = sqrt(cabs2<R>(X[fi]));
+            const R ym = Y ? sqrt(cabs2<R>(Y[fi])) : R(0);
+            const R zm = Z ? sqrt(cabs2<R>(Z[fi])) : R(0);
+            I += Y || Z ? xm * xm + ym * ym + zm * zm : xm * xm;
         }
     const R den = (R)(ss * ss);
     if (accumulate) D[i] += scale * I / den;
@@ -190,19 +204,27 @@ using namespace bimsim;
 
 extern "C" {
 
-int bimsim_field_to_scalar(bimsim_ctx* ctx, int prec, int op, const void* F, long long n, void* S) {
+int bimsim_field3_to_scalar(bimsim_ctx* ctx, int prec, int op, const void* X, const void* Y,
+                            const void* Z, long long n, void* S) {
     if (!ctx) return BIMSIM_E_INVALID;
-    BIMSIM_CHECK_ARG(ctx, F && S && n > 0 && prec_ok(prec), "bad field-to-scalar arguments");
+    BIMSIM_CHECK_ARG(ctx, X && S && n > 0 && prec_ok(prec), "bad field-to-scalar arguments");
+    BIMSIM_CHECK_ARG(ctx, (Y == nullptr) == (Z == nullptr), "Y and Z must both be given or both NULL");
     BIMSIM_CHECK_ARG(ctx, op >= BIMSIM_SLICE_INTENSITY && op <= BIMSIM_SLICE_IMAG, "unknown op");
     BIMSIM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     if (prec == BIMSIM_FP32_FAST)
         hipLaunchKernelGGL(field_to_scalar_kernel<float>, grid1(n), dim3(kThreads), 0, ctx->stream,
-                           (const float2*)F, (size_t)n, op, (float*)S);
+                           (const float2*)X, (const float2*)Y, (const float2*)Z, (size_t)n, op,
+                           (float*)S);
     else
         hipLaunchKernelGGL(field_to_scalar_kernel<double>, grid1(n), dim3(kThreads), 0,
-                           ctx->stream, (const double2*)F, (size_t)n, op, (double*)S);
+                           ctx->stream, (const double2*)X, (const double2*)Y, (const double2*)Z,
+                           (size_t)n, op, (double*)S);
     BIMSIM_HIP_CHECK(ctx, hipGetLastError());
     return BIMSIM_OK;
+}
+
+int bimsim_field_to_scalar(bimsim_ctx* ctx, int prec, int op, const void* F, long long n, void* S) {
+    return bimsim_field3_to_scalar(ctx, prec, op, F, nullptr, nullptr, n, S);
 }
 
 int bimsim_field_scale(bimsim_ctx* ctx, int prec, void* F, long long n, double v) {
@@ -237,21 +259,29 @@ int bimsim_field_crop(bimsim_ctx* ctx, int prec, const void* src, int uR, int vR
     return BIMSIM_OK;
 }
 
-int bimsim_detector_resample(bimsim_ctx* ctx, int prec, const void* F, int Du, int Dv, int ss,
-                             double scale, int accumulate, void* D) {
+int bimsim_detector_resample3(bimsim_ctx* ctx, int prec, const void* X, const void* Y,
+                              const void* Z, int Du, int Dv, int ss, double scale, int accumulate,
+                              void* D) {
     if (!ctx) return BIMSIM_E_INVALID;
-    BIMSIM_CHECK_ARG(ctx, F && D && Du > 0 && Dv > 0 && ss > 0 && prec_ok(prec),
+    BIMSIM_CHECK_ARG(ctx, X && D && Du > 0 && Dv > 0 && ss > 0 && prec_ok(prec),
                      "bad detector-resample arguments");
     BIMSIM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     const size_t n = (size_t)Du * Dv;
     if (prec == BIMSIM_FP32_FAST)
         hipLaunchKernelGGL(resample_kernel<float>, grid1(n), dim3(kThreads), 0, ctx->stream,
-                           (const float2*)F, Du, Dv, ss, (float)scale, accumulate, (float*)D);
+                           (const float2*)X, (const float2*)Y, (const float2*)Z, Du, Dv, ss,
+                           (float)scale, accumulate, (float*)D);
     else
         hipLaunchKernelGGL(resample_kernel<double>, grid1(n), dim3(kThreads), 0, ctx->stream,
-                           (const double2*)F, Du, Dv, ss, scale, accumulate, (double*)D);
+                           (const double2*)X, (const double2*)Y, (const double2*)Z, Du, Dv, ss,
+                           scale, accumulate, (double*)D);
     BIMSIM_HIP_CHECK(ctx, hipGetLastError());
     return BIMSIM_OK;
+}
+
+int bimsim_detector_resample(bimsim_ctx* ctx, int prec, const void* F, int Du, int Dv, int ss,
+                             double scale, int accumulate, void* D) {
+    return bimsim_detector_resample3(ctx, prec, F, nullptr, nullptr, Du, Dv, ss, scale, accumulate, D);
 }
 
 int bimsim_slice_ratio(bimsim_ctx* ctx, int prec, int op, const void* D, const void* Di,

@@ -233,8 +233,10 @@ int series_plan_launch_sums(bimsim_ctx* ctx, const SeriesPlan& plan, const doubl
 // Order capacity for Nl + 1 orders, or -1.
 int series_capacity(int nl1);
 
-// Plan for `batch`, reused while the batch tables are unchanged (the
-// context keeps its last plan; the key is the tables' bytes).
+// Plan for `batch`, reused while the batch tables are unchanged (the key is
+// the tables' bytes).  The context keeps the last few plans, and every plan
+// used during a stream capture until it ends; a plan missing during a
+// capture is an error (BIMSIM_E_UNSUPPORTED).
 int series_cached_plan(bimsim_ctx* ctx, SeriesBatch& batch, const double* waves, const double2* A,
                        const double2* B, const SeriesPlan** out);
 void series_cache_release(bimsim_ctx* ctx);

@@ -715,10 +715,21 @@ int alloc(bimsim_ctx* ctx, T** dst, long long n) {
     return BIMSIM_OK;
 }
 
-struct SeriesCache {
+// Plans of per-call scattered fields, keyed by their tables' bytes.  A plan
+// used while the stream is being captured is pinned: the captured graph keeps
+// its device pointers, so it stays until the context ends.  Other plans are
+// evicted least recently used beyond kPlanCacheSize.
+struct SeriesCacheEntry {
     std::vector<char> key;
     SeriesPlan plan;
+    bool pinned = false;
+    unsigned long long last_use = 0;
 };
+struct SeriesCache {
+    std::vector<SeriesCacheEntry*> entries;
+    unsigned long long clock = 0;
+};
+constexpr int kPlanCacheSize = 4;
 
 }  // namespace
 
@@ -1378,28 +1389,60 @@ int series_cached_plan(bimsim_ctx* ctx, SeriesBatch& batch, const double* waves,
         std::memcpy(key.data() + sizeof(SeriesInst) * batch.inst.size(), batch.sph.data(),
                     sizeof(SeriesSph) * batch.sph.size());
     SeriesCache* c = (SeriesCache*)ctx->series_cache;
-    if (c && c->key == key) {
-        *out = &c->plan;
-        return BIMSIM_OK;
-    }
-    if (c) {
-        BIMSIM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-        series_plan_destroy(&c->plan);
-    } else {
+    if (!c) {
         c = new SeriesCache();
         ctx->series_cache = c;
     }
-    c->key.clear();
-    BIMSIM_TRY(series_plan_create_tuned(ctx, batch, (size_t)8 << 30, waves, A, B, &c->plan));
-    c->key = key;
-    *out = &c->plan;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    BIMSIM_HIP_CHECK(ctx, hipStreamIsCapturing(ctx->stream, &cap));
+    const bool capturing = cap != hipStreamCaptureStatusNone;
+    for (SeriesCacheEntry* e : c->entries)
+        if (e->key == key) {
+            e->last_use = ++c->clock;
+            e->pinned = e->pinned || capturing;
+            *out = &e->plan;
+            return BIMSIM_OK;
+        }
+    // a new plan allocates and times launches: not possible inside a capture
+    if (capturing)
+        return set_error(ctx, BIMSIM_E_UNSUPPORTED,
+                         "scattered-field plan for this call is not cached: run the call once "
+                         "outside the capture first");
+    int unpinned = 0;
+    for (SeriesCacheEntry* e : c->entries) unpinned += e->pinned ? 0 : 1;
+    if (unpinned >= kPlanCacheSize) {
+        size_t victim = c->entries.size();
+        for (size_t i = 0; i < c->entries.size(); ++i)
+            if (!c->entries[i]->pinned &&
+                (victim == c->entries.size() || c->entries[i]->last_use < c->entries[victim]->last_use))
+                victim = i;
+        BIMSIM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        series_plan_destroy(&c->entries[victim]->plan);
+        delete c->entries[victim];
+        c->entries.erase(c->entries.begin() + victim);
+    }
+    SeriesCacheEntry* e = new SeriesCacheEntry();
+    const int rc = series_plan_create_tuned(ctx, batch, (size_t)8 << 30, waves, A, B, &e->plan);
+    if (rc != BIMSIM_OK) {
+        (void)hipStreamSynchronize(ctx->stream);
+        series_plan_destroy(&e->plan);
+        delete e;
+        return rc;
+    }
+    e->key = std::move(key);
+    e->last_use = ++c->clock;
+    c->entries.push_back(e);
+    *out = &e->plan;
     return BIMSIM_OK;
 }
 
 void series_cache_release(bimsim_ctx* ctx) {
     SeriesCache* c = (SeriesCache*)ctx->series_cache;
     if (!c) return;
-    series_plan_destroy(&c->plan);
+    for (SeriesCacheEntry* e : c->entries) {
+        series_plan_destroy(&e->plan);
+        delete e;
+    }
     delete c;
     ctx->series_cache = nullptr;
 }

@@ -1,7 +1,8 @@
 // fieldslice (reference fieldslice.h:13-68, fieldslice.cpp, fieldslice.cu):
 // an owning device field of R[0] x R[1] complex samples, row-major with u
 // fastest (index iv*R[0] + iu), complex64 for the FP32 form and complex128
-// for the FP64 forms.  Scalar fields use x_hat only.
+// for the FP64 forms.  Scalar fields use x_hat only; vector fields
+// (scalarField false) hold x_hat, y_hat and z_hat.
 #pragma once
 
 #include <string>
@@ -34,6 +35,8 @@ struct fieldslice {
     void ResampleAndSave(scalarslice* detector, unsigned int supersample);
 
     void ScaleField(ptype v);
+    // zero y_hat and z_hat (vector fields)
+    void clear_yz();
 
     void toAngularSpectrum();
     void fromAngularSpectrum();

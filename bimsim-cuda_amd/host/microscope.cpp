@@ -128,7 +128,9 @@ void microscopeStruct::SimulateInterpolation() { nf.Interpolate(); }
 void microscopeStruct::SimulateImaging() {
     const int uR = (int)nf.U.R[0], vR = (int)nf.U.R[1];
     const bool square = uR == vR && Ud.R[0] == Ud.R[1];
-    if (fused_detector && square) {
+    // the fused detector sums |x|^2 only: vector fields take the separate
+    // steps (band-pass on x as the reference, crop and |x|^2+|y|^2+|z|^2)
+    if (fused_detector && square && nf.U.scalarField) {
         // band-pass, crop and |.|^2 integration in one pass; the filtered
         // fields stay in nf.U / nf.Uf as after applyBandpass
         const double du = 1.0 / nf.pos.X.len(), dv = 1.0 / nf.pos.Y.len();

@@ -253,11 +253,19 @@ void nearfieldStruct::calcUf() {
     else scalarUf();
 }
 
+// Vector simulation (nearfieldStruct::vectorUf, nfVectorUf.cu:162-191): the
+// scalar focused field (or plane wave) in the x component, y and z zero.
 void nearfieldStruct::vectorUf() {
-    throw std::runtime_error("bimsim: vector field simulation is not supported");
+    scalarUf();
+    Uf.clear_yz();
 }
 
-void nearfieldStruct::vectorUfLut() { vectorUf(); }
+// The reference's vectorUfLut has an empty body (nfVectorUfLut.cu:7-10), which
+// leaves Uf zero; this one evaluates the tabulated form into x.
+void nearfieldStruct::vectorUfLut() {
+    scalarUfLut();
+    Uf.clear_yz();
+}
 
 static float stage_ms(bimsim_ctx* ctx, int stage) {
     float ms[4] = {0, 0, 0, 0};
@@ -317,7 +325,13 @@ void nearfieldStruct::calcBesselLut(ptype* j, ptype dmin, ptype dmax, int dR) {
 }
 
 void nearfieldStruct::calcUs() {
-    if (!scalarSim) throw std::runtime_error("bimsim: vector field simulation is not supported");
+    if (!scalarSim) {
+        // the reference has no vector scattered field: it prints a note and
+        // leaves U as it is (nearfield.cpp:160-165); here U is zero, so the
+        // vector image is that of the incident field outside the spheres
+        U.clear_gpu();
+        return;
+    }
     if (lut_us) scalarUpLut();
     else scalarUs();
 }
@@ -427,9 +441,14 @@ void nearfieldStruct::sumUf() {
     const bimsim_quad q = pos.abi();
     std::vector<bimsim_sphere> S;
     for (const sphere& s : sVector) S.push_back(s.abi(0));
-    check(bimsim_nf_sum_uf(ctx, &q, (int)U.R[0], (int)U.R[1], S.empty() ? nullptr : S.data(),
-                           (int)S.size(), U.prec, Uf.x_hat, U.x_hat),
-          "sumUf");
+    // each component of a vector field (gpuScalarUsp, nfSumUf.cu:6-48)
+    void* const uf[3] = {Uf.x_hat, Uf.y_hat, Uf.z_hat};
+    void* const u[3] = {U.x_hat, U.y_hat, U.z_hat};
+    for (int c = 0; c < 3; ++c)
+        if (uf[c] && u[c])
+            check(bimsim_nf_sum_uf(ctx, &q, (int)U.R[0], (int)U.R[1], S.empty() ? nullptr : S.data(),
+                                   (int)S.size(), U.prec, uf[c], u[c]),
+                  "sumUf");
 }
 
 void nearfieldStruct::saveFieldsToTexture() {

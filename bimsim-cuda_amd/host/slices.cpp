@@ -458,12 +458,14 @@ void fieldslice::download(double* out) const {
     }
 }
 
+// Vector fields (scalarField false) carry x_hat, y_hat and z_hat; the
+// intensity and magnitude views and the detector sums use all three
+// (field_intensity / resample_intensity, fieldslice.cu:6-108).
 static scalarslice field_view(fieldslice& f, int op) {
     scalarslice s;
     s.init((int)f.R[0], (int)f.R[1], f.prec);
-    if (!f.scalarField && (f.y_hat || f.z_hat))
-        throw std::runtime_error("bimsim: vector fields are not supported");
-    check(bimsim_field_to_scalar(context(), f.prec, op, f.x_hat, (long long)f.samples(), s.S),
+    check(bimsim_field3_to_scalar(context(), f.prec, op, f.x_hat, f.y_hat, f.z_hat,
+                                  (long long)f.samples(), s.S),
           "field view");
     return s;
 }
@@ -474,19 +476,27 @@ scalarslice fieldslice::Imag() { return field_view(*this, BIMSIM_SLICE_IMAG); }
 scalarslice fieldslice::Intensity() { return field_view(*this, BIMSIM_SLICE_INTENSITY); }
 
 void fieldslice::IntegrateAndResample(scalarslice* detector, unsigned int supersample, float scale) {
-    check(bimsim_detector_resample(context(), prec, x_hat, detector->R[0], detector->R[1],
-                                   (int)supersample, scale, 1, detector->S),
+    check(bimsim_detector_resample3(context(), prec, x_hat, y_hat, z_hat, detector->R[0],
+                                    detector->R[1], (int)supersample, scale, 1, detector->S),
           "IntegrateAndResample");
 }
 
 void fieldslice::ResampleAndSave(scalarslice* detector, unsigned int supersample) {
-    check(bimsim_detector_resample(context(), prec, x_hat, detector->R[0], detector->R[1],
-                                   (int)supersample, 1.0, 0, detector->S),
+    check(bimsim_detector_resample3(context(), prec, x_hat, y_hat, z_hat, detector->R[0],
+                                    detector->R[1], (int)supersample, 1.0, 0, detector->S),
           "ResampleAndSave");
 }
 
+// field_scale scales every component (fieldslice.cu:141-153)
 void fieldslice::ScaleField(ptype v) {
-    check(bimsim_field_scale(context(), prec, x_hat, (long long)samples(), v), "ScaleField");
+    for (void* c : {x_hat, y_hat, z_hat})
+        if (c) check(bimsim_field_scale(context(), prec, c, (long long)samples(), v), "ScaleField");
+}
+
+void fieldslice::clear_yz() {
+    const size_t bytes = field_bytes(prec) * samples();
+    if (y_hat) dev_zero(y_hat, bytes);
+    if (z_hat) dev_zero(z_hat, bytes);
 }
 
 void fieldslice::toAngularSpectrum() {
@@ -504,8 +514,15 @@ fieldslice fieldslice::crop(int u, int v, int su, int sv) {
     r.R[0] = su;
     r.R[1] = sv;
     r.init_gpu();
-    check(bimsim_field_crop(context(), prec, x_hat, (int)R[0], (int)R[1], u, v, su, sv, r.x_hat),
-          "crop");
+    // every component (the reference's crop result holds x_hat only, then
+    // copies y_hat and z_hat of a vector field into it: fieldslice.cu:345-359)
+    const void* src[3] = {x_hat, y_hat, z_hat};
+    void* dst[3] = {r.x_hat, r.y_hat, r.z_hat};
+    for (int c = 0; c < 3; ++c)
+        if (src[c] && dst[c])
+            check(bimsim_field_crop(context(), prec, src[c], (int)R[0], (int)R[1], u, v, su, sv,
+                                    dst[c]),
+                  "crop");
     return r;
 }
 

@@ -215,7 +215,11 @@ int bimsim_nf_uf_mc(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int vR,
  * waves: device, nk*3 doubles.  subA: amplitude per the reference
  * (nfScalarUs.cu:572 for Monte-Carlo, A for a plane wave).
  * A, B: device complex128 packed coefficients (bimsim_mie_coeffs).
- * accumulate = 0 clears U first (U.clear_gpu(), nfScalarUs.cu:507). */
+ * accumulate = 0 clears U first (U.clear_gpu(), nfScalarUs.cu:507).
+ * BIMSIM_FP32_FAST serves orders up to Nl = 111 with the FP32 series; a
+ * call with a larger Nl computes the field in the FP64 form and rounds it
+ * to complex64 (same result, FP64 speed; batches do the same per
+ * instance). */
 int bimsim_nf_us(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int vR,
                  const double focus[3], double kmag,
                  const bimsim_sphere* spheres, int ns, const void* A,
@@ -305,6 +309,15 @@ int bimsim_field_backpropagate(bimsim_ctx* ctx, int prec, int quirks, void* F, i
 int bimsim_field_to_scalar(bimsim_ctx* ctx, int prec, int op, const void* F, long long n,
                            void* S);
 
+/* The same for a vector field of three components (a fieldslice with
+ * scalarField = false: x_hat, y_hat, z_hat): INTENSITY and MAGNITUDE use
+ * |X|^2 + |Y|^2 + |Z|^2 (field_intensity with y and z, reference
+ * fieldslice.cu:6-27,156-171), REAL and IMAG use X alone
+ * (fieldslice.cu:110-138,173-203).  Y and Z are both given or both NULL
+ * (then as bimsim_field_to_scalar). */
+int bimsim_field3_to_scalar(bimsim_ctx* ctx, int prec, int op, const void* X, const void* Y,
+                            const void* Z, long long n, void* S);
+
 /* F *= v (fieldslice::ScaleField, reference fieldslice.cu:141-153). */
 int bimsim_field_scale(bimsim_ctx* ctx, int prec, void* F, long long n, double v);
 
@@ -319,6 +332,13 @@ int bimsim_field_crop(bimsim_ctx* ctx, int prec, const void* src, int uR, int vR
  * fieldslice.cu:30-108. */
 int bimsim_detector_resample(bimsim_ctx* ctx, int prec, const void* F, int Du, int Dv, int ss,
                              double scale, int accumulate, void* D);
+
+/* The same over a vector field: each pixel sums |X|^2 + |Y|^2 + |Z|^2 per
+ * sample (resample_intensity with y_hat and z_hat, reference
+ * fieldslice.cu:30-108); Y and Z may be NULL. */
+int bimsim_detector_resample3(bimsim_ctx* ctx, int prec, const void* X, const void* Y,
+                              const void* Z, int Du, int Dv, int ss, double scale, int accumulate,
+                              void* D);
 
 /* out = -log10(D/Di) or D/Di (calc_absorbance / calc_transmittance,
  * reference microscope.cu:722-791); op = bimsim_ratio_op. */

@@ -274,3 +274,35 @@ def test_cli_interpolate_gaussian_rings(exe, orc, tmp_path):
                        timeout=120)
     assert p.returncode != 0 and "does not terminate" in (p.stdout + p.stderr)
     assert not (tmp_path / "I2").exists()
+
+
+@pytest.mark.parametrize("prec", ["parity", "fp32"])
+def test_cli_vector_field(exe, orc, tmp_path, prec):
+    """--vector (nearfieldStruct::vectorUf, nfVectorUf.cu:162-191): the scalar
+    focused field in x, y and z zero, and no scattered field (the reference
+    computes none, nearfield.cpp:160-165), so the images are those of Uf with
+    the spheres' interiors cut out (sumUf, nfSumUf.cu:6-48); the intensity
+    and near-field views sum |x|^2 + |y|^2 + |z|^2."""
+    import oracle
+    out = run_cli(tmp_path, "--vector", "--precision", prec, "--recursive",
+                  "--near-field", "n.bmp")
+    c = CASE
+    lam = f32(c["lam"])
+    N = c["res"] * (2 * c["pad"] + 1)
+    L = c["size"] * (2 * c["pad"] + 1)
+    q = oracle.Quad.xy(L, 0.0)
+    x, y, z, a = (f32(v) for v in c["sphere"])
+    s = dict(p=(x, y, z), a=a, n=complex(f32(c["n"][0]), f32(c["n"][1])),
+             Nl=orc.sphere_order(a, lam), coeff_off=0)
+    ca, cb = 1.0, math.cos(math.asin(f32(c["cond"])))
+    Uf = orc.nf_uf(q, N, N, (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 2.0 * PI_F / lam, 1.0, ca, cb,
+                   c["m"])
+    U = orc.nf_sum_uf(q, N, N, [s], Uf, np.zeros_like(Uf))
+    D, Di, Ab = oracle_detector(orc, U, Uf)
+    assert np.min(D) < 0.5 * np.max(D)            # the cut-out is visible
+    bright = Di > 1e-3 * Di.max()
+    if prec == "parity":
+        check_images(out, D, Di)
+    else:
+        assert np.max(np.abs(out["A"][0][0][bright] - Ab[bright])) <= 1e-4
+    assert (tmp_path / "n.bmp").read_bytes()[:2] == b"BM"
