@@ -51,67 +51,101 @@ __device__ __forceinline__ RadialTrig harm_trig(const SphereDev& S, double kmag,
 // reference divides, with FMAs.  The threshold rules act only where l > kd
 // (the y lower bound only where y is that large, which needs l > kd as
 // well): an order skips its selects when no lane of the wave is past kd.
-template <int L>
-__device__ __forceinline__ void harm_radial(const SphereDev& S, double kmag, double d,
-                                            double inv_d, double inv_kmag, const RadialTrig& T,
-                                            const double2* __restrict__ A,
-                                            const double2* __restrict__ B, const float* ar,
-                                            const float* ai, float& acc_r, float& acc_i) {
-    const int nl1 = S.Nl + 1;
-    auto mac = [&](int l, double cr, double ci) {
-        const float fr = (float)cr, fi = (float)ci;
-        acc_r = fmaf(fr, ar[l], fmaf(-fi, ai[l], acc_r));
-        acc_i = fmaf(fr, ai[l], fmaf(fi, ar[l], acc_i));
-    };
+// The recurrences' state carries from one range of orders to the next.
+// One set of registers serves both cases: outside, cj0 = (j_{l-2}, y_{l-2}),
+// cj1 = (j_{l-1}, y_{l-1}) and cinv.r = 1/kd; inside, cj0 and cj1 are the
+// complex j of the last two orders and cinv = 1/knd.
+struct RadialState {
+    int mode;          // 0 outside the sphere, 1 inside, 2 at the centre (l = 0 only)
+    dc cj0, cj1, cinv;
+    double kx;         // kd (outside) or Re knd (inside)
+};
+
+__device__ __forceinline__ RadialState harm_radial_init(const SphereDev& S, double kmag, double d,
+                                                        double inv_d, double inv_kmag,
+                                                        const RadialTrig& T) {
+    RadialState st;
     if (d <= S.a) {
-        const double2* As = A + S.coeff_off;
         const dc knd = cscl(dc{S.n[0], S.n[1]}, kmag * d);
-        if (knd.r < (double)kEpsFloat) {
-            // calc_Ui returns A_0 for every wave: only the l = 0 term
-            const double2 a0 = As[0];
-            mac(0, a0.x, a0.y);
-            return;
-        }
+        st.kx = knd.r;
+        // calc_Ui returns A_0 for every wave at the centre: only l = 0
+        st.mode = knd.r < (double)kEpsFloat ? 2 : 1;
         // init_jc / shift_jc (csin, ccos as csin_r / ccos_r)
-        const dc inv = cdiv_naive(dc{1.0, 0.0}, knd);
+        st.cinv = cdiv_naive(dc{1.0, 0.0}, knd);
         const dc sn = {T.t0 * T.t3, T.t1 * T.t2}, cs = {T.t1 * T.t3, -(T.t0 * T.t2)};
-        dc j0 = cmul(sn, inv);
-        dc j1 = csub(cmul(j0, inv), cmul(cs, inv));
+        st.cj0 = cmul(sn, st.cinv);
+        st.cj1 = csub(cmul(st.cj0, st.cinv), cmul(cs, st.cinv));
+        return st;
+    }
+    st.mode = 0;
+    st.kx = kmag * d;
+    const double inv = inv_d * inv_kmag;
+    const double sn = T.t0, cs = T.t1;
+    const double j0 = sn * inv, y0 = -(cs * inv);
+    st.cinv = dc{inv, 0.0};
+    st.cj0 = dc{j0, y0};
+    st.cj1 = dc{fma(j0, inv, -(cs * inv)), fma(y0, inv, -(sn * inv))};
+    return st;
+}
+
+// Orders L0 .. L0 + C - 1 (those <= Nl) of the radial sum; ar / ai hold
+// the order sums of these orders at [l - L0].
+template <int L0, int C>
+__device__ __forceinline__ void harm_radial_range(RadialState& st, const SphereDev& S,
+                                                  const double2* __restrict__ A,
+                                                  const double2* __restrict__ B, const float* ar,
+                                                  const float* ai, float& acc_r, float& acc_i) {
+    const int nl1 = S.Nl + 1;
+    auto mac = [&](int i, double cr, double ci) {
+        const float fr = (float)cr, fi = (float)ci;
+        acc_r = fmaf(fr, ar[i], fmaf(-fi, ai[i], acc_r));
+        acc_i = fmaf(fr, ai[i], fmaf(fi, ar[i], acc_i));
+    };
+    if (st.mode == 2) {
+        if (L0 == 0) {
+            const double2 a0 = A[S.coeff_off];
+            mac(0, a0.x, a0.y);
+        }
+        return;
+    }
+    if (st.mode == 1) {
+        const double2* As = A + S.coeff_off;
+        const dc inv = st.cinv;
 #pragma clang loop unroll(full)
-        for (int l = 0; l < L; ++l) {
+        for (int i = 0; i < C; ++i) {
+            const int l = L0 + i;
             if (l >= nl1) continue;
-            dc jl = l == 0 ? j0 : j1;
+            dc jl = l == 0 ? st.cj0 : st.cj1;
             if (l >= 2) {
                 // (2l - 1)/knd times j_{l-1}, minus j_{l-2}
                 const double tr = (double)(2 * l - 1) * inv.r, ti = (double)(2 * l - 1) * inv.i;
-                dc jn = {fma(tr, j1.r, fma(-ti, j1.i, -j0.r)), fma(tr, j1.i, fma(ti, j1.r, -j0.i))};
+                dc jn = {fma(tr, st.cj1.r, fma(-ti, st.cj1.i, -st.cj0.r)),
+                         fma(tr, st.cj1.i, fma(ti, st.cj1.r, -st.cj0.i))};
                 // the threshold rule can only act where l > Re knd: skip its
                 // selects when no lane of the wave is there
-                const bool past = (double)l > knd.r;
+                const bool past = (double)l > st.kx;
                 if (__any(past))
                     if (past && (jn.r < kJMin || jn.r > kJMax)) jn = dc{0.0, 0.0};
-                j0 = j1;
-                j1 = jn;
+                st.cj0 = st.cj1;
+                st.cj1 = jn;
                 jl = jn;
             }
             const double2 a = As[l];
-            mac(l, fma(a.x, jl.r, -(a.y * jl.i)), fma(a.x, jl.i, a.y * jl.r));
+            mac(i, fma(a.x, jl.r, -(a.y * jl.i)), fma(a.x, jl.i, a.y * jl.r));
         }
         return;
     }
     const double2* Bs = B + S.coeff_off;
-    const double kd = kmag * d;
-    const double inv = inv_d * inv_kmag;
-    const double sn = T.t0, cs = T.t1;
-    double j0 = sn * inv, j1 = fma(j0, inv, -(cs * inv));
-    double y0 = -(cs * inv), y1 = fma(y0, inv, -(sn * inv));
+    const double inv = st.cinv.r;
+    double j0 = st.cj0.r, y0 = st.cj0.i, j1 = st.cj1.r, y1 = st.cj1.i;
     // B_l read four orders at a time (indices clamped to Nl), so that one
     // wait on the scalar loads covers four orders
     double2 bq[4];
 #pragma clang loop unroll(full)
-    for (int l = 0; l < L; ++l) {
+    for (int i = 0; i < C; ++i) {
+        const int l = L0 + i;
         if (l >= nl1) continue;
-        if ((l & 3) == 0)
+        if ((i & 3) == 0)
 #pragma clang loop unroll(full)
             for (int q = 0; q < 4; ++q) bq[q] = Bs[min(l + q, nl1 - 1)];
         double jl = l == 0 ? j0 : j1, yl = l == 0 ? y0 : y1;
@@ -120,7 +154,7 @@ __device__ __forceinline__ void harm_radial(const SphereDev& S, double kmag, dou
             double jn = fma(t, j1, -j0);
             double yn = fma(t, y1, -y0);
             // (y_l < -1e33 needs l > kd as well)
-            const bool past = (double)l > kd;
+            const bool past = (double)l > st.kx;
             if (__any(past)) {
                 if (past && (jn < kJMin || jn > kJMax)) jn = 0.0;
                 if (yn < kYMin || (past && yn > 0.0)) {
@@ -135,9 +169,22 @@ __device__ __forceinline__ void harm_radial(const SphereDev& S, double kmag, dou
             jl = jn;
             yl = yn;
         }
-        const double2 b = bq[l & 3];
-        mac(l, fma(b.x, jl, -(b.y * yl)), fma(b.x, yl, b.y * jl));
+        const double2 b = bq[i & 3];
+        mac(i, fma(b.x, jl, -(b.y * yl)), fma(b.x, yl, b.y * jl));
     }
+    st.cj0 = dc{j0, y0};
+    st.cj1 = dc{j1, y1};
+}
+
+// All orders at once (the sums of orders 0 .. L - 1 in ar / ai).
+template <int L>
+__device__ __forceinline__ void harm_radial(const SphereDev& S, double kmag, double d,
+                                            double inv_d, double inv_kmag, const RadialTrig& T,
+                                            const double2* __restrict__ A,
+                                            const double2* __restrict__ B, const float* ar,
+                                            const float* ai, float& acc_r, float& acc_i) {
+    RadialState st = harm_radial_init(S, kmag, d, inv_d, inv_kmag, T);
+    harm_radial_range<0, L>(st, S, A, B, ar, ai, acc_r, acc_i);
 }
 
 // Order sums of a planar sphere (x = 0 at every pixel): the Legendre
@@ -184,6 +231,73 @@ __device__ __forceinline__ void harm_planar_sums(const float4* __restrict__ R, i
 }
 
 
+// The planar order sums of orders L0 .. L0 + C - 1 only (L0 a multiple of
+// 4): rows m < L0 + C (phases from e^{i 0 phi} again), the records of
+// k = L0/2 .. (L0 + C)/2 - 1 in each, in the blocks of two k of
+// harm_planar_sums; ar / ai at [l - L0].  Every sum meets the same terms
+// in the same order as in harm_planar_sums.
+template <int L, int C, int L0>
+__device__ __forceinline__ void harm_planar_chunk(const float4* __restrict__ R, int nl1, float z1r,
+                                                  float z1i, float* ar, float* ai) {
+    constexpr int H = L / 2;
+    constexpr int KB = 2;
+    constexpr int K0 = L0 / 2;
+    constexpr int NB = ((L0 + C <= L ? C : L - L0) / 2 + KB - 1) / KB;   // blocks in the chunk
+    const int ke = (nl1 + 1) >> 1;
+    const int mend = min(nl1, L0 + C);
+    float zr = 1.f, zi = 0.f;
+#pragma clang loop unroll(disable)
+    for (int m = 0; m < mend; m += 2) {
+        const float4* Ra = R + m * H;
+        const float4* Rb = Ra + H;
+        const float zar = zr, zai = zi;
+        const float zbr = zr * z1r - zi * z1i, zbi = fmaf(zr, z1i, zi * z1r);
+        const int bf = (m >> 1) / KB;   // block of k = m / 2
+#pragma clang loop unroll(full)
+        for (int b = 0; b < NB; ++b) {
+            constexpr int gb0 = K0 / KB;
+            const int gb = gb0 + b;
+            if (gb < bf || KB * gb >= ke) continue;
+            float4 ra[KB], rb[KB];
+#pragma clang loop unroll(full)
+            for (int j = 0; j < KB; ++j) {
+                ra[j] = Ra[KB * gb + j];
+                rb[j] = Rb[KB * gb + j];
+            }
+#pragma clang loop unroll(full)
+            for (int j = 0; j < KB; ++j) {
+                const int le = 2 * (KB * b + j);
+                ar[le] = fmaf(zai, ra[j].z, fmaf(zar, ra[j].x, ar[le]));
+                ai[le] = fmaf(zai, ra[j].w, fmaf(zar, ra[j].y, ai[le]));
+                ar[le + 1] = fmaf(zbi, rb[j].z, fmaf(zbr, rb[j].x, ar[le + 1]));
+                ai[le + 1] = fmaf(zbi, rb[j].w, fmaf(zbr, rb[j].y, ai[le + 1]));
+            }
+        }
+        zr = zbr * z1r - zbi * z1i;
+        zi = fmaf(zbr, z1i, zbi * z1r);
+    }
+}
+
+// Chunk c of the chunked planar form and the ones after it.
+template <int L, int C, int c>
+__device__ __forceinline__ void harm_planar_chunks(const float4* __restrict__ R, const SphereDev& S,
+                                                   float z1r, float z1i, RadialState& st,
+                                                   const double2* __restrict__ A,
+                                                   const double2* __restrict__ B, float& acc_r,
+                                                   float& acc_i) {
+    if constexpr (c * C < L) {
+        const int nl1 = S.Nl + 1;
+        if (c * C >= nl1) return;
+        float ar[C], ai[C];
+#pragma unroll
+        for (int i = 0; i < C; ++i) ar[i] = ai[i] = 0.f;
+        harm_planar_chunk<L, C, c * C>(R, nl1, z1r, z1i, ar, ai);
+        harm_radial_range<c * C, C>(st, S, A, B, ar, ai, acc_r, acc_i);
+        harm_planar_chunks<L, C, c + 1>(R, S, z1r, z1i, st, A, B, acc_r, acc_i);
+    }
+}
+
+
 // Per-pixel harmonic sum, persistent over (instance, pixel block) items.
 // Row m of the order sums runs the scaled recurrence
 //   q_mm = Pn_mm,  q_lm = 2x q_{l-1,m} - c_lm q_{l-2,m}
@@ -194,7 +308,7 @@ __device__ __forceinline__ void harm_planar_sums(const float4* __restrict__ R, i
 // holds c_lm (l >= m + 2) and the diagonal factor sqrt((2m+1)/(2m)) at
 // l = m (where the recurrence does not read c); cst[2 (m kCS + l) + 1] is 1
 // at l = m, else 0, and injects the diagonal value (q is 0 below it).
-template <int L, bool kPlanar, int kWaves>
+template <int L, bool kPlanar, int kWaves, int kChunk>
 __global__ __launch_bounds__(kHarmThreads, kWaves) void series_harm_kernel(
     const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph,
     const SeriesGroup* __restrict__ grps, const SeriesItem* __restrict__ items, int n_items,
@@ -232,6 +346,14 @@ __global__ __launch_bounds__(kHarmThreads, kWaves) void series_harm_kernel(
             // direction; the centre (knd below float epsilon) keeps only l = 0
             const bool centre = d <= S.a && S.n[0] * (I.kmag * d) < (double)kEpsFloat;
             r = centre ? d3{0.0, 0.0, 0.0} : scale(r, inv_d);
+            if constexpr (kChunk > 0) {
+                // planar buckets only; orders in chunks of kChunk
+                const RadialTrig trig = harm_trig(S, I.kmag, d);
+                RadialState st = harm_radial_init(S, I.kmag, d, inv_d, inv_kmag, trig);
+                harm_planar_chunks<L, kChunk, 0>(rec + SS.toff, S, (float)r.x, (float)r.y, st, A,
+                                                 B, acc_r, acc_i);
+                continue;
+            }
             if (kPlanar || SS.planar) {
                 // r.z = 0, so sin(theta) = 1 and e^{i phi} = r.x + i r.y
                 const RadialTrig trig = harm_trig(S, I.kmag, d);

@@ -14,13 +14,23 @@ void launch_variant(const HarmLaunch& a) {
     // buckets whose spheres are all planar take the kernel without the
     // general branch (fewer registers live)
     if (a.planar)
-        hipLaunchKernelGGL((harm::series_harm_kernel<L, true, W>), a.grid,
+        hipLaunchKernelGGL((harm::series_harm_kernel<L, true, W, 0>), a.grid,
                            dim3(harm::kHarmThreads), 0, a.stream, a.inst, a.sph, a.grp, a.items,
                            a.n_items, a.A, a.B, a.rec, a.cst, a.perm, a.U, a.counter);
     else
-        hipLaunchKernelGGL((harm::series_harm_kernel<L, false, W>), a.grid,
+        hipLaunchKernelGGL((harm::series_harm_kernel<L, false, W, 0>), a.grid,
                            dim3(harm::kHarmThreads), 0, a.stream, a.inst, a.sph, a.grp, a.items,
                            a.n_items, a.A, a.B, a.rec, a.cst, a.perm, a.U, a.counter);
+}
+
+// the chunked planar form (planar buckets only)
+template <int L, int W, int C>
+int launch_chunked(const HarmLaunch& a) {
+    if (!a.planar) return -1;
+    hipLaunchKernelGGL((harm::series_harm_kernel<L, true, W, C>), a.grid, dim3(harm::kHarmThreads),
+                       0, a.stream, a.inst, a.sph, a.grp, a.items, a.n_items, a.A, a.B, a.rec,
+                       a.cst, a.perm, a.U, a.counter);
+    return 0;
 }
 
 }  // namespace
@@ -43,6 +53,12 @@ int harm_launch(const HarmLaunch& a, int variant) {
                 return 0;
             }
             return -1;
+        case 3:
+            return launch_chunked<L, harm_variant_waves(L, 3), harm_variant_chunk(L, 3)>(a);
+        case 4:
+            return launch_chunked<L, harm_variant_waves(L, 4), harm_variant_chunk(L, 4)>(a);
+        case 5:
+            return launch_chunked<L, harm_variant_waves(L, 5), harm_variant_chunk(L, 5)>(a);
         default:
             return -1;
     }

@@ -133,13 +133,19 @@ struct HarmLaunch {
 
 // Register budgets of the harmonic kernel: variant v of the bucket of L
 // orders runs harm_variant_waves(L, v) waves per SIMD (0: no such variant).
-// The order sums alone take 2L VGPRs, so the larger budgets spill some
-// values to scratch; more waves in flight hide the latency of the FP64
-// radial chains and of the record loads better, and the plan's tuner
+// Variants 0-2 hold the order sums of all L orders at once (2L VGPRs), so
+// the larger budgets spill some values to scratch; more waves in flight
+// hide the latency of the FP64 radial chains and of the record loads
+// better.  Variants 3-5 exist for planar buckets only: they sum the orders
+// in chunks of harm_variant_chunk(L, v) (each chunk's rows of records, then
+// its radial factors, the recurrences carried from chunk to chunk), which
+// holds 2C sums instead of 2L and adds the phase steps of the rows below
+// each chunk; they give the same bits as variants 0-2.  The plan's tuner
 // times the variants on the batch's own inputs.
-constexpr int kHarmVariants = 3;
+constexpr int kHarmVariants = 6;
 constexpr int kHarmDefaultVariant = 1;
 constexpr int harm_variant_waves(int L, int v) {
+    if (v >= 3) return v == 3 ? 6 : v == 4 ? 7 : 8;
     return L <= 16 ? 6 + v
            : L <= 24 ? 5 + v
            : L <= 40 ? 4 + v
@@ -147,6 +153,13 @@ constexpr int harm_variant_waves(int L, int v) {
            : L <= 64 ? 2 + v
            : L <= 80 ? (v < 2 ? 2 + v : 0)
                      : (v < 2 ? 1 + v : 0);
+}
+constexpr int harm_variant_chunk(int L, int v) {
+    return v < 3 ? 0 : v == 5 ? 8 : (L <= 16 ? 8 : 16);
+}
+// whether variant v runs on a bucket (chunked variants: planar buckets)
+constexpr bool harm_variant_exists(int L, int v, bool planar) {
+    return v >= 0 && v < kHarmVariants && harm_variant_waves(L, v) > 0 && (v < 3 || planar);
 }
 
 // Launches bucket L in variant v (defined per bucket in us_harm.hip);

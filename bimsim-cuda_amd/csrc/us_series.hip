@@ -1229,11 +1229,22 @@ namespace {
 // The register budget of each bucket of a harmonic plan, chosen by timing
 // its variants on the plan's own inputs (into a scratch field); the choice
 // is kept in the context per batch shape.
-int harm_tune(bimsim_ctx* ctx, const std::string& key, const double* waves, const double2* A,
+int harm_tune(bimsim_ctx* ctx, const std::string& shape, const double* waves, const double2* A,
               const double2* B, SeriesPlan* P) {
     std::vector<SeriesPlan::Bucket*> bks;
     for (SeriesPlan::Chunk& ch : P->chunks)
         for (SeriesPlan::Bucket& K : ch.buckets) bks.push_back(&K);
+    // a fixed variant for every bucket that has it (for comparing variants)
+    if (const char* fixed = std::getenv("BIMSIM_HARM_VARIANT")) {
+        const int v = std::atoi(fixed);
+        for (SeriesPlan::Bucket* K : bks)
+            if (harm_variant_exists(K->L, v, K->planar)) K->variant = v;
+        return BIMSIM_OK;
+    }
+    // the key holds the buckets' orders and planar flags: the variants that
+    // exist differ with them
+    std::string key = shape;
+    for (const SeriesPlan::Bucket* K : bks) key += "|" + std::to_string(K->L) + (K->planar ? "p" : "g");
     auto hit = ctx->tune_harm.find(key);
     if (hit != ctx->tune_harm.end() && hit->second.size() == bks.size()) {
         for (size_t i = 0; i < bks.size(); ++i) bks[i]->variant = hit->second[i];
@@ -1256,7 +1267,7 @@ int harm_tune(bimsim_ctx* ctx, const std::string& key, const double* waves, cons
         float best = 1e30f;
         int best_v = K->variant;
         for (int v = 0; v < kHarmVariants && rc == BIMSIM_OK; ++v) {
-            if (harm_variant_waves(K->L, v) <= 0) continue;
+            if (!harm_variant_exists(K->L, v, K->planar)) continue;
             K->variant = v;
             float ms = 0.f;
             for (int r = 0; r < 2 && rc == BIMSIM_OK; ++r) {   // a warm run, then the timed one
@@ -1287,7 +1298,9 @@ int harm_tune(bimsim_ctx* ctx, const std::string& key, const double* waves, cons
         std::string msg;
         for (size_t i = 0; i < bks.size(); ++i)
             msg += " L" + std::to_string(bks[i]->L) + ":" +
-                   std::to_string(harm_variant_waves(bks[i]->L, chosen[i]));
+                   std::to_string(harm_variant_waves(bks[i]->L, chosen[i])) +
+                   (chosen[i] >= 3 ? "/c" + std::to_string(harm_variant_chunk(bks[i]->L, chosen[i]))
+                                   : std::string());
         std::fprintf(stderr, "bimsim: harmonic waves per SIMD%s\n", msg.c_str());
     }
     return BIMSIM_OK;

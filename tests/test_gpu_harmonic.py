@@ -162,3 +162,35 @@ def test_mixed_planar_and_off_plane_spheres(pkg, ctx, orc, W):
     inst = W.make_instance(sph, 2.0, 40, 30.0, (0.0, 0.5), (0.0, 0.5), 48, 23)
     got, ref = _harmonic_field(pkg, ctx, orc, W, inst)
     assert normrel(got, ref) <= 2e-5
+
+
+@pytest.mark.parametrize("variant", [0, 3, 4, 5])
+def test_register_variants_give_the_same_bits(pkg, W, variant, monkeypatch):
+    """Every register-budget variant of the harmonic kernel, the chunked
+    planar ones (orders summed in chunks of 16 or 8) included, adds the
+    same terms in the same order: the images equal the default variant's
+    bit for bit, on a planar sweep whose orders span several buckets."""
+    import bimsim_amd.pipeline as P
+    insts = W.config4(n_nu=256, N=48, ns=4, nk=24, stride=37)
+    assert max(s["Nl"] for i in insts for s in i["spheres"]) > 24
+    # planar: the slice is normal to z through every centre
+    assert all(i["X"][2] == 0 and i["Y"][2] == 0 and i["A"][2] == s["p"][2]
+               for i in insts for s in i["spheres"])
+    waves = [W.mc_waves(i, pkg.mc_sample_na) for i in insts]
+    arr, spheres, Wv, n_img = P.batch_tables(insts, waves)
+    wd = torch.as_tensor(Wv.ravel()).cuda()
+
+    def image(v):
+        monkeypatch.setenv("BIMSIM_HARM_VARIANT", str(v))
+        c = pkg.Context(0)
+        try:
+            D = torch.empty(n_img, dtype=torch.float32, device="cuda")
+            Di, Ab = torch.empty_like(D), torch.empty_like(D)
+            b = c.batch(arr, spheres, wd, D, Di, Ab, prec=pkg.FP32_FAST, use_graph=False)
+            out = host(D).copy()
+            b.close()
+            return out
+        finally:
+            c.close()
+
+    assert np.array_equal(image(variant), image(1))
