@@ -286,8 +286,11 @@ def headline(pkg, P, W, ctx, torch, args):
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS if achieved else None,
             "traffic": traffic,
-            "kernel": "series_harm_kernel (harmonic FP32 scattered-field sums, planar form; one "
-                      "persistent launch per order bucket)",
+            "traffic_note": "HBM bytes per pass (the six bucket launches) from rocprofv3 "
+                            "FETCH_SIZE x 2 + WRITE_SIZE, scaled by a 256 MiB copy; "
+                            "algorithmic bytes per pass 5.68e8 (fields once, records, B_l)",
+            "kernel": "series_harm_kernel (harmonic FP32 scattered-field sums, planar form in "
+                      "chunks of orders; one persistent launch per order bucket)",
             "flops_per_pass": flops,
             "series_ms_per_pass": series_ms_pass,
             "peak_note": "157.3 is AMD's published FP32 vector peak; this build compiles "

@@ -1,14 +1,17 @@
 // The harmonic pixel kernel of one order bucket (HARM_L, set by the build:
-// this file is compiled once per bucket) in its register-budget variants.
+// this file is compiled twice per bucket) in its register-budget variants:
+// HARM_PART 0 the variants that hold all order sums at once, HARM_PART 1
+// the chunked planar variants.
 #include "us_harm.h"
 
-#ifndef HARM_L
-#error "us_harm.hip is compiled once per order bucket with -DHARM_L=<orders>"
+#if !defined(HARM_L) || !defined(HARM_PART)
+#error "us_harm.hip is compiled per order bucket with -DHARM_L=<orders> -DHARM_PART=<0|1>"
 #endif
 
 namespace bimsim {
 namespace {
 
+#if HARM_PART == 0
 template <int L, int W>
 void launch_variant(const HarmLaunch& a) {
     // buckets whose spheres are all planar take the kernel without the
@@ -23,6 +26,7 @@ void launch_variant(const HarmLaunch& a) {
                            a.n_items, a.A, a.B, a.rec, a.cst, a.perm, a.U, a.counter);
 }
 
+#else
 // the chunked planar form (planar buckets only)
 template <int L, int W, int C>
 int launch_chunked(const HarmLaunch& a) {
@@ -32,11 +36,13 @@ int launch_chunked(const HarmLaunch& a) {
                        a.cst, a.perm, a.U, a.counter);
     return 0;
 }
+#endif
 
 }  // namespace
 
+#if HARM_PART == 0
 template <int L>
-int harm_launch(const HarmLaunch& a, int variant) {
+int harm_launch_full(const HarmLaunch& a, int variant) {
     constexpr int w0 = harm_variant_waves(L, 0);
     constexpr int w1 = harm_variant_waves(L, 1);
     constexpr int w2 = harm_variant_waves(L, 2);
@@ -53,6 +59,16 @@ int harm_launch(const HarmLaunch& a, int variant) {
                 return 0;
             }
             return -1;
+        default:
+            return -1;
+    }
+}
+
+template int harm_launch_full<HARM_L>(const HarmLaunch& a, int variant);
+#else
+template <int L>
+int harm_launch_chunked(const HarmLaunch& a, int variant) {
+    switch (variant) {
         case 3:
             return launch_chunked<L, harm_variant_waves(L, 3), harm_variant_chunk(L, 3)>(a);
         case 4:
@@ -64,6 +80,7 @@ int harm_launch(const HarmLaunch& a, int variant) {
     }
 }
 
-template int harm_launch<HARM_L>(const HarmLaunch& a, int variant);
+template int harm_launch_chunked<HARM_L>(const HarmLaunch& a, int variant);
+#endif
 
 }  // namespace bimsim

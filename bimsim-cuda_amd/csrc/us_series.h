@@ -162,10 +162,17 @@ constexpr bool harm_variant_exists(int L, int v, bool planar) {
     return v >= 0 && v < kHarmVariants && harm_variant_waves(L, v) > 0 && (v < 3 || planar);
 }
 
-// Launches bucket L in variant v (defined per bucket in us_harm.hip);
-// returns 0, or -1 for a variant that does not exist.
+// Launches bucket L in variant v (defined per bucket in us_harm.hip, in two
+// objects: the chunked variants are built with machine LICM off, the others
+// with it); returns 0, or -1 for a variant that does not exist.
 template <int L>
-int harm_launch(const HarmLaunch& a, int variant);
+int harm_launch_full(const HarmLaunch& a, int variant);
+template <int L>
+int harm_launch_chunked(const HarmLaunch& a, int variant);
+template <int L>
+inline int harm_launch(const HarmLaunch& a, int variant) {
+    return variant < 3 ? harm_launch_full<L>(a, variant) : harm_launch_chunked<L>(a, variant);
+}
 
 struct SeriesPlan {
     int n_inst = 0, n_sph = 0;
