@@ -555,10 +555,11 @@ def main():
         "dtype": "fp32",
         "data": "synthetic: SplitMix64-seeded spheres and Monte-Carlo waves, float32-rounded",
         "config": h["config"],
-        "terms_per_s": h["terms_per_s"],
+        "reference_equivalent_terms_per_s": h["terms_per_s"],
         "terms_note": "Mie series terms counted as the reference's per-wave series would evaluate "
                       "them (N^2 nk sum(Nl+1) + N^2 (m+1) per image); the harmonic kernel "
-                      "reaches them with fewer operations, so this is a reference-equivalent rate",
+                      "reaches them with fewer operations (roofline.flops_per_pass is its own "
+                      "count), so this is a reference-equivalent rate",
         "eager_pass": h["eager_pass"],
         "fp64_plain": h["fp64_plain"],
         "fp64_hoisted": h["fp64_hoisted"],

@@ -306,3 +306,24 @@ def test_cli_vector_field(exe, orc, tmp_path, prec):
     else:
         assert np.max(np.abs(out["A"][0][0][bright] - Ab[bright])) <= 1e-4
     assert (tmp_path / "n.bmp").read_bytes()[:2] == b"BM"
+
+
+def test_cli_fp32_beyond_the_series_orders(exe, tmp_path):
+    """A sphere of Nl = 122 (a = 40 um at 2.5 um, past the FP32 series
+    kernels' 112 orders) with the default FP32 arithmetic: the scattered
+    field comes from the FP64 form, so the images agree with the parity
+    form's to the float32 output rounding."""
+    def run(sub, *extra):
+        d = tmp_path / sub
+        d.mkdir()
+        args = [EXE, "--spheres", 3, -2, 0, 40, "--materials", 1.2, 0.01, "--lambda", 2.5,
+                "--xy", 100, 0, "--resolution", 32, "--padding", 0, "--condenser", 0.5,
+                "--objective", 0.5, "--samples", 16, "--field-order", 40, "--recursive",
+                "--intensity", "I", "--incident", "I0", "--absorbance", "A", *extra]
+        p = subprocess.run([str(a) for a in args], cwd=d, capture_output=True, text=True,
+                           timeout=120)
+        assert p.returncode == 0, p.stdout + p.stderr
+        return {k: read_envi(str(d / k))[0][0] for k in ["I", "I0", "A"]}
+    f32 = run("fp32")
+    ref = run("parity", "--precision", "parity")
+    assert normrel(f32["I"], ref["I"]) <= 1e-5 and normrel(f32["I0"], ref["I0"]) <= 1e-5
