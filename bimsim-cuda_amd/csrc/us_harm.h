@@ -89,12 +89,16 @@ __device__ __forceinline__ RadialState harm_radial_init(const SphereDev& S, doub
 }
 
 // Orders L0 .. L0 + C - 1 (those <= Nl) of the radial sum; ar / ai hold
-// the order sums of these orders at [l - L0].
+// the order sums of these orders at [l - L0], formed from records that
+// already carry the coefficient: B_l for the pixels outside the sphere
+// (pass 0), A_l for those inside it (pass 1).  A lane takes part only in
+// its own pass: outside, acc += (j_l + i y_l)(kd) times the sum; inside,
+// acc += j_l(knd) times the sum; at the centre (knd below float epsilon)
+// only the l = 0 sum, A_0 being in it (calc_Ui returns A_0 there).
 template <int L0, int C>
-__device__ __forceinline__ void harm_radial_range(RadialState& st, const SphereDev& S,
-                                                  const double2* __restrict__ A,
-                                                  const double2* __restrict__ B, const float* ar,
-                                                  const float* ai, float& acc_r, float& acc_i) {
+__device__ __forceinline__ void harm_radial_range(int pass, RadialState& st, const SphereDev& S,
+                                                  const float* ar, const float* ai, float& acc_r,
+                                                  float& acc_i) {
     const int nl1 = S.Nl + 1;
     auto mac = [&](int i, double cr, double ci) {
         const float fr = (float)cr, fi = (float)ci;
@@ -102,14 +106,14 @@ __device__ __forceinline__ void harm_radial_range(RadialState& st, const SphereD
         acc_i = fmaf(fr, ai[i], fmaf(fi, ar[i], acc_i));
     };
     if (st.mode == 2) {
-        if (L0 == 0) {
-            const double2 a0 = A[S.coeff_off];
-            mac(0, a0.x, a0.y);
+        if (pass == 1 && L0 == 0) {
+            acc_r += ar[0];
+            acc_i += ai[0];
         }
         return;
     }
     if (st.mode == 1) {
-        const double2* As = A + S.coeff_off;
+        if (pass != 1) return;
         const dc inv = st.cinv;
 #pragma clang loop unroll(full)
         for (int i = 0; i < C; ++i) {
@@ -130,37 +134,33 @@ __device__ __forceinline__ void harm_radial_range(RadialState& st, const SphereD
                 st.cj1 = jn;
                 jl = jn;
             }
-            const double2 a = As[l];
-            mac(i, fma(a.x, jl.r, -(a.y * jl.i)), fma(a.x, jl.i, a.y * jl.r));
+            mac(i, jl.r, jl.i);
         }
         return;
     }
-    const double2* Bs = B + S.coeff_off;
+    if (pass != 0) return;
     const double inv = st.cinv.r;
     double j0 = st.cj0.r, y0 = st.cj0.i, j1 = st.cj1.r, y1 = st.cj1.i;
-    // B_l read four orders at a time (indices clamped to Nl), so that one
-    // wait on the scalar loads covers four orders
-    double2 bq[4];
 #pragma clang loop unroll(full)
     for (int i = 0; i < C; ++i) {
         const int l = L0 + i;
         if (l >= nl1) continue;
-        if ((i & 3) == 0)
-#pragma clang loop unroll(full)
-            for (int q = 0; q < 4; ++q) bq[q] = Bs[min(l + q, nl1 - 1)];
         double jl = l == 0 ? j0 : j1, yl = l == 0 ? y0 : y1;
         if (l >= 2) {
             const double t = (double)(2 * l - 1) * inv;
             double jn = fma(t, j1, -j0);
             double yn = fma(t, y1, -y0);
-            // (y_l < -1e33 needs l > kd as well)
             const bool past = (double)l > st.kx;
             if (__any(past)) {
                 if (past && (jn < kJMin || jn > kJMax)) jn = 0.0;
-                if (yn < kYMin || (past && yn > 0.0)) {
-                    yn = 0.0;
-                    y1 = 0.0;
-                }
+                // (y_l < -1e33 needs l > kd as well; neither rule acts in
+                // practice, so its selects run only where a lane needs them)
+                const bool yz = yn < kYMin || (past && yn > 0.0);
+                if (__any(yz))
+                    if (yz) {
+                        yn = 0.0;
+                        y1 = 0.0;
+                    }
             }
             j0 = j1;
             j1 = jn;
@@ -169,8 +169,7 @@ __device__ __forceinline__ void harm_radial_range(RadialState& st, const SphereD
             jl = jn;
             yl = yn;
         }
-        const double2 b = bq[i & 3];
-        mac(i, fma(b.x, jl, -(b.y * yl)), fma(b.x, yl, b.y * jl));
+        mac(i, jl, yl);
     }
     st.cj0 = dc{j0, y0};
     st.cj1 = dc{j1, y1};
@@ -178,13 +177,12 @@ __device__ __forceinline__ void harm_radial_range(RadialState& st, const SphereD
 
 // All orders at once (the sums of orders 0 .. L - 1 in ar / ai).
 template <int L>
-__device__ __forceinline__ void harm_radial(const SphereDev& S, double kmag, double d,
+__device__ __forceinline__ void harm_radial(int pass, const SphereDev& S, double kmag, double d,
                                             double inv_d, double inv_kmag, const RadialTrig& T,
-                                            const double2* __restrict__ A,
-                                            const double2* __restrict__ B, const float* ar,
-                                            const float* ai, float& acc_r, float& acc_i) {
+                                            const float* ar, const float* ai, float& acc_r,
+                                            float& acc_i) {
     RadialState st = harm_radial_init(S, kmag, d, inv_d, inv_kmag, T);
-    harm_radial_range<0, L>(st, S, A, B, ar, ai, acc_r, acc_i);
+    harm_radial_range<0, L>(pass, st, S, ar, ai, acc_r, acc_i);
 }
 
 // Order sums of a planar sphere (x = 0 at every pixel): the Legendre
@@ -278,13 +276,11 @@ __device__ __forceinline__ void harm_planar_chunk(const float4* __restrict__ R, 
     }
 }
 
-// Chunk c of the chunked planar form and the ones after it.
+// Chunk c of the chunked planar form and the ones after it (one pass).
 template <int L, int C, int c>
 __device__ __forceinline__ void harm_planar_chunks(const float4* __restrict__ R, const SphereDev& S,
-                                                   float z1r, float z1i, RadialState& st,
-                                                   const double2* __restrict__ A,
-                                                   const double2* __restrict__ B, float& acc_r,
-                                                   float& acc_i) {
+                                                   float z1r, float z1i, int pass, RadialState& st,
+                                                   float& acc_r, float& acc_i) {
     if constexpr (c * C < L) {
         const int nl1 = S.Nl + 1;
         if (c * C >= nl1) return;
@@ -292,8 +288,8 @@ __device__ __forceinline__ void harm_planar_chunks(const float4* __restrict__ R,
 #pragma unroll
         for (int i = 0; i < C; ++i) ar[i] = ai[i] = 0.f;
         harm_planar_chunk<L, C, c * C>(R, nl1, z1r, z1i, ar, ai);
-        harm_radial_range<c * C, C>(st, S, A, B, ar, ai, acc_r, acc_i);
-        harm_planar_chunks<L, C, c + 1>(R, S, z1r, z1i, st, A, B, acc_r, acc_i);
+        harm_radial_range<c * C, C>(pass, st, S, ar, ai, acc_r, acc_i);
+        harm_planar_chunks<L, C, c + 1>(R, S, z1r, z1i, pass, st, acc_r, acc_i);
     }
 }
 
@@ -308,11 +304,15 @@ __device__ __forceinline__ void harm_planar_chunks(const float4* __restrict__ R,
 // holds c_lm (l >= m + 2) and the diagonal factor sqrt((2m+1)/(2m)) at
 // l = m (where the recurrence does not read c); cst[2 (m kCS + l) + 1] is 1
 // at l = m, else 0, and injects the diagonal value (q is 0 below it).
+// rec holds the records times B_l, rec_in the same times A_l: a wave with
+// pixels outside the sphere sums the first (pass 0), one with pixels inside
+// the second (pass 1); a wave with both does both passes, each lane
+// keeping its own.
 template <int L, bool kPlanar, int kWaves, int kChunk>
 __global__ __launch_bounds__(kHarmThreads, kWaves) void series_harm_kernel(
     const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph,
     const SeriesGroup* __restrict__ grps, const SeriesItem* __restrict__ items, int n_items,
-    const double2* __restrict__ A, const double2* __restrict__ B, const float4* __restrict__ rec,
+    const float4* __restrict__ rec, const float4* __restrict__ rec_in,
     const float* __restrict__ cst, const int* __restrict__ perm, float2* __restrict__ U,
     int* __restrict__ counter) {
     __shared__ int s_item;
@@ -346,22 +346,35 @@ __global__ __launch_bounds__(kHarmThreads, kWaves) void series_harm_kernel(
             // direction; the centre (knd below float epsilon) keeps only l = 0
             const bool centre = d <= S.a && S.n[0] * (I.kmag * d) < (double)kEpsFloat;
             r = centre ? d3{0.0, 0.0, 0.0} : scale(r, inv_d);
+            const bool inside = d <= S.a;
+            const int pass0 = __any(!inside) ? 0 : 1, pass1 = __any(inside) ? 1 : 0;
             if constexpr (kChunk > 0) {
                 // planar buckets only; orders in chunks of kChunk
                 const RadialTrig trig = harm_trig(S, I.kmag, d);
                 RadialState st = harm_radial_init(S, I.kmag, d, inv_d, inv_kmag, trig);
-                harm_planar_chunks<L, kChunk, 0>(rec + SS.toff, S, (float)r.x, (float)r.y, st, A,
-                                                 B, acc_r, acc_i);
+                if (pass0 == 0)
+                    harm_planar_chunks<L, kChunk, 0>(rec + SS.toff, S, (float)r.x, (float)r.y, 0,
+                                                     st, acc_r, acc_i);
+                if (pass1 == 1)
+                    harm_planar_chunks<L, kChunk, 0>(rec_in + SS.toff, S, (float)r.x, (float)r.y,
+                                                     1, st, acc_r, acc_i);
                 continue;
             }
             if (kPlanar || SS.planar) {
                 // r.z = 0, so sin(theta) = 1 and e^{i phi} = r.x + i r.y
                 const RadialTrig trig = harm_trig(S, I.kmag, d);
-                float ar[L], ai[L];
+                auto sums = [&](const float4* R, int pass) {
+                    float ar[L], ai[L];
 #pragma unroll
-                for (int l = 0; l < L; ++l) ar[l] = ai[l] = 0.f;
-                harm_planar_sums<L>(rec + SS.toff, nl1, (float)r.x, (float)r.y, ar, ai);
-                harm_radial<L>(S, I.kmag, d, inv_d, inv_kmag, trig, A, B, ar, ai, acc_r, acc_i);
+                    for (int l = 0; l < L; ++l) ar[l] = ai[l] = 0.f;
+                    harm_planar_sums<L>(R + SS.toff, nl1, (float)r.x, (float)r.y, ar, ai);
+                    harm_radial<L>(pass, S, I.kmag, d, inv_d, inv_kmag, trig, ar, ai, acc_r,
+                                   acc_i);
+                };
+                // (two calls rather than a loop over the passes: a loop here
+                // keeps the order sums in scratch)
+                if (pass0 == 0) sums(rec, 0);
+                if (pass1 == 1) sums(rec_in, 1);
                 continue;
             }
             const float x = (float)r.z;
@@ -373,69 +386,74 @@ __global__ __launch_bounds__(kHarmThreads, kWaves) void series_harm_kernel(
             }
             const float x2 = 2.f * x;
             const RadialTrig trig = harm_trig(S, I.kmag, d);
-            float ar[L], ai[L];
+            auto sums = [&](const float4* Rp, int pass) {
+                float ar[L], ai[L];
 #pragma unroll
-            for (int l = 0; l < L; ++l) ar[l] = ai[l] = 0.f;
-            const float4* R = rec + SS.toff;   // row stride L: the instance's bucket
-            const int be = (nl1 + 3) >> 2;
-            float zr = 1.f, zi = 0.f;
-            float pmm = 0.28209479177387814f;   // 1/sqrt(4 pi)
-            // rows in pairs (m, m + 1), m even: both start in block m / 4, so
-            // one wait covers two rows' loads and the two recurrences
-            // interleave (a row m + 1 = Nl + 1 has zero records)
+                for (int l = 0; l < L; ++l) ar[l] = ai[l] = 0.f;
+                // row stride L: the instance's bucket
+                const float4* R = Rp + SS.toff;
+                const int be = (nl1 + 3) >> 2;
+                float zr = 1.f, zi = 0.f;
+                float pmm = 0.28209479177387814f;   // 1/sqrt(4 pi)
+                // rows in pairs (m, m + 1), m even: both start in block m / 4, so
+                // one wait covers two rows' loads and the two recurrences
+                // interleave (a row m + 1 = Nl + 1 has zero records)
 #pragma clang loop unroll(disable)
-            for (int m = 0; m < nl1; m += 2) {
-                const float* Ca = cst + 2 * m * kCS;
-                const float* Cb = Ca + 2 * kCS;
-                const float4* Ra = R + m * L;
-                const float4* Rb = Ra + L;
-                const float pa = pmm, pb = Cb[2 * (m + 1)] * st * pmm;
-                const float zar = zr, zai = zi;
-                const float zbr = zr * z1r - zi * z1i, zbi = fmaf(zr, z1i, zi * z1r);
-                const int bf = m >> 2;
-                float qa0 = 0.f, qa1 = 0.f, qb0 = 0.f, qb1 = 0.f;
+                for (int m = 0; m < nl1; m += 2) {
+                    const float* Ca = cst + 2 * m * kCS;
+                    const float* Cb = Ca + 2 * kCS;
+                    const float4* Ra = R + m * L;
+                    const float4* Rb = Ra + L;
+                    const float pa = pmm, pb = Cb[2 * (m + 1)] * st * pmm;
+                    const float zar = zr, zai = zi;
+                    const float zbr = zr * z1r - zi * z1i, zbi = fmaf(zr, z1i, zi * z1r);
+                    const int bf = m >> 2;
+                    float qa0 = 0.f, qa1 = 0.f, qb0 = 0.f, qb1 = 0.f;
 #pragma clang loop unroll(full)
-                for (int b = 0; b < L / 4; ++b) {
-                    if (b < bf || b >= be) continue;
-                    const float4 ca0 = *reinterpret_cast<const float4*>(Ca + 8 * b);
-                    const float4 ca1 = *reinterpret_cast<const float4*>(Ca + 8 * b + 4);
-                    const float4 cb0 = *reinterpret_cast<const float4*>(Cb + 8 * b);
-                    const float4 cb1 = *reinterpret_cast<const float4*>(Cb + 8 * b + 4);
-                    const float cca[4] = {ca0.x, ca0.z, ca1.x, ca1.z};
-                    const float eea[4] = {ca0.y, ca0.w, ca1.y, ca1.w};
-                    const float ccb[4] = {cb0.x, cb0.z, cb1.x, cb1.z};
-                    const float eeb[4] = {cb0.y, cb0.w, cb1.y, cb1.w};
-                    float4 ra[4], rb[4];
+                    for (int b = 0; b < L / 4; ++b) {
+                        if (b < bf || b >= be) continue;
+                        const float4 ca0 = *reinterpret_cast<const float4*>(Ca + 8 * b);
+                        const float4 ca1 = *reinterpret_cast<const float4*>(Ca + 8 * b + 4);
+                        const float4 cb0 = *reinterpret_cast<const float4*>(Cb + 8 * b);
+                        const float4 cb1 = *reinterpret_cast<const float4*>(Cb + 8 * b + 4);
+                        const float cca[4] = {ca0.x, ca0.z, ca1.x, ca1.z};
+                        const float eea[4] = {ca0.y, ca0.w, ca1.y, ca1.w};
+                        const float ccb[4] = {cb0.x, cb0.z, cb1.x, cb1.z};
+                        const float eeb[4] = {cb0.y, cb0.w, cb1.y, cb1.w};
+                        float4 ra[4], rb[4];
 #pragma clang loop unroll(full)
-                    for (int j = 0; j < 4; ++j) {
-                        ra[j] = Ra[4 * b + j];
-                        rb[j] = Rb[4 * b + j];
+                        for (int j = 0; j < 4; ++j) {
+                            ra[j] = Ra[4 * b + j];
+                            rb[j] = Rb[4 * b + j];
+                        }
+#pragma clang loop unroll(full)
+                        for (int j = 0; j < 4; ++j) {
+                            // the diagonal value enters at l = m (ee = 1 there, else
+                            // 0); below it q stays 0
+                            const int l = 4 * b + j;
+                            const float qa = fmaf(x2, qa1, fmaf(-cca[j], qa0, eea[j] * pa));
+                            const float qb = fmaf(x2, qb1, fmaf(-ccb[j], qb0, eeb[j] * pb));
+                            qa0 = qa1;
+                            qa1 = qa;
+                            qb0 = qb1;
+                            qb1 = qb;
+                            const float ta1 = qa * zar, ta2 = qa * zai;
+                            const float tb1 = qb * zbr, tb2 = qb * zbi;
+                            ar[l] = fmaf(ta2, ra[j].z, fmaf(ta1, ra[j].x, ar[l]));
+                            ai[l] = fmaf(ta2, ra[j].w, fmaf(ta1, ra[j].y, ai[l]));
+                            ar[l] = fmaf(tb2, rb[j].z, fmaf(tb1, rb[j].x, ar[l]));
+                            ai[l] = fmaf(tb2, rb[j].w, fmaf(tb1, rb[j].y, ai[l]));
+                        }
                     }
-#pragma clang loop unroll(full)
-                    for (int j = 0; j < 4; ++j) {
-                        // the diagonal value enters at l = m (ee = 1 there, else
-                        // 0); below it q stays 0
-                        const int l = 4 * b + j;
-                        const float qa = fmaf(x2, qa1, fmaf(-cca[j], qa0, eea[j] * pa));
-                        const float qb = fmaf(x2, qb1, fmaf(-ccb[j], qb0, eeb[j] * pb));
-                        qa0 = qa1;
-                        qa1 = qa;
-                        qb0 = qb1;
-                        qb1 = qb;
-                        const float ta1 = qa * zar, ta2 = qa * zai;
-                        const float tb1 = qb * zbr, tb2 = qb * zbi;
-                        ar[l] = fmaf(ta2, ra[j].z, fmaf(ta1, ra[j].x, ar[l]));
-                        ai[l] = fmaf(ta2, ra[j].w, fmaf(ta1, ra[j].y, ai[l]));
-                        ar[l] = fmaf(tb2, rb[j].z, fmaf(tb1, rb[j].x, ar[l]));
-                        ai[l] = fmaf(tb2, rb[j].w, fmaf(tb1, rb[j].y, ai[l]));
-                    }
+                    // the next pair's diagonal value and phase
+                    pmm = Cb[2 * (kCS + m + 2)] * st * pb;
+                    zr = zbr * z1r - zbi * z1i;
+                    zi = fmaf(zbr, z1i, zbi * z1r);
                 }
-                // the next pair's diagonal value and phase
-                pmm = Cb[2 * (kCS + m + 2)] * st * pb;
-                zr = zbr * z1r - zbi * z1i;
-                zi = fmaf(zbr, z1i, zbi * z1r);
-            }
-            harm_radial<L>(S, I.kmag, d, inv_d, inv_kmag, trig, A, B, ar, ai, acc_r, acc_i);
+                harm_radial<L>(pass, S, I.kmag, d, inv_d, inv_kmag, trig, ar, ai, acc_r, acc_i);
+            };
+            if (pass0 == 0) sums(rec, 0);
+            if (pass1 == 1) sums(rec_in, 1);
         }
         float2* o = U + I.u_off + pix;
         float2 v = make_float2(0.f, 0.f);

@@ -19,11 +19,11 @@ void launch_variant(const HarmLaunch& a) {
     if (a.planar)
         hipLaunchKernelGGL((harm::series_harm_kernel<L, true, W, 0>), a.grid,
                            dim3(harm::kHarmThreads), 0, a.stream, a.inst, a.sph, a.grp, a.items,
-                           a.n_items, a.A, a.B, a.rec, a.cst, a.perm, a.U, a.counter);
+                           a.n_items, a.rec, a.rec_in, a.cst, a.perm, a.U, a.counter);
     else
         hipLaunchKernelGGL((harm::series_harm_kernel<L, false, W, 0>), a.grid,
                            dim3(harm::kHarmThreads), 0, a.stream, a.inst, a.sph, a.grp, a.items,
-                           a.n_items, a.A, a.B, a.rec, a.cst, a.perm, a.U, a.counter);
+                           a.n_items, a.rec, a.rec_in, a.cst, a.perm, a.U, a.counter);
 }
 
 #else
@@ -32,7 +32,7 @@ template <int L, int W, int C>
 int launch_chunked(const HarmLaunch& a) {
     if (!a.planar) return -1;
     hipLaunchKernelGGL((harm::series_harm_kernel<L, true, W, C>), a.grid, dim3(harm::kHarmThreads),
-                       0, a.stream, a.inst, a.sph, a.grp, a.items, a.n_items, a.A, a.B, a.rec,
+                       0, a.stream, a.inst, a.sph, a.grp, a.items, a.n_items, a.rec, a.rec_in,
                        a.cst, a.perm, a.U, a.counter);
     return 0;
 }

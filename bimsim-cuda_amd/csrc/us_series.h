@@ -121,9 +121,8 @@ struct HarmLaunch {
     const SeriesGroup* grp;
     const SeriesItem* items;
     int n_items;
-    const double2* A;
-    const double2* B;
-    const float4* rec;
+    const float4* rec;      // records times B_l
+    const float4* rec_in;   // records times A_l
     const float* cst;
     const int* perm;
     float2* U;
@@ -145,7 +144,7 @@ struct HarmLaunch {
 constexpr int kHarmVariants = 6;
 constexpr int kHarmDefaultVariant = 1;
 constexpr int harm_variant_waves(int L, int v) {
-    if (v >= 3) return v == 3 ? 6 : v == 4 ? 7 : 8;
+    if (v >= 3) return v == 3 ? 7 : 8;
     return L <= 16 ? 6 + v
            : L <= 24 ? 5 + v
            : L <= 40 ? 4 + v
@@ -210,7 +209,8 @@ struct SeriesPlan {
     double* d_pl = nullptr;       // normalised Legendre values per (m, wave, l)
     double2* d_e = nullptr;       // exp(i m phi_k) per (m, wave)
     double2* d_phi = nullptr;     // wave weights (1/nk) subA exp(i k k.c) per sphere
-    float4* d_rec = nullptr;      // harmonic records per (sphere, m, l)
+    float4* d_rec = nullptr;      // harmonic records per (sphere, m, l), then times B_l
+    float4* d_rec_in = nullptr;   // the same records times A_l
     float* d_cst = nullptr;       // scaled Legendre recurrence constants per (m, l)
     int* d_perm = nullptr;        // pixel orders of the instances with perm_off >= 0
     SeriesGroup* d_grp = nullptr; // sphere groups of the harmonic form

@@ -640,6 +640,46 @@ __global__ __launch_bounds__(kRecThreads) void harm_records_tile_kernel(
                       h64, rec);
 }
 
+// The records times the sphere's coefficients, once both are formed:
+// rec (in place) times B_l, rec_in times A_l, l the record's order; the
+// records of orders below the row or past Nl stay zero.  Rows 0 .. tl - 1
+// of each table (tl / 2 records each for a planar sphere, else tl) hold
+// every record the pixel kernel reads.  Grid: (record blocks, spheres).
+__global__ __launch_bounds__(kRecThreads) void harm_fold_kernel(const SeriesSph* __restrict__ sph,
+                                                                int n_sph,
+                                                                const double2* __restrict__ A,
+                                                                const double2* __restrict__ B,
+                                                                float4* __restrict__ rec,
+                                                                float4* __restrict__ rec_in) {
+    for (int s = blockIdx.y; s < n_sph; s += gridDim.y) {
+        const SeriesSph& SS = sph[s];
+        const int tl = SS.tl;
+        const bool planar = SS.planar != 0;
+        const int width = planar ? tl / 2 : tl;
+        const int n = tl * width;
+        for (int o = blockIdx.x * kRecThreads + (int)threadIdx.x; o < n;
+             o += gridDim.x * kRecThreads) {
+            const int m = o / width, k = o - m * width;
+            const int l = planar ? 2 * k + (m & 1) : k;
+            float4 ob = make_float4(0.f, 0.f, 0.f, 0.f), oa = ob;
+            if (l >= m && l <= SS.s.Nl) {
+                const float4 r = rec[SS.toff + o];
+                const double2 b = B[SS.s.coeff_off + l], a = A[SS.s.coeff_off + l];
+                // U1 = (r.x, r.y) and U2 = (r.z, r.w), each times the coefficient
+                auto mul = [](double x, double y, double2 c) {
+                    return make_float2((float)(x * c.x - y * c.y), (float)(x * c.y + y * c.x));
+                };
+                const float2 b1 = mul(r.x, r.y, b), b2 = mul(r.z, r.w, b);
+                const float2 a1 = mul(r.x, r.y, a), a2 = mul(r.z, r.w, a);
+                ob = make_float4(b1.x, b1.y, b2.x, b2.y);
+                oa = make_float4(a1.x, a1.y, a2.x, a2.y);
+            }
+            rec[SS.toff + o] = ob;
+            rec_in[SS.toff + o] = oa;
+        }
+    }
+}
+
 // The FP32 batch's pre-pass as one launch: Mie coefficients (blocks
 // [0, n_mie), mie_block over spb spheres each, Bessel arrays in dynamic
 // LDS), the harmonic records (the next n_rec blocks, one per (m, instance)
@@ -752,7 +792,7 @@ void series_plan_destroy(SeriesPlan* P) {
     (void)hipFree(P->d_part);
     (void)hipFree(P->d_coef);
     (void)hipFree(P->d_rhat);
-    void* harm[] = {P->d_pl, P->d_e, P->d_phi, P->d_rec, P->d_cst, P->d_h64};
+    void* harm[] = {P->d_pl, P->d_e, P->d_phi, P->d_rec, P->d_rec_in, P->d_cst, P->d_h64};
     for (void* q : harm) (void)hipFree(q);
     *P = SeriesPlan();
 }
@@ -1056,14 +1096,17 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
          (rc = alloc(ctx, &P->d_pl, P->n_pl)) != BIMSIM_OK ||
          (rc = alloc(ctx, &P->d_e, P->n_e)) != BIMSIM_OK ||
          (rc = alloc(ctx, &P->d_phi, std::max<long long>(1, P->n_wave_rec))) != BIMSIM_OK ||
-         (rc = alloc(ctx, &P->d_rec, P->n_rec)) != BIMSIM_OK)) {
+         (rc = alloc(ctx, &P->d_rec, P->n_rec)) != BIMSIM_OK ||
+         (rc = alloc(ctx, &P->d_rec_in, P->n_rec)) != BIMSIM_OK)) {
         series_plan_destroy(P);
         return rc;
     }
     // the records kernel writes only the records the spheres use: the rest
     // of each table stays zero, as the pixel kernel's blocks may read past Nl
-    if (P->harmonic && P->n_rec > 0)
+    if (P->harmonic && P->n_rec > 0) {
         BIMSIM_HIP_CHECK(ctx, hipMemset(P->d_rec, 0, sizeof(float4) * (size_t)P->n_rec));
+        BIMSIM_HIP_CHECK(ctx, hipMemset(P->d_rec_in, 0, sizeof(float4) * (size_t)P->n_rec));
+    }
     if ((rc = upload(ctx, &P->d_inst, batch.inst)) != BIMSIM_OK ||
         (rc = upload(ctx, &P->d_sph, batch.sph)) != BIMSIM_OK ||
         (rc = upload(ctx, &P->d_items, items)) != BIMSIM_OK ||
@@ -1146,13 +1189,23 @@ int series_plan_launch_prepass(bimsim_ctx* ctx, const SeriesPlan& P, const doubl
 namespace {
 
 // One bucket's persistent launch (the counter must be zero).
+// The coefficients into the records (harm_fold_kernel).
+int fold_launch(bimsim_ctx* ctx, const SeriesPlan& P, const double2* A, const double2* B) {
+    const dim3 grid((P.max_tl * P.max_tl / 2 + kRecThreads - 1) / kRecThreads,
+                    std::min(P.n_sph, 65535));
+    hipLaunchKernelGGL(harm_fold_kernel, grid, dim3(kRecThreads), 0, ctx->stream, P.d_sph,
+                       P.n_sph, A, B, P.d_rec, P.d_rec_in);
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return BIMSIM_OK;
+}
+
 int launch_one_bucket(bimsim_ctx* ctx, const SeriesPlan& P, const SeriesPlan::Bucket& K,
                       const double2* A, const double2* B, float2* U) {
     const hipStream_t st = ctx->stream;
     // resident workgroups: the queue keeps them busy until it drains
     const dim3 grid(std::min(K.item_count, P.groups_per_cu * ctx->cu_count));
     const HarmLaunch hl{grid, st, P.d_inst, P.d_sph, P.d_grp, P.d_items + K.item_begin,
-                        K.item_count, A, B, P.d_rec, P.d_cst, P.d_perm, U,
+                        K.item_count, P.d_rec, P.d_rec_in, P.d_cst, P.d_perm, U,
                         P.d_counters + K.counter, K.planar};
     int hv = 0;
     switch (K.L) {
@@ -1187,6 +1240,7 @@ int series_plan_launch_sums(bimsim_ctx* ctx, const SeriesPlan& P, const double2*
                             const double2* B, float2* U) {
     if (P.n_inst == 0 || P.n_sph == 0) return BIMSIM_OK;
     const hipStream_t st = ctx->stream;
+    if (P.harmonic) BIMSIM_TRY(fold_launch(ctx, P, A, B));
     for (const SeriesPlan::Chunk& ch : P.chunks) {
         if (!P.harmonic) {
             const dim3 cgrid((P.max_npix + kThreads - 1) / kThreads, std::min(ch.sph_count, 65535));
@@ -1262,6 +1316,7 @@ int harm_tune(bimsim_ctx* ctx, const std::string& shape, const double* waves, co
     ctx->timing = false;
     int rc = zero_launch(ctx, tmp, sizeof(float2) * (size_t)std::max<long long>(1, field));
     if (rc == BIMSIM_OK) rc = series_plan_launch_waves(ctx, *P, waves);
+    if (rc == BIMSIM_OK) rc = fold_launch(ctx, *P, A, B);
     std::vector<int> chosen;
     for (SeriesPlan::Bucket* K : bks) {
         if (rc != BIMSIM_OK) break;
