@@ -203,7 +203,7 @@ __device__ __forceinline__ void harm_planar_sums(const float4* __restrict__ R, i
         const float4* Ra = R + m * H;
         const float4* Rb = Ra + H;
         const float zar = zr, zai = zi;
-        const float zbr = zr * z1r - zi * z1i, zbi = fmaf(zr, z1i, zi * z1r);
+        const float zbr = fmaf(zr, z1r, -(zi * z1i)), zbi = fmaf(zr, z1i, zi * z1r);
         const int bf = (m >> 1) / KB;   // block of k = m / 2
 #pragma clang loop unroll(full)
         for (int b = 0; b < H / KB; ++b) {
@@ -223,7 +223,7 @@ __device__ __forceinline__ void harm_planar_sums(const float4* __restrict__ R, i
                 ai[le + 1] = fmaf(zbi, rb[j].w, fmaf(zbr, rb[j].y, ai[le + 1]));
             }
         }
-        zr = zbr * z1r - zbi * z1i;
+        zr = fmaf(zbr, z1r, -(zbi * z1i));
         zi = fmaf(zbr, z1i, zbi * z1r);
     }
 }
@@ -249,7 +249,7 @@ __device__ __forceinline__ void harm_planar_chunk(const float4* __restrict__ R, 
         const float4* Ra = R + m * H;
         const float4* Rb = Ra + H;
         const float zar = zr, zai = zi;
-        const float zbr = zr * z1r - zi * z1i, zbi = fmaf(zr, z1i, zi * z1r);
+        const float zbr = fmaf(zr, z1r, -(zi * z1i)), zbi = fmaf(zr, z1i, zi * z1r);
         const int bf = (m >> 1) / KB;   // block of k = m / 2
 #pragma clang loop unroll(full)
         for (int b = 0; b < NB; ++b) {
@@ -271,7 +271,7 @@ __device__ __forceinline__ void harm_planar_chunk(const float4* __restrict__ R, 
                 ai[le + 1] = fmaf(zbi, rb[j].w, fmaf(zbr, rb[j].y, ai[le + 1]));
             }
         }
-        zr = zbr * z1r - zbi * z1i;
+        zr = fmaf(zbr, z1r, -(zbi * z1i));
         zi = fmaf(zbr, z1i, zbi * z1r);
     }
 }
@@ -406,7 +406,7 @@ __global__ __launch_bounds__(kHarmThreads, kWaves) void series_harm_kernel(
                     const float4* Rb = Ra + L;
                     const float pa = pmm, pb = Cb[2 * (m + 1)] * st * pmm;
                     const float zar = zr, zai = zi;
-                    const float zbr = zr * z1r - zi * z1i, zbi = fmaf(zr, z1i, zi * z1r);
+                    const float zbr = fmaf(zr, z1r, -(zi * z1i)), zbi = fmaf(zr, z1i, zi * z1r);
                     const int bf = m >> 2;
                     float qa0 = 0.f, qa1 = 0.f, qb0 = 0.f, qb1 = 0.f;
 #pragma clang loop unroll(full)
@@ -447,7 +447,7 @@ __global__ __launch_bounds__(kHarmThreads, kWaves) void series_harm_kernel(
                     }
                     // the next pair's diagonal value and phase
                     pmm = Cb[2 * (kCS + m + 2)] * st * pb;
-                    zr = zbr * z1r - zbi * z1i;
+                    zr = fmaf(zbr, z1r, -(zbi * z1i));
                     zi = fmaf(zbr, z1i, zbi * z1r);
                 }
                 harm_radial<L>(pass, S, I.kmag, d, inv_d, inv_kmag, trig, ar, ai, acc_r, acc_i);
