@@ -58,20 +58,22 @@ def log(msg):
 
 # ------------------------------------------------------------ work counts
 def harmonic_flops(Nl, planar=False):
-    """FLOPs of the harmonic series kernel per (pixel, sphere).  General form,
+    """FLOPs of the harmonic series kernel per (pixel, sphere), with the
+    coefficients B_l (A_l inside) already in the records.  General form,
     P = (Nl+1)(Nl+2)/2 pairs (m, l >= m): per pair the scaled Legendre step
-    (3), the rotation by e^{i m phi} (2) and the accumulation (8); per order
-    the FP64 radial steps of j_l and y_l (6), B_l h_l (6) and its product with
-    the order sum (8); per row the next diagonal value and phase (8).
-    Planar form (the slice contains the sphere centre and is normal to z, so
-    x = 0 and Pn_lm(0) sits in the records): only the pairs with l + m even,
-    8 each (cos and sin times the two complex records); per row the next
-    phase (6); per order as above (28)."""
+    (3), the rotation by e^{i m phi} (2) and the accumulation (8); per row
+    the next diagonal value and phase (8).  Planar form (the slice contains
+    the sphere centre and is normal to z, so x = 0 and Pn_lm(0) sits in the
+    records): only the pairs with l + m even, 8 each (cos and sin times the
+    two complex records); per row the next phase (6).  Both: per order the
+    FP64 steps of j_l and y_l (6) and the product of j_l + i y_l with the
+    order sum, accumulated (8).  The chunked variants repeat the phase steps
+    of the rows below each chunk; that extra work is not counted."""
     n = Nl + 1
     if planar:
         pairs = sum((n - m + 1) // 2 for m in range(n))
-        return 8 * pairs + 6 * n + 28 * n
-    return 13 * n * (n + 1) // 2 + 28 * n
+        return 8 * pairs + 6 * n + 14 * n
+    return 13 * n * (n + 1) // 2 + 8 * n + 14 * n
 
 
 def is_planar(inst, s):
