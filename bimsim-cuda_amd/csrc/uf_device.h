@@ -128,6 +128,10 @@ __device__ inline dc uf_fast_value(const QuadArg& q, int uR, int vR, int pix, d3
         };
         int l = 2;
         for (; l + kB - 1 <= m; l += kB) {
+            // two orders zeroed by the threshold rule zero every later one
+            // (the recurrence then stays at 0): the rest of the series adds
+            // nothing once that holds on every lane
+            if (__all(j0 == 0.0 && j1 == 0.0)) break;
             double tb[4 * kB];
 #pragma unroll
             for (int x = 0; x < 4 * kB; ++x) tb[x] = T[4 * l + x];
@@ -139,7 +143,8 @@ __device__ inline dc uf_fast_value(const QuadArg& q, int uR, int vR, int pix, d3
                 for (int b = 0; b < kB; ++b) fstep(l + b, tb + 4 * b, false);
             }
         }
-        for (; l <= m; ++l) fstep(l, T + 4 * l, true);
+        if (!(j0 == 0.0 && j1 == 0.0))
+            for (; l <= m; ++l) fstep(l, T + 4 * l, true);
         return dc{((ar * 2.0) * (double)kPIf) * amp, 0.0};
     }
     const double ct = dot(k, r);
@@ -155,6 +160,7 @@ __device__ inline dc uf_fast_value(const QuadArg& q, int uR, int vR, int pix, d3
     };
     int l = 2;
     for (; l + kB - 1 <= m; l += kB) {   // l even: ar, ai, ar, ai
+        if (__all(j0 == 0.0 && j1 == 0.0)) break;   // (as in the flat case)
         double tb[4 * kB];
 #pragma unroll
         for (int x = 0; x < 4 * kB; ++x) tb[x] = T[4 * l + x];
@@ -170,10 +176,11 @@ __device__ inline dc uf_fast_value(const QuadArg& q, int uR, int vR, int pix, d3
             ai = step(l + 3, tb + 12, ai, false);
         }
     }
-    for (; l <= m; ++l) {
-        if (l & 1) ai = step(l, T + 4 * l, ai, true);
-        else ar = step(l, T + 4 * l, ar, true);
-    }
+    if (!(j0 == 0.0 && j1 == 0.0))
+        for (; l <= m; ++l) {
+            if (l & 1) ai = step(l, T + 4 * l, ai, true);
+            else ar = step(l, T + 4 * l, ar, true);
+        }
     return dc{((ar * 2.0) * (double)kPIf) * amp, ((ai * 2.0) * (double)kPIf) * amp};
 }
 
