@@ -142,7 +142,9 @@ struct HarmLaunch {
 // each chunk; they give the same bits as variants 0-2.  The plan's tuner
 // times the variants on the batch's own inputs.
 constexpr int kHarmVariants = 6;
-constexpr int kHarmDefaultVariant = 1;
+// untuned choice: all order sums at the middle budget (general buckets), or
+// chunks of 16 orders at 8 waves per SIMD (planar buckets)
+constexpr int harm_default_variant(bool planar) { return planar ? 4 : 1; }
 constexpr int harm_variant_waves(int L, int v) {
     if (v >= 3) return v == 3 ? 7 : 8;
     return L <= 16 ? 6 + v
@@ -156,6 +158,15 @@ constexpr int harm_variant_waves(int L, int v) {
 constexpr int harm_variant_chunk(int L, int v) {
     return v < 3 ? 0 : v == 5 ? 8 : (L <= 16 ? 8 : 16);
 }
+// Whether the tuner times variant v on a bucket: planar buckets take the
+// chunked variants only (the variants holding all order sums ran 1.3 to 4
+// times slower on every planar bucket measured), general buckets variants
+// 0-2 (which are all that exist for them).
+constexpr bool harm_variant_tuned(int L, int v, bool planar) {
+    return planar ? v >= 3 && v < kHarmVariants
+                  : v >= 0 && v < 3 && harm_variant_waves(L, v) > 0;
+}
+
 // whether variant v runs on a bucket (chunked variants: planar buckets)
 constexpr bool harm_variant_exists(int L, int v, bool planar) {
     return v >= 0 && v < kHarmVariants && harm_variant_waves(L, v) > 0 && (v < 3 || planar);

@@ -969,7 +969,8 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
             const int count = (int)items.size() - begin;
             if (count > 0)
                 ch.buckets.push_back(SeriesPlan::Bucket{L, begin, count, P->n_counters++,
-                                                        all_planar, kHarmDefaultVariant});
+                                                        all_planar,
+                                                        harm_default_variant(all_planar != 0)});
         }
         P->chunks.push_back(ch);
         i0 = i1;
@@ -1323,7 +1324,7 @@ int harm_tune(bimsim_ctx* ctx, const std::string& shape, const double* waves, co
         float best = 1e30f;
         int best_v = K->variant;
         for (int v = 0; v < kHarmVariants && rc == BIMSIM_OK; ++v) {
-            if (!harm_variant_exists(K->L, v, K->planar)) continue;
+            if (!harm_variant_tuned(K->L, v, K->planar)) continue;
             K->variant = v;
             float ms = 0.f;
             for (int r = 0; r < 2 && rc == BIMSIM_OK; ++r) {   // a warm run, then the timed one
