@@ -404,11 +404,15 @@ constexpr int kRecThreads = 256;
 // plan and pixel-sphere tests on the host to build them.
 constexpr long long kPermMaxPixels = 16LL << 20;
 constexpr long long kPermMaxTests = 128LL << 20;
-constexpr int kRecW = 16;         // waves per LDS chunk
-constexpr int kRecL = 128;        // orders per tile
-constexpr int kRecS = 64;         // spheres per tile (16 groups of four)
 constexpr int kRecPer = 2;        // combinations per thread and pass
 
+// NT threads per block, tiles of kRecL orders and kRecS spheres (groups of
+// four), kRecW waves per LDS chunk: 256 / 128 / 64 / 16, or 64 / 64 / 32 /
+// 16 where a row has few combinations (a 256-thread block would leave most
+// of its threads idle; on the sweep's rows of at most 25 orders and five
+// groups the 64-thread form takes 0.82 against 1.02 ms, and chunks of 32
+// waves were slower, 0.99 ms).
+template <int NT, int kRecL, int kRecS, int kRecW>
 __device__ __forceinline__ void harm_records_row(
     int m, int i, const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph,
     const double* __restrict__ pl, const double2* __restrict__ e, const double2* __restrict__ phi,
@@ -431,7 +435,7 @@ __device__ __forceinline__ void harm_records_row(
         for (int j0 = 0; j0 < nlv; j0 += kRecL) {
             const int nlt = min(kRecL, nlv - j0);
             const int combos = nq * nlt;
-            for (int c0 = 0; c0 < combos; c0 += kRecPer * kRecThreads) {
+            for (int c0 = 0; c0 < combos; c0 += kRecPer * NT) {
                 // combination c: order index c % nlt, sphere group c / nlt
                 double acc[kRecPer][4][4];
 #pragma unroll
@@ -443,12 +447,12 @@ __device__ __forceinline__ void harm_records_row(
                 for (int w0 = 0; w0 < I.nk; w0 += kRecW) {
                     const int nw = min(kRecW, I.nk - w0);
                     __syncthreads();   // the previous chunk is consumed
-                    for (int x = t; x < nw * nlt; x += kRecThreads) {
+                    for (int x = t; x < nw * nlt; x += NT) {
                         const int w = x / nlt, jj = x - w * nlt;
                         const int l = m + (j0 + jj) * lstep;
                         sP[w][jj] = Prow[(long long)(l - m) * I.nk + w0 + w];
                     }
-                    for (int x = t; x < nw * nst; x += kRecThreads) {
+                    for (int x = t; x < nw * nst; x += NT) {
                         const int w = x / nst, ss = x - w * nst;
                         sF[w][ss] = phi[sph[I.s_begin + s0 + ss].woff + w0 + w];
                     }
@@ -456,7 +460,7 @@ __device__ __forceinline__ void harm_records_row(
                     __syncthreads();
 #pragma unroll
                     for (int u = 0; u < kRecPer; ++u) {
-                        const int c = c0 + u * kRecThreads + t;
+                        const int c = c0 + u * NT + t;
                         if (c >= combos) continue;
                         const int jj = c % nlt, g = c / nlt;
                         for (int w = 0; w < nw; ++w) {
@@ -477,7 +481,7 @@ __device__ __forceinline__ void harm_records_row(
                 // the records of this pass's combinations
 #pragma unroll
                 for (int u = 0; u < kRecPer; ++u) {
-                    const int c = c0 + u * kRecThreads + t;
+                    const int c = c0 + u * NT + t;
                     if (c >= combos) continue;
                     const int jj = c % nlt, g = c / nlt;
                     const int l = m + (j0 + jj) * lstep;
@@ -624,12 +628,14 @@ __device__ __forceinline__ void harm_records_tile(
     }
 }
 
-__global__ __launch_bounds__(kRecThreads) void harm_records_kernel(
+template <int NT, int kRecL, int kRecS, int kRecW>
+__global__ __launch_bounds__(NT) void harm_records_kernel(
     const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph, int n_inst,
     const double* __restrict__ pl, const double2* __restrict__ e, const double2* __restrict__ phi,
     const double* __restrict__ h64, float4* __restrict__ rec) {
     for (int i = blockIdx.y; i < n_inst; i += gridDim.y)
-        harm_records_row(blockIdx.x, i, insts, sph, pl, e, phi, h64, rec);
+        harm_records_row<NT, kRecL, kRecS, kRecW>(blockIdx.x, i, insts, sph, pl, e, phi, h64,
+                                                  rec);
 }
 
 __global__ __launch_bounds__(kRecThreads) void harm_records_tile_kernel(
@@ -990,6 +996,12 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
             I.e_off = ne;
             ne += (long long)lh * I.nk;
             P->max_lh = std::max(P->max_lh, lh);
+            // combinations (order, group of four spheres) of the instance's row 0
+            bool planar_all = true;
+            for (int t = I.s_begin; t < I.s_begin + I.ns; ++t)
+                planar_all = planar_all && harm_planar(I.q, batch.sph[t].s);
+            const int nlv = planar_all ? (lh + 1) / 2 : lh;
+            P->max_rec_combos = std::max(P->max_rec_combos, nlv * ((I.ns + 3) / 4));
         }
         for (int t = 0; t < P->n_sph; ++t) {
             SeriesSph& S = batch.sph[t];
@@ -1143,8 +1155,14 @@ int series_plan_launch_waves(bimsim_ctx* ctx, const SeriesPlan& P, const double*
         if ((long long)P.max_lh * P.n_inst >= 2LL * ctx->cu_count) {
             // one block per (row, instance) over all its spheres
             const dim3 rgrid(P.max_lh, std::min(P.n_inst, 65535));
-            hipLaunchKernelGGL(harm_records_kernel, rgrid, dim3(kRecThreads), 0, st, P.d_inst,
-                               P.d_sph, P.n_inst, P.d_pl, P.d_e, P.d_phi, P.d_h64, P.d_rec);
+            if (P.max_rec_combos <= 2 * kRecPer * 64)
+                hipLaunchKernelGGL((harm_records_kernel<64, 64, 32, 16>), rgrid, dim3(64), 0, st,
+                                   P.d_inst, P.d_sph, P.n_inst, P.d_pl, P.d_e, P.d_phi, P.d_h64,
+                                   P.d_rec);
+            else
+                hipLaunchKernelGGL((harm_records_kernel<kRecThreads, 128, 64, 16>), rgrid,
+                                   dim3(kRecThreads), 0, st, P.d_inst, P.d_sph, P.n_inst, P.d_pl,
+                                   P.d_e, P.d_phi, P.d_h64, P.d_rec);
         } else {
             // few instances: blocks per (row, four records, sphere)
             const dim3 rgrid(P.max_tl, (P.max_tl + kRecTile - 1) / kRecTile,
