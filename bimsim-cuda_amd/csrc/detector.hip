@@ -319,12 +319,22 @@ __device__ __forceinline__ typename C2<R>::T* fixed_fft(typename C2<R>::T* a,
 // A batch instance's Uf + Us at pixel pix from U = Us (a) and Uf (b), by
 // the rule of the batch's sum kernel: Uf without spheres, else U + Uf
 // outside every sphere (d < a is inside, nfSumUf.cu:31-48).
+// Only the spheres of `near` (bit s: sphere s) are tested when fz.ns <= 64;
+// the others hold no pixel of the caller's rows.
 template <class C>
-__device__ __forceinline__ C fused_sum(const SumFuse& fz, int pix, C a, C b) {
+__device__ __forceinline__ C fused_sum(const SumFuse& fz, int pix, C a, C b,
+                                       unsigned long long near = ~0ull) {
     if (fz.ns == 0) return b;
     const int iv = pix / fz.uR, iu = pix - iv * fz.uR;
     const d3 p = quad_at(fz.q, (double)iu / (double)fz.uR, (double)iv / (double)fz.vR);
+    const bool masked = fz.ns <= 64;
+    unsigned long long left = near;
     for (int s = 0; s < fz.ns; ++s) {
+        if (masked) {
+            if (left == 0) break;
+            s = __builtin_ctzll(left);
+            left &= left - 1;
+        }
         const double4 S = fz.sph[s];
         // |p - c| < a from the squared distance, the square root taken only
         // within a relative 1e-12 of the boundary, where the rounded distance
@@ -356,9 +366,39 @@ __device__ __forceinline__ void rows_fwd_body(
     const int row0 = blk * B;
     const int nb = min(B, nrows - row0);
     if (kSum && f == 0) {
+        // the spheres within their radius of these rows' bounding box (the
+        // box of the four corner pixels holds every pixel of the rows): the
+        // others cannot hold one of their pixels
+        __shared__ unsigned long long s_near;
+        if (threadIdx.x < 64) {
+            bool near = false;
+            if ((int)threadIdx.x < fz.ns && fz.ns <= 64) {
+                const double v0 = (double)row0 / (double)fz.vR;
+                const double v1 = (double)(row0 + nb - 1) / (double)fz.vR;
+                const double u1 = (double)(fz.uR - 1) / (double)fz.uR;
+                const d3 c0 = quad_at(fz.q, 0.0, v0), c1 = quad_at(fz.q, u1, v0);
+                const d3 c2 = quad_at(fz.q, 0.0, v1), c3 = quad_at(fz.q, u1, v1);
+                const double4 S = fz.sph[threadIdx.x];
+                const double cc[3] = {S.x, S.y, S.z};
+                const double q0[3] = {c0.x, c0.y, c0.z}, q1[3] = {c1.x, c1.y, c1.z};
+                const double q2[3] = {c2.x, c2.y, c2.z}, q3[3] = {c3.x, c3.y, c3.z};
+                double d2 = 0.0;
+                for (int x = 0; x < 3; ++x) {
+                    const double lo = fmin(fmin(q0[x], q1[x]), fmin(q2[x], q3[x]));
+                    const double hi = fmax(fmax(q0[x], q1[x]), fmax(q2[x], q3[x]));
+                    const double g = fmax(fmax(lo - cc[x], cc[x] - hi), 0.0);
+                    d2 += g * g;
+                }
+                near = d2 <= S.w * S.w * (1.0 + 1e-9);
+            }
+            const unsigned long long m = __ballot(near);
+            if (threadIdx.x == 0) s_near = m;
+        }
+        __syncthreads();
+        const unsigned long long near = s_near;
         for (int t = threadIdx.x; t < nb * N; t += kThreads) {
             const size_t i = (size_t)row0 * N + t;
-            a[t] = fused_sum(fz, (int)i, U[i], Uf[i]);
+            a[t] = fused_sum(fz, (int)i, U[i], Uf[i], near);
         }
     } else {
         for (int t = threadIdx.x; t < nb * N; t += kThreads) a[t] = F[(size_t)row0 * N + t];
