@@ -297,6 +297,328 @@ __device__ __forceinline__ void harm_planar_chunks(const float4* __restrict__ R,
 }
 
 
+// ---- The planar order sums on the matrix cores ----
+// v_mfma_f32_4x4x1_16b_f32 with its A operand broadcast from one block of
+// four lanes (CBSZ 4, ABID a) computes in every lane
+//   d[i] = fmaf(A[4a + i], b, d[i]),  i = 0..3,
+// A[4a + i] being lane 4a + i's value and b the lane's own: the four FMAs
+// of one record quad and one pixel, rounded as fmaf, in one instruction at
+// twice the issue rate of four VALU FMAs.  A lane keeps its pixel: b is its
+// cos(m phi) or sin(m phi), d four of its order sums.  The instruction
+// holds the SIMD for its 8 cycles (VALU work of other waves does not run
+// beside it), so only the products get faster; the radial factors and the
+// phase steps stay on the VALU, and the tuner takes these variants where
+// they win.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// One row pair (m, m + 1) of the products for n blocks of a chunk, as one
+// asm statement: blocks n - 1 .. 0 in turn, each adding its cos terms
+// (records of slots 2b in ae for row m, in ao for row m + 1, times zar,
+// zbr), then the same blocks their sin terms (slots 2b + 1, times zai,
+// zbi), so that each sum takes its cos term before its sin term as
+// harm_planar_chunk does.  Block b runs while b >= first (the row pair's
+// first block of the chunk; the blocks below the diagonal are skipped by
+// scalar branches, the later ones then too).  The sums stay in place in
+// AGPRs: the MFMAs are written with their accumulators tied, which the
+// compiler's forms of them are not (it then moves the sums through other
+// registers with copies every row).  The compiler does not see into the
+// asm, so the statement keeps its own wait states: two before the first
+// product (a VALU write of an operand just before it), two before the sin
+// terms (a sum's cos term written just before), one after the last (a
+// VALU write of an operand read by the last product).  Slot layouts:
+// packed (both rows in one register, row m + 1 from lane 32) or one
+// register per row.
+#define QUAD_C(b, ce, co)                                                                     \
+    "s_cmp_gt_i32 %[first], " #b "\n\ts_cbranch_scc1 .Lqc%=\n\t"                               \
+    "v_mfma_f32_4x4x1_16b_f32 %[e" #b "], %[ae], %[zar], %[e" #b "] cbsz:4 abid:" #ce "\n\t" \
+    "v_mfma_f32_4x4x1_16b_f32 %[o" #b "], %[ao], %[zbr], %[o" #b "] cbsz:4 abid:" #co "\n\t"
+#define QUAD_S(b, ce, co)                                                                     \
+    "s_cmp_gt_i32 %[first], " #b "\n\ts_cbranch_scc1 .Lqs%=\n\t"                               \
+    "v_mfma_f32_4x4x1_16b_f32 %[e" #b "], %[ae], %[zai], %[e" #b "] cbsz:4 abid:" #ce "\n\t" \
+    "v_mfma_f32_4x4x1_16b_f32 %[o" #b "], %[ao], %[zbi], %[o" #b "] cbsz:4 abid:" #co "\n\t"
+// the first row pair of a chunk (every block; the cos products start the
+// sums: accumulator 0)
+#define QUAD_C0(b, ce, co)                                                                    \
+    "v_mfma_f32_4x4x1_16b_f32 %[e" #b "], %[ae], %[zar], 0 cbsz:4 abid:" #ce "\n\t"           \
+    "v_mfma_f32_4x4x1_16b_f32 %[o" #b "], %[ao], %[zbr], 0 cbsz:4 abid:" #co "\n\t"
+#define QUAD_S0(b, ce, co)                                                                    \
+    "v_mfma_f32_4x4x1_16b_f32 %[e" #b "], %[ae], %[zai], %[e" #b "] cbsz:4 abid:" #ce "\n\t" \
+    "v_mfma_f32_4x4x1_16b_f32 %[o" #b "], %[ao], %[zbi], %[o" #b "] cbsz:4 abid:" #co "\n\t"
+// packed slots: row m's slot s at broadcast block s, row m + 1's at 8 + s
+#define QUAD_CP1 QUAD_C(0, 0, 8)
+#define QUAD_CP2 QUAD_C(1, 2, 10) QUAD_CP1
+#define QUAD_CP3 QUAD_C(2, 4, 12) QUAD_CP2
+#define QUAD_CP4 QUAD_C(3, 6, 14) QUAD_CP3
+#define QUAD_SP1 QUAD_S(0, 1, 9)
+#define QUAD_SP2 QUAD_S(1, 3, 11) QUAD_SP1
+#define QUAD_SP3 QUAD_S(2, 5, 13) QUAD_SP2
+#define QUAD_SP4 QUAD_S(3, 7, 15) QUAD_SP3
+// a register per row: slot s at broadcast block s of each
+#define QUAD_CU1 QUAD_C(0, 0, 0)
+#define QUAD_CU2 QUAD_C(1, 2, 2) QUAD_CU1
+#define QUAD_CU3 QUAD_C(2, 4, 4) QUAD_CU2
+#define QUAD_CU4 QUAD_C(3, 6, 6) QUAD_CU3
+#define QUAD_CU5 QUAD_C(4, 8, 8) QUAD_CU4
+#define QUAD_CU6 QUAD_C(5, 10, 10) QUAD_CU5
+#define QUAD_CU7 QUAD_C(6, 12, 12) QUAD_CU6
+#define QUAD_CU8 QUAD_C(7, 14, 14) QUAD_CU7
+#define QUAD_SU1 QUAD_S(0, 1, 1)
+#define QUAD_SU2 QUAD_S(1, 3, 3) QUAD_SU1
+#define QUAD_SU3 QUAD_S(2, 5, 5) QUAD_SU2
+#define QUAD_SU4 QUAD_S(3, 7, 7) QUAD_SU3
+#define QUAD_SU5 QUAD_S(4, 9, 9) QUAD_SU4
+#define QUAD_SU6 QUAD_S(5, 11, 11) QUAD_SU5
+#define QUAD_SU7 QUAD_S(6, 13, 13) QUAD_SU6
+#define QUAD_SU8 QUAD_S(7, 15, 15) QUAD_SU7
+#define QUAD_CP01 QUAD_C0(0, 0, 8)
+#define QUAD_SP01 QUAD_S0(0, 1, 9)
+#define QUAD_CP02 QUAD_C0(1, 2, 10) QUAD_CP01
+#define QUAD_SP02 QUAD_S0(1, 3, 11) QUAD_SP01
+#define QUAD_CP03 QUAD_C0(2, 4, 12) QUAD_CP02
+#define QUAD_SP03 QUAD_S0(2, 5, 13) QUAD_SP02
+#define QUAD_CP04 QUAD_C0(3, 6, 14) QUAD_CP03
+#define QUAD_SP04 QUAD_S0(3, 7, 15) QUAD_SP03
+#define QUAD_CU01 QUAD_C0(0, 0, 0)
+#define QUAD_SU01 QUAD_S0(0, 1, 1)
+#define QUAD_CU02 QUAD_C0(1, 2, 2) QUAD_CU01
+#define QUAD_SU02 QUAD_S0(1, 3, 3) QUAD_SU01
+#define QUAD_CU03 QUAD_C0(2, 4, 4) QUAD_CU02
+#define QUAD_SU03 QUAD_S0(2, 5, 5) QUAD_SU02
+#define QUAD_CU04 QUAD_C0(3, 6, 6) QUAD_CU03
+#define QUAD_SU04 QUAD_S0(3, 7, 7) QUAD_SU03
+#define QUAD_CU05 QUAD_C0(4, 8, 8) QUAD_CU04
+#define QUAD_SU05 QUAD_S0(4, 9, 9) QUAD_SU04
+#define QUAD_CU06 QUAD_C0(5, 10, 10) QUAD_CU05
+#define QUAD_SU06 QUAD_S0(5, 11, 11) QUAD_SU05
+#define QUAD_CU07 QUAD_C0(6, 12, 12) QUAD_CU06
+#define QUAD_SU07 QUAD_S0(6, 13, 13) QUAD_SU06
+#define QUAD_CU08 QUAD_C0(7, 14, 14) QUAD_CU07
+#define QUAD_SU08 QUAD_S0(7, 15, 15) QUAD_SU07
+#define QUAD_OUT1 [e0] "=a"(E[0]), [o0] "=a"(O[0])
+#define QUAD_OUT2 QUAD_OUT1, [e1] "=a"(E[1]), [o1] "=a"(O[1])
+#define QUAD_OUT3 QUAD_OUT2, [e2] "=a"(E[2]), [o2] "=a"(O[2])
+#define QUAD_OUT4 QUAD_OUT3, [e3] "=a"(E[3]), [o3] "=a"(O[3])
+#define QUAD_OUT5 QUAD_OUT4, [e4] "=a"(E[4]), [o4] "=a"(O[4])
+#define QUAD_OUT6 QUAD_OUT5, [e5] "=a"(E[5]), [o5] "=a"(O[5])
+#define QUAD_OUT7 QUAD_OUT6, [e6] "=a"(E[6]), [o6] "=a"(O[6])
+#define QUAD_OUT8 QUAD_OUT7, [e7] "=a"(E[7]), [o7] "=a"(O[7])
+#define QUAD_OPS1 [e0] "+a"(E[0]), [o0] "+a"(O[0])
+#define QUAD_OPS2 QUAD_OPS1, [e1] "+a"(E[1]), [o1] "+a"(O[1])
+#define QUAD_OPS3 QUAD_OPS2, [e2] "+a"(E[2]), [o2] "+a"(O[2])
+#define QUAD_OPS4 QUAD_OPS3, [e3] "+a"(E[3]), [o3] "+a"(O[3])
+#define QUAD_OPS5 QUAD_OPS4, [e4] "+a"(E[4]), [o4] "+a"(O[4])
+#define QUAD_OPS6 QUAD_OPS5, [e5] "+a"(E[5]), [o5] "+a"(O[5])
+#define QUAD_OPS7 QUAD_OPS6, [e6] "+a"(E[6]), [o6] "+a"(O[6])
+#define QUAD_OPS8 QUAD_OPS7, [e7] "+a"(E[7]), [o7] "+a"(O[7])
+
+template <int N, bool kPacked>
+struct QuadRow;
+#define QUAD_ROW(N, P, CASC_C, CASC_S, SEED_C, SEED_S)                                           \
+    template <>                                                                                  \
+    struct QuadRow<N, P> {                                                                       \
+        __device__ static __forceinline__ void run(f32x4* E, f32x4* O, float ae, float ao,       \
+                                                   float zar, float zai, float zbr, float zbi,   \
+                                                   int first) {                                  \
+            asm volatile("s_nop 1\n\t" CASC_C ".Lqc%=:\n\ts_nop 1\n\t" CASC_S ".Lqs%=:\n\ts_nop 0" \
+                         : QUAD_OPS##N                                                           \
+                         : [ae] "v"(ae), [ao] "v"(ao), [zar] "v"(zar), [zai] "v"(zai),           \
+                           [zbr] "v"(zbr), [zbi] "v"(zbi), [first] "s"(first));                  \
+        }                                                                                        \
+        /* the chunk's first row pair (rows 0, 1: every block) */                                \
+        __device__ static __forceinline__ void seed(f32x4* E, f32x4* O, float ae, float ao,      \
+                                                    float zar, float zai, float zbr, float zbi) { \
+            asm volatile("s_nop 1\n\t" SEED_C "s_nop 1\n\t" SEED_S "s_nop 0"                   \
+                         : QUAD_OUT##N                                                           \
+                         : [ae] "v"(ae), [ao] "v"(ao), [zar] "v"(zar), [zai] "v"(zai),           \
+                           [zbr] "v"(zbr), [zbi] "v"(zbi));                                      \
+        }                                                                                        \
+        /* wait states between the sums' last products and their first read */                 \
+        __device__ static __forceinline__ void settle(f32x4* E, f32x4* O) {                      \
+            asm volatile("s_nop 7" : QUAD_OPS##N);                                                \
+        }                                                                                        \
+    };
+QUAD_ROW(1, true, QUAD_CP1, QUAD_SP1, QUAD_CP01, QUAD_SP01)
+QUAD_ROW(2, true, QUAD_CP2, QUAD_SP2, QUAD_CP02, QUAD_SP02)
+QUAD_ROW(3, true, QUAD_CP3, QUAD_SP3, QUAD_CP03, QUAD_SP03)
+QUAD_ROW(4, true, QUAD_CP4, QUAD_SP4, QUAD_CP04, QUAD_SP04)
+QUAD_ROW(1, false, QUAD_CU1, QUAD_SU1, QUAD_CU01, QUAD_SU01)
+QUAD_ROW(2, false, QUAD_CU2, QUAD_SU2, QUAD_CU02, QUAD_SU02)
+QUAD_ROW(3, false, QUAD_CU3, QUAD_SU3, QUAD_CU03, QUAD_SU03)
+QUAD_ROW(4, false, QUAD_CU4, QUAD_SU4, QUAD_CU04, QUAD_SU04)
+QUAD_ROW(5, false, QUAD_CU5, QUAD_SU5, QUAD_CU05, QUAD_SU05)
+QUAD_ROW(6, false, QUAD_CU6, QUAD_SU6, QUAD_CU06, QUAD_SU06)
+QUAD_ROW(7, false, QUAD_CU7, QUAD_SU7, QUAD_CU07, QUAD_SU07)
+QUAD_ROW(8, false, QUAD_CU8, QUAD_SU8, QUAD_CU08, QUAD_SU08)
+#undef QUAD_ROW
+#undef QUAD_C
+#undef QUAD_S
+#undef QUAD_C0
+#undef QUAD_S0
+
+// Record operands of a chunk of C orders.  Row m's part of the chunk is
+// F = 2C floats: float4 (U1, U2) per order l = L0 + (m & 1) + 2k.  Its
+// quads are slots s = 2b + sigma, b the block of the orders k = 2b, 2b + 1
+// (l and l + 2 of the row's parity), sigma 0 for the cos records (U1 of
+// both orders), 1 for the sin records (U2); slot s holds floats
+// 4 (2b + i/2) + 2 sigma + (i & 1), i = 0..3, in lanes 4s + i.  Rows m
+// and m + 1 load together, into one register (row m + 1 from lane 32) when
+// F <= 32, else into NV registers each.
+template <int L, int C>
+struct QuadRegs {
+    static constexpr int F = 2 * C;
+    static constexpr bool kPacked = F <= 32;
+    static constexpr int NV = kPacked ? 1 : (F + 63) / 64;
+    static constexpr int NR = kPacked ? 1 : 2 * NV;   // registers per row pair
+    // register and broadcast block of slot (row r of the pair, block b, sigma)
+    static constexpr int reg(int r, int b, int sg) {
+        return kPacked ? 0 : r * NV + (2 * b + sg) / 16;
+    }
+    static constexpr int abid(int r, int b, int sg) {
+        return kPacked ? 8 * r + 2 * b + sg : (2 * b + sg) % 16;
+    }
+};
+
+// The byte offsets (from row m's part of the chunk) that a lane loads into
+// each register of a row pair; lanes past F load the part's first float,
+// which no slot broadcasts.
+template <int L, int C>
+__device__ __forceinline__ void quad_offsets(int lane, int* off) {
+    using G = QuadRegs<L, C>;
+    auto perm = [](int u) {
+        return 4 * (2 * (u >> 3) + ((u & 3) >> 1)) + 2 * ((u >> 2) & 1) + (u & 1);
+    };
+    if constexpr (G::kPacked) {
+        const int r = lane >> 5, u = lane & 31;
+        off[0] = 4 * (u < G::F ? r * 2 * L + perm(u) : 0);
+    } else {
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int v = 0; v < G::NV; ++v) {
+                const int u = 64 * v + lane;
+                off[r * G::NV + v] = 4 * (u < G::F ? r * 2 * L + perm(u) : 0);
+            }
+    }
+}
+
+// The record table of one sphere (L x L float4) as a buffer resource: the
+// row offset of a load is a scalar operand, so a row's loads take no VALU
+// address arithmetic (and a load past the table returns zero).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t quad_table(const float* Rf, int L) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(Rf), 0, L * L * 16, 0x00020000);
+}
+
+template <int NR>
+__device__ __forceinline__ void quad_load(__amdgpu_buffer_rsrc_t rs, const int* off, int row_bytes,
+                                          float* a) {
+#pragma unroll
+    for (int q = 0; q < NR; ++q)
+        a[q] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off[q], row_bytes, 0));
+}
+
+// The blocks of a row pair in asm statements of up to eight blocks each
+// (group k: blocks 8k .. of the chunk; its registers: row m's register k
+// and row m + 1's, or the one packed register).  Block gb of the bucket
+// (= L0/4 + b) runs from row 4 gb on, as in harm_planar_chunk; the blocks
+// that it skips past Nl run here on the zero records the fold writes
+// there, which leave the sums as they are.
+template <int L, int C, int L0, int k, bool kSeed>
+__device__ __forceinline__ void quad_groups(const float* a, float zar, float zai, float zbr,
+                                            float zbi, int m, f32x4* De, f32x4* Do) {
+    constexpr int Qv = (L0 + C <= L ? C : L - L0) / 4;
+    if constexpr (8 * k < Qv) {
+        using G = QuadRegs<L, C>;
+        constexpr int n = Qv - 8 * k < 8 ? Qv - 8 * k : 8;
+        const float ae = a[G::kPacked ? 0 : k], ao = a[G::kPacked ? 0 : G::NV + k];
+        if constexpr (kSeed) {
+            QuadRow<n, G::kPacked>::seed(De + 8 * k, Do + 8 * k, ae, ao, zar, zai, zbr, zbi);
+        } else {
+            const int first = (m >> 2) - (L0 / 4 + 8 * k);
+            QuadRow<n, G::kPacked>::run(De + 8 * k, Do + 8 * k, ae, ao, zar, zai, zbr, zbi,
+                                        first);
+        }
+        quad_groups<L, C, L0, k + 1, kSeed>(a, zar, zai, zbr, zbi, m, De, Do);
+    }
+}
+
+template <int L, int C, int L0, int k>
+__device__ __forceinline__ void quad_settle(f32x4* De, f32x4* Do) {
+    constexpr int Qv = (L0 + C <= L ? C : L - L0) / 4;
+    if constexpr (8 * k < Qv) {
+        constexpr int n = Qv - 8 * k < 8 ? Qv - 8 * k : 8;
+        QuadRow<n, QuadRegs<L, C>::kPacked>::settle(De + 8 * k, Do + 8 * k);
+        quad_settle<L, C, L0, k + 1>(De, Do);
+    }
+}
+
+// harm_planar_chunk on the matrix cores: De[b] holds the sums of orders
+// L0 + 4b and L0 + 4b + 2 (re, im, re, im), Do[b] those of L0 + 4b + 1 and
+// L0 + 4b + 3.  The next row pair's records load while this one's
+// products run.
+template <int L, int C, int L0>
+__device__ __forceinline__ void quad_chunk(__amdgpu_buffer_rsrc_t rs, const int* off, int nl1,
+                                           float z1r, float z1i, f32x4* De, f32x4* Do) {
+    using G = QuadRegs<L, C>;
+    static_assert(!G::kPacked || C <= 16, "packed record registers hold four blocks");
+    const int mend = min(nl1, L0 + C);
+    // bytes of row m's part of the chunk
+    auto row = [](int m) { return 4 * (m * (2 * L) + 2 * L0); };
+    float a0[G::NR], a1[G::NR];
+    quad_load<G::NR>(rs, off, row(0), a0);
+    // (a second register set for alternate row pairs, instead of the copy
+    // below, doubled the asm and raised the register count)
+    // rows 0 and 1 (every block runs there, and starts its sums): phases
+    // 1 and e^{i phi}
+    quad_load<G::NR>(rs, off, row(2), a1);
+    quad_groups<L, C, L0, 0, true>(a0, 1.f, 0.f, z1r, z1i, 0, De, Do);
+    float zr = fmaf(z1r, z1r, -(z1i * z1i)), zi = fmaf(z1r, z1i, z1i * z1r);
+#pragma clang loop unroll(disable)
+    for (int m = 2; m < mend; m += 2) {
+#pragma unroll
+        for (int q = 0; q < G::NR; ++q) a0[q] = a1[q];
+        quad_load<G::NR>(rs, off, row(m + 2), a1);
+        const float zar = zr, zai = zi;
+        const float zbr = fmaf(zr, z1r, -(zi * z1i)), zbi = fmaf(zr, z1i, zi * z1r);
+        quad_groups<L, C, L0, 0, false>(a0, zar, zai, zbr, zbi, m, De, Do);
+        zr = fmaf(zbr, z1r, -(zbi * z1i));
+        zi = fmaf(zbr, z1i, zbi * z1r);
+    }
+    quad_settle<L, C, L0, 0>(De, Do);
+}
+
+// The radial factors of the chunk's orders, block by block (orders
+// L0 + 4b .. L0 + 4b + 3 from De[b] and Do[b]), the recurrences carried
+// as in harm_radial_range over the whole chunk.
+template <int L, int C, int L0, int b>
+__device__ __forceinline__ void quad_radial(const f32x4* De, const f32x4* Do, int pass,
+                                            RadialState& st, const SphereDev& S, float& acc_r,
+                                            float& acc_i) {
+    constexpr int Qv = (L0 + C <= L ? C : L - L0) / 4;
+    if constexpr (b < Qv) {
+        if (L0 + 4 * b > S.Nl) return;
+        const float ar[4] = {De[b][0], Do[b][0], De[b][2], Do[b][2]};
+        const float ai[4] = {De[b][1], Do[b][1], De[b][3], Do[b][3]};
+        harm_radial_range<L0 + 4 * b, 4>(pass, st, S, ar, ai, acc_r, acc_i);
+        quad_radial<L, C, L0, b + 1>(De, Do, pass, st, S, acc_r, acc_i);
+    }
+}
+
+// harm_planar_chunks on the matrix cores: chunk c and the ones after it.
+template <int L, int C, int c>
+__device__ __forceinline__ void quad_chunks(__amdgpu_buffer_rsrc_t rs, const int* off,
+                                            const SphereDev& S, float z1r, float z1i, int pass,
+                                            RadialState& st, float& acc_r, float& acc_i) {
+    if constexpr (c * C < L) {
+        const int nl1 = S.Nl + 1;
+        if (c * C >= nl1) return;
+        constexpr int Q = C / 4;
+        f32x4 De[Q], Do[Q];   // set by the chunk's first row pair
+        quad_chunk<L, C, c * C>(rs, off, nl1, z1r, z1i, De, Do);
+        quad_radial<L, C, c * C, 0>(De, Do, pass, st, S, acc_r, acc_i);
+        quad_chunks<L, C, c + 1>(rs, off, S, z1r, z1i, pass, st, acc_r, acc_i);
+    }
+}
+
 // Per-pixel harmonic sum, persistent over (instance, pixel block) items.
 // Row m of the order sums runs the scaled recurrence
 //   q_mm = Pn_mm,  q_lm = 2x q_{l-1,m} - c_lm q_{l-2,m}
@@ -310,15 +632,20 @@ __device__ __forceinline__ void harm_planar_chunks(const float4* __restrict__ R,
 // rec holds the records times B_l, rec_in the same times A_l: a wave with
 // pixels outside the sphere sums the first (pass 0), one with pixels inside
 // the second (pass 1); a wave with both does both passes, each lane
-// keeping its own.
-template <int L, bool kPlanar, int kWaves, int kChunk>
+// keeping its own.  kMfma (planar buckets, orders in chunks of kChunk):
+// the order sums on the matrix cores (quad_chunks), which give the same
+// bits as the VALU chunks.
+template <int L, bool kPlanar, int kWaves, int kChunk, bool kMfma = false>
 __global__ __launch_bounds__(kHarmThreads, kWaves) void series_harm_kernel(
     const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph,
     const SeriesGroup* __restrict__ grps, const SeriesItem* __restrict__ items, int n_items,
     const float4* __restrict__ rec, const float4* __restrict__ rec_in,
     const float* __restrict__ cst, const int* __restrict__ perm, float2* __restrict__ U,
     int* __restrict__ counter) {
+    static_assert(!kMfma || (kPlanar && kChunk > 0), "matrix-core sums: planar chunks only");
     __shared__ int s_item;
+    int qoff[QuadRegs<L, (kChunk > 0 ? kChunk : 4)>::NR];
+    if constexpr (kMfma) quad_offsets<L, kChunk>((int)(threadIdx.x & 63), qoff);
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
         __syncthreads();
@@ -328,8 +655,12 @@ __global__ __launch_bounds__(kHarmThreads, kWaves) void series_harm_kernel(
         const SeriesItem item = items[it];
         const SeriesInst& I = insts[item.inst];
         const int npix = I.uR * I.vR;
-        const int j = item.pix0 + (int)threadIdx.x;
-        if (j >= npix) continue;
+        const int j0 = item.pix0 + (int)threadIdx.x;
+        // lanes past the instance's pixels repeat its last one and store
+        // nothing: every lane stays active (the matrix-core sums broadcast
+        // record operands from lanes of the wave)
+        const bool live = j0 < npix;
+        const int j = live ? j0 : npix - 1;
         // pixels in the instance's order of uniform waves (inside each
         // sphere, then outside every sphere), so that a wave takes one
         // radial path
@@ -351,7 +682,19 @@ __global__ __launch_bounds__(kHarmThreads, kWaves) void series_harm_kernel(
             r = centre ? d3{0.0, 0.0, 0.0} : scale(r, inv_d);
             const bool inside = d <= S.a;
             const int pass0 = __any(!inside) ? 0 : 1, pass1 = __any(inside) ? 1 : 0;
-            if constexpr (kChunk > 0) {
+            if constexpr (kMfma) {
+                const RadialTrig trig = harm_trig(S, I.kmag, d);
+                RadialState st = harm_radial_init(S, I.kmag, d, inv_d, inv_kmag, trig);
+                if (pass0 == 0)
+                    quad_chunks<L, kChunk, 0>(
+                        quad_table(reinterpret_cast<const float*>(rec + SS.toff), L), qoff, S,
+                        (float)r.x, (float)r.y, 0, st, acc_r, acc_i);
+                if (pass1 == 1)
+                    quad_chunks<L, kChunk, 0>(
+                        quad_table(reinterpret_cast<const float*>(rec_in + SS.toff), L), qoff, S,
+                        (float)r.x, (float)r.y, 1, st, acc_r, acc_i);
+                continue;
+            } else if constexpr (kChunk > 0) {
                 // planar buckets only; orders in chunks of kChunk
                 const RadialTrig trig = harm_trig(S, I.kmag, d);
                 RadialState st = harm_radial_init(S, I.kmag, d, inv_d, inv_kmag, trig);
@@ -458,10 +801,12 @@ __global__ __launch_bounds__(kHarmThreads, kWaves) void series_harm_kernel(
             if (pass0 == 0) sums(rec, 0);
             if (pass1 == 1) sums(rec_in, 1);
         }
-        float2* o = U + I.u_off + pix;
-        float2 v = make_float2(0.f, 0.f);
-        if (G.accumulate) v = *o;
-        *o = make_float2(v.x + acc_r, v.y + acc_i);
+        if (live) {
+            float2* o = U + I.u_off + pix;
+            float2 v = make_float2(0.f, 0.f);
+            if (G.accumulate) v = *o;
+            *o = make_float2(v.x + acc_r, v.y + acc_i);
+        }
     }
 }
 

@@ -135,17 +135,32 @@ struct HarmLaunch {
 // Variants 0-2 hold the order sums of all L orders at once (2L VGPRs), so
 // the larger budgets spill some values to scratch; more waves in flight
 // hide the latency of the FP64 radial chains and of the record loads
-// better.  Variants 3-5 exist for planar buckets only: they sum the orders
+// better.  Variants 3-8 exist for planar buckets only: they sum the orders
 // in chunks of harm_variant_chunk(L, v) (each chunk's rows of records, then
 // its radial factors, the recurrences carried from chunk to chunk), which
 // holds 2C sums instead of 2L and adds the phase steps of the rows below
-// each chunk; they give the same bits as variants 0-2.  The plan's tuner
-// times the variants on the batch's own inputs.
-constexpr int kHarmVariants = 6;
+// each chunk; 3-5 form the sums with VALU FMAs, 6-8 on the matrix cores
+// (harm_variant_mfma).  All give the same bits as variants 0-2.  The plan's
+// tuner times the variants on the batch's own inputs.
+constexpr int kHarmVariants = 9;
 // untuned choice: all order sums at the middle budget (general buckets), or
 // chunks of 16 orders at 8 waves per SIMD (planar buckets)
 constexpr int harm_default_variant(bool planar) { return planar ? 4 : 1; }
+constexpr bool harm_variant_mfma(int v) { return v >= 6; }
+constexpr int harm_variant_chunk(int L, int v) {
+    return v < 3 ? 0 : v == 5 ? 8 : v == 6 ? 16 : v == 7 ? 32 : v == 8 ? L / 2 : (L <= 16 ? 8 : 16);
+}
 constexpr int harm_variant_waves(int L, int v) {
+    if (v >= 6) {
+        // the matrix-core chunks: 16, 32 or L/2 orders (2C AGPRs), none that
+        // repeats another variant's chunk (all orders at once ran slowest on
+        // every bucket, at two or three waves per SIMD).  The budgets are the
+        // largest wave counts that spill nothing.
+        if (v == 7 && L <= 32) return 0;
+        if (v == 8 && (L <= 16 || L == 32 || L == 64)) return 0;
+        const int c = harm_variant_chunk(L, v);
+        return c <= 12 ? 6 : c <= 16 ? 5 : c <= 24 ? 4 : 3;
+    }
     if (v >= 3) return v == 3 ? 7 : 8;
     return L <= 16 ? 6 + v
            : L <= 24 ? 5 + v
@@ -155,15 +170,12 @@ constexpr int harm_variant_waves(int L, int v) {
            : L <= 80 ? (v < 2 ? 2 + v : 0)
                      : (v < 2 ? 1 + v : 0);
 }
-constexpr int harm_variant_chunk(int L, int v) {
-    return v < 3 ? 0 : v == 5 ? 8 : (L <= 16 ? 8 : 16);
-}
 // Whether the tuner times variant v on a bucket: planar buckets take the
 // chunked variants only (the variants holding all order sums ran 1.3 to 4
 // times slower on every planar bucket measured), general buckets variants
 // 0-2 (which are all that exist for them).
 constexpr bool harm_variant_tuned(int L, int v, bool planar) {
-    return planar ? v >= 3 && v < kHarmVariants
+    return planar ? v >= 3 && v < kHarmVariants && harm_variant_waves(L, v) > 0
                   : v >= 0 && v < 3 && harm_variant_waves(L, v) > 0;
 }
 
@@ -172,16 +184,20 @@ constexpr bool harm_variant_exists(int L, int v, bool planar) {
     return v >= 0 && v < kHarmVariants && harm_variant_waves(L, v) > 0 && (v < 3 || planar);
 }
 
-// Launches bucket L in variant v (defined per bucket in us_harm.hip, in two
-// objects: the chunked variants are built with machine LICM off, the others
-// with it); returns 0, or -1 for a variant that does not exist.
+// Launches bucket L in variant v (defined per bucket in us_harm.hip, in
+// three objects: the chunked variants are built with machine LICM off, the
+// others with it); returns 0, or -1 for a variant that does not exist.
 template <int L>
 int harm_launch_full(const HarmLaunch& a, int variant);
 template <int L>
 int harm_launch_chunked(const HarmLaunch& a, int variant);
 template <int L>
+int harm_launch_mfma(const HarmLaunch& a, int variant);
+template <int L>
 inline int harm_launch(const HarmLaunch& a, int variant) {
-    return variant < 3 ? harm_launch_full<L>(a, variant) : harm_launch_chunked<L>(a, variant);
+    return variant < 3   ? harm_launch_full<L>(a, variant)
+           : variant < 6 ? harm_launch_chunked<L>(a, variant)
+                         : harm_launch_mfma<L>(a, variant);
 }
 
 struct SeriesPlan {

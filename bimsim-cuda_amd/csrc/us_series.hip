@@ -1344,15 +1344,19 @@ int harm_tune(bimsim_ctx* ctx, const std::string& shape, const double* waves, co
         for (int v = 0; v < kHarmVariants && rc == BIMSIM_OK; ++v) {
             if (!harm_variant_tuned(K->L, v, K->planar)) continue;
             K->variant = v;
-            float ms = 0.f;
-            for (int r = 0; r < 2 && rc == BIMSIM_OK; ++r) {   // a warm run, then the timed one
+            // a warm run, then the better of two timed ones (variants a few
+            // per cent apart were ranked wrongly by a single run)
+            float ms = 1e30f;
+            for (int r = 0; r < 3 && rc == BIMSIM_OK; ++r) {
                 rc = zero_launch(ctx, P->d_counters + K->counter, sizeof(int));
                 if (rc != BIMSIM_OK) break;
                 (void)hipEventRecord(e0, ctx->stream);
                 rc = launch_one_bucket(ctx, *P, *K, A, B, tmp);
                 (void)hipEventRecord(e1, ctx->stream);
                 (void)hipEventSynchronize(e1);
-                (void)hipEventElapsedTime(&ms, e0, e1);
+                float t = 0.f;
+                (void)hipEventElapsedTime(&t, e0, e1);
+                if (r > 0) ms = std::min(ms, t);
             }
             if (rc == BIMSIM_OK && ms < best) {
                 best = ms;
@@ -1375,7 +1379,8 @@ int harm_tune(bimsim_ctx* ctx, const std::string& shape, const double* waves, co
             msg += " L" + std::to_string(bks[i]->L) + ":" +
                    std::to_string(harm_variant_waves(bks[i]->L, chosen[i])) +
                    (chosen[i] >= 3 ? "/c" + std::to_string(harm_variant_chunk(bks[i]->L, chosen[i]))
-                                   : std::string());
+                                   : std::string()) +
+                   (harm_variant_mfma(chosen[i]) ? "m" : "");
         std::fprintf(stderr, "bimsim: harmonic waves per SIMD%s\n", msg.c_str());
     }
     return BIMSIM_OK;

@@ -164,12 +164,15 @@ def test_mixed_planar_and_off_plane_spheres(pkg, ctx, orc, W):
     assert normrel(got, ref) <= 2e-5
 
 
-@pytest.mark.parametrize("variant", [0, 3, 4, 5])
+@pytest.mark.parametrize("variant", [0, 3, 4, 5, 6, 7, 8])
 def test_register_variants_give_the_same_bits(pkg, W, variant, monkeypatch):
     """Every register-budget variant of the harmonic kernel, the chunked
-    planar ones (orders summed in chunks of 16 or 8) included, adds the
-    same terms in the same order: the images equal the default variant's
-    bit for bit, on a planar sweep whose orders span several buckets."""
+    planar ones (orders summed in chunks of 16 or 8 by VALU FMAs, or in
+    chunks of 16, 32 or half the orders on the matrix cores, whose FP32 products
+    round as fmaf) included, adds the same terms in the same order: the
+    images equal the default variant's bit for bit, on a planar sweep whose
+    orders span several buckets and whose 48 x 48 images end in a partial
+    pixel block."""
     import bimsim_amd.pipeline as P
     insts = W.config4(n_nu=256, N=48, ns=4, nk=24, stride=37)
     assert max(s["Nl"] for i in insts for s in i["spheres"]) > 24
