@@ -62,7 +62,17 @@ struct RadialState {
     int mode;          // 0 outside the sphere, 1 inside, 2 at the centre (l = 0 only)
     dc cj0, cj1, cinv;
     double kx;         // kd (outside) or Re knd (inside)
+    int kfl;           // floor(kx), at most 2^30: order l is past kx where l > kfl
 };
+
+// whether the condition holds on some lane of the wave: the ballot of the
+// lane mask itself (HIP's __any(int) turns the mask into 0/1 values and
+// compares them, two VALU instructions)
+__device__ __forceinline__ bool wave_any(bool b) { return __builtin_amdgcn_ballot_w64(b) != 0; }
+
+__device__ __forceinline__ int harm_floor_order(double kx) {
+    return kx < 1073741824.0 ? (int)kx : 1073741824;
+}
 
 __device__ __forceinline__ RadialState harm_radial_init(const SphereDev& S, double kmag, double d,
                                                         double inv_d, double inv_kmag,
@@ -71,6 +81,7 @@ __device__ __forceinline__ RadialState harm_radial_init(const SphereDev& S, doub
     if (d <= S.a) {
         const dc knd = cscl(dc{S.n[0], S.n[1]}, kmag * d);
         st.kx = knd.r;
+        st.kfl = harm_floor_order(knd.r);
         // calc_Ui returns A_0 for every wave at the centre: only l = 0
         st.mode = knd.r < (double)kEpsFloat ? 2 : 1;
         // init_jc / shift_jc (csin, ccos as csin_r / ccos_r)
@@ -82,6 +93,7 @@ __device__ __forceinline__ RadialState harm_radial_init(const SphereDev& S, doub
     }
     st.mode = 0;
     st.kx = kmag * d;
+    st.kfl = harm_floor_order(st.kx);
     const double inv = inv_d * inv_kmag;
     const double sn = T.t0, cs = T.t1;
     const double j0 = sn * inv, y0 = -(cs * inv);
@@ -89,6 +101,64 @@ __device__ __forceinline__ RadialState harm_radial_init(const SphereDev& S, doub
     st.cj0 = dc{j0, y0};
     st.cj1 = dc{fma(j0, inv, -(cs * inv)), fma(y0, inv, -(sn * inv))};
     return st;
+}
+
+// The outside-sphere steps of harm_radial_range.  The recurrences keep
+// even orders in (je, ye) and odd ones in (jo, yo), each step overwriting
+// the order two below (the registers of a skipped order then stay as they
+// are, with no moves).  kCheck: the threshold rules (shift_sbesselj /
+// shift_sbessely, spherical_bessel.h:107-141: where l > kd, j outside
+// [kJMin, kJMax] becomes 0; y below kYMin, or positive past kd, becomes 0
+// with the order below it), on ranges where some lane of the wave is past
+// kd at one of their orders; their conditions are combined bitwise (no
+// branches per lane), and y's selects run only where a lane needs them (in
+// practice none).
+template <int L0, int C, bool kCheck>
+__device__ __forceinline__ void harm_radial_out(RadialState& st, int nl1, const float* ar,
+                                                const float* ai, float& acc_r, float& acc_i) {
+    static_assert(L0 % 2 == 0 && C % 2 == 0, "ranges start at even orders");
+    const double inv = st.cinv.r;
+    double je = st.cj0.r, ye = st.cj0.i, jo = st.cj1.r, yo = st.cj1.i;
+#pragma clang loop unroll(full)
+    for (int i = 0; i < C; ++i) {
+        const int l = L0 + i;
+        if (l >= nl1) continue;
+        const bool even = (l & 1) == 0;
+        double& jn = even ? je : jo;   // order l - 2, then l
+        double& yn = even ? ye : yo;
+        double& jp = even ? jo : je;   // order l - 1
+        double& yp = even ? yo : ye;
+        if (l >= 2) {
+            const double t = (double)(2 * l - 1) * inv;
+            jn = fma(t, jp, -jn);
+            yn = fma(t, yp, -yn);
+            if constexpr (kCheck) {
+                // j: a compare and a select on every order of the range (a
+                // branch around them cost more in register moves at its
+                // merge); y: the lane masks of the compares combined as
+                // ballots (scalar), the selects in a branch of their own
+                const bool past = l > st.kfl;
+                jn = past & ((jn < kJMin) | (jn > kJMax)) ? 0.0 : jn;
+                const bool ylo = yn < kYMin, ypos = yn > 0.0;
+                const uint64_t yz = __builtin_amdgcn_ballot_w64(ylo) |
+                                    (__builtin_amdgcn_ballot_w64(past) &
+                                     __builtin_amdgcn_ballot_w64(ypos));
+                if (yz != 0) {
+                    // (kept a branch: the compiler would run these selects
+                    // on every order)
+                    asm volatile("");
+                    const bool z = ylo | (past & ypos);
+                    yn = z ? 0.0 : yn;
+                    yp = z ? 0.0 : yp;
+                }
+            }
+        }
+        const float fr = (float)jn, fi = (float)yn;
+        acc_r = fmaf(fr, ar[i], fmaf(-fi, ai[i], acc_r));
+        acc_i = fmaf(fr, ai[i], fmaf(fi, ar[i], acc_i));
+    }
+    st.cj0 = dc{je, ye};
+    st.cj1 = dc{jo, yo};
 }
 
 // Orders L0 .. L0 + C - 1 (those <= Nl) of the radial sum; ar / ai hold
@@ -142,40 +212,12 @@ __device__ __forceinline__ void harm_radial_range(int pass, RadialState& st, con
         return;
     }
     if (pass != 0) return;
-    const double inv = st.cinv.r;
-    double j0 = st.cj0.r, y0 = st.cj0.i, j1 = st.cj1.r, y1 = st.cj1.i;
-#pragma clang loop unroll(full)
-    for (int i = 0; i < C; ++i) {
-        const int l = L0 + i;
-        if (l >= nl1) continue;
-        double jl = l == 0 ? j0 : j1, yl = l == 0 ? y0 : y1;
-        if (l >= 2) {
-            const double t = (double)(2 * l - 1) * inv;
-            double jn = fma(t, j1, -j0);
-            double yn = fma(t, y1, -y0);
-            const bool past = (double)l > st.kx;
-            if (__any(past)) {
-                if (past && (jn < kJMin || jn > kJMax)) jn = 0.0;
-                // (y_l < -1e33 needs l > kd as well; neither rule acts in
-                // practice, so its selects run only where a lane needs them)
-                const bool yz = yn < kYMin || (past && yn > 0.0);
-                if (__any(yz))
-                    if (yz) {
-                        yn = 0.0;
-                        y1 = 0.0;
-                    }
-            }
-            j0 = j1;
-            j1 = jn;
-            y0 = y1;
-            y1 = yn;
-            jl = jn;
-            yl = yn;
-        }
-        mac(i, jl, yl);
-    }
-    st.cj0 = dc{j0, y0};
-    st.cj1 = dc{j1, y1};
+    // the rules only where some lane of the wave is past kd at an order of
+    // the range (orders l <= kfl of every lane: none)
+    if (wave_any(L0 + C - 1 > st.kfl))
+        harm_radial_out<L0, C, true>(st, nl1, ar, ai, acc_r, acc_i);
+    else
+        harm_radial_out<L0, C, false>(st, nl1, ar, ai, acc_r, acc_i);
 }
 
 // All orders at once (the sums of orders 0 .. L - 1 in ar / ai).
