@@ -282,7 +282,11 @@ def headline(pkg, P, W, ctx, torch, args):
         "speedup_vs_fp64_plain": (n / t32) * t64,
         "absorbance_max_abs_err_vs_fp64": err,
         "roofline": {
-            "bound": "valu",
+            "bound": "mfma",
+            "bound_note": "compute-bound on the SIMDs, not HBM: the order-sum products run on "
+                          "the matrix cores (v_mfma_f32_4x4x1_16b_f32) in the variants the "
+                          "tuner picks, the FP64 radial recurrences and phase steps on the "
+                          "VALU, and the two do not issue beside each other on MI355X",
             "achieved": achieved,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
@@ -292,13 +296,14 @@ def headline(pkg, P, W, ctx, torch, args):
                             "FETCH_SIZE x 2 + WRITE_SIZE, scaled by a 256 MiB copy; "
                             "algorithmic bytes per pass 5.68e8 (fields once, records, B_l)",
             "kernel": "series_harm_kernel (harmonic FP32 scattered-field sums, planar form in "
-                      "chunks of orders; one persistent launch per order bucket)",
+                      "chunks of orders, products on VALU FMAs or MFMA; one persistent launch "
+                      "per order bucket)",
             "flops_per_pass": flops,
             "series_ms_per_pass": series_ms_pass,
-            "peak_note": "157.3 is AMD's published FP32 vector peak; this build compiles "
-                         "without packed FP32 instructions (the direct series kernel still "
-                         "executes 95.8 TFLOP/s of plain FP32 VALU work by the SQ counters, so "
-                         "the plain-instruction ceiling is above half of 157.3)",
+            "peak_note": "157.3 is AMD's published FP32 peak (vector and matrix); this build "
+                         "compiles without packed FP32 instructions: plain v_fma_f32 issues "
+                         "at about 4.9 cycles per wave, MFMA 4x4x1 at about 10 for four FMAs "
+                         "per lane (profiles/valu_mfma_probe.md)",
         },
     }
 
