@@ -2,6 +2,9 @@
 // slice, spheres, wavelength and waves in one pass).
 #pragma once
 
+#include <algorithm>
+#include <cmath>
+
 #include <vector>
 
 #include "internal.h"
@@ -21,7 +24,54 @@ struct UfInst {
     int nk;          // Monte-Carlo sum: waves [wave_off, wave_off + nk)
     long long wave_off;
     double sA;       // Monte-Carlo sum: weight (1.0f/nk) subA of every wave
+    // Mirror symmetry of the series field about the focus (uf_mirror): with
+    // ncx > 0, columns iu and mx - iu (rows iv and my - iv) hold the same
+    // values, and the ncx x ncy canonical pixels cover the slice: column
+    // index c < nlx is column c, else column hx + c - nlx (rows alike).
+    int ncx, ncy, mx, my, nlx, nly, hx, hy;
 };
+
+// The mirror axis of one slice direction: positions r(i) = (a + x i/n) - f
+// (as quad_at and sub form them), i in [0, n).  Returns m with
+// r(m - i) == -r(i) for every pair in range and at least one pair, or -1.
+inline int uf_mirror_axis(double a, double x, double f, int n) {
+    if (!(x != 0.0) || n < 2) return -1;
+    const double c = 2.0 * (f - a) / x * (double)n;
+    if (!(c > 0.5 && c < 2.0 * n - 2.5)) return -1;
+    const long long m = (long long)std::llround(c);
+    auto r = [&](long long i) { return (a + x * ((double)i / (double)n)) - f; };
+    for (long long i = 0; i < n; ++i) {
+        const long long j = m - i;
+        if (j < 0 || j >= n || j == i) continue;
+        if (!(r(j) == -r(i))) return -1;
+    }
+    return (int)m;
+}
+
+// Fills I's mirror fields when its field is the series form, the slice is
+// normal to z with edges along x and y, k is along z and the pixel
+// positions mirror exactly about the focus on both axes (the value then
+// depends on the pixel only through r.x^2 + r.y^2 and r.z, so mirrored
+// pixels get the same bits).
+inline void uf_mirror(UfInst& I) {
+    I.ncx = I.ncy = 0;
+    const QuadArg& q = I.q;
+    if (I.mode != BIMSIM_UF_SERIES || q.X.y != 0.0 || q.X.z != 0.0 || q.Y.x != 0.0 ||
+        q.Y.z != 0.0 || I.k.x != 0.0 || I.k.y != 0.0)
+        return;
+    const int mx = uf_mirror_axis(q.A.x, q.X.x, I.f.x, I.uR);
+    const int my = uf_mirror_axis(q.A.y, q.Y.y, I.f.y, I.vR);
+    if (mx < 0 || my < 0) return;
+    auto axis = [](int m, int n, int& nl, int& h) {
+        nl = std::max(0, m - n + 1);   // [0, m - n]: partners past n - 1
+        h = (m + 1) / 2;               // [h, n): the larger of each pair, or unpaired
+        return nl + (n - h);
+    };
+    I.mx = mx;
+    I.my = my;
+    I.ncx = axis(mx, I.uR, I.nlx, I.hx);
+    I.ncy = axis(my, I.vR, I.nly, I.hy);
+}
 
 // The incident field's plane case: k along z and a slice normal to z
 // through the focus, so that k.(p - f) = 0 exactly at every pixel.

@@ -290,6 +290,7 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
                        I.uf_mode, field, tab, uf_flat(q, mk(I.focus), mk(I.kdir)) ? 1 : 0,
                        plane ? 1 : I.nk, plane ? 0 : (long long)I.wave_begin,
                        plane ? I.amp : (double)(1.0f / (float)I.nk) * b->subA[i]};
+        uf_mirror(uf[i]);
         if (b->f32 && I.uf_mode == BIMSIM_UF_MC) {
             mc_list.push_back(i);
             b->mc_tiles_max = std::max(b->mc_tiles_max, mc_tiles(I.N, I.N));

@@ -60,13 +60,14 @@ __global__ __launch_bounds__(kThreads) void uf_batch_kernel(const UfInst* __rest
                                                             R* __restrict__ D, R* __restrict__ Di) {
     for (int i = blockIdx.y; i < n; i += gridDim.y) {
         const UfInst& I = insts[i];
-        const int npix = I.uR * I.vR;
-        for (int pix = blockIdx.x * kThreads + threadIdx.x; pix < npix; pix += gridDim.x * kThreads) {
-            if (sum && pix < sum[i].nd2) {
+        const int nw = uf_batch_work(I), nd2 = sum ? sum[i].nd2 : 0;   // nd2 <= uR vR
+        for (int pix = blockIdx.x * kThreads + threadIdx.x; pix < max(nw, nd2);
+             pix += gridDim.x * kThreads) {
+            if (pix < nd2) {
                 D[sum[i].image_off + pix] = R(0);
                 Di[sum[i].image_off + pix] = R(0);
             }
-            uf_batch_pixel(I, pix, tabs, waves, Uf);
+            if (pix < nw) uf_batch_item(I, pix, tabs, waves, Uf);
         }
     }
 }

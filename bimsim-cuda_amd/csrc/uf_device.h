@@ -234,5 +234,40 @@ __device__ inline void uf_batch_pixel(const UfInst& I, int pix, const double* __
     store(Uf + I.off, pix, v);
 }
 
+// Work items of batch instance I's incident field: its pixels, or with
+// mirror symmetry (uf_mirror) its canonical pixels.
+__device__ inline int uf_batch_work(const UfInst& I) {
+    return I.ncx > 0 ? I.ncx * I.ncy : I.uR * I.vR;
+}
+
+// Work item w of batch instance I: the pixel w, or the canonical pixel w
+// stored at each of its mirror images in the slice (its own column and
+// mx - that column, its row and my - that row, those inside the slice).
+template <class Out>
+__device__ inline void uf_batch_item(const UfInst& I, int w, const double* __restrict__ tabs,
+                                     const double* __restrict__ waves, Out* __restrict__ Uf) {
+    if (I.ncx <= 0) {
+        uf_batch_pixel(I, w, tabs, waves, Uf);
+        return;
+    }
+    const int cy = w / I.ncx, cx = w - cy * I.ncx;
+    const int iu = cx < I.nlx ? cx : I.hx + (cx - I.nlx);
+    const int iv = cy < I.nly ? cy : I.hy + (cy - I.nly);
+    const int pu = I.mx - iu, pv = I.my - iv;
+    const bool mu = pu >= 0 && pu < I.uR && pu != iu, mv = pv >= 0 && pv < I.vR && pv != iv;
+    dc v;
+    if constexpr (std::is_same<Out, float2>::value)
+        v = uf_fast_value(I.q, I.uR, I.vR, iv * I.uR + iu, I.f, I.k, I.kmag, I.amp, I.ca, I.cb,
+                          I.m, tabs + I.tab, I.flat != 0);
+    else
+        v = uf_series_value(I.q, I.uR, I.vR, iv * I.uR + iu, I.f, I.k, I.kmag, I.amp, I.ca, I.cb,
+                            I.m);
+    Out* F = Uf + I.off;
+    store(F, iv * I.uR + iu, v);
+    if (mu) store(F, iv * I.uR + pu, v);
+    if (mv) store(F, pv * I.uR + iu, v);
+    if (mu && mv) store(F, pv * I.uR + pu, v);
+}
+
 }  // namespace ufd
 }  // namespace bimsim

@@ -727,7 +727,7 @@ __global__ __launch_bounds__(kRecThreads) void prepass_kernel(
     b -= n_rec;
     const int i = b / a.uf_blocks, pix = (b - i * a.uf_blocks) * kRecThreads + t;
     const UfInst& I = a.uf[i];
-    if (pix < I.uR * I.vR) ufd::uf_batch_pixel(I, pix, a.uf_tabs, a.uf_waves, a.Uf);
+    if (pix < ufd::uf_batch_work(I)) ufd::uf_batch_item(I, pix, a.uf_tabs, a.uf_waves, a.Uf);
     if (a.sum) {
         const SumInst& S = a.sum[i];   // nd2 <= uR vR
         if (pix < S.nd2) {

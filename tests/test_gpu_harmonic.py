@@ -174,8 +174,10 @@ def test_register_variants_give_the_same_bits(pkg, W, variant, monkeypatch):
     orders span several buckets and whose 48 x 48 images end in a partial
     pixel block."""
     import bimsim_amd.pipeline as P
-    insts = W.config4(n_nu=256, N=48, ns=4, nk=24, stride=37)
-    assert max(s["Nl"] for i in insts for s in i["spheres"]) > 24
+    insts = W.config4(n_nu=256, N=48, ns=20, nk=24, stride=51)
+    # orders in every bucket from 16 to 64 (all the chunk sizes of the
+    # matrix-core variants: 12, 16, 20, 24 and 32)
+    assert max(s["Nl"] + 1 for i in insts for s in i["spheres"]) > 48
     # planar: the slice is normal to z through every centre
     assert all(i["X"][2] == 0 and i["Y"][2] == 0 and i["A"][2] == s["p"][2]
                for i in insts for s in i["spheres"])
