@@ -171,10 +171,13 @@ def test_register_variants_give_the_same_bits(pkg, W, variant, monkeypatch):
     chunks of 16, 32 or half the orders on the matrix cores, whose FP32 products
     round as fmaf) included, adds the same terms in the same order: the
     images equal the default variant's bit for bit, on a planar sweep whose
-    orders span several buckets and whose 48 x 48 images end in a partial
-    pixel block."""
+    orders span several buckets and whose 50 x 50 images end in a partial
+    wave."""
     import bimsim_amd.pipeline as P
-    insts = W.config4(n_nu=256, N=48, ns=20, nk=24, stride=51)
+    insts = W.config4(n_nu=256, N=50, ns=20, nk=24, stride=51)
+    # 50 x 50 = 2500 pixels: the last wave of each instance has 4 of its 64
+    # lanes on pixels (the others repeat the last pixel and store nothing)
+    assert (50 * 50) % 64 == 4
     # orders in every bucket from 16 to 64 (all the chunk sizes of the
     # matrix-core variants: 12, 16, 20, 24 and 32)
     assert max(s["Nl"] + 1 for i in insts for s in i["spheres"]) > 48
