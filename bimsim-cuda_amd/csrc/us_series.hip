@@ -1344,10 +1344,10 @@ int harm_tune(bimsim_ctx* ctx, const std::string& shape, const double* waves, co
         for (int v = 0; v < kHarmVariants && rc == BIMSIM_OK; ++v) {
             if (!harm_variant_tuned(K->L, v, K->planar)) continue;
             K->variant = v;
-            // a warm run, then the better of two timed ones (variants a few
-            // per cent apart were ranked wrongly by a single run)
+            // a warm run, then the best of three timed ones (variants a few
+            // per cent apart were ranked wrongly by one or two)
             float ms = 1e30f;
-            for (int r = 0; r < 3 && rc == BIMSIM_OK; ++r) {
+            for (int r = 0; r < 4 && rc == BIMSIM_OK; ++r) {
                 rc = zero_launch(ctx, P->d_counters + K->counter, sizeof(int));
                 if (rc != BIMSIM_OK) break;
                 (void)hipEventRecord(e0, ctx->stream);
