@@ -367,7 +367,10 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // asm, so the statement keeps its own wait states: two before the first
 // product (a VALU write of an operand just before it), two before the sin
 // terms (a sum's cos term written just before), one after the last (a
-// VALU write of an operand read by the last product).  Slot layouts:
+// VALU write of an operand read by the last product); the sums are first
+// read after `settle` (eight more), and tests/test_mfma_wait_states.py
+// checks in every bucket's compiled code that nothing else touches them
+// sooner (the compiler did, where it spilled them).  Slot layouts:
 // packed (both rows in one register, row m + 1 from lane 32) or one
 // register per row.
 #define QUAD_C(b, ce, co)                                                                     \
