@@ -155,15 +155,17 @@ constexpr int harm_variant_waves(int L, int v) {
         // the matrix-core chunks: 16, 32 or L/2 orders (2C AGPRs; L/2 = 8 on
         // L = 16 ties the VALU chunks there, at 7 waves), none that
         // repeats another variant's chunk (all orders at once ran slowest on
-        // the buckets above 32, at two or three waves per SIMD; on L = 32 the
-        // 32-order chunk at four is the fastest by 1 %), and no chunk over 40
+        // the buckets above 32, at two or three waves per SIMD; 32-order
+        // chunks at four waves ran 1 % faster on L = 32 but spilled, 0.5 GB
+        // more HBM writes per cfg4 pass), and no chunk over 40
         // orders (48 and 56 spilled their sums, and the compiler then read
         // them right after the products, inside the wait the MFMA needs).
-        // The budgets are the largest wave counts that spill nothing.
+        // The budgets are the largest wave counts that spill nothing (the
+        // 16-order chunks spill two registers on L >= 48).
         if (v == 7 && L < 32) return 0;
         if (v == 8 && (L < 16 || L == 32 || L == 64 || L > 80)) return 0;
         const int c = harm_variant_chunk(L, v);
-        return c <= 8 ? 7 : c <= 12 ? 6 : c <= 16 ? 5 : c <= 32 ? 4 : 3;
+        return c <= 8 ? 7 : c <= 12 ? 6 : c <= 16 ? 5 : c <= 24 ? 4 : 3;
     }
     if (v >= 3) return v == 3 ? 7 : 8;
     return L <= 16 ? 6 + v
