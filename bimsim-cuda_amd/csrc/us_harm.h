@@ -65,6 +65,29 @@ struct RadialState {
     int kfl;           // floor(kx), at most 2^30: order l is past kx where l > kfl
 };
 
+// The distance d = sqrt(s) of a pixel from a sphere's centre and 1/d, from
+// s = |r|^2 >= 0: v_rsq_f64's estimate y, one step of Goldschmidt's coupled
+// iteration from g = s y -> sqrt(s), h = y/2 -> 1/(2 sqrt(s)), two
+// corrections g += (s - g^2) h, and a Newton step for 1/g from 2h.  d comes
+// out correctly rounded and 1/d within two ulps (for estimates y within
+// 2^-22), in some 15 VALU instructions against some 30 of the square root
+// and the division.  s is never denormal here; s = 0 (a pixel at the centre
+// itself) gives d = 0 and 1/d = inf, as the division does.
+__device__ __forceinline__ void harm_dist(double s, double& d, double& inv_d) {
+    const double y = __builtin_amdgcn_rsq(s);
+    double g = s * y, h = 0.5 * y;
+    const double e = fma(-g, h, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    g = fma(fma(-g, g, s), h, g);
+    g = fma(fma(-g, g, s), h, g);
+    double v = h + h;
+    v = fma(v, fma(-g, v, 1.0), v);
+    const bool zero = s == 0.0;
+    d = zero ? 0.0 : g;
+    inv_d = zero ? __builtin_inf() : v;
+}
+
 // whether the condition holds on some lane of the wave: the ballot of the
 // lane mask itself (HIP's __any(int) turns the mask into 0/1 values and
 // compares them, two VALU instructions)
@@ -112,17 +135,23 @@ __device__ __forceinline__ RadialState harm_radial_init(const SphereDev& S, doub
 // with the order below it), on ranges where some lane of the wave is past
 // kd at one of their orders; their conditions are combined bitwise (no
 // branches per lane), and y's selects run only where a lane needs them (in
-// practice none).
-template <int L0, int C, bool kCheck>
+// practice none).  kAll (only without the rules): every order of the
+// range is stepped and summed, also those past Nl, whose order sums are
+// zero there (the record rows hold zeros past Nl), so that they add
+// nothing; the values stay moderate at orders no lane is past kd at, and
+// the range takes no branch per order (nor the register copies at their
+// merges).
+template <int L0, int C, bool kCheck, bool kAll = false>
 __device__ __forceinline__ void harm_radial_out(RadialState& st, int nl1, const float* ar,
                                                 const float* ai, float& acc_r, float& acc_i) {
     static_assert(L0 % 2 == 0 && C % 2 == 0, "ranges start at even orders");
+    static_assert(!(kCheck && kAll), "the threshold rules keep the orders past Nl out");
     const double inv = st.cinv.r;
     double je = st.cj0.r, ye = st.cj0.i, jo = st.cj1.r, yo = st.cj1.i;
 #pragma clang loop unroll(full)
     for (int i = 0; i < C; ++i) {
         const int l = L0 + i;
-        if (l >= nl1) continue;
+        if (!kAll && l >= nl1) continue;
         const bool even = (l & 1) == 0;
         double& jn = even ? je : jo;   // order l - 2, then l
         double& yn = even ? ye : yo;
@@ -617,8 +646,13 @@ __device__ __forceinline__ void quad_chunk(__amdgpu_buffer_rsrc_t rs, const int*
     quad_load<G::NR>(rs, off, row(2), a1);
     quad_groups<L, C, L0, 0, true>(a0, 1.f, 0.f, z1r, z1i, 0, De, Do);
     float zr = fmaf(z1r, z1r, -(z1i * z1i)), zi = fmaf(z1r, z1i, z1i * z1r);
+    // rows 2, 3 run even where Nl < 2: their records are zero there (rows
+    // past Nl hold zeros), so they leave the sums as they are, and with one
+    // path out of the loop the sums keep their registers (an exit that could
+    // skip the loop cost a shuffle of the accumulators after it)
+    int m = 2;
 #pragma clang loop unroll(disable)
-    for (int m = 2; m < mend; m += 2) {
+    do {
 #pragma unroll
         for (int q = 0; q < G::NR; ++q) a0[q] = a1[q];
         quad_load<G::NR>(rs, off, row(m + 2), a1);
@@ -627,24 +661,47 @@ __device__ __forceinline__ void quad_chunk(__amdgpu_buffer_rsrc_t rs, const int*
         quad_groups<L, C, L0, 0, false>(a0, zar, zai, zbr, zbi, m, De, Do);
         zr = fmaf(zbr, z1r, -(zbi * z1i));
         zi = fmaf(zbr, z1i, zbi * z1r);
-    }
+        m += 2;
+    } while (m < mend);
     quad_settle<L, C, L0, 0>(De, Do);
 }
 
 // The radial factors of the chunk's orders, block by block (orders
 // L0 + 4b .. L0 + 4b + 3 from De[b] and Do[b]), the recurrences carried
-// as in harm_radial_range over the whole chunk.
+// as in harm_radial_range over the whole chunk.  Outside the sphere (the
+// lanes of pass 0 whose pixel is outside, tested once per chunk): the
+// steps of harm_radial_out, the threshold rules on the blocks some lane
+// is past kd at; a block with no such lane steps all four orders.
 template <int L, int C, int L0, int b>
-__device__ __forceinline__ void quad_radial(const f32x4* De, const f32x4* Do, int pass,
-                                            RadialState& st, const SphereDev& S, float& acc_r,
-                                            float& acc_i) {
+__device__ __forceinline__ void quad_radial_out(const f32x4* De, const f32x4* Do,
+                                                RadialState& st, int nl1, float& acc_r,
+                                                float& acc_i) {
+    constexpr int Qv = (L0 + C <= L ? C : L - L0) / 4;
+    if constexpr (b < Qv) {
+        constexpr int l0 = L0 + 4 * b;
+        if (l0 >= nl1) return;
+        const float ar[4] = {De[b][0], Do[b][0], De[b][2], Do[b][2]};
+        const float ai[4] = {De[b][1], Do[b][1], De[b][3], Do[b][3]};
+        if (wave_any(l0 + 3 > st.kfl))
+            harm_radial_out<l0, 4, true>(st, nl1, ar, ai, acc_r, acc_i);
+        else
+            harm_radial_out<l0, 4, false, true>(st, nl1, ar, ai, acc_r, acc_i);
+        quad_radial_out<L, C, L0, b + 1>(De, Do, st, nl1, acc_r, acc_i);
+    }
+}
+
+// The same for the lanes of pass 1 (inside the sphere, or at its centre).
+template <int L, int C, int L0, int b>
+__device__ __forceinline__ void quad_radial_in(const f32x4* De, const f32x4* Do,
+                                               RadialState& st, const SphereDev& S,
+                                               float& acc_r, float& acc_i) {
     constexpr int Qv = (L0 + C <= L ? C : L - L0) / 4;
     if constexpr (b < Qv) {
         if (L0 + 4 * b > S.Nl) return;
         const float ar[4] = {De[b][0], Do[b][0], De[b][2], Do[b][2]};
         const float ai[4] = {De[b][1], Do[b][1], De[b][3], Do[b][3]};
-        harm_radial_range<L0 + 4 * b, 4>(pass, st, S, ar, ai, acc_r, acc_i);
-        quad_radial<L, C, L0, b + 1>(De, Do, pass, st, S, acc_r, acc_i);
+        harm_radial_range<L0 + 4 * b, 4>(1, st, S, ar, ai, acc_r, acc_i);
+        quad_radial_in<L, C, L0, b + 1>(De, Do, st, S, acc_r, acc_i);
     }
 }
 
@@ -659,7 +716,11 @@ __device__ __forceinline__ void quad_chunks(__amdgpu_buffer_rsrc_t rs, const int
         constexpr int Q = C / 4;
         f32x4 De[Q], Do[Q];   // set by the chunk's first row pair
         quad_chunk<L, C, c * C>(rs, off, nl1, z1r, z1i, De, Do);
-        quad_radial<L, C, c * C, 0>(De, Do, pass, st, S, acc_r, acc_i);
+        if (pass == 0) {
+            if (st.mode == 0) quad_radial_out<L, C, c * C, 0>(De, Do, st, nl1, acc_r, acc_i);
+        } else {
+            quad_radial_in<L, C, c * C, 0>(De, Do, st, S, acc_r, acc_i);
+        }
         quad_chunks<L, C, c + 1>(rs, off, S, z1r, z1i, pass, st, acc_r, acc_i);
     }
 }
@@ -720,8 +781,8 @@ __global__ __launch_bounds__(kHarmThreads, kWaves) void series_harm_kernel(
             const SphereDev S = SS.s;
             const int nl1 = S.Nl + 1;
             d3 r = sub(p, mk(S.p));
-            const double d = len(r);
-            const double inv_d = 1.0 / d;
+            double d, inv_d;
+            harm_dist(dot(r, r), d, inv_d);
             // direction; the centre (knd below float epsilon) keeps only l = 0
             const bool centre = d <= S.a && S.n[0] * (I.kmag * d) < (double)kEpsFloat;
             r = centre ? d3{0.0, 0.0, 0.0} : scale(r, inv_d);
