@@ -625,32 +625,71 @@ __device__ __forceinline__ void quad_settle(f32x4* De, f32x4* Do) {
     }
 }
 
+// Phases of the row pairs kept in LDS from a pixel's first chunk for its
+// later ones, which would otherwise step them again: per lane, pair
+// m = 2, 4, ... at slot m/2 - 1 as (cos, sin) of rows m and m + 1, slots
+// kHarmThreads apart (a wave's 64 lanes read 1 KB in a row).  As many pairs
+// as the chunk's first rows hold (C/2 - 1) and as leave the kernel's waves
+// per SIMD in the 160 KB of LDS of a CU (a block of 256 lanes per wave per
+// SIMD, 4 KB per pair, 1 KB kept for the rest).
+constexpr int quad_phase_pairs(int C, int waves) {
+    const int budget = waves > 0 ? (163840 / waves - 1024) / 4096 : 0;
+    const int need = C / 2 - 1;
+    return need < budget ? need : budget;
+}
+
 // harm_planar_chunk on the matrix cores: De[b] holds the sums of orders
 // L0 + 4b and L0 + 4b + 2 (re, im, re, im), Do[b] those of L0 + 4b + 1 and
 // L0 + 4b + 3.  The next row pair's records load while this one's
-// products run.
-template <int L, int C, int L0>
+// products run.  The first chunk (L0 = 0) leaves the phases of its first
+// P row pairs past rows 0, 1 in ph; the later ones read them there and
+// step the phases on from the last (the same values as stepping them
+// again).
+template <int L, int C, int L0, int P>
 __device__ __forceinline__ void quad_chunk(__amdgpu_buffer_rsrc_t rs, const int* off, int nl1,
-                                           float z1r, float z1i, f32x4* De, f32x4* Do) {
+                                           float z1r, float z1i, float4* ph, f32x4* De,
+                                           f32x4* Do) {
     using G = QuadRegs<L, C>;
     static_assert(!G::kPacked || C <= 16, "packed record registers hold four blocks");
+    static_assert(2 + 2 * P <= C, "the first chunk holds the kept pairs");
     const int mend = min(nl1, L0 + C);
     // bytes of row m's part of the chunk
     auto row = [](int m) { return 4 * (m * (2 * L) + 2 * L0); };
     float a0[G::NR], a1[G::NR];
     quad_load<G::NR>(rs, off, row(0), a0);
-    // (a second register set for alternate row pairs, instead of the copy
+    // (a second register set for alternate row pairs, instead of the copies
     // below, doubled the asm and raised the register count)
     // rows 0 and 1 (every block runs there, and starts its sums): phases
     // 1 and e^{i phi}
     quad_load<G::NR>(rs, off, row(2), a1);
     quad_groups<L, C, L0, 0, true>(a0, 1.f, 0.f, z1r, z1i, 0, De, Do);
-    float zr = fmaf(z1r, z1r, -(z1i * z1i)), zi = fmaf(z1r, z1i, z1i * z1r);
+    float zr, zi;
     // rows 2, 3 run even where Nl < 2: their records are zero there (rows
     // past Nl hold zeros), so they leave the sums as they are, and with one
-    // path out of the loop the sums keep their registers (an exit that could
-    // skip the loop cost a shuffle of the accumulators after it)
+    // path out of each loop the sums keep their registers (an exit that
+    // could skip a loop cost a shuffle of the accumulators after it)
     int m = 2;
+    if constexpr (L0 > 0 && P > 0) {
+        // (a later chunk runs past row C - 1 >= 2P + 1: both loops run; the
+        // first unrolled, so that each read has its own registers and a
+        // constant offset)
+        float4 p = ph[0];
+#pragma unroll
+        do {
+#pragma unroll
+            for (int q = 0; q < G::NR; ++q) a0[q] = a1[q];
+            quad_load<G::NR>(rs, off, row(m + 2), a1);
+            const float4 z = p;
+            p = ph[min(m >> 1, P - 1) * kHarmThreads];
+            quad_groups<L, C, L0, 0, false>(a0, z.x, z.y, z.z, z.w, m, De, Do);
+            zr = fmaf(z.z, z1r, -(z.w * z1i));
+            zi = fmaf(z.z, z1i, z.w * z1r);
+            m += 2;
+        } while (m < 2 + 2 * P);
+    } else {
+        zr = fmaf(z1r, z1r, -(z1i * z1i));
+        zi = fmaf(z1r, z1i, z1i * z1r);
+    }
 #pragma clang loop unroll(disable)
     do {
 #pragma unroll
@@ -658,6 +697,9 @@ __device__ __forceinline__ void quad_chunk(__amdgpu_buffer_rsrc_t rs, const int*
         quad_load<G::NR>(rs, off, row(m + 2), a1);
         const float zar = zr, zai = zi;
         const float zbr = fmaf(zr, z1r, -(zi * z1i)), zbi = fmaf(zr, z1i, zi * z1r);
+        if constexpr (L0 == 0 && P > 0) {
+            if (m < 2 + 2 * P) ph[((m >> 1) - 1) * kHarmThreads] = make_float4(zar, zai, zbr, zbi);
+        }
         quad_groups<L, C, L0, 0, false>(a0, zar, zai, zbr, zbi, m, De, Do);
         zr = fmaf(zbr, z1r, -(zbi * z1i));
         zi = fmaf(zbr, z1i, zbi * z1r);
@@ -706,22 +748,23 @@ __device__ __forceinline__ void quad_radial_in(const f32x4* De, const f32x4* Do,
 }
 
 // harm_planar_chunks on the matrix cores: chunk c and the ones after it.
-template <int L, int C, int c>
+template <int L, int C, int P, int c>
 __device__ __forceinline__ void quad_chunks(__amdgpu_buffer_rsrc_t rs, const int* off,
                                             const SphereDev& S, float z1r, float z1i, int pass,
-                                            RadialState& st, float& acc_r, float& acc_i) {
+                                            RadialState& st, float4* ph, float& acc_r,
+                                            float& acc_i) {
     if constexpr (c * C < L) {
         const int nl1 = S.Nl + 1;
         if (c * C >= nl1) return;
         constexpr int Q = C / 4;
         f32x4 De[Q], Do[Q];   // set by the chunk's first row pair
-        quad_chunk<L, C, c * C>(rs, off, nl1, z1r, z1i, De, Do);
+        quad_chunk<L, C, c * C, P>(rs, off, nl1, z1r, z1i, ph, De, Do);
         if (pass == 0) {
             if (st.mode == 0) quad_radial_out<L, C, c * C, 0>(De, Do, st, nl1, acc_r, acc_i);
         } else {
             quad_radial_in<L, C, c * C, 0>(De, Do, st, S, acc_r, acc_i);
         }
-        quad_chunks<L, C, c + 1>(rs, off, S, z1r, z1i, pass, st, acc_r, acc_i);
+        quad_chunks<L, C, P, c + 1>(rs, off, S, z1r, z1i, pass, st, ph, acc_r, acc_i);
     }
 }
 
@@ -752,6 +795,10 @@ __global__ __launch_bounds__(kHarmThreads, kWaves) void series_harm_kernel(
     __shared__ int s_item;
     int qoff[QuadRegs<L, (kChunk > 0 ? kChunk : 4)>::NR];
     if constexpr (kMfma) quad_offsets<L, kChunk>((int)(threadIdx.x & 63), qoff);
+    // the row pairs' phases that a pixel's first chunk keeps for its later ones
+    constexpr int kPhasePairs = kMfma ? quad_phase_pairs(kChunk, kWaves) : 0;
+    __shared__ float4 s_phase[(kPhasePairs > 0 ? kPhasePairs : 1) * kHarmThreads];
+    float4* const ph = s_phase + threadIdx.x;
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
         __syncthreads();
@@ -792,13 +839,13 @@ __global__ __launch_bounds__(kHarmThreads, kWaves) void series_harm_kernel(
                 const RadialTrig trig = harm_trig(S, I.kmag, d);
                 RadialState st = harm_radial_init(S, I.kmag, d, inv_d, inv_kmag, trig);
                 if (pass0 == 0)
-                    quad_chunks<L, kChunk, 0>(
+                    quad_chunks<L, kChunk, kPhasePairs, 0>(
                         quad_table(reinterpret_cast<const float*>(rec + SS.toff), L), qoff, S,
-                        (float)r.x, (float)r.y, 0, st, acc_r, acc_i);
+                        (float)r.x, (float)r.y, 0, st, ph, acc_r, acc_i);
                 if (pass1 == 1)
-                    quad_chunks<L, kChunk, 0>(
+                    quad_chunks<L, kChunk, kPhasePairs, 0>(
                         quad_table(reinterpret_cast<const float*>(rec_in + SS.toff), L), qoff, S,
-                        (float)r.x, (float)r.y, 1, st, acc_r, acc_i);
+                        (float)r.x, (float)r.y, 1, st, ph, acc_r, acc_i);
                 continue;
             } else if constexpr (kChunk > 0) {
                 // planar buckets only; orders in chunks of kChunk

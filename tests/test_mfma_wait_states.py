@@ -7,7 +7,11 @@ before other instructions touch its registers.  This compiles each order
 bucket's matrix-core object (the Makefile's flags) to assembly and checks
 that no instruction reads or writes an AGPR that a statement's MFMAs wrote
 within 8 wait states after the statement (the chunk sizes whose sums spill
-did exactly that and were withdrawn).  CPU only: hipcc cross-compiles.
+did exactly that and were withdrawn).  The one exception is the next
+statement's own products, which add to the same accumulators (destination
+and addend the same registers): they keep at least the 2 wait states that a
+statement keeps between its cos and sin products on one accumulator.  CPU
+only: hipcc cross-compiles.
 """
 import os
 import re
@@ -21,6 +25,7 @@ PKG = os.path.join(ROOT, "bimsim-cuda_amd")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 BUCKETS = (16, 24, 32, 40, 48, 64, 80, 96, 112)
 WAIT = 8
+WAIT_ACCUMULATE = 2
 
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Xclang", "-target-feature",
          "-Xclang", "-packed-fp32-ops", "-ffp-contract=off", "-mllvm", "-disable-machine-licm",
@@ -66,7 +71,9 @@ def early_uses(asm):
                     ws += int(x.split()[1]) + 1
                     continue
                 if agprs(x) & written:
-                    hits.append((name[:80], ws, x))
+                    acc = re.match(r"v_mfma\S*\s+(a\[\d+:\d+\]),\s*v\d+,\s*v\d+,\s*(a\[\d+:\d+\])", x)
+                    if not (acc and acc.group(1) == acc.group(2) and ws >= WAIT_ACCUMULATE):
+                        hits.append((name[:80], ws, x))
                 ws += 1
     return statements, hits
 
@@ -101,3 +108,21 @@ def test_the_check_finds_an_early_read():
     statements, hits = early_uses(asm)
     assert statements == 2
     assert [(w, x) for _, w, x in hits] == [(0, "v_accvgpr_read_b32 v27, a19")]
+
+
+def test_the_check_allows_the_next_products_on_the_sums():
+    """The next statement's products on the same accumulators pass after 2
+    wait states (counted from the statement's end) and are found with none;
+    a product into other registers that reads the sums as its addend is
+    found."""
+    def stmt(*mfma):
+        return [";;#ASMSTART", "s_nop 1", *mfma, "s_nop 0", ";;#ASMEND"]
+    head = ["_ZN6bimsim4harm18series_harm_kernelILi16E: ; kernel"]
+    first = stmt("v_mfma_f32_4x4x1_16b_f32 a[0:3], v2, v3, a[0:3] cbsz:4 abid:2")
+    ok = head + first + stmt("v_mfma_f32_4x4x1_16b_f32 a[0:3], v10, v4, a[0:3] cbsz:4 abid:2")
+    assert early_uses("\n".join(ok))[1] == []
+    early = head + first + [";;#ASMSTART", "v_mfma_f32_4x4x1_16b_f32 a[0:3], v10, v4, a[0:3]",
+                            ";;#ASMEND"]
+    assert [w for _, w, _ in early_uses("\n".join(early))[1]] == [0]
+    other = head + first + stmt("v_mfma_f32_4x4x1_16b_f32 a[4:7], v10, v4, a[0:3] cbsz:4 abid:2")
+    assert [w for _, w, _ in early_uses("\n".join(other))[1]] == [2]
