@@ -2,9 +2,12 @@
 // sequence (reference microscope.cu:849-857, nearfield.cpp:194-218) for many
 // instances at once, optionally captured into one HIP graph.
 //
-//   Mie coefficients   one launch, three waves per block of spheres; for
-//                      FP32 on a side stream beside the next two rows
-//   incident fields    one launch over (pixel blocks, instances)
+//   Mie coefficients   one launch, three waves per block of spheres
+//   incident fields    one launch over (pixel blocks, instances); for FP32
+//                      both on a side stream beside the wave tables and
+//                      records, joined before the scattered fields (beside
+//                      those persistent launches it ran starved of CUs, and
+//                      slowed them)
 //   FP32 pre-pass      with one harmonic plan and few spheres instead: the
 //                      Legendre tables, then coefficients, records and
 //                      incident fields in one launch (series_plan_launch_prepass);
@@ -60,10 +63,11 @@ struct bimsim_batch {
     SeriesPlan plan_mc, plan_pw;    // FP32 series: Monte-Carlo / plane-wave instances
     bool have_mc = false, have_pw = false;
     hipStream_t capture_stream = nullptr;
-    // FP32 passes compute the Mie coefficients on a stream of their own,
-    // beside the incident fields and wave tables (joined before the sums)
+    // FP32 passes compute the Mie coefficients and then the incident fields
+    // on a stream of their own, beside the wave tables and records (joined
+    // before the sums)
     hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_fields = nullptr;
     // FP32 passes of one harmonic plan whose coefficient blocks fit
     // kPrepassLds: coefficients, records and incident fields in one launch
     // on the one stream instead
@@ -107,7 +111,7 @@ void release(bimsim_batch* b) {
     if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
     if (b->side) (void)hipStreamDestroy(b->side);
     if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
-    if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+    if (b->ev_fields) (void)hipEventDestroy(b->ev_fields);
     series_plan_destroy(&b->plan_mc);
     series_plan_destroy(&b->plan_pw);
     for (DetBatch& d : b->det) det_batch_destroy(&d);
@@ -169,24 +173,28 @@ int enqueue(bimsim_batch* b) {
         BIMSIM_TRY(series_plan_launch_sums(ctx, mc ? b->plan_mc : b->plan_pw, b->d_A, b->d_B,
                                            (float2*)b->d_U));
     } else if (b->f32) {
-        // fork: coefficients on the side stream, fields and wave tables here
+        // fork: coefficients, then the incident fields, on the side stream;
+        // wave tables and records here
         BIMSIM_HIP_CHECK(ctx, hipEventRecord(b->ev_fork, ctx->stream));
         BIMSIM_HIP_CHECK(ctx, hipStreamWaitEvent(b->side, b->ev_fork, 0));
         const hipStream_t main = ctx->stream;
         ctx->stream = b->side;
-        const int rc = mie_batch_launch(ctx, b->d_sph, b->d_kmag, b->n_sph, b->lmax, b->quirks,
-                                        b->d_work, b->d_A, b->d_B);
+        int rc = mie_batch_launch(ctx, b->d_sph, b->d_kmag, b->n_sph, b->lmax, b->quirks,
+                                  b->d_work, b->d_A, b->d_B);
+        // (with the sum formed by the detectors, this launch clears the images)
+        if (rc == BIMSIM_OK)
+            rc = uf_batch_launch(ctx, b->d_uf, b->n, b->max_npix, b->f32, b->d_uftab, b->waves,
+                                 b->d_Uf, b->fuse_sum ? b->d_sum : nullptr, b->D, b->Di);
+        if (rc == BIMSIM_OK)
+            rc = uf_mc_batch_launch(ctx, b->d_uf, b->d_mc_list, b->n_mc_uf, b->mc_tiles_max,
+                                    b->waves, (float2*)b->d_Uf);
+        if (rc == BIMSIM_OK && hipEventRecord(b->ev_fields, b->side) != hipSuccess)
+            rc = set_error(ctx, BIMSIM_E_HIP, "hipEventRecord failed");
         ctx->stream = main;
         BIMSIM_TRY(rc);
-        BIMSIM_HIP_CHECK(ctx, hipEventRecord(b->ev_join, b->side));
         if (b->have_mc) BIMSIM_TRY(series_plan_launch_waves(ctx, b->plan_mc, b->waves));
         if (b->have_pw) BIMSIM_TRY(series_plan_launch_waves(ctx, b->plan_pw, b->d_kdirs));
-        // (with the sum formed by the detectors, this launch clears the images)
-        BIMSIM_TRY(uf_batch_launch(ctx, b->d_uf, b->n, b->max_npix, b->f32, b->d_uftab, b->waves,
-                                   b->d_Uf, b->fuse_sum ? b->d_sum : nullptr, b->D, b->Di));
-        BIMSIM_TRY(uf_mc_batch_launch(ctx, b->d_uf, b->d_mc_list, b->n_mc_uf, b->mc_tiles_max,
-                                      b->waves, (float2*)b->d_Uf));
-        BIMSIM_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, b->ev_join, 0));
+        BIMSIM_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, b->ev_fields, 0));
         if (b->have_mc)
             BIMSIM_TRY(series_plan_launch_sums(ctx, b->plan_mc, b->d_A, b->d_B, (float2*)b->d_U));
         if (b->have_pw)
@@ -369,7 +377,7 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
     if (b->f32) {
         if (hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess)
+            hipEventCreateWithFlags(&b->ev_fields, hipEventDisableTiming) != hipSuccess)
             return set_error(ctx, BIMSIM_E_HIP, "cannot create the batch's side stream");
         // the series plans are timed on real inputs: coefficients first
         const size_t budget = (size_t)8 << 30;
