@@ -240,6 +240,8 @@ struct SeriesPlan {
     long long n_pl = 0, n_e = 0, n_rec = 0;
     int max_lh = 0, max_tl = 0;
     int max_rec_combos = 0;   // harmonic records: the largest row's (order, 4 spheres) count
+    int max_rec_nlv = 0;      // and its largest order count and sphere count
+    int max_rec_ns = 0;
     double* d_pl = nullptr;       // normalised Legendre values per (m, wave, l)
     double2* d_e = nullptr;       // exp(i m phi_k) per (m, wave)
     double2* d_phi = nullptr;     // wave weights (1/nk) subA exp(i k k.c) per sphere

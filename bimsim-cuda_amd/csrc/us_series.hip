@@ -406,20 +406,25 @@ constexpr long long kPermMaxPixels = 16LL << 20;
 constexpr long long kPermMaxTests = 128LL << 20;
 constexpr int kRecPer = 2;        // combinations per thread and pass
 
-// NT threads per block, tiles of kRecL orders and kRecS spheres (groups of
-// four), kRecW waves per LDS chunk: 256 / 128 / 64 / 16, or 64 / 64 / 32 /
-// 16 where a row has few combinations (a 256-thread block would leave most
-// of its threads idle; on the sweep's rows of at most 25 orders and five
-// groups the 64-thread form takes 0.82 against 1.02 ms, and chunks of 32
-// waves were slower, 0.99 ms).
-template <int NT, int kRecL, int kRecS, int kRecW>
+// NT threads per block, tiles of tl orders and ts spheres (a multiple of
+// four), kRecW waves per LDS chunk: 256 threads, tiles of up to 128 orders
+// and 64 spheres, or 64 threads, up to 64 and 32, where a row has few
+// combinations (a 256-thread block would leave most of its threads idle;
+// on the sweep's rows of at most 25 orders and five groups the 64-thread
+// form took 0.82 against 1.02 ms, and chunks of 32 waves were slower,
+// 0.99 ms).  The tiles are the plan's largest row and sphere count within
+// those limits (rec_lds_bytes), so that the LDS a block takes, and with it
+// the blocks a CU holds, follows the batch (the sweep's 25 orders and 20
+// spheres: 8.6 KB, against 16.3 KB for fixed tiles of 64 and 32).
+template <int NT, int kRecW>
 __device__ __forceinline__ void harm_records_row(
     int m, int i, const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph,
     const double* __restrict__ pl, const double2* __restrict__ e, const double2* __restrict__ phi,
-    const double* __restrict__ h64, float4* __restrict__ rec) {
-    __shared__ double sP[kRecW][kRecL];      // Pn_lm(cos th_w) of the tile's orders
-    __shared__ double2 sF[kRecW][kRecS];     // wave weights of the tile's spheres
-    __shared__ double2 sE[kRecW];            // e^{i m phi_w}
+    const double* __restrict__ h64, float4* __restrict__ rec, int tl, int ts) {
+    extern __shared__ double2 rec_lds[];
+    double2* const sE = rec_lds;                                       // e^{i m phi_w}
+    double2* const sF = sE + kRecW;                                    // [w][ts] sphere weights
+    double* const sP = reinterpret_cast<double*>(sF + kRecW * ts);  // [w][tl] Pn_lm(cos th_w)
     const SeriesInst& I = insts[i];
     if (m >= I.lh) return;
     const int t = (int)threadIdx.x;
@@ -429,11 +434,11 @@ __device__ __forceinline__ void harm_records_row(
     const int nlv = (I.lh - m + lstep - 1) / lstep;   // orders l = m + j lstep
     const double* Prow = pl + I.pl_off + tri_off(m, I.lh) * I.nk;
     const double2* E = e + I.e_off + (long long)m * I.nk;
-    for (int s0 = 0; s0 < I.ns; s0 += kRecS) {
-        const int nst = min(kRecS, I.ns - s0);
+    for (int s0 = 0; s0 < I.ns; s0 += ts) {
+        const int nst = min(ts, I.ns - s0);
         const int nq = (nst + 3) >> 2;
-        for (int j0 = 0; j0 < nlv; j0 += kRecL) {
-            const int nlt = min(kRecL, nlv - j0);
+        for (int j0 = 0; j0 < nlv; j0 += tl) {
+            const int nlt = min(tl, nlv - j0);
             const int combos = nq * nlt;
             for (int c0 = 0; c0 < combos; c0 += kRecPer * NT) {
                 // combination c: order index c % nlt, sphere group c / nlt
@@ -450,11 +455,11 @@ __device__ __forceinline__ void harm_records_row(
                     for (int x = t; x < nw * nlt; x += NT) {
                         const int w = x / nlt, jj = x - w * nlt;
                         const int l = m + (j0 + jj) * lstep;
-                        sP[w][jj] = Prow[(long long)(l - m) * I.nk + w0 + w];
+                        sP[w * tl + jj] = Prow[(long long)(l - m) * I.nk + w0 + w];
                     }
                     for (int x = t; x < nw * nst; x += NT) {
                         const int w = x / nst, ss = x - w * nst;
-                        sF[w][ss] = phi[sph[I.s_begin + s0 + ss].woff + w0 + w];
+                        sF[w * ts + ss] = phi[sph[I.s_begin + s0 + ss].woff + w0 + w];
                     }
                     if (t < nw) sE[t] = E[w0 + t];
                     __syncthreads();
@@ -464,12 +469,12 @@ __device__ __forceinline__ void harm_records_row(
                         if (c >= combos) continue;
                         const int jj = c % nlt, g = c / nlt;
                         for (int w = 0; w < nw; ++w) {
-                            const double p = sP[w][jj];
+                            const double p = sP[w * tl + jj];
                             const double2 ew = sE[w];
                             const double gr = p * ew.x, gi = p * ew.y;
 #pragma unroll
                             for (int v = 0; v < 4; ++v) {
-                                const double2 f = sF[w][min(4 * g + v, nst - 1)];
+                                const double2 f = sF[w * ts + min(4 * g + v, nst - 1)];
                                 acc[u][v][0] = fma(f.x, gr, acc[u][v][0]);
                                 acc[u][v][1] = fma(f.y, gi, acc[u][v][1]);
                                 acc[u][v][2] = fma(f.x, gi, acc[u][v][2]);
@@ -628,14 +633,27 @@ __device__ __forceinline__ void harm_records_tile(
     }
 }
 
-template <int NT, int kRecL, int kRecS, int kRecW>
+template <int NT, int kRecW>
 __global__ __launch_bounds__(NT) void harm_records_kernel(
     const SeriesInst* __restrict__ insts, const SeriesSph* __restrict__ sph, int n_inst,
     const double* __restrict__ pl, const double2* __restrict__ e, const double2* __restrict__ phi,
-    const double* __restrict__ h64, float4* __restrict__ rec) {
+    const double* __restrict__ h64, float4* __restrict__ rec, int tl, int ts) {
     for (int i = blockIdx.y; i < n_inst; i += gridDim.y)
-        harm_records_row<NT, kRecL, kRecS, kRecW>(blockIdx.x, i, insts, sph, pl, e, phi, h64,
-                                                  rec);
+        harm_records_row<NT, kRecW>(blockIdx.x, i, insts, sph, pl, e, phi, h64, rec, tl, ts);
+}
+
+// Tiles of harm_records_row for the plan's rows (orders) and spheres, and
+// the LDS they take with chunks of kW waves.
+struct RecTiles {
+    int tl, ts;
+    size_t lds;
+};
+RecTiles rec_tiles(int max_nlv, int max_ns, int l_cap, int s_cap, int kW) {
+    RecTiles r;
+    r.tl = std::max(1, std::min(max_nlv, l_cap));
+    r.ts = std::min((std::max(1, max_ns) + 3) / 4 * 4, s_cap);
+    r.lds = sizeof(double2) * (size_t)kW * (1 + r.ts) + sizeof(double) * (size_t)kW * r.tl;
+    return r;
 }
 
 __global__ __launch_bounds__(kRecThreads) void harm_records_tile_kernel(
@@ -1002,6 +1020,8 @@ int series_plan_create(bimsim_ctx* ctx, SeriesBatch& batch, size_t coef_budget, 
                 planar_all = planar_all && harm_planar(I.q, batch.sph[t].s);
             const int nlv = planar_all ? (lh + 1) / 2 : lh;
             P->max_rec_combos = std::max(P->max_rec_combos, nlv * ((I.ns + 3) / 4));
+            P->max_rec_nlv = std::max(P->max_rec_nlv, nlv);
+            P->max_rec_ns = std::max(P->max_rec_ns, I.ns);
         }
         for (int t = 0; t < P->n_sph; ++t) {
             SeriesSph& S = batch.sph[t];
@@ -1155,14 +1175,17 @@ int series_plan_launch_waves(bimsim_ctx* ctx, const SeriesPlan& P, const double*
         if ((long long)P.max_lh * P.n_inst >= 2LL * ctx->cu_count) {
             // one block per (row, instance) over all its spheres
             const dim3 rgrid(P.max_lh, std::min(P.n_inst, 65535));
-            if (P.max_rec_combos <= 2 * kRecPer * 64)
-                hipLaunchKernelGGL((harm_records_kernel<64, 64, 32, 16>), rgrid, dim3(64), 0, st,
+            if (P.max_rec_combos <= 2 * kRecPer * 64) {
+                const RecTiles T = rec_tiles(P.max_rec_nlv, P.max_rec_ns, 64, 32, 16);
+                hipLaunchKernelGGL((harm_records_kernel<64, 16>), rgrid, dim3(64), T.lds, st,
                                    P.d_inst, P.d_sph, P.n_inst, P.d_pl, P.d_e, P.d_phi, P.d_h64,
-                                   P.d_rec);
-            else
-                hipLaunchKernelGGL((harm_records_kernel<kRecThreads, 128, 64, 16>), rgrid,
-                                   dim3(kRecThreads), 0, st, P.d_inst, P.d_sph, P.n_inst, P.d_pl,
-                                   P.d_e, P.d_phi, P.d_h64, P.d_rec);
+                                   P.d_rec, T.tl, T.ts);
+            } else {
+                const RecTiles T = rec_tiles(P.max_rec_nlv, P.max_rec_ns, 128, 64, 16);
+                hipLaunchKernelGGL((harm_records_kernel<kRecThreads, 16>), rgrid,
+                                   dim3(kRecThreads), T.lds, st, P.d_inst, P.d_sph, P.n_inst,
+                                   P.d_pl, P.d_e, P.d_phi, P.d_h64, P.d_rec, T.tl, T.ts);
+            }
         } else {
             // few instances: blocks per (row, four records, sphere)
             const dim3 rgrid(P.max_tl, (P.max_tl + kRecTile - 1) / kRecTile,
