@@ -387,29 +387,44 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // (records of slots 2b in ae for row m, in ao for row m + 1, times zar,
 // zbr), then the same blocks their sin terms (slots 2b + 1, times zai,
 // zbi), so that each sum takes its cos term before its sin term as
-// harm_planar_chunk does.  Block b runs while b >= first (the row pair's
-// first block of the chunk; the blocks below the diagonal are skipped by
-// scalar branches, the later ones then too).  The sums stay in place in
-// AGPRs: the MFMAs are written with their accumulators tied, which the
-// compiler's forms of them are not (it then moves the sums through other
-// registers with copies every row).  The compiler does not see into the
-// asm, so the statement keeps its own wait states: two before the first
-// product (a VALU write of an operand just before it), two before the sin
-// terms (a sum's cos term written just before), one after the last (a
-// VALU write of an operand read by the last product); the sums are first
-// read after `settle` (eight more), and tests/test_mfma_wait_states.py
-// checks in every bucket's compiled code that nothing else touches them
-// sooner (the compiler did, where it spilled them).  Slot layouts:
-// packed (both rows in one register, row m + 1 from lane 32) or one
-// register per row.
+// harm_planar_chunk does.  Block b runs while first <= b <= last (first:
+// the row pair's first block of the chunk, below which the records are
+// zero; last: the block of the sphere's order Nl, past which they are zero
+// too): each cascade enters at block min(last, n - 1) by a scalar compare
+// per block above it, and leaves at the first block below first.  The
+// sums stay in place in AGPRs: the MFMAs are written with their
+// accumulators tied, which the compiler's forms of them are not (it then
+// moves the sums through other registers with copies every row).  The
+// compiler does not see into the asm, so the statement keeps its own wait
+// states: two before the first product (a VALU write of an operand just
+// before it), two before the sin terms (a sum's cos term written just
+// before), one after the last (a VALU write of an operand read by the
+// last product); the sums are first read after `settle` (eight more), and
+// tests/test_mfma_wait_states.py checks in every bucket's compiled code
+// that nothing else touches them sooner (the compiler did, where it
+// spilled them).  Slot layouts: packed (both rows in one register, row
+// m + 1 from lane 32) or one register per row.
 #define QUAD_C(b, ce, co)                                                                     \
+    ".Lqcb" #b "_%=:\n\t"                                                                      \
     "s_cmp_gt_i32 %[first], " #b "\n\ts_cbranch_scc1 .Lqc%=\n\t"                               \
     "v_mfma_f32_4x4x1_16b_f32 %[e" #b "], %[ae], %[zar], %[e" #b "] cbsz:4 abid:" #ce "\n\t" \
     "v_mfma_f32_4x4x1_16b_f32 %[o" #b "], %[ao], %[zbr], %[o" #b "] cbsz:4 abid:" #co "\n\t"
 #define QUAD_S(b, ce, co)                                                                     \
+    ".Lqsb" #b "_%=:\n\t"                                                                      \
     "s_cmp_gt_i32 %[first], " #b "\n\ts_cbranch_scc1 .Lqs%=\n\t"                               \
     "v_mfma_f32_4x4x1_16b_f32 %[e" #b "], %[ae], %[zai], %[e" #b "] cbsz:4 abid:" #ce "\n\t" \
     "v_mfma_f32_4x4x1_16b_f32 %[o" #b "], %[ao], %[zbi], %[o" #b "] cbsz:4 abid:" #co "\n\t"
+// the entry of a cascade (x: c or s): the highest block b <= last, or
+// none (last < 0)
+#define QUAD_DB(x, b) "s_cmp_le_i32 " #b ", %[last]\n\ts_cbranch_scc1 .Lq" #x "b" #b "_%=\n\t"
+#define QUAD_D1(x) QUAD_DB(x, 0) "s_branch .Lq" #x "%=\n\t"
+#define QUAD_D2(x) QUAD_DB(x, 1) QUAD_D1(x)
+#define QUAD_D3(x) QUAD_DB(x, 2) QUAD_D2(x)
+#define QUAD_D4(x) QUAD_DB(x, 3) QUAD_D3(x)
+#define QUAD_D5(x) QUAD_DB(x, 4) QUAD_D4(x)
+#define QUAD_D6(x) QUAD_DB(x, 5) QUAD_D5(x)
+#define QUAD_D7(x) QUAD_DB(x, 6) QUAD_D6(x)
+#define QUAD_D8(x) QUAD_DB(x, 7) QUAD_D7(x)
 // the first row pair of a chunk (every block; the cos products start the
 // sums: accumulator 0)
 #define QUAD_C0(b, ce, co)                                                                    \
@@ -492,11 +507,12 @@ struct QuadRow;
     struct QuadRow<N, P> {                                                                       \
         __device__ static __forceinline__ void run(f32x4* E, f32x4* O, float ae, float ao,       \
                                                    float zar, float zai, float zbr, float zbi,   \
-                                                   int first) {                                  \
-            asm volatile("s_nop 1\n\t" CASC_C ".Lqc%=:\n\ts_nop 1\n\t" CASC_S ".Lqs%=:\n\ts_nop 0" \
+                                                   int first, int last) {                        \
+            asm volatile("s_nop 1\n\t" QUAD_D##N(c) CASC_C ".Lqc%=:\n\ts_nop 1\n\t" QUAD_D##N(s)  \
+                         CASC_S ".Lqs%=:\n\ts_nop 0"                                             \
                          : QUAD_OPS##N                                                           \
                          : [ae] "v"(ae), [ao] "v"(ao), [zar] "v"(zar), [zai] "v"(zai),           \
-                           [zbr] "v"(zbr), [zbi] "v"(zbi), [first] "s"(first));                  \
+                           [zbr] "v"(zbr), [zbi] "v"(zbi), [first] "s"(first), [last] "s"(last)); \
         }                                                                                        \
         /* the chunk's first row pair (rows 0, 1: every block) */                                \
         __device__ static __forceinline__ void seed(f32x4* E, f32x4* O, float ae, float ao,      \
@@ -528,6 +544,7 @@ QUAD_ROW(8, false, QUAD_CU8, QUAD_SU8, QUAD_CU08, QUAD_SU08)
 #undef QUAD_S
 #undef QUAD_C0
 #undef QUAD_S0
+#undef QUAD_DB
 
 // Record operands of a chunk of C orders.  Row m's part of the chunk is
 // F = 2C floats: float4 (U1, U2) per order l = L0 + (m & 1) + 2k.  Its
@@ -593,12 +610,13 @@ __device__ __forceinline__ void quad_load(__amdgpu_buffer_rsrc_t rs, const int* 
 // The blocks of a row pair in asm statements of up to eight blocks each
 // (group k: blocks 8k .. of the chunk; its registers: row m's register k
 // and row m + 1's, or the one packed register).  Block gb of the bucket
-// (= L0/4 + b) runs from row 4 gb on, as in harm_planar_chunk; the blocks
-// that it skips past Nl run here on the zero records the fold writes
-// there, which leave the sums as they are.
+// (= L0/4 + b) runs from row 4 gb on and up to the block of order Nl, as
+// in harm_planar_chunk.  The chunk's first row pair runs every block (the
+// blocks past Nl there take the zero records the fold writes, which start
+// their sums at zero; no later product or read touches them).
 template <int L, int C, int L0, int k, bool kSeed>
 __device__ __forceinline__ void quad_groups(const float* a, float zar, float zai, float zbr,
-                                            float zbi, int m, f32x4* De, f32x4* Do) {
+                                            float zbi, int m, int nl1, f32x4* De, f32x4* Do) {
     constexpr int Qv = (L0 + C <= L ? C : L - L0) / 4;
     if constexpr (8 * k < Qv) {
         using G = QuadRegs<L, C>;
@@ -608,10 +626,11 @@ __device__ __forceinline__ void quad_groups(const float* a, float zar, float zai
             QuadRow<n, G::kPacked>::seed(De + 8 * k, Do + 8 * k, ae, ao, zar, zai, zbr, zbi);
         } else {
             const int first = (m >> 2) - (L0 / 4 + 8 * k);
+            const int last = ((nl1 - 1) >> 2) - (L0 / 4 + 8 * k);
             QuadRow<n, G::kPacked>::run(De + 8 * k, Do + 8 * k, ae, ao, zar, zai, zbr, zbi,
-                                        first);
+                                        first, last);
         }
-        quad_groups<L, C, L0, k + 1, kSeed>(a, zar, zai, zbr, zbi, m, De, Do);
+        quad_groups<L, C, L0, k + 1, kSeed>(a, zar, zai, zbr, zbi, m, nl1, De, Do);
     }
 }
 
@@ -662,7 +681,7 @@ __device__ __forceinline__ void quad_chunk(__amdgpu_buffer_rsrc_t rs, const int*
     // rows 0 and 1 (every block runs there, and starts its sums): phases
     // 1 and e^{i phi}
     quad_load<G::NR>(rs, off, row(2), a1);
-    quad_groups<L, C, L0, 0, true>(a0, 1.f, 0.f, z1r, z1i, 0, De, Do);
+    quad_groups<L, C, L0, 0, true>(a0, 1.f, 0.f, z1r, z1i, 0, nl1, De, Do);
     float zr, zi;
     // rows 2, 3 run even where Nl < 2: their records are zero there (rows
     // past Nl hold zeros), so they leave the sums as they are, and with one
@@ -681,7 +700,7 @@ __device__ __forceinline__ void quad_chunk(__amdgpu_buffer_rsrc_t rs, const int*
             quad_load<G::NR>(rs, off, row(m + 2), a1);
             const float4 z = p;
             p = ph[min(m >> 1, P - 1) * kHarmThreads];
-            quad_groups<L, C, L0, 0, false>(a0, z.x, z.y, z.z, z.w, m, De, Do);
+            quad_groups<L, C, L0, 0, false>(a0, z.x, z.y, z.z, z.w, m, nl1, De, Do);
             zr = fmaf(z.z, z1r, -(z.w * z1i));
             zi = fmaf(z.z, z1i, z.w * z1r);
             m += 2;
@@ -700,7 +719,7 @@ __device__ __forceinline__ void quad_chunk(__amdgpu_buffer_rsrc_t rs, const int*
         if constexpr (L0 == 0 && P > 0) {
             if (m < 2 + 2 * P) ph[((m >> 1) - 1) * kHarmThreads] = make_float4(zar, zai, zbr, zbi);
         }
-        quad_groups<L, C, L0, 0, false>(a0, zar, zai, zbr, zbi, m, De, Do);
+        quad_groups<L, C, L0, 0, false>(a0, zar, zai, zbr, zbi, m, nl1, De, Do);
         zr = fmaf(zbr, z1r, -(zbi * z1i));
         zi = fmaf(zbr, z1i, zbi * z1r);
         m += 2;
