@@ -4,10 +4,10 @@
 //
 //   Mie coefficients   one launch, three waves per block of spheres
 //   incident fields    one launch over (pixel blocks, instances); for FP32
-//                      both on a side stream beside the wave tables and
-//                      records, joined before the scattered fields (beside
-//                      those persistent launches it ran starved of CUs, and
-//                      slowed them)
+//                      each on a side stream of its own beside the wave
+//                      tables and records, joined before the scattered
+//                      fields (beside those persistent launches the fields
+//                      ran starved of CUs, and slowed them)
 //   FP32 pre-pass      with one harmonic plan and few spheres instead: the
 //                      Legendre tables, then coefficients, records and
 //                      incident fields in one launch (series_plan_launch_prepass);
@@ -63,11 +63,11 @@ struct bimsim_batch {
     SeriesPlan plan_mc, plan_pw;    // FP32 series: Monte-Carlo / plane-wave instances
     bool have_mc = false, have_pw = false;
     hipStream_t capture_stream = nullptr;
-    // FP32 passes compute the Mie coefficients and then the incident fields
-    // on a stream of their own, beside the wave tables and records (joined
+    // FP32 passes compute the Mie coefficients and the incident fields on
+    // streams of their own, beside the wave tables and records (joined
     // before the sums)
-    hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_fields = nullptr;
+    hipStream_t side = nullptr, side_f = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_coef = nullptr, ev_fields = nullptr;
     // FP32 passes of one harmonic plan whose coefficient blocks fit
     // kPrepassLds: coefficients, records and incident fields in one launch
     // on the one stream instead
@@ -110,7 +110,9 @@ void release(bimsim_batch* b) {
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
     if (b->side) (void)hipStreamDestroy(b->side);
+    if (b->side_f) (void)hipStreamDestroy(b->side_f);
     if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    if (b->ev_coef) (void)hipEventDestroy(b->ev_coef);
     if (b->ev_fields) (void)hipEventDestroy(b->ev_fields);
     series_plan_destroy(&b->plan_mc);
     series_plan_destroy(&b->plan_pw);
@@ -173,14 +175,18 @@ int enqueue(bimsim_batch* b) {
         BIMSIM_TRY(series_plan_launch_sums(ctx, mc ? b->plan_mc : b->plan_pw, b->d_A, b->d_B,
                                            (float2*)b->d_U));
     } else if (b->f32) {
-        // fork: coefficients, then the incident fields, on the side stream;
-        // wave tables and records here
+        // fork: coefficients on one side stream, incident fields on the
+        // other, wave tables and records here
         BIMSIM_HIP_CHECK(ctx, hipEventRecord(b->ev_fork, ctx->stream));
         BIMSIM_HIP_CHECK(ctx, hipStreamWaitEvent(b->side, b->ev_fork, 0));
+        BIMSIM_HIP_CHECK(ctx, hipStreamWaitEvent(b->side_f, b->ev_fork, 0));
         const hipStream_t main = ctx->stream;
         ctx->stream = b->side;
         int rc = mie_batch_launch(ctx, b->d_sph, b->d_kmag, b->n_sph, b->lmax, b->quirks,
                                   b->d_work, b->d_A, b->d_B);
+        if (rc == BIMSIM_OK && hipEventRecord(b->ev_coef, b->side) != hipSuccess)
+            rc = set_error(ctx, BIMSIM_E_HIP, "hipEventRecord failed");
+        ctx->stream = b->side_f;
         // (with the sum formed by the detectors, this launch clears the images)
         if (rc == BIMSIM_OK)
             rc = uf_batch_launch(ctx, b->d_uf, b->n, b->max_npix, b->f32, b->d_uftab, b->waves,
@@ -188,12 +194,13 @@ int enqueue(bimsim_batch* b) {
         if (rc == BIMSIM_OK)
             rc = uf_mc_batch_launch(ctx, b->d_uf, b->d_mc_list, b->n_mc_uf, b->mc_tiles_max,
                                     b->waves, (float2*)b->d_Uf);
-        if (rc == BIMSIM_OK && hipEventRecord(b->ev_fields, b->side) != hipSuccess)
+        if (rc == BIMSIM_OK && hipEventRecord(b->ev_fields, b->side_f) != hipSuccess)
             rc = set_error(ctx, BIMSIM_E_HIP, "hipEventRecord failed");
         ctx->stream = main;
         BIMSIM_TRY(rc);
         if (b->have_mc) BIMSIM_TRY(series_plan_launch_waves(ctx, b->plan_mc, b->waves));
         if (b->have_pw) BIMSIM_TRY(series_plan_launch_waves(ctx, b->plan_pw, b->d_kdirs));
+        BIMSIM_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, b->ev_coef, 0));
         BIMSIM_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, b->ev_fields, 0));
         if (b->have_mc)
             BIMSIM_TRY(series_plan_launch_sums(ctx, b->plan_mc, b->d_A, b->d_B, (float2*)b->d_U));
@@ -376,9 +383,11 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
     BIMSIM_TRY(dev_alloc(ctx, &b->d_U, fb * (size_t)field));
     if (b->f32) {
         if (hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
+            hipStreamCreateWithFlags(&b->side_f, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&b->ev_coef, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&b->ev_fields, hipEventDisableTiming) != hipSuccess)
-            return set_error(ctx, BIMSIM_E_HIP, "cannot create the batch's side stream");
+            return set_error(ctx, BIMSIM_E_HIP, "cannot create the batch's side streams");
         // the series plans are timed on real inputs: coefficients first
         const size_t budget = (size_t)8 << 30;
         BIMSIM_TRY(mie_batch_launch(ctx, b->d_sph, b->d_kmag, b->n_sph, b->lmax, b->quirks,
