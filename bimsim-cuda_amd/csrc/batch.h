@@ -133,7 +133,14 @@ struct SumFuse {
     QuadArg q;
     int uR, vR, ns;
     const double4* sph;
+    // pixels inside a sphere as bits (pixel p: bit p % 32 of word p / 32),
+    // formed once by sum_mask_launch from the same test; null: the test runs
+    // per pixel as the rows load
+    const unsigned* inside = nullptr;
 };
+
+// fz's inside bits into mask (ceil(uR vR / 32) words), on ctx->stream.
+int sum_mask_launch(bimsim_ctx* ctx, const SumFuse& fz, unsigned* mask);
 
 // bimsim_detector's work; with fz (complex64 fields on the band-limited
 // path, detector_fusable), U holds the scattered field alone and the sum

@@ -78,6 +78,10 @@ struct bimsim_batch {
     // pre-pass launch)
     bool fuse_sum = false;
     std::vector<SumInst> sum_host;
+    // with fuse_sum: each instance's inside-a-sphere pixel bits (the
+    // geometry's, formed once), at inside_off[i] words
+    unsigned* d_inside = nullptr;
+    std::vector<long long> inside_off;
     // detectors: one batched pass per FFT size, and the instances that
     // need their own (supersampling, zero fill, sizes without a fixed plan)
     std::vector<DetBatch> det;
@@ -118,7 +122,8 @@ void release(bimsim_batch* b) {
     series_plan_destroy(&b->plan_pw);
     for (DetBatch& d : b->det) det_batch_destroy(&d);
     void* ptrs[] = {b->d_sph, b->d_kmag, b->d_work, b->d_A, b->d_B, b->d_uf, b->d_uftab,
-                    b->d_sum, b->d_sph4, b->d_Uf, b->d_U, b->d_kdirs, b->d_mc_list};
+                    b->d_sum, b->d_sph4, b->d_Uf, b->d_U, b->d_kdirs, b->d_mc_list,
+                    b->d_inside};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -229,7 +234,8 @@ int enqueue(bimsim_batch* b) {
         SumFuse fz{};
         if (b->fuse_sum) {
             const SumInst& S = b->sum_host[i];
-            fz = SumFuse{S.q, S.uR, S.vR, S.ns, b->d_sph4 + S.s_begin};
+            fz = SumFuse{S.q, S.uR, S.vR, S.ns, b->d_sph4 + S.s_begin,
+                         b->d_inside + b->inside_off[i]};
         }
         BIMSIM_TRY(detector_run(ctx, b->prec, b->quirks, (char*)b->d_U + fb * b->field_off[i],
                                 (char*)b->d_Uf + fb * b->field_off[i], I.N, I.N, b->du[i], b->dv[i],
@@ -418,6 +424,20 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
             b->pre.sum = b->d_sum;
             b->pre.D = (float*)b->D;
             b->pre.Di = (float*)b->Di;
+            // the pixels inside a sphere, once: the detectors' first row
+            // pass then reads a bit per pixel instead of testing the spheres
+            long long words = 0;
+            b->inside_off.resize(n);
+            for (int i = 0; i < n; ++i) {
+                b->inside_off[i] = words;
+                words += ((long long)sum[i].uR * sum[i].vR + 31) / 32;
+            }
+            BIMSIM_TRY(dev_alloc(ctx, (void**)&b->d_inside, sizeof(unsigned) * (size_t)words));
+            for (int i = 0; i < n; ++i) {
+                const SumInst& S = sum[i];
+                BIMSIM_TRY(sum_mask_launch(ctx, SumFuse{S.q, S.uR, S.vR, S.ns, b->d_sph4 + S.s_begin},
+                                           b->d_inside + b->inside_off[i]));
+            }
         }
     }
     // detectors by FFT size
@@ -440,7 +460,8 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
         q.lambda = I.lambda;
         if (b->fuse_sum) {
             const SumInst& S = sum[i];
-            q.fz = SumFuse{S.q, S.uR, S.vR, S.ns, b->d_sph4 + S.s_begin};
+            q.fz = SumFuse{S.q, S.uR, S.vR, S.ns, b->d_sph4 + S.s_begin,
+                           b->d_inside + b->inside_off[i]};
         }
         by_size[I.N].push_back(q);
     }

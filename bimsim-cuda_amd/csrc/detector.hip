@@ -316,15 +316,12 @@ __device__ __forceinline__ typename C2<R>::T* fixed_fft(typename C2<R>::T* a,
     }
 }
 
-// A batch instance's Uf + Us at pixel pix from U = Us (a) and Uf (b), by
-// the rule of the batch's sum kernel: Uf without spheres, else U + Uf
-// outside every sphere (d < a is inside, nfSumUf.cu:31-48).
+// Whether pixel pix of a batch instance lies inside one of its spheres, by
+// the rule of the batch's sum kernel (d < a is inside, nfSumUf.cu:31-48).
 // Only the spheres of `near` (bit s: sphere s) are tested when fz.ns <= 64;
 // the others hold no pixel of the caller's rows.
-template <class C>
-__device__ __forceinline__ C fused_sum(const SumFuse& fz, int pix, C a, C b,
-                                       unsigned long long near = ~0ull) {
-    if (fz.ns == 0) return b;
+__device__ __forceinline__ bool fused_inside(const SumFuse& fz, int pix,
+                                             unsigned long long near = ~0ull) {
     const int iv = pix / fz.uR, iu = pix - iv * fz.uR;
     const d3 p = quad_at(fz.q, (double)iu / (double)fz.uR, (double)iv / (double)fz.vR);
     const bool masked = fz.ns <= 64;
@@ -341,12 +338,38 @@ __device__ __forceinline__ C fused_sum(const SumFuse& fz, int pix, C a, C b,
         // could decide otherwise than its square
         const d3 r = sub(p, d3{S.x, S.y, S.z});
         const double d2 = dot(r, r), a2 = S.w * S.w;
-        if (d2 < a2 * (1.0 - 1e-12)) return a;
-        if (d2 <= a2 * (1.0 + 1e-12) && sqrt(d2) < S.w) return a;
+        if (d2 < a2 * (1.0 - 1e-12)) return true;
+        if (d2 <= a2 * (1.0 + 1e-12) && sqrt(d2) < S.w) return true;
     }
+    return false;
+}
+
+// A batch instance's Uf + Us at pixel pix from U = Us (a) and Uf (b): Uf
+// without spheres, U inside a sphere, else U + Uf; inside from fz's bits
+// where it has them.
+template <class C>
+__device__ __forceinline__ C fused_sum(const SumFuse& fz, int pix, C a, C b,
+                                       unsigned long long near = ~0ull) {
+    if (fz.ns == 0) return b;
+    const bool in = fz.inside ? ((fz.inside[pix >> 5] >> (pix & 31)) & 1u) != 0
+                              : fused_inside(fz, pix, near);
+    if (in) return a;
     a.x = a.x + b.x;
     a.y = a.y + b.y;
     return a;
+}
+
+// fz's inside bits: one word of 32 pixels per lane.
+__global__ __launch_bounds__(kThreads) void sum_mask_kernel(SumFuse fz, unsigned* __restrict__ mask) {
+    const int npix = fz.uR * fz.vR;
+    const int w = blockIdx.x * kThreads + threadIdx.x;
+    if (w >= (npix + 31) / 32) return;
+    unsigned bits = 0u;
+    for (int i = 0; i < 32; ++i) {
+        const int pix = 32 * w + i;
+        if (pix < npix && fz.ns > 0 && fused_inside(fz, pix)) bits |= 1u << i;
+    }
+    mask[w] = bits;
 }
 
 // Forward FFT of B rows (length N) of U (blockIdx.y = 0) or Uf (= 1); the
@@ -365,7 +388,12 @@ __device__ __forceinline__ void rows_fwd_body(
     const C* F = f ? Uf : U;
     const int row0 = blk * B;
     const int nb = min(B, nrows - row0);
-    if (kSum && f == 0) {
+    if (kSum && f == 0 && fz.inside) {
+        for (int t = threadIdx.x; t < nb * N; t += kThreads) {
+            const size_t i = (size_t)row0 * N + t;
+            a[t] = fused_sum(fz, (int)i, U[i], Uf[i]);
+        }
+    } else if (kSum && f == 0) {
         // the spheres within their radius of these rows' bounding box (the
         // box of the four corner pixels holds every pixel of the rows): the
         // others cannot hold one of their pixels
@@ -980,6 +1008,16 @@ int field_fft_impl(bimsim_ctx* ctx, void* Fin, int uR, int vR, int inverse) {
 using namespace bimsim;
 
 namespace bimsim {
+
+int sum_mask_launch(bimsim_ctx* ctx, const SumFuse& fz, unsigned* mask) {
+    SumFuse f = fz;
+    f.inside = nullptr;
+    const int words = (fz.uR * fz.vR + 31) / 32;
+    hipLaunchKernelGGL(sum_mask_kernel, dim3((words + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                       ctx->stream, f, mask);
+    BIMSIM_HIP_CHECK(ctx, hipGetLastError());
+    return BIMSIM_OK;
+}
 
 bool detector_fusable(const bimsim_ctx* ctx, int uR, int vR) {
     return band_path(ctx, sizeof(float2), uR, vR);
