@@ -28,7 +28,9 @@ struct UfInst {
     // ncx > 0, columns iu and mx - iu (rows iv and my - iv) hold the same
     // values, and the ncx x ncy canonical pixels cover the slice: column
     // index c < nlx is column c, else column hx + c - nlx (rows alike).
-    int ncx, ncy, mx, my, nlx, nly, hx, hy;
+    // diag: pixel (iu, iv) and the transposed (iv, iu) hold the same values
+    // too (ncx = ncy), and the canonical pixels with cx <= cy cover the slice.
+    int ncx, ncy, mx, my, nlx, nly, hx, hy, diag;
 };
 
 // The mirror axis of one slice direction: positions r(i) = (a + x i/n) - f
@@ -55,6 +57,7 @@ inline int uf_mirror_axis(double a, double x, double f, int n) {
 // pixels get the same bits).
 inline void uf_mirror(UfInst& I) {
     I.ncx = I.ncy = 0;
+    I.diag = 0;
     const QuadArg& q = I.q;
     if (I.mode != BIMSIM_UF_SERIES || q.X.y != 0.0 || q.X.z != 0.0 || q.Y.x != 0.0 ||
         q.Y.z != 0.0 || I.k.x != 0.0 || I.k.y != 0.0)
@@ -71,6 +74,10 @@ inline void uf_mirror(UfInst& I) {
     I.my = my;
     I.ncx = axis(mx, I.uR, I.nlx, I.hx);
     I.ncy = axis(my, I.vR, I.nly, I.hy);
+    // the transposed pixel's offsets from the focus are this one's swapped,
+    // formed from the same numbers: the value, a function of r.x^2 + r.y^2
+    // and r.z, keeps its bits
+    I.diag = I.uR == I.vR && mx == my && q.A.x == q.A.y && q.X.x == q.Y.y && I.f.x == I.f.y ? 1 : 0;
 }
 
 // The incident field's plane case: k along z and a slice normal to z

@@ -235,14 +235,34 @@ __device__ inline void uf_batch_pixel(const UfInst& I, int pix, const double* __
 }
 
 // Work items of batch instance I's incident field: its pixels, or with
-// mirror symmetry (uf_mirror) its canonical pixels.
+// mirror symmetry (uf_mirror) its canonical pixels, or with the diagonal
+// symmetry as well the canonical pixels on and below the diagonal.
 __device__ inline int uf_batch_work(const UfInst& I) {
-    return I.ncx > 0 ? I.ncx * I.ncy : I.uR * I.vR;
+    if (I.ncx <= 0) return I.uR * I.vR;
+    return I.diag ? I.ncx * (I.ncx + 1) / 2 : I.ncx * I.ncy;
+}
+
+// Slice column (row) of canonical column (row) index c.
+__device__ inline int uf_canonical(int c, int nl, int h) { return c < nl ? c : h + (c - nl); }
+
+// Stores v at pixel (iu, iv) and at its mirror images in the slice (column
+// mx - iu, row my - iv, both), those inside the slice and distinct.
+template <class Out>
+__device__ inline void uf_store_mirrors(const UfInst& I, Out* F, int iu, int iv, dc v) {
+    const int pu = I.mx - iu, pv = I.my - iv;
+    const bool mu = pu >= 0 && pu < I.uR && pu != iu, mv = pv >= 0 && pv < I.vR && pv != iv;
+    store(F, iv * I.uR + iu, v);
+    if (mu) store(F, iv * I.uR + pu, v);
+    if (mv) store(F, pv * I.uR + iu, v);
+    if (mu && mv) store(F, pv * I.uR + pu, v);
 }
 
 // Work item w of batch instance I: the pixel w, or the canonical pixel w
 // stored at each of its mirror images in the slice (its own column and
-// mx - that column, its row and my - that row, those inside the slice).
+// mx - that column, its row and my - that row, those inside the slice),
+// and with the diagonal symmetry (uf_mirror: column i and row i at the same
+// offsets from the focus) at the transposed pixel and its mirror images
+// too.
 template <class Out>
 __device__ inline void uf_batch_item(const UfInst& I, int w, const double* __restrict__ tabs,
                                      const double* __restrict__ waves, Out* __restrict__ Uf) {
@@ -250,11 +270,18 @@ __device__ inline void uf_batch_item(const UfInst& I, int w, const double* __res
         uf_batch_pixel(I, w, tabs, waves, Uf);
         return;
     }
-    const int cy = w / I.ncx, cx = w - cy * I.ncx;
-    const int iu = cx < I.nlx ? cx : I.hx + (cx - I.nlx);
-    const int iv = cy < I.nly ? cy : I.hy + (cy - I.nly);
-    const int pu = I.mx - iu, pv = I.my - iv;
-    const bool mu = pu >= 0 && pu < I.uR && pu != iu, mv = pv >= 0 && pv < I.vR && pv != iv;
+    int cx, cy;
+    if (I.diag) {
+        // w = cy (cy + 1) / 2 + cx, cx <= cy
+        cy = (int)((sqrtf(8.f * (float)w + 1.f) - 1.f) * 0.5f);
+        while (cy > 0 && cy * (cy + 1) / 2 > w) --cy;
+        while ((cy + 1) * (cy + 2) / 2 <= w) ++cy;
+        cx = w - cy * (cy + 1) / 2;
+    } else {
+        cy = w / I.ncx;
+        cx = w - cy * I.ncx;
+    }
+    const int iu = uf_canonical(cx, I.nlx, I.hx), iv = uf_canonical(cy, I.nly, I.hy);
     dc v;
     if constexpr (std::is_same<Out, float2>::value)
         v = uf_fast_value(I.q, I.uR, I.vR, iv * I.uR + iu, I.f, I.k, I.kmag, I.amp, I.ca, I.cb,
@@ -263,10 +290,8 @@ __device__ inline void uf_batch_item(const UfInst& I, int w, const double* __res
         v = uf_series_value(I.q, I.uR, I.vR, iv * I.uR + iu, I.f, I.k, I.kmag, I.amp, I.ca, I.cb,
                             I.m);
     Out* F = Uf + I.off;
-    store(F, iv * I.uR + iu, v);
-    if (mu) store(F, iv * I.uR + pu, v);
-    if (mv) store(F, pv * I.uR + iu, v);
-    if (mu && mv) store(F, pv * I.uR + pu, v);
+    uf_store_mirrors(I, F, iu, iv, v);
+    if (I.diag && cx != cy) uf_store_mirrors(I, F, iv, iu, v);
 }
 
 }  // namespace ufd
