@@ -93,7 +93,7 @@ struct SumInst {
     int uR, vR, s_begin, ns;
     long long off;
     long long image_off;   // its Nd x Nd images in D and Di
-    int nd2;               // Nd * Nd (<= uR * vR)
+    int nd2;               // Nd * Nd (<= uR * vR) where the pass clears the images, else 0
 };
 
 // Mie coefficients of n spheres, sphere i at wavenumber kmag[i]; writes

@@ -316,7 +316,10 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
             mc_list.push_back(i);
             b->mc_tiles_max = std::max(b->mc_tiles_max, mc_tiles(I.N, I.N));
         }
-        sum[i] = SumInst{q, I.N, I.N, (int)sph_dev.size(), I.ns, field, I.image_off, I.Nd * I.Nd};
+        // images to clear each pass: those of the instances whose detector
+        // adds to them (the batched detector writes its images whole)
+        const int nd2 = det_batchable(ctx, b->prec, I.N, I.ss, I.zero_fill) ? 0 : I.Nd * I.Nd;
+        sum[i] = SumInst{q, I.N, I.N, (int)sph_dev.size(), I.ns, field, I.image_off, nd2};
         for (int c = 0; c < 3; ++c) kdirs[3 * (size_t)i + c] = I.kdir[c];
         // spheres of this instance, coefficients packed
         SeriesBatch& sb = plane ? pw : mc;

@@ -525,7 +525,9 @@ __global__ __launch_bounds__(kThreads) void rows_inv_kernel(
 // filtered-field output, detect_from_intensity on those rows: D += |U|^2,
 // Di += |Uf|^2 (|.| squared, as the intensity rows), Abs = -log10(D / Di)
 // over the crop columns [off, off + Nd); the rows are those of the crop.
-template <class R, int N>
+// kAdd false: D = |U|^2 and Di = |Uf|^2 (the same bits as adding to
+// cleared images), for the batches, whose passes then clear nothing.
+template <class R, int N, bool kAdd = true>
 __device__ __forceinline__ void rows_inv_detect_body(
     const typename C2<R>::T* __restrict__ G, int Nc, int vR, const int* __restrict__ cols,
     int row_begin, int nrows, int B, int off, int Nd, R* __restrict__ D, R* __restrict__ Di,
@@ -550,7 +552,7 @@ __device__ __forceinline__ void rows_inv_detect_body(
         const C v = res[g * N + off + iu], w = res[(nb + g) * N + off + iu];
         const R m = sqrt(v.x * v.x + v.y * v.y), mf = sqrt(w.x * w.x + w.y * w.y);
         const size_t i = (size_t)(r0 - row_begin + g) * Nd + iu;
-        const R d = D[i] + m * m, di = Di[i] + mf * mf;
+        const R d = kAdd ? D[i] + m * m : m * m, di = kAdd ? Di[i] + mf * mf : mf * mf;
         D[i] = d;
         Di[i] = di;
         if (Abs) Abs[i] = -log10(d / di);
@@ -606,9 +608,9 @@ __global__ __launch_bounds__(kThreads) void rows_inv_detect_batch_kernel(
     for (int z = blockIdx.z; z < n; z += gridDim.z) {
         const DetInst& I = insts[z];
         if ((int)blockIdx.x * B >= I.Nd) continue;
-        rows_inv_detect_body<R, N>(G + I.g_off, I.Nc, N, cols + I.cols_off, I.off, I.Nd, B, I.off,
-                                   I.Nd, D + I.image_off, Di + I.image_off,
-                                   Abs ? Abs + I.image_off : nullptr, tw, blockIdx.x);
+        rows_inv_detect_body<R, N, false>(G + I.g_off, I.Nc, N, cols + I.cols_off, I.off, I.Nd, B,
+                                          I.off, I.Nd, D + I.image_off, Di + I.image_off,
+                                          Abs ? Abs + I.image_off : nullptr, tw, blockIdx.x);
         __syncthreads();
     }
 }
