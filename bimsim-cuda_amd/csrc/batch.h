@@ -80,6 +80,14 @@ inline void uf_mirror(UfInst& I) {
     I.diag = I.uR == I.vR && mx == my && q.A.x == q.A.y && q.X.x == q.Y.y && I.f.x == I.f.y ? 1 : 0;
 }
 
+// Work items of instance I's incident field: its pixels, or with mirror
+// symmetry its canonical pixels (on and below the diagonal with the
+// diagonal symmetry).
+__host__ __device__ inline int uf_work_items(const UfInst& I) {
+    if (I.ncx <= 0) return I.uR * I.vR;
+    return I.diag ? I.ncx * (I.ncx + 1) / 2 : I.ncx * I.ncy;
+}
+
 // The incident field's plane case: k along z and a slice normal to z
 // through the focus, so that k.(p - f) = 0 exactly at every pixel.
 inline bool uf_flat(const QuadArg& q, const d3& f, const d3& k) {
@@ -112,9 +120,11 @@ void uf_fast_table(double ca, double cb, int m, std::vector<double>* out);
 // complex128); tabs holds the instances' uf_fast_table rows (f32 only).
 // Monte-Carlo instances read their waves from `waves` (complex128 only).
 // With sum (one SumInst per instance), the launch also clears the
-// instances' images in D and Di (float when f32, else double).
+// instances' images in D and Di (float when f32, else double) where its
+// nd2 asks.  items: at least every instance's uf_work_items and nd2 (the
+// launch's pixel blocks per instance; it strides over more).
 struct SumInst;
-int uf_batch_launch(bimsim_ctx* ctx, const UfInst* d_inst, int n, int max_npix, int f32,
+int uf_batch_launch(bimsim_ctx* ctx, const UfInst* d_inst, int n, int items, int f32,
                     const double* tabs, const double* waves, void* Uf,
                     const SumInst* sum = nullptr, void* D = nullptr, void* Di = nullptr);
 

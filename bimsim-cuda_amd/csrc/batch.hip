@@ -40,6 +40,9 @@ struct bimsim_batch {
     void *D = nullptr, *Di = nullptr, *Abs = nullptr;
     size_t image_elems = 0;
     int n_sph = 0, lmax = 0, max_npix = 0;
+    // the incident-field launch's items per instance: the most of any
+    // instance's work items and the image pixels it clears
+    int uf_items = 1;
     long long ncoef = 0, field_elems = 0;
     std::vector<long long> field_off;
     // device
@@ -135,7 +138,7 @@ int enqueue_fp64_fields(bimsim_batch* b) {
     const size_t fb = sizeof(double2);
     BIMSIM_TRY(mie_batch_launch(ctx, b->d_sph, b->d_kmag, b->n_sph, b->lmax, b->quirks, b->d_work,
                                 b->d_A, b->d_B));
-    BIMSIM_TRY(uf_batch_launch(ctx, b->d_uf, b->n, b->max_npix, b->f32, b->d_uftab, b->waves,
+    BIMSIM_TRY(uf_batch_launch(ctx, b->d_uf, b->n, b->uf_items, b->f32, b->d_uftab, b->waves,
                                b->d_Uf));
     for (int i = 0; i < b->n; ++i) {
         const bimsim_instance& I = b->inst[i];
@@ -194,7 +197,7 @@ int enqueue(bimsim_batch* b) {
         ctx->stream = b->side_f;
         // (with the sum formed by the detectors, this launch clears the images)
         if (rc == BIMSIM_OK)
-            rc = uf_batch_launch(ctx, b->d_uf, b->n, b->max_npix, b->f32, b->d_uftab, b->waves,
+            rc = uf_batch_launch(ctx, b->d_uf, b->n, b->uf_items, b->f32, b->d_uftab, b->waves,
                                  b->d_Uf, b->fuse_sum ? b->d_sum : nullptr, b->D, b->Di);
         if (rc == BIMSIM_OK)
             rc = uf_mc_batch_launch(ctx, b->d_uf, b->d_mc_list, b->n_mc_uf, b->mc_tiles_max,
@@ -380,6 +383,8 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
     BIMSIM_TRY(dev_upload(ctx, &b->d_kmag, sph_k));
     BIMSIM_TRY(dev_upload(ctx, &b->d_uf, uf));
     BIMSIM_TRY(dev_upload(ctx, &b->d_uftab, uftab));
+    for (int i = 0; i < n; ++i)
+        b->uf_items = std::max(b->uf_items, std::max(uf_work_items(uf[i]), sum[i].nd2));
     BIMSIM_TRY(dev_upload(ctx, &b->d_sum, sum));
     BIMSIM_TRY(dev_upload(ctx, &b->d_sph4, sph4));
     BIMSIM_TRY(dev_upload(ctx, &b->d_kdirs, kdirs));
@@ -412,7 +417,7 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
         const int wstride = b->lmax + 2;
         b->pre = PrepassArgs{b->d_sph, b->d_kmag, b->n_sph, mie::mie_spheres_per_block(wstride),
                              wstride, b->quirks, b->d_A, b->d_B, b->d_uf, b->n,
-                             (b->max_npix + 255) / 256, b->d_uftab, b->waves, (float2*)b->d_Uf};
+                             (b->uf_items + 255) / 256, b->d_uftab, b->waves, (float2*)b->d_Uf};
         const SeriesPlan& P = b->have_mc ? b->plan_mc : b->plan_pw;
         b->prepass = b->have_mc != b->have_pw && P.harmonic && b->n_sph > 0 &&
                      prepass_lds(b->pre) <= kPrepassLds;

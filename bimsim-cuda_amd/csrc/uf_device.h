@@ -237,10 +237,7 @@ __device__ inline void uf_batch_pixel(const UfInst& I, int pix, const double* __
 // Work items of batch instance I's incident field: its pixels, or with
 // mirror symmetry (uf_mirror) its canonical pixels, or with the diagonal
 // symmetry as well the canonical pixels on and below the diagonal.
-__device__ inline int uf_batch_work(const UfInst& I) {
-    if (I.ncx <= 0) return I.uR * I.vR;
-    return I.diag ? I.ncx * (I.ncx + 1) / 2 : I.ncx * I.ncy;
-}
+__device__ inline int uf_batch_work(const UfInst& I) { return uf_work_items(I); }
 
 // Slice column (row) of canonical column (row) index c.
 __device__ inline int uf_canonical(int c, int nl, int h) { return c < nl ? c : h + (c - nl); }
