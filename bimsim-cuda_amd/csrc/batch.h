@@ -113,8 +113,11 @@ int mie_batch_launch(bimsim_ctx* ctx, const SphereDev* d_sph, const double* d_km
 // form: for l = 0..m the four values (i^l sign) * D_l, (2l-1)/l, (l-1)/l
 // and 2l-1 (the ratios unused at l = 0); D_l from the same FP64 Legendre
 // recurrences as the per-pixel reference form, so the values are
-// identical.  Appends 4 (m + 1) doubles.
-void uf_fast_table(double ca, double cb, int m, std::vector<double>* out);
+// identical.  flat (uf_flat: k.r = 0 at every pixel): the first value of
+// each even order l >= 2 is D_l times P_l(0) instead (P_l(0) =
+// -((l-1)/l) P_{l-2}(0), in that order), the whole weight of its term.
+// Appends 4 (m + 1) doubles.
+void uf_fast_table(double ca, double cb, int m, std::vector<double>* out, bool flat = false);
 
 // Incident fields of n instances into Uf (complex64 when f32, else
 // complex128); tabs holds the instances' uf_fast_table rows (f32 only).

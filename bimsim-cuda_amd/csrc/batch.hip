@@ -308,7 +308,7 @@ int build(bimsim_batch* b, const bimsim_instance* inst, int n, const bimsim_sphe
         long long tab = -1;
         if (b->f32 && I.uf_mode == BIMSIM_UF_SERIES) {
             tab = (long long)uftab.size();
-            uf_fast_table(ca, cb, I.m, &uftab);
+            uf_fast_table(ca, cb, I.m, &uftab, uf_flat(q, mk(I.focus), mk(I.kdir)));
         }
         uf[i] = UfInst{q, mk(I.focus), mk(I.kdir), b->kmag[i], I.amp, ca, cb, I.m, I.N, I.N,
                        I.uf_mode, field, tab, uf_flat(q, mk(I.focus), mk(I.kdir)) ? 1 : 0,

@@ -253,7 +253,7 @@ __global__ void add_outside_kernel(int npix, const unsigned char* __restrict__ m
 
 }  // namespace
 
-void uf_fast_table(double ca, double cb, int m, std::vector<double>* out) {
+void uf_fast_table(double ca, double cb, int m, std::vector<double>* out, bool flat) {
     const size_t base = out->size();
     out->resize(base + 4 * (size_t)(m + 1), 0.0);
     double* T = out->data() + base;
@@ -278,6 +278,13 @@ void uf_fast_table(double ca, double cb, int m, std::vector<double>* out) {
             T[4 * l + 2] = (l - 1.0) / l;
         }
         T[4 * l + 3] = 2.0 * l - 1.0;
+    }
+    if (flat) {
+        double P = 1.0;
+        for (int l = 2; l <= m; l += 2) {
+            P = -(T[4 * l + 2] * P);
+            T[4 * l] = P * T[4 * l];
+        }
     }
 }
 
@@ -373,7 +380,7 @@ extern "C" int bimsim_nf_uf(bimsim_ctx* ctx, const bimsim_quad* pos, int uR, int
     } else {
         if (f32) {
             std::vector<double> tab;
-            uf_fast_table(cos_alpha, cos_beta, m, &tab);
+            uf_fast_table(cos_alpha, cos_beta, m, &tab, uf_flat(q, f, k));
             const double* T = nullptr;
             BIMSIM_TRY(dbl_table_get(ctx, tab, &T));
             hipLaunchKernelGGL(uf_fast_kernel, grid, dim3(kThreads), 0, ctx->stream, q, uR, vR, f, k,

@@ -11,6 +11,7 @@
 #include "geometry.h"
 #include "internal.h"
 #include "refmath.h"
+#include "sincos_fast.h"
 
 namespace bimsim {
 namespace ufd {
@@ -90,15 +91,19 @@ __device__ inline dc uf_fast_value(const QuadArg& q, int uR, int vR, int pix, d3
     pixel_of(pix, uR, iu, iv);
     const d3 p = quad_at(q, (double)iu / (double)uR, (double)iv / (double)vR);
     d3 r = sub(p, f);
-    const double d = len(r);
+    // the distance and sin, cos of kd by the FP32 forms' set-up helpers
+    // (sincos_fast.h), their reciprocals by Newton steps
+    double d, inv_d;
+    dist_rsq(dot(r, r), d, inv_d);
     if (d < (double)kEpsFloat) return dc{amp * 2.0 * (double)kPIf * (ca - cb), 0.0};
-    r = norm(r);
+    r = scale(r, inv_d);
     const double kd = kmag * d;
-    const double inv = 1.0 / kd;
+    const double inv = rcp_newton(kd);
     // l > kd  <=>  l >= floor(kd) + 1 for kd >= 0
     const int lk = (int)fmin(floor(kd), 1e9) + 1;
     double sn, cs;
-    sincos(kd, &sn, &cs);
+    if (__all(kd < 1048576.0)) sincos_reduced(kd, &sn, &cs);
+    else sincos(kd, &sn, &cs);
     double j0 = sn * inv;
     double j1 = fma(j0, inv, -(cs * inv));
     // one order: j_l from the recurrence (threshold rule where l > kd), then
@@ -117,14 +122,12 @@ __device__ inline dc uf_fast_value(const QuadArg& q, int uR, int vR, int pix, d3
     constexpr int kB = 4;
     if (flat) {
         // k.r = 0 at every pixel: P_l(0) = -((l-1)/l) P_{l-2}(0) and 0 for odd
-        // l, so the series is real; the values of the general form at ct = 0
-        double ar = j0 * T[0], P = 1.0;
+        // l, so the series is real; the flat table holds D_l P_l(0) for the
+        // even orders (uf_fast_table), one FMA per term
+        double ar = j0 * T[0];
         auto fstep = [&](int l, const double* t, bool chk) {
             const double jn = jstep(l, t, chk);
-            if (!(l & 1)) {
-                P = -(t[2] * P);
-                ar = fma(jn * P, t[0], ar);
-            }
+            if (!(l & 1)) ar = fma(jn, t[0], ar);
         };
         int l = 2;
         for (; l + kB - 1 <= m; l += kB) {

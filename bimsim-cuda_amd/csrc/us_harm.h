@@ -65,29 +65,6 @@ struct RadialState {
     int kfl;           // floor(kx), at most 2^30: order l is past kx where l > kfl
 };
 
-// The distance d = sqrt(s) of a pixel from a sphere's centre and 1/d, from
-// s = |r|^2 >= 0: v_rsq_f64's estimate y, one step of Goldschmidt's coupled
-// iteration from g = s y -> sqrt(s), h = y/2 -> 1/(2 sqrt(s)), two
-// corrections g += (s - g^2) h, and a Newton step for 1/g from 2h.  d comes
-// out correctly rounded and 1/d within two ulps (for estimates y within
-// 2^-22), in some 15 VALU instructions against some 30 of the square root
-// and the division.  s is never denormal here; s = 0 (a pixel at the centre
-// itself) gives d = 0 and 1/d = inf, as the division does.
-__device__ __forceinline__ void harm_dist(double s, double& d, double& inv_d) {
-    const double y = __builtin_amdgcn_rsq(s);
-    double g = s * y, h = 0.5 * y;
-    const double e = fma(-g, h, 0.5);
-    g = fma(g, e, g);
-    h = fma(h, e, h);
-    g = fma(fma(-g, g, s), h, g);
-    g = fma(fma(-g, g, s), h, g);
-    double v = h + h;
-    v = fma(v, fma(-g, v, 1.0), v);
-    const bool zero = s == 0.0;
-    d = zero ? 0.0 : g;
-    inv_d = zero ? __builtin_inf() : v;
-}
-
 // whether the condition holds on some lane of the wave: the ballot of the
 // lane mask itself (HIP's __any(int) turns the mask into 0/1 values and
 // compares them, two VALU instructions)
@@ -848,7 +825,7 @@ __global__ __launch_bounds__(kHarmThreads, kWaves) void series_harm_kernel(
             const int nl1 = S.Nl + 1;
             d3 r = sub(p, mk(S.p));
             double d, inv_d;
-            harm_dist(dot(r, r), d, inv_d);
+            dist_rsq(dot(r, r), d, inv_d);
             // direction; the centre (knd below float epsilon) keeps only l = 0
             const bool centre = d <= S.a && S.n[0] * (I.kmag * d) < (double)kEpsFloat;
             r = centre ? d3{0.0, 0.0, 0.0} : scale(r, inv_d);
