@@ -81,11 +81,15 @@ inline void uf_mirror(UfInst& I) {
 }
 
 // Work items of instance I's incident field: its pixels, or with mirror
-// symmetry its canonical pixels (on and below the diagonal with the
-// diagonal symmetry).
+// symmetry its canonical pixels, or with the diagonal symmetry too 64 per
+// 8 x 8 tile of canonical pixels on and below the diagonal (so that the
+// transposed pixels a wave stores lie in eight runs of eight, as its own
+// do).
 __host__ __device__ inline int uf_work_items(const UfInst& I) {
     if (I.ncx <= 0) return I.uR * I.vR;
-    return I.diag ? I.ncx * (I.ncx + 1) / 2 : I.ncx * I.ncy;
+    if (!I.diag) return I.ncx * I.ncy;
+    const int t = (I.ncx + 7) / 8;
+    return 64 * (t * (t + 1) / 2);
 }
 
 // The incident field's plane case: k along z and a slice normal to z

@@ -272,11 +272,15 @@ __device__ inline void uf_batch_item(const UfInst& I, int w, const double* __res
     }
     int cx, cy;
     if (I.diag) {
-        // w = cy (cy + 1) / 2 + cx, cx <= cy
-        cy = (int)((sqrtf(8.f * (float)w + 1.f) - 1.f) * 0.5f);
-        while (cy > 0 && cy * (cy + 1) / 2 > w) --cy;
-        while ((cy + 1) * (cy + 2) / 2 <= w) ++cy;
-        cx = w - cy * (cy + 1) / 2;
+        // tile t = ty (ty + 1) / 2 + tx, tx <= ty, of 8 x 8 canonical pixels
+        const int t = w >> 6, lane = w & 63;
+        int ty = (int)((sqrtf(8.f * (float)t + 1.f) - 1.f) * 0.5f);
+        while (ty > 0 && ty * (ty + 1) / 2 > t) --ty;
+        while ((ty + 1) * (ty + 2) / 2 <= t) ++ty;
+        const int tx = t - ty * (ty + 1) / 2;
+        cx = 8 * tx + (lane & 7);
+        cy = 8 * ty + (lane >> 3);
+        if (cx > cy || cy >= I.ncx) return;
     } else {
         cy = w / I.ncx;
         cx = w - cy * I.ncx;
