@@ -434,6 +434,11 @@ __device__ __forceinline__ void harm_records_row(
     const int nlv = (I.lh - m + lstep - 1) / lstep;   // orders l = m + j lstep
     const double* Prow = pl + I.pl_off + tri_off(m, I.lh) * I.nk;
     const double2* E = e + I.e_off + (long long)m * I.nk;
+    // the wave weights of the instance's spheres follow each other, nk_pad
+    // apart
+    const long long wbase = sph[I.s_begin].woff;
+    static_assert((kRecW & (kRecW - 1)) == 0, "chunks of a power of two waves");
+    constexpr int kRecWLog = __builtin_ctz(kRecW);
     for (int s0 = 0; s0 < I.ns; s0 += ts) {
         const int nst = min(ts, I.ns - s0);
         const int nq = (nst + 3) >> 2;
@@ -452,14 +457,18 @@ __device__ __forceinline__ void harm_records_row(
                 for (int w0 = 0; w0 < I.nk; w0 += kRecW) {
                     const int nw = min(kRecW, I.nk - w0);
                     __syncthreads();   // the previous chunk is consumed
-                    for (int x = t; x < nw * nlt; x += NT) {
-                        const int w = x / nlt, jj = x - w * nlt;
-                        const int l = m + (j0 + jj) * lstep;
-                        sP[w * tl + jj] = Prow[(long long)(l - m) * I.nk + w0 + w];
+                    // (consecutive threads take consecutive waves: the
+                    // element's wave and order or sphere by a mask and a
+                    // shift, no division)
+                    for (int x = t; x < kRecW * nlt; x += NT) {
+                        const int w = x & (kRecW - 1), jj = x >> kRecWLog;
+                        if (w < nw)
+                            sP[w * tl + jj] =
+                                Prow[(long long)((j0 + jj) * lstep) * I.nk + w0 + w];
                     }
-                    for (int x = t; x < nw * nst; x += NT) {
-                        const int w = x / nst, ss = x - w * nst;
-                        sF[w * ts + ss] = phi[sph[I.s_begin + s0 + ss].woff + w0 + w];
+                    for (int x = t; x < kRecW * nst; x += NT) {
+                        const int w = x & (kRecW - 1), ss = x >> kRecWLog;
+                        if (w < nw) sF[w * ts + ss] = phi[wbase + (long long)(s0 + ss) * I.nk_pad + w0 + w];
                     }
                     if (t < nw) sE[t] = E[w0 + t];
                     __syncthreads();
