@@ -291,7 +291,9 @@ void uf_fast_table(double ca, double cb, int m, std::vector<double>* out, bool f
 int uf_batch_launch(bimsim_ctx* ctx, const UfInst* d_inst, int n, int items, int f32,
                     const double* tabs, const double* waves, void* Uf, const SumInst* sum,
                     void* D, void* Di) {
-    const dim3 grid((items + kThreads - 1) / kThreads, std::min(n, 65535));
+    // four work items a thread (the launch strides over them): a block's
+    // items are short, and fewer blocks spend less on starting them
+    const dim3 grid((items + 4 * kThreads - 1) / (4 * kThreads), std::min(n, 65535));
     if (f32)
         hipLaunchKernelGGL((uf_batch_kernel<float2, float>), grid, dim3(kThreads), 0, ctx->stream,
                            d_inst, n, tabs, waves, (float2*)Uf, sum, (float*)D, (float*)Di);
