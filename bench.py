@@ -302,8 +302,8 @@ def headline(pkg, P, W, ctx, torch, args):
             "series_ms_per_pass": series_ms_pass,
             "peak_note": "157.3 is AMD's published FP32 peak (vector and matrix); this build "
                          "compiles without packed FP32 instructions: plain v_fma_f32 issues "
-                         "at about 4.9 cycles per wave, MFMA 4x4x1 at about 10 for four FMAs "
-                         "per lane (profiles/valu_mfma_probe.md)",
+                         "at about 4.7 cycles per wave, MFMA 4x4x1 at about 10 for four FMAs "
+                         "per lane, and the two hardly overlap (profiles/valu_mfma_probe.md)",
         },
     }
 
