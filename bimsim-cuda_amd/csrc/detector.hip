@@ -372,23 +372,45 @@ __global__ __launch_bounds__(kThreads) void sum_mask_kernel(SumFuse fz, unsigned
     mask[w] = bits;
 }
 
-// Forward FFT of B rows (length N) of U (blockIdx.y = 0) or Uf (= 1); the
+// Forward FFT of B rows (length N) of U (f = 0) or Uf (f = 1); the
 // pass-band columns cols[0..Nc) go to the compact spectrum
 // G[(field*Nc + c)*nrows + row].  kSum: U is the scattered field alone and
-// the rows of U load as fused_sum(fz, ...).
-template <class R, int N, bool kSum>
+// the rows of U load as fused_sum(fz, ...).  kBoth: the rows of both
+// fields in one transform of 2 nb rows (U's, then Uf's; f unused), each Uf
+// element loaded once for both.  kTwLds: the twiddles read from a copy in
+// LDS (N more elements past the transform's buffers), not through the
+// vector caches at every stage.
+template <class R, int N, bool kSum, bool kBoth = false, bool kTwLds = false>
 __device__ __forceinline__ void rows_fwd_body(
     const typename C2<R>::T* __restrict__ U, const typename C2<R>::T* __restrict__ Uf, int nrows,
     int B, const int* __restrict__ cols, int Nc, typename C2<R>::T* __restrict__ G,
     const typename C2<R>::T* __restrict__ tw, const SumFuse& fz, int blk, int f) {
     using C = typename C2<R>::T;
+    constexpr int kF = kBoth ? 2 : 1;   // fields in the transform
     extern __shared__ unsigned char smem_raw[];
     C* a = reinterpret_cast<C*>(smem_raw);
-    C* b = a + (size_t)B * N;
+    C* b = a + (size_t)kF * B * N;
+    if (kBoth) f = 0;
     const C* F = f ? Uf : U;
     const int row0 = blk * B;
     const int nb = min(B, nrows - row0);
-    if (kSum && f == 0 && fz.inside) {
+    const C* twp = tw;
+    if constexpr (kTwLds) {
+        C* tws = b + (size_t)kF * B * N;
+        for (int t = threadIdx.x; t < N; t += kThreads) tws[t] = tw[t];
+        twp = tws;   // (the barrier after the rows' loads covers the copy)
+    }
+    if (kBoth && !(kSum && fz.ns > 0 && !fz.inside)) {
+        // (the rows' loads issued together: four elements of each field a
+        // step)
+#pragma unroll 4
+        for (int t = threadIdx.x; t < nb * N; t += kThreads) {
+            const size_t i = (size_t)row0 * N + t;
+            const C u = U[i], w = Uf[i];
+            a[t] = kSum ? fused_sum(fz, (int)i, u, w) : u;
+            a[nb * N + t] = w;
+        }
+    } else if (kSum && f == 0 && fz.inside) {
         for (int t = threadIdx.x; t < nb * N; t += kThreads) {
             const size_t i = (size_t)row0 * N + t;
             a[t] = fused_sum(fz, (int)i, U[i], Uf[i]);
@@ -426,16 +448,19 @@ __device__ __forceinline__ void rows_fwd_body(
         const unsigned long long near = s_near;
         for (int t = threadIdx.x; t < nb * N; t += kThreads) {
             const size_t i = (size_t)row0 * N + t;
-            a[t] = fused_sum(fz, (int)i, U[i], Uf[i], near);
+            const C w = Uf[i];
+            a[t] = fused_sum(fz, (int)i, U[i], w, near);
+            if (kBoth) a[nb * N + t] = w;
         }
     } else {
         for (int t = threadIdx.x; t < nb * N; t += kThreads) a[t] = F[(size_t)row0 * N + t];
     }
     __syncthreads();
-    const C* res = fixed_fft<R, N>(a, b, nb, tw, false);
-    for (int t = threadIdx.x; t < nb * Nc; t += kThreads) {
-        const int g = t / Nc, c = t - g * Nc;
-        G[((size_t)f * Nc + c) * nrows + row0 + g] = res[g * N + cols[c]];
+    const C* res = fixed_fft<R, N>(a, b, kF * nb, twp, false);
+    for (int t = threadIdx.x; t < kF * nb * Nc; t += kThreads) {
+        const int h = kBoth ? t / (nb * Nc) : 0, u = t - h * (nb * Nc);
+        const int g = u / Nc, c = u - g * Nc;
+        G[((size_t)(f + h) * Nc + c) * nrows + row0 + g] = res[(h * nb + g) * N + cols[c]];
     }
 }
 
@@ -527,7 +552,8 @@ __global__ __launch_bounds__(kThreads) void rows_inv_kernel(
 // over the crop columns [off, off + Nd); the rows are those of the crop.
 // kAdd false: D = |U|^2 and Di = |Uf|^2 (the same bits as adding to
 // cleared images), for the batches, whose passes then clear nothing.
-template <class R, int N, bool kAdd = true>
+// kTwLds: the twiddles read from a copy in LDS, as in rows_fwd_body.
+template <class R, int N, bool kAdd = true, bool kTwLds = false>
 __device__ __forceinline__ void rows_inv_detect_body(
     const typename C2<R>::T* __restrict__ G, int Nc, int vR, const int* __restrict__ cols,
     int row_begin, int nrows, int B, int off, int Nd, R* __restrict__ D, R* __restrict__ Di,
@@ -538,6 +564,12 @@ __device__ __forceinline__ void rows_inv_detect_body(
     C* b = a + (size_t)2 * B * N;
     const int r0 = row_begin + blk * B;
     const int nb = min(B, row_begin + nrows - r0);
+    const C* twp = tw;
+    if constexpr (kTwLds) {
+        C* tws = b + (size_t)2 * B * N;
+        for (int t = threadIdx.x; t < N; t += kThreads) tws[t] = tw[t];
+        twp = tws;
+    }
     for (int t = threadIdx.x; t < 2 * nb * N; t += kThreads) a[t] = cmk<C>((R)0, (R)0);
     __syncthreads();
     for (int t = threadIdx.x; t < 2 * nb * Nc; t += kThreads) {
@@ -546,7 +578,7 @@ __device__ __forceinline__ void rows_inv_detect_body(
         a[(f * nb + g) * N + cols[c]] = G[((size_t)f * Nc + c) * vR + r0 + g];
     }
     __syncthreads();
-    const C* res = fixed_fft<R, N>(a, b, 2 * nb, tw, true);
+    const C* res = fixed_fft<R, N>(a, b, 2 * nb, twp, true);
     for (int t = threadIdx.x; t < nb * Nd; t += kThreads) {
         const int g = t / Nd, iu = t - g * Nd;
         const C v = res[g * N + off + iu], w = res[(nb + g) * N + off + iu];
@@ -573,7 +605,7 @@ __global__ __launch_bounds__(kThreads) void rows_inv_detect_kernel(
 // block's instance from DetInst[blockIdx.z], its fields, columns, compact
 // spectrum and images at their offsets.  Blocks past an instance's rows or
 // columns return (uniformly per block).
-template <class R, int N, bool kSum>
+template <class R, int N, bool kSum, bool kTwLds>
 __global__ __launch_bounds__(kThreads) void rows_fwd_batch_kernel(
     const DetInst* __restrict__ insts, int n, const typename C2<R>::T* __restrict__ U,
     const typename C2<R>::T* __restrict__ Uf, int B, const int* __restrict__ cols,
@@ -581,8 +613,8 @@ __global__ __launch_bounds__(kThreads) void rows_fwd_batch_kernel(
     for (int z = blockIdx.z; z < n; z += gridDim.z) {
         const DetInst& I = insts[z];
         if (I.Nc == 0) continue;
-        rows_fwd_body<R, N, kSum>(U + I.uoff, Uf + I.uoff, N, B, cols + I.cols_off, I.Nc,
-                                  G + I.g_off, tw, I.fz, blockIdx.x, blockIdx.y);
+        rows_fwd_body<R, N, kSum, true, kTwLds>(U + I.uoff, Uf + I.uoff, N, B, cols + I.cols_off,
+                                                I.Nc, G + I.g_off, tw, I.fz, blockIdx.x, 0);
         __syncthreads();   // the block's LDS is reused by its next instance
     }
 }
@@ -600,7 +632,7 @@ __global__ __launch_bounds__(kThreads) void cols_band_batch_kernel(
     }
 }
 
-template <class R, int N>
+template <class R, int N, bool kTwLds>
 __global__ __launch_bounds__(kThreads) void rows_inv_detect_batch_kernel(
     const DetInst* __restrict__ insts, int n, int B, const int* __restrict__ cols,
     const typename C2<R>::T* __restrict__ G, R* __restrict__ D, R* __restrict__ Di,
@@ -608,9 +640,10 @@ __global__ __launch_bounds__(kThreads) void rows_inv_detect_batch_kernel(
     for (int z = blockIdx.z; z < n; z += gridDim.z) {
         const DetInst& I = insts[z];
         if ((int)blockIdx.x * B >= I.Nd) continue;
-        rows_inv_detect_body<R, N, false>(G + I.g_off, I.Nc, N, cols + I.cols_off, I.off, I.Nd, B,
-                                          I.off, I.Nd, D + I.image_off, Di + I.image_off,
-                                          Abs ? Abs + I.image_off : nullptr, tw, blockIdx.x);
+        rows_inv_detect_body<R, N, false, kTwLds>(G + I.g_off, I.Nc, N, cols + I.cols_off, I.off,
+                                                  I.Nd, B, I.off, I.Nd, D + I.image_off,
+                                                  Di + I.image_off,
+                                                  Abs ? Abs + I.image_off : nullptr, tw, blockIdx.x);
         __syncthreads();
     }
 }
@@ -1063,32 +1096,42 @@ int det_batch_launch_impl(bimsim_ctx* ctx, const DetBatch& b, const void* U, con
     const DetInst* I = (const DetInst*)b.d_inst;
     const C* tw = (const C*)b.tw;
     const unsigned nz = (unsigned)std::min(b.n, 65535);
-    const size_t lds_r = sizeof(C) * 2 * (size_t)b.B * N;
+    const size_t lds_r = sizeof(C) * 2 * 2 * (size_t)b.B * N;   // both fields' rows
     const size_t lds_c = sizeof(C) * 2 * (size_t)b.Bc * N;
+    const size_t lds_i = sizeof(C) * 2 * 2 * (size_t)b.B2 * N;
+    // the row passes read their twiddles from LDS where the N more elements fit
+    const size_t lds_tw = sizeof(C) * (size_t)N;
+    const bool tw_r = lds_r + lds_tw <= ctx->lds_max, tw_i = lds_i + lds_tw <= ctx->lds_max;
     const hipStream_t st = ctx->stream;
     constexpr bool f32 = std::is_same<R, float>::value;
+    const bool sum = f32 && b.fused;
+    const dim3 grid_r((N + b.B - 1) / b.B, 1, nz), grid_i((b.max_nd + b.B2 - 1) / b.B2, 1, nz);
     switch (N) {
+#define BIMSIM_FWD(NN, S, T)                                                                       \
+    hipLaunchKernelGGL((rows_fwd_batch_kernel<R, NN, S, T>), grid_r, dim3(kThreads),               \
+                       lds_r + (T ? lds_tw : 0), st, I, b.n, (const C*)U, (const C*)Uf, b.B,       \
+                       b.d_cols, (C*)b.d_G, tw)
+#define BIMSIM_INV(NN, T)                                                                          \
+    hipLaunchKernelGGL((rows_inv_detect_batch_kernel<R, NN, T>), grid_i, dim3(kThreads),           \
+                       lds_i + (T ? lds_tw : 0), st, I, b.n, b.B2, b.d_cols, (const C*)b.d_G,      \
+                       (R*)D, (R*)Di, (R*)Abs, tw)
 #define BIMSIM_X(NN)                                                                               \
     case NN:                                                                                       \
-        if (f32 && b.fused)                                                                        \
-            hipLaunchKernelGGL((rows_fwd_batch_kernel<R, NN, f32>), dim3((NN + b.B - 1) / b.B, 2, nz), \
-                               dim3(kThreads), lds_r, st, I, b.n, (const C*)U, (const C*)Uf, b.B,  \
-                               b.d_cols, (C*)b.d_G, tw);                                           \
-        else                                                                                       \
-            hipLaunchKernelGGL((rows_fwd_batch_kernel<R, NN, false>),                              \
-                               dim3((NN + b.B - 1) / b.B, 2, nz), dim3(kThreads), lds_r, st, I,    \
-                               b.n, (const C*)U, (const C*)Uf, b.B, b.d_cols, (C*)b.d_G, tw);     \
+        if (sum && tw_r) BIMSIM_FWD(NN, f32, true);                                                \
+        else if (sum) BIMSIM_FWD(NN, f32, false);                                                  \
+        else if (tw_r) BIMSIM_FWD(NN, false, true);                                                \
+        else BIMSIM_FWD(NN, false, false);                                                         \
         if (b.max_nc > 0)                                                                          \
             hipLaunchKernelGGL((cols_band_batch_kernel<R, NN>),                                    \
                                dim3((b.max_nc + b.Bc - 1) / b.Bc, 2, nz), dim3(kThreads), lds_c,   \
                                st, I, b.n, b.Bc, b.d_cols, (C*)b.d_G, tw);                        \
-        hipLaunchKernelGGL((rows_inv_detect_batch_kernel<R, NN>),                                  \
-                           dim3((b.max_nd + b.B2 - 1) / b.B2, 1, nz), dim3(kThreads),              \
-                           2 * sizeof(C) * 2 * (size_t)b.B2 * NN, st, I, b.n, b.B2, b.d_cols,      \
-                           (const C*)b.d_G, (R*)D, (R*)Di, (R*)Abs, tw);                          \
+        if (tw_i) BIMSIM_INV(NN, true);                                                            \
+        else BIMSIM_INV(NN, false);                                                                \
         break;
         BIMSIM_FFT_SIZES(BIMSIM_X)
 #undef BIMSIM_X
+#undef BIMSIM_INV
+#undef BIMSIM_FWD
         default:
             return set_error(ctx, BIMSIM_E_UNSUPPORTED, "no batched detector for N = %d", N);
     }
