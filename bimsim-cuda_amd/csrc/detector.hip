@@ -372,6 +372,23 @@ __global__ __launch_bounds__(kThreads) void sum_mask_kernel(SumFuse fz, unsigned
     mask[w] = bits;
 }
 
+// A thread's share of tw[0, N), loaded into registers at a kernel's start
+// (beside its other first loads) and stored to a copy in LDS later.
+template <class C, int N>
+struct TwShare {
+    static constexpr int kPer = (N + kThreads - 1) / kThreads;
+    C v[kPer];
+    __device__ __forceinline__ void load(const C* __restrict__ tw) {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) v[k] = tw[min((int)threadIdx.x + k * kThreads, N - 1)];
+    }
+    __device__ __forceinline__ void store(C* T) const {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+            if ((int)threadIdx.x + k * kThreads < N) T[threadIdx.x + k * kThreads] = v[k];
+    }
+};
+
 // Forward FFT of B rows (length N) of U (f = 0) or Uf (f = 1); the
 // pass-band columns cols[0..Nc) go to the compact spectrum
 // G[(field*Nc + c)*nrows + row].  kSum: U is the scattered field alone and
@@ -395,20 +412,45 @@ __device__ __forceinline__ void rows_fwd_body(
     const int row0 = blk * B;
     const int nb = min(B, nrows - row0);
     const C* twp = tw;
+    TwShare<C, N> tws;
     if constexpr (kTwLds) {
-        C* tws = b + (size_t)kF * B * N;
-        for (int t = threadIdx.x; t < N; t += kThreads) tws[t] = tw[t];
-        twp = tws;   // (the barrier after the rows' loads covers the copy)
+        tws.load(tw);
+        twp = b + (size_t)kF * B * N;
     }
     if (kBoth && !(kSum && fz.ns > 0 && !fz.inside)) {
-        // (the rows' loads issued together: four elements of each field a
-        // step)
-#pragma unroll 4
-        for (int t = threadIdx.x; t < nb * N; t += kThreads) {
-            const size_t i = (size_t)row0 * N + t;
-            const C u = U[i], w = Uf[i];
-            a[t] = kSum ? fused_sum(fz, (int)i, u, w) : u;
-            a[nb * N + t] = w;
+        // fused_sum with the inside bits (or no spheres), written out so that
+        // the loop body has no branch: four elements of each field (and
+        // their mask words) a step, every load issued before the first
+        // value is used
+        const int n = nb * N;
+        const bool none = !kSum || fz.ns == 0;
+        for (int t0 = threadIdx.x; t0 < n; t0 += 4 * kThreads) {
+            C u[4], w[4];
+            unsigned mk[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int t = min(t0 + k * kThreads, n - 1);
+                const size_t i = (size_t)row0 * N + t;
+                u[k] = U[i];
+                w[k] = Uf[i];
+                mk[k] = kSum && !none ? fz.inside[i >> 5] : 0u;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int t = t0 + k * kThreads;
+                if (t >= n) break;
+                const int i = row0 * N + t;
+                C v = u[k];
+                if (kSum) {
+                    const bool in = ((mk[k] >> (i & 31)) & 1u) != 0;
+                    C s = v;
+                    s.x = v.x + w[k].x;
+                    s.y = v.y + w[k].y;
+                    v = none ? w[k] : (in ? v : s);
+                }
+                a[t] = v;
+                a[n + t] = w[k];
+            }
         }
     } else if (kSum && f == 0 && fz.inside) {
         for (int t = threadIdx.x; t < nb * N; t += kThreads) {
@@ -455,12 +497,19 @@ __device__ __forceinline__ void rows_fwd_body(
     } else {
         for (int t = threadIdx.x; t < nb * N; t += kThreads) a[t] = F[(size_t)row0 * N + t];
     }
+    // (the barrier below covers the copy)
+    if constexpr (kTwLds) tws.store(b + (size_t)kF * B * N);
+    // the column of the thread's first output, read before the transform
+    // (the rest, if any, as they come)
+    const int nout = kF * nb * Nc;
+    const int col0 = (int)threadIdx.x < nout ? cols[(int)threadIdx.x % Nc] : 0;
     __syncthreads();
     const C* res = fixed_fft<R, N>(a, b, kF * nb, twp, false);
-    for (int t = threadIdx.x; t < kF * nb * Nc; t += kThreads) {
+    for (int t = threadIdx.x; t < nout; t += kThreads) {
         const int h = kBoth ? t / (nb * Nc) : 0, u = t - h * (nb * Nc);
         const int g = u / Nc, c = u - g * Nc;
-        G[((size_t)(f + h) * Nc + c) * nrows + row0 + g] = res[(h * nb + g) * N + cols[c]];
+        const int col = t == (int)threadIdx.x ? col0 : cols[c];
+        G[((size_t)(f + h) * Nc + c) * nrows + row0 + g] = res[(h * nb + g) * N + col];
     }
 }
 
@@ -565,17 +614,31 @@ __device__ __forceinline__ void rows_inv_detect_body(
     const int r0 = row_begin + blk * B;
     const int nb = min(B, row_begin + nrows - r0);
     const C* twp = tw;
+    TwShare<C, N> tws;
     if constexpr (kTwLds) {
-        C* tws = b + (size_t)2 * B * N;
-        for (int t = threadIdx.x; t < N; t += kThreads) tws[t] = tw[t];
-        twp = tws;
+        tws.load(tw);
+        twp = b + (size_t)2 * B * N;
     }
-    for (int t = threadIdx.x; t < 2 * nb * N; t += kThreads) a[t] = cmk<C>((R)0, (R)0);
-    __syncthreads();
-    for (int t = threadIdx.x; t < 2 * nb * Nc; t += kThreads) {
+    // the spectrum element of the thread's first scatter and its column,
+    // loaded beside the twiddles before the clearing (the rest, if any, as
+    // they come)
+    const int nsc = 2 * nb * Nc;
+    auto spec = [&](int t, int* col) {
         const int f = t / (nb * Nc), u = t - f * (nb * Nc);
         const int g = u / Nc, c = u - g * Nc;
-        a[(f * nb + g) * N + cols[c]] = G[((size_t)f * Nc + c) * vR + r0 + g];
+        *col = (f * nb + g) * N + cols[c];
+        return G[((size_t)f * Nc + c) * vR + r0 + g];
+    };
+    int col0 = 0;
+    C v0 = cmk<C>((R)0, (R)0);
+    if ((int)threadIdx.x < nsc) v0 = spec(threadIdx.x, &col0);
+    for (int t = threadIdx.x; t < 2 * nb * N; t += kThreads) a[t] = cmk<C>((R)0, (R)0);
+    if constexpr (kTwLds) tws.store(b + (size_t)2 * B * N);
+    __syncthreads();
+    for (int t = threadIdx.x; t < nsc; t += kThreads) {
+        int col = col0;
+        const C v = t == (int)threadIdx.x ? v0 : spec(t, &col);
+        a[col] = v;
     }
     __syncthreads();
     const C* res = fixed_fft<R, N>(a, b, 2 * nb, twp, true);
