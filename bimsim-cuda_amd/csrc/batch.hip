@@ -208,12 +208,18 @@ int enqueue(bimsim_batch* b) {
         BIMSIM_TRY(rc);
         if (b->have_mc) BIMSIM_TRY(series_plan_launch_waves(ctx, b->plan_mc, b->waves));
         if (b->have_pw) BIMSIM_TRY(series_plan_launch_waves(ctx, b->plan_pw, b->d_kdirs));
+        // the incident fields join before the series sums (beside their
+        // persistent launches they would wait for free slots to the end),
+        // after the coefficient fold
         BIMSIM_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, b->ev_coef, 0));
-        BIMSIM_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, b->ev_fields, 0));
         if (b->have_mc)
-            BIMSIM_TRY(series_plan_launch_sums(ctx, b->plan_mc, b->d_A, b->d_B, (float2*)b->d_U));
+            BIMSIM_TRY(series_plan_launch_sums(ctx, b->plan_mc, b->d_A, b->d_B, (float2*)b->d_U,
+                                               b->ev_fields));
         if (b->have_pw)
-            BIMSIM_TRY(series_plan_launch_sums(ctx, b->plan_pw, b->d_A, b->d_B, (float2*)b->d_U));
+            BIMSIM_TRY(series_plan_launch_sums(ctx, b->plan_pw, b->d_A, b->d_B, (float2*)b->d_U,
+                                               b->ev_fields));
+        if (!b->have_mc && !b->have_pw)
+            BIMSIM_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, b->ev_fields, 0));
     } else {
         BIMSIM_TRY(enqueue_fp64_fields(b));
     }

@@ -283,8 +283,10 @@ int series_plan_launch_waves(bimsim_ctx* ctx, const SeriesPlan& plan, const doub
 // P must be a harmonic plan and prepass_lds(a) <= kPrepassLds.
 int series_plan_launch_prepass(bimsim_ctx* ctx, const SeriesPlan& P, const double* waves,
                                const PrepassArgs& a);
+// after_fold: an event the sums (not the records' coefficient fold before
+// them) wait for.
 int series_plan_launch_sums(bimsim_ctx* ctx, const SeriesPlan& plan, const double2* A,
-                            const double2* B, float2* U);
+                            const double2* B, float2* U, hipEvent_t after_fold = nullptr);
 
 // Order capacity for Nl + 1 orders, or -1.
 int series_capacity(int nl1);

@@ -454,24 +454,49 @@ __device__ __forceinline__ void harm_records_row(
                     for (int v = 0; v < 4; ++v)
 #pragma unroll
                         for (int q = 0; q < 4; ++q) acc[u][v][q] = 0.0;
+                // the azimuth phase a thread stages, loaded a chunk ahead
+                double2 ev = E[min(t, I.nk - 1)];
                 for (int w0 = 0; w0 < I.nk; w0 += kRecW) {
                     const int nw = min(kRecW, I.nk - w0);
                     __syncthreads();   // the previous chunk is consumed
                     // (consecutive threads take consecutive waves: the
                     // element's wave and order or sphere by a mask and a
-                    // shift, no division)
-                    for (int x = t; x < kRecW * nlt; x += NT) {
-                        const int w = x & (kRecW - 1), jj = x >> kRecWLog;
-                        if (w < nw)
-                            sP[w * tl + jj] =
-                                Prow[(long long)((j0 + jj) * lstep) * I.nk + w0 + w];
+                    // shift, no division.  Four elements of a table a step,
+                    // all loaded before any is stored, so that a step waits
+                    // on its loads once; the stores take no condition (one
+                    // would draw each load down beside its store): an index
+                    // past the table repeats its last element, and the rows
+                    // of waves past the chunk's, which no product reads, get
+                    // the chunk's last wave)
+                    for (int x0 = t; x0 < kRecW * nlt; x0 += 4 * NT) {
+                        double v[4];
+                        int xs[4];
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            xs[k] = min(x0 + k * NT, kRecW * nlt - 1);
+                            const int w = min(xs[k] & (kRecW - 1), nw - 1), jj = xs[k] >> kRecWLog;
+                            v[k] = Prow[(long long)((j0 + jj) * lstep) * I.nk + w0 + w];
+                        }
+#pragma unroll
+                        for (int k = 0; k < 4; ++k)
+                            sP[(xs[k] & (kRecW - 1)) * tl + (xs[k] >> kRecWLog)] = v[k];
                     }
-                    for (int x = t; x < kRecW * nst; x += NT) {
-                        const int w = x & (kRecW - 1), ss = x >> kRecWLog;
-                        if (w < nw) sF[w * ts + ss] = phi[wbase + (long long)(s0 + ss) * I.nk_pad + w0 + w];
+                    for (int x0 = t; x0 < kRecW * nst; x0 += 4 * NT) {
+                        double2 v[4];
+                        int xs[4];
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            xs[k] = min(x0 + k * NT, kRecW * nst - 1);
+                            const int w = min(xs[k] & (kRecW - 1), nw - 1), ss = xs[k] >> kRecWLog;
+                            v[k] = phi[wbase + (long long)(s0 + ss) * I.nk_pad + w0 + w];
+                        }
+#pragma unroll
+                        for (int k = 0; k < 4; ++k)
+                            sF[(xs[k] & (kRecW - 1)) * ts + (xs[k] >> kRecWLog)] = v[k];
                     }
-                    if (t < nw) sE[t] = E[w0 + t];
+                    if (t < nw) sE[t] = ev;
                     __syncthreads();
+                    ev = E[min(w0 + kRecW + t, I.nk - 1)];   // the next chunk's
 #pragma unroll
                     for (int u = 0; u < kRecPer; ++u) {
                         const int c = c0 + u * NT + t;
@@ -1288,10 +1313,14 @@ int launch_one_bucket(bimsim_ctx* ctx, const SeriesPlan& P, const SeriesPlan::Bu
 }  // namespace
 
 int series_plan_launch_sums(bimsim_ctx* ctx, const SeriesPlan& P, const double2* A,
-                            const double2* B, float2* U) {
-    if (P.n_inst == 0 || P.n_sph == 0) return BIMSIM_OK;
+                            const double2* B, float2* U, hipEvent_t after_fold) {
     const hipStream_t st = ctx->stream;
+    if (P.n_inst == 0 || P.n_sph == 0) {
+        if (after_fold) BIMSIM_HIP_CHECK(ctx, hipStreamWaitEvent(st, after_fold, 0));
+        return BIMSIM_OK;
+    }
     if (P.harmonic) BIMSIM_TRY(fold_launch(ctx, P, A, B));
+    if (after_fold) BIMSIM_HIP_CHECK(ctx, hipStreamWaitEvent(st, after_fold, 0));
     for (const SeriesPlan::Chunk& ch : P.chunks) {
         if (!P.harmonic) {
             const dim3 cgrid((P.max_npix + kThreads - 1) / kThreads, std::min(ch.sph_count, 65535));
